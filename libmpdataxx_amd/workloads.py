@@ -1,0 +1,215 @@
+"""Initial conditions and advector fields for the standard MPDATA workloads.
+
+Pure numpy, FP64.  Every generator returns ``(psi, [advector(d) ...])`` laid
+out like the reference's ``advectee()`` / ``advector(d)`` views:
+``advector(d)`` spans faces -1..n_d-2 along d (index i <-> face i+1/2) and,
+for 3D components d > 0, x = -1..n0-1.  Arrays are C-order ``[x][y][z]``.
+
+Test problems follow the reference's own setups:
+  * revolving sphere 3D  -- tests/paper_2015_GMD/4_revolving_sphere_3d/
+                            revolving_sphere_3d.cpp:29-97
+  * rotating cone 2D     -- tests/paper_2015_GMD/3_rotating_cone_2d/
+                            rotating_cone_2d.cpp:32-140
+Benchmark workloads (no reference numbers exist for these shapes):
+  * gaussian_2d_cyclic   -- 256^2 cyclic, uniform Courant (0.25, 0.25)
+  * rotation_sphere_3d   -- N^3 open, solid-body rotation about z
+  * deform_multitracer   -- 512x512x128 cyclic, 8 tracers, reversing flow
+Random numbers come from numpy's PCG64 with fixed seeds.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def face_index(n, lo=-1):
+    return np.arange(lo, lo + n)
+
+
+# ---------------------------------------------------------------- reference
+
+def revolving_sphere_3d(nx=59):
+    """revolving_sphere_3d.cpp:29-97 (FP64, open boundaries)."""
+    dt = 0.018 * 2 * math.pi
+    L = 100.0
+    dx = L / (nx - 1)
+    dy = dz = dx
+    h = 4.0
+    r = 15.0
+    d = 25 / math.sqrt(3)
+    x0, y0, z0 = 50 - d, 50 + d, 50 + d
+    i = np.arange(nx, dtype=np.float64)[:, None, None]
+    j = np.arange(nx, dtype=np.float64)[None, :, None]
+    k = np.arange(nx, dtype=np.float64)[None, None, :]
+    tmp = (i * dx - x0) ** 2 + (j * dx - y0) ** 2 + (k * dx - z0) ** 2
+    psi = np.where(tmp - r ** 2 <= 0, h, 0.0)
+    omega = 0.1
+    xc = yc = zc = 50.0
+    c = omega / math.sqrt(3)
+    ia = np.arange(-1, nx, dtype=np.float64)            # x = -1..n-1
+    gx = np.empty((nx, nx, nx))
+    jj = np.arange(nx, dtype=np.float64)[None, :, None]
+    kk = np.arange(nx, dtype=np.float64)[None, None, :]
+    gx[:] = c * (-(jj * dy - yc) + (kk * dz - zc)) * dt / dx
+    gy = np.empty((nx + 1, nx - 1, nx))
+    gy[:] = c * ((ia[:, None, None] * dx - xc) - (kk * dz - zc)) * dt / dy
+    gz = np.empty((nx + 1, nx, nx - 1))
+    gz[:] = c * (-(ia[:, None, None] * dx - xc) + (jj * dy - yc)) * dt / dz
+    return psi, [gx, gy, gz], dt
+
+
+def revolving_sphere_stats(ic, fc, timestep, dt):
+    """revolving_sphere_stats.hpp:46-65"""
+    n = ic.shape[0]
+    diff = fc - ic
+    return {
+        "max": float(fc.max()),
+        "min": float(fc.min()),
+        "Linf": float(np.abs(diff).max()),
+        "L2": float(1.0 / (timestep * dt) * math.sqrt(1.0 / n ** 3 * float((diff ** 2).sum()))),
+        "L1": float(1.0 / (timestep * dt) * (1.0 / n ** 3 * float(np.abs(diff).sum()))),
+    }
+
+
+def rotating_cone_2d(n=101):
+    """rotating_cone_2d.cpp:32-140 (FP64, open boundaries)."""
+    dt, dx, dy, omega, h, h0 = 0.1, 1.0, 1.0, 0.1, 4.0, 1.0
+    r = 15.0 * dx
+    x0, y0 = 50 * dx, 75 * dy
+    xc = 0.5 * (n - 1) * dx
+    yc = 0.5 * (n - 1) * dy
+    i = np.arange(n, dtype=np.float64)[:, None]
+    j = np.arange(n, dtype=np.float64)[None, :]
+    tmp = (i * dx - x0) ** 2 + (j * dy - y0) ** 2
+    psi = h0 + np.where(tmp - r ** 2 <= 0, h - np.sqrt(tmp / (r / h) ** 2), 0.0)
+    gx = np.empty((n, n))
+    gx[:] = omega * (j * dy - yc) * dt / dx
+    ia = np.arange(-1, n, dtype=np.float64)[:, None]
+    gy = np.empty((n + 1, n - 1))
+    gy[:] = -omega * (ia * dx - xc) * dt / dy
+    return psi, [gx, gy], dt
+
+
+def rotating_cone_stats(ic, fc, timestep, dt):
+    """rotating_cone_stats.hpp:54-69"""
+    err2 = (ic - fc) ** 2
+    return {
+        "min": float(fc.min()),
+        "max": float(fc.max()),
+        "rms": float(math.sqrt(err2.sum() / ic.shape[0] / ic.shape[1]) / timestep / dt),
+    }
+
+
+# ---------------------------------------------------------------- benchmarks
+
+def gaussian_2d_cyclic(n=256, courant=(0.25, 0.25), amp=1.0, sigma=16.0, bg=0.5):
+    """2D cyclic Gaussian cone on a background, uniform advector.
+
+    The cyclic domain repeats with period n-1 (cells 0 and n-1 coincide), so
+    the distance is periodic with period n-1 and psi[0] == psi[n-1]."""
+    per = n - 1
+    c = n // 2
+    idx = np.arange(n, dtype=np.float64)
+    dist = np.abs(idx - c)
+    dist = np.minimum(dist, per - dist)
+    r2 = dist[:, None] ** 2 + dist[None, :] ** 2
+    psi = bg + amp * np.exp(-r2 / (2 * sigma ** 2))
+    gx = np.full((n, n), courant[0])
+    gy = np.full((n + 1, n - 1), courant[1])
+    return psi, [gx, gy]
+
+
+def rotation_sphere_3d(n=256, radius=None, offset=None, max_courant=0.5):
+    """Open-boundary N^3 solid-body rotation about z with a discontinuous
+    sphere (1 inside, 0 outside).  GC_x = -w (j - c), GC_y = w (i - c),
+    GC_z = 0 with c = (n-1)/2 and w = max_courant / c, so the discrete
+    divergence is exactly zero."""
+    if radius is None:
+        radius = 40.0 * n / 256
+    if offset is None:
+        offset = 48.0 * n / 256
+    c = (n - 1) / 2.0
+    w = max_courant / c
+    i = np.arange(n, dtype=np.float64)
+    r2 = ((i[:, None, None] - (c + offset)) ** 2 + (i[None, :, None] - c) ** 2
+          + (i[None, None, :] - c) ** 2)
+    psi = np.where(r2 <= radius ** 2, 1.0, 0.0)
+    gx = np.empty((n, n, n))
+    gx[:] = (-w * (i - c))[None, :, None]
+    ia = np.arange(-1, n, dtype=np.float64)
+    gy = np.empty((n + 1, n - 1, n))
+    gy[:] = (w * (ia - c))[:, None, None]
+    gz = np.zeros((n + 1, n, n - 1))
+    return psi, [gx, gy, gz]
+
+
+def random_field(shape, seed, lo=0.1, hi=1.0):
+    """Seeded positive field for parity tests."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(lo, hi, size=shape)
+
+
+def streamfunction_advector(grid, seed, max_courant=0.5, n_modes=4, periodic=True,
+                            quantum=2.0 ** -20):
+    """Discretely divergence-free random advector from a streamfunction.
+
+    The streamfunction lives on cell corners and is quantised to a dyadic
+    grid so that every advector value and every divergence sum is exact.
+    For periodic domains the corner field has period n-1 so the advector is
+    consistent across the duplicated edge cell.  3D: horizontal (x, y) flow
+    per z-level plus a w built from a second streamfunction in (x, z)."""
+    rng = np.random.default_rng(seed)
+    nd = len(grid)
+
+    def corner_field(nx, ny):
+        # corners at x = i+1/2 for i = -2..nx-1, same in y (covers all faces)
+        xs = np.arange(-2, nx) + 0.5
+        ys = np.arange(-2, ny) + 0.5
+        px = (nx - 1) if periodic else 2 * nx
+        py = (ny - 1) if periodic else 2 * ny
+        f = np.zeros((xs.size, ys.size))
+        for _ in range(n_modes):
+            kx = rng.integers(1, 3)
+            ky = rng.integers(1, 3)
+            ph1, ph2 = rng.uniform(0, 2 * np.pi, 2)
+            a = rng.normal()
+            f += a * np.sin(2 * np.pi * kx * xs[:, None] / px + ph1) * np.sin(2 * np.pi * ky * ys[None, :] / py + ph2)
+        return f
+
+    if nd == 2:
+        nx, ny = grid
+        f = corner_field(nx, ny)
+        # GC_x(face i+1/2, cell j) = f(i+1/2, j+1/2) - f(i+1/2, j-1/2)
+        gx_full = f[:, 1:] - f[:, :-1]          # x corners -2..nx-1 -> faces; y cells -1..ny-1
+        gy_full = -(f[1:, :] - f[:-1, :])       # x cells -1..nx-1; y faces -2..ny-1
+        scale = max_courant / max(np.abs(gx_full).max(), np.abs(gy_full).max())
+        f = np.round(f * scale / quantum) * quantum
+        gx_full = f[:, 1:] - f[:, :-1]
+        gy_full = -(f[1:, :] - f[:-1, :])
+        # views: gx faces -1..nx-2 (corner index +1), y cells 0..ny-1 (index +1)
+        gx = gx_full[1:nx + 1, 1:ny + 1]
+        gy = gy_full[0:nx + 1, 2:ny]
+        return [np.ascontiguousarray(gx), np.ascontiguousarray(gy)]
+    nx, ny, nz = grid
+    f = corner_field(nx, ny)
+    g = corner_field(nx, nz)
+    gx_full = f[:, 1:] - f[:, :-1]
+    gy_full = -(f[1:, :] - f[:-1, :])
+    hx_full = g[:, 1:] - g[:, :-1]           # x faces, z cells
+    hz_full = -(g[1:, :] - g[:-1, :])        # x cells, z faces
+    scale = max_courant / max(np.abs(gx_full).max() + np.abs(hx_full).max(),
+                              np.abs(gy_full).max(), np.abs(hz_full).max())
+    f = np.round(f * scale / quantum) * quantum
+    g = np.round(g * scale / quantum) * quantum
+    gx_full = f[:, 1:] - f[:, :-1]
+    gy_full = -(f[1:, :] - f[:-1, :])
+    hx_full = g[:, 1:] - g[:, :-1]
+    hz_full = -(g[1:, :] - g[:-1, :])
+    gx = np.empty((nx, ny, nz))
+    gx[:] = gx_full[1:nx + 1, 1:ny + 1][:, :, None] + hx_full[1:nx + 1, 1:nz + 1][:, None, :]
+    gy = np.empty((nx + 1, ny - 1, nz))
+    gy[:] = gy_full[0:nx + 1, 2:ny][:, :, None]
+    gz = np.empty((nx + 1, ny, nz - 1))
+    gz[:] = hz_full[0:nx + 1, 2:nz][:, None, :]
+    return [gx, gy, gz]

@@ -1,0 +1,69 @@
+/*
+ * mpdx_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * FP64 CPU restatement of libmpdata++'s MPDATA(-FCT) advection step
+ * (solvers::mpdata<ct_params_t> run under concurr::serial), used as the
+ * parity oracle for the HIP library and as the timed CPU baseline.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it; the product library never links or calls it.
+ *
+ * The option bits, halo widths, advector view extents and error texts follow
+ * the reference (libmpdata++/opts.hpp:34-49, concurr/detail/sharedmem.hpp:
+ * 411-456,522-577, solvers/detail/solver_common.hpp:246-248,
+ * solvers/detail/solver_3d.hpp:375-388).
+ */
+#ifndef MPDX_ORACLE_H
+#define MPDX_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_EQNS 64
+
+typedef struct orc orc_t;
+
+typedef struct {
+  int n_dims;                 /* 2 or 3 */
+  int n[3];                   /* grid_size */
+  unsigned opts;              /* libmpdata++ opts bitmask */
+  int n_iters;
+  int n_eqns;
+  int bc[3][2];               /* [dim][left,right]: 0 = cyclic, 1 = open */
+  int upwind_filter_freq;
+  int hint_scale[ORC_MAX_EQNS];
+  double dt;
+  double max_abs_div_eps;     /* < 0 selects numeric_limits<double>::epsilon() */
+} orc_params_t;
+
+/* returns NULL and writes a message into err on failure */
+orc_t *orc_create(const orc_params_t *p, char *err, int errlen);
+void orc_destroy(orc_t *s);
+
+/* dense C-order copies of the interior advectee (n0 x n1 [x n2]) */
+int orc_set_advectee(orc_t *s, int e, const double *dense);
+int orc_get_advectee(const orc_t *s, int e, double *dense);
+
+/* dense C-order copy of the reference advector(d) view: component d spans
+ * face indices -1..n_d-2 along d (n_d entries); in 3D the components d>0
+ * additionally span x = -1..n0-1 (n0+1 entries); other dims are interior. */
+int orc_advector_extents(const orc_t *s, int d, int ext[3]);
+int orc_set_advector(orc_t *s, int d, const double *dense);
+int orc_get_advector(const orc_t *s, int d, double *dense);
+
+/* runs nt more time steps; nonzero return = error (message in err) */
+int orc_advance(orc_t *s, long nt, char *err, int errlen);
+
+long orc_timestep(const orc_t *s);
+double orc_time(const orc_t *s);
+int *orc_panic_ptr(orc_t *s);
+
+/* raw access to one full halo-extended array for white-box tests:
+ * what: 0 psi[e] (current level), 1 GC[idx] */
+int orc_full_shape(const orc_t *s, int *hw, int shape[3]);
+int orc_get_full(const orc_t *s, int what, int idx, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,73 @@
+"""Regenerate the golden fixtures under tests/golden/ from the reference's own
+test data (run in a container that has the reference tree and h5dump).
+
+Fixtures written (all data, no code):
+  revolving_sphere_3d/<variant>_556.f32.gz   float32 psi[x][y][z], 59^3, after
+                                              556 steps (refdata/<variant>/
+                                              timestep0000000556.h5, dataset psi)
+  revolving_sphere_3d/initial_0.f32.gz       the t=0 field (refdata/common/...)
+  revolving_sphere_3d/stats_<variant>.txt    refdata/stats_<variant>.txt.gz
+  rotating_cone_2d/stats_<variant>_.txt      refdata/stats_<variant>_.txt.gz
+  SHA256SUMS                                 checksums of the above
+
+The HDF5 datasets are float32 little-endian, C order (output/hdf5.hpp:56,
+327-357); h5dump -b LE writes their raw bytes.
+"""
+import gzip
+import hashlib
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+REF = "/root/reference/tests/paper_2015_GMD"
+H5DUMP = "/opt/conda/bin/h5dump"
+HERE = os.path.dirname(os.path.abspath(__file__))
+SPHERE_VARIANTS = ["upwind", "basic", "iga", "iga_fct", "fct"]
+CONE_VARIANTS = ["basic", "fct", "iga_fct", "iga_tot_fct", "iters3_tot_fct"]
+
+
+def dump_psi(h5path, out_gz):
+    with tempfile.TemporaryDirectory() as tmp:
+        raw = os.path.join(tmp, "psi.bin")
+        subprocess.run([H5DUMP, "-d", "/psi", "-b", "LE", "-o", raw, h5path],
+                       check=True, stdout=subprocess.DEVNULL, cwd=tmp)
+        data = open(raw, "rb").read()
+    assert len(data) == 59 ** 3 * 4, len(data)
+    with gzip.GzipFile(out_gz, "wb", mtime=0) as f:
+        f.write(data)
+
+
+def main():
+    sphere = os.path.join(HERE, "revolving_sphere_3d")
+    cone = os.path.join(HERE, "rotating_cone_2d")
+    os.makedirs(sphere, exist_ok=True)
+    os.makedirs(cone, exist_ok=True)
+    rs = os.path.join(REF, "4_revolving_sphere_3d", "refdata")
+    dump_psi(os.path.join(rs, "common", "timestep0000000000.h5"),
+             os.path.join(sphere, "initial_0.f32.gz"))
+    for v in SPHERE_VARIANTS:
+        dump_psi(os.path.join(rs, v, "timestep0000000556.h5"),
+                 os.path.join(sphere, f"{v}_556.f32.gz"))
+        with gzip.open(os.path.join(rs, f"stats_{v}.txt.gz"), "rb") as f, \
+                open(os.path.join(sphere, f"stats_{v}.txt"), "wb") as g:
+            shutil.copyfileobj(f, g)
+    rc = os.path.join(REF, "3_rotating_cone_2d", "refdata")
+    for v in CONE_VARIANTS:
+        with gzip.open(os.path.join(rc, f"stats_{v}_.txt.gz"), "rb") as f, \
+                open(os.path.join(cone, f"stats_{v}_.txt"), "wb") as g:
+            shutil.copyfileobj(f, g)
+    lines = []
+    for sub in ("revolving_sphere_3d", "rotating_cone_2d"):
+        for name in sorted(os.listdir(os.path.join(HERE, sub))):
+            path = os.path.join(HERE, sub, name)
+            digest = hashlib.sha256(open(path, "rb").read()).hexdigest()
+            lines.append(f"{digest}  {sub}/{name}")
+    with open(os.path.join(HERE, "SHA256SUMS"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+if __name__ == "__main__":
+    sys.exit(main())
