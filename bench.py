@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Benchmark of the MPDATA advection step on one MI355X.
+
+Workload (the north-star configuration): 3D MPDATA with FCT, n_iters = 3,
+open boundaries, 512^3 grid, FP32 state/advector/fluxes with FP64 guards and
+FP64 Kahan mass sums; solid-body rotation about z (max Courant 0.5) moving a
+discontinuous sphere.  A "step" is one time step of the whole grid.  The
+same problem is also run in FP64 for the FP32/FP64 ratio and as the
+accuracy reference (the FP64 GPU path agrees with the FP64 CPU oracle to
+<= 1e-12).  Inputs are generated on the device and resident in HBM before
+the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+METRIC = "Gcell-updates/s, 3D MPDATA-FCT iters=3, FP32 vs FP64, % of HBM peak"
+
+# Compulsory HBM words per interior cell for one launch of each kernel class
+# (reads of the arrays it consumes + writes of the arrays it produces; halo
+# re-reads, intermediates kept on chip and reductions excluded).
+ALG_WORDS = {
+    "fused_step": 5,        # psi^n, GC x3 in; psi^{n+1} out
+    "fused_pass0": 5,       # psi^n, GC x3 in; psi^1 out
+    "fused_corr_first": 9,  # psi^k, GC x3, psi^n in; psi^{k+1}, V x3 out
+    "fused_corr_mid": 9,    # psi^k, V^{k-1} x3, psi^n in; psi^{k+1}, V^k x3 out
+    "fused_corr_last": 6,   # psi^k, U x3, psi^n in; psi^{k+1} out
+    "pp_antidiff": 5,       # psi, U_d, U_d+1, U_d+2 in; V_d out (per component)
+    "pp_flux": 3,           # psi, C_d in; F_d out (per component)
+    "pp_donorcell": 5,      # psi, F x3 in; psi' out
+    "pp_betas": 10,         # psi, psi_min, psi_max, F' x3 in... (x6 faces); beta x2 out
+    "pp_gc_mono": 5,        # V_d, beta x2, psi in; M_d out
+    "pp_fct_init": 3,       # psi in; psi_min, psi_max out
+    "pp_halo": 0,
+    "pp_scale": 2,
+}
+
+
+def rotation_sphere_device(torch, n, dtype, dev):
+    """rotation_sphere_3d of libmpdataxx_amd.workloads, generated on the GPU."""
+    c = (n - 1) / 2.0
+    w = 0.5 / c
+    radius, offset = 40.0 * n / 256, 48.0 * n / 256
+    i = torch.arange(n, dtype=torch.float64, device=dev)
+    r2 = ((i[:, None, None] - (c + offset)) ** 2 + (i[None, :, None] - c) ** 2
+          + (i[None, None, :] - c) ** 2)
+    psi = torch.where(r2 <= radius ** 2, 1.0, 0.0).to(dtype)
+    del r2
+    gx = (-w * (i - c)).to(dtype)[None, :, None].expand(n, n, n).contiguous()
+    ia = torch.arange(-1, n, dtype=torch.float64, device=dev)
+    gy = (w * (ia - c)).to(dtype)[:, None, None].expand(n + 1, n - 1, n).contiguous()
+    gz = torch.zeros((n + 1, n, n - 1), dtype=dtype, device=dev)
+    return psi, [gx, gy, gz]
+
+
+def run_gpu(torch, M, n, dtype, steps, warmup, path):
+    dev = torch.device("cuda:0")
+    s = M.Solver((n, n, n), opts=M.FCT, n_iters=3, dtype=dtype, path=path,
+                 bcs=[(M.OPEN, M.OPEN)] * 3)
+    psi, gc = rotation_sphere_device(torch, n, torch.float32 if dtype == "float32" else torch.float64, dev)
+    s.set_advectee_device(psi)
+    for d in range(3):
+        s.set_advector_device(gc[d], d)
+    del psi, gc
+    torch.cuda.synchronize()
+    mass0 = s.mass()
+    s.advance(warmup)
+    torch.cuda.synchronize()
+    s.profile(True)
+    t0 = time.perf_counter()
+    s.advance(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = s.profile_read()
+    s.profile(False)
+    return s, (t1 - t0), prof, mass0
+
+
+def cpu_baseline(seconds=10.0):
+    """FP64 oracle (reference release-style flags, OpenMP) on a 128^3 sample
+    of the same workload; FP64 like the reference's double-precision runs."""
+    threads = int(os.environ.get("MPDX_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import oracle as O
+    from libmpdataxx_amd import workloads as W
+    n = 128
+    psi, gc = W.rotation_sphere_3d(n)
+    o = O.Oracle((n, n, n), opts=O.FCT, n_iters=3, fast=True)
+    o.set_advectee(psi)
+    for d in range(3):
+        o.set_advector(gc[d], d)
+    o.advance(1)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.advance(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 200:
+            break
+    return {"value": n ** 3 * steps / el / 1e9, "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n}^3 open, FCT n_iters=3, FP64, {steps} steps after 1 warm-up step, "
+                      f"{el:.1f} s, OpenMP {threads} threads, -Ofast"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--path", type=int, default=0, help="0 auto, 1 per-pass, 2 fused")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharding design is "
+              "described in DESIGN.md", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    import libmpdataxx_amd as M
+
+    n, K, W = args.grid, args.steps, args.warmup
+    cells = n ** 3
+    s32, el32, prof32, m0_32 = run_gpu(torch, M, n, "float32", K, W, args.path)
+    value = cells * K / el32 / 1e9
+    path_name = {1: "per-pass", 2: "fused"}.get(s32.active_path, "?")
+
+    # dominant kernel: largest summed device time in the timed region
+    dom = max(prof32.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dlaunch) = dom
+    avg_ms = dms / max(dlaunch, 1)
+    words = ALG_WORDS.get(dname, 0)
+    alg_bytes = words * 4 * cells
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": round(el32 * 1e3 / K, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (solid-body rotation of a discontinuous sphere, generated on device)",
+        "config": {"workload": f"3D MPDATA-FCT n_iters=3, {n}^3, open bconds, FP32 state, FP64 guards",
+                   "grid": [n, n, n], "n_iters": 3, "opts": "fct", "bcond": "open", "path": path_name},
+        "roofline": {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_ms": round(avg_ms, 4), "launches": dlaunch,
+                     "alg_bytes_per_launch": alg_bytes},
+        "step_roofline": {"alg_bytes_per_cell": 20, "achieved_GBs": round(value * 20, 2),
+                          "frac": round(value * 20 / HBM_PEAK_GBS, 4)},
+        "kernels_ms": {k: round(v[0], 3) for k, v in prof32.items()},
+    }
+
+    if not args.no_fp64:
+        s64, el64, prof64, m0_64 = run_gpu(torch, M, n, "float64", K, W, args.path)
+        v64 = cells * K / el64 / 1e9
+        a32 = torch.empty((n, n, n), dtype=torch.float32, device="cuda:0")
+        a64 = torch.empty((n, n, n), dtype=torch.float64, device="cuda:0")
+        s32.get_advectee_device(a32)
+        s64.get_advectee_device(a64)
+        diff = a32.double() - a64
+        l2 = float(torch.sqrt((diff ** 2).sum() / (a64 ** 2).sum()))
+        linf = float(diff.abs().max())
+        m32, m64 = s32.mass(), s64.mass()
+        out["fp64"] = {"value": round(v64, 4), "ms_per_step": round(el64 * 1e3 / K, 4),
+                       "kernels_ms": {k: round(v[0], 3) for k, v in prof64.items()}}
+        out["fp32_over_fp64"] = round(value / v64, 4)
+        out["accuracy_vs_fp64"] = {"l2_rel": l2, "linf": linf,
+                                   "mass_rel_err_fp32": abs(m32 - m0_32) / abs(m0_32),
+                                   "mass_rel_err_fp64": abs(m64 - m0_64) / abs(m0_64),
+                                   "min_fp32": s32.min(), "steps": K + W}
+        del a32, a64, diff
+        s64.close()
+    s32.close()
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

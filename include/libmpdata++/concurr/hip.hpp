@@ -1,0 +1,232 @@
+/** @file
+ * concurr::hip -- the MI355X concurrency backend.
+ *
+ * Drop-in for concurr::serial / concurr::openmp (concurr/serial.hpp:15-53,
+ * concurr/openmp.hpp:21-83): same template parameters
+ * <solver_t, bcxl, bcxr, bcyl, bcyr, bczl, bczr>, same public interface
+ * (concurr/any.hpp, concurr_common.hpp:380-453) and the same exception texts.
+ * The state lives in GPU memory; advectee()/advector() return views of host
+ * mirrors that are uploaded before and refreshed after every advance().
+ * All work goes through the C ABI of include/mpdx.h.
+ */
+#pragma once
+
+#include <array>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include <mpdx.h>
+
+#include <libmpdata++/bcond/bcond.hpp>
+#include <libmpdata++/concurr/any.hpp>
+#include <libmpdata++/solvers/mpdata.hpp>
+
+namespace libmpdataxx
+{
+  namespace concurr
+  {
+    template <class solver_t_,
+              bcond::bcond_e bcxl, bcond::bcond_e bcxr,
+              bcond::bcond_e bcyl = bcond::null, bcond::bcond_e bcyr = bcond::null,
+              bcond::bcond_e bczl = bcond::null, bcond::bcond_e bczr = bcond::null>
+    class hip : public any<typename solver_t_::real_t, solver_t_::n_dims, typename solver_t_::advance_arg_t>
+    {
+      public:
+
+      using solver_t = solver_t_;
+      using real_t = typename solver_t::real_t;
+      using ct_params_t = typename solver_t::ct_params_t_;
+      using advance_arg_t = typename solver_t::advance_arg_t;
+      enum { n_dims = solver_t::n_dims, n_eqns = solver_t::n_eqns };
+      using arr_t = libmpdataxx::arr_t<real_t, n_dims>;
+
+      // concurr_common.hpp:134-148
+      static_assert(n_dims == 3 || (n_dims == 2 && bczl == bcond::null && bczr == bcond::null),
+                    "more boundary conditions than dimensions");
+
+      private:
+
+      mpdx_solver *h_ = nullptr;
+      std::vector<std::vector<real_t>> psi_host_;
+      std::vector<bool> psi_dirty_;
+      std::array<std::vector<real_t>, 3> gc_host_;
+      std::array<bool, 3> gc_dirty_{};
+      bool panic_ = false;
+
+      static int bc_code(bcond::bcond_e b)
+      {
+        if (b == bcond::cyclic) return MPDX_BC_CYCLIC;
+        if (b == bcond::open) return MPDX_BC_OPEN;
+        throw std::runtime_error("libmpdata++: boundary condition not supported by the HIP backend");
+      }
+
+      void check(int rc) const
+      {
+        if (rc != MPDX_OK) throw std::runtime_error(mpdx_last_error(h_));
+      }
+
+      std::array<int, n_dims> interior_extent() const
+      {
+        std::array<int, n_dims> e{};
+        for (int d = 0; d < n_dims; ++d) e[d] = grid_[d];
+        return e;
+      }
+
+      static void dense_strides(const std::array<int, n_dims> &ext, int64_t st[3])
+      {
+        st[0] = st[1] = st[2] = 0;
+        int64_t s = 1;
+        for (int d = n_dims - 1; d >= 0; --d) { st[d] = s; s *= ext[d]; }
+      }
+
+      std::array<int, n_dims> grid_{};
+
+      void upload()
+      {
+        int64_t st[3];
+        for (int e = 0; e < n_eqns; ++e)
+          if (psi_dirty_[e])
+          {
+            dense_strides(interior_extent(), st);
+            check(mpdx_set_advectee(h_, e, psi_host_[e].data(), st, 0));
+            psi_dirty_[e] = false;
+          }
+        for (int d = 0; d < n_dims; ++d)
+          if (gc_dirty_[d])
+          {
+            int ext3[3];
+            check(mpdx_advector_extents(h_, d, ext3));
+            std::array<int, n_dims> ext{};
+            for (int q = 0; q < n_dims; ++q) ext[q] = ext3[q];
+            dense_strides(ext, st);
+            check(mpdx_set_advector(h_, d, gc_host_[d].data(), st, 0));
+            gc_dirty_[d] = false;
+          }
+      }
+
+      void download()
+      {
+        int64_t st[3];
+        dense_strides(interior_extent(), st);
+        for (int e = 0; e < n_eqns; ++e) check(mpdx_get_advectee(h_, e, psi_host_[e].data(), st, 0));
+      }
+
+      public:
+
+      // concurr_common ctor (concurr_common.hpp:176-187) + solver ctor checks
+      explicit hip(const typename solver_t::rt_params_t &p)
+      {
+        mpdx_params mp;
+        mpdx_params_default(&mp);
+        mp.n_dims = n_dims;
+        for (int d = 0; d < n_dims; ++d) { mp.grid[d] = p.grid_size[d]; grid_[d] = p.grid_size[d]; }
+        mp.real_bytes = sizeof(real_t);
+        mp.opts = (uint32_t)ct_params_t::opts;
+        mp.n_iters = p.n_iters;
+        mp.n_eqns = n_eqns;
+        const bcond::bcond_e bcs[3][2] = {{bcxl, bcxr}, {bcyl, bcyr}, {bczl, bczr}};
+        for (int d = 0; d < n_dims; ++d)
+          for (int s = 0; s < 2; ++s) mp.bc[d][s] = bc_code(bcs[d][s]);
+        mp.upwind_filter_freq = p.upwind_filter_freq;
+        for (int e = 0; e < n_eqns && e < MPDX_MAX_EQNS; ++e) mp.hint_scale[e] = ct_params_t::hint_scale(e);
+        mp.dt = p.dt;
+        mp.max_abs_div_eps = p.max_abs_div_eps;
+        if (mpdx_create(&mp, &h_) != MPDX_OK) throw std::runtime_error(mpdx_last_error(nullptr));
+        std::size_t n = 1;
+        for (int d = 0; d < n_dims; ++d) n *= grid_[d];
+        psi_host_.assign(n_eqns, std::vector<real_t>(n, real_t(0)));
+        psi_dirty_.assign(n_eqns, false);
+        for (int d = 0; d < n_dims; ++d)
+        {
+          int ext3[3];
+          check(mpdx_advector_extents(h_, d, ext3));
+          std::size_t m = 1;
+          for (int q = 0; q < n_dims; ++q) m *= ext3[q];
+          gc_host_[d].assign(m, real_t(0));
+        }
+      }
+
+      hip(const hip &) = delete;
+      hip &operator=(const hip &) = delete;
+      ~hip() { mpdx_destroy(h_); }
+
+      // concurr_common::advance (concurr_common.hpp:380-385); the panic flag is
+      // polled between time steps like solver_common::solve (solver_common.hpp:301-302)
+      void advance(advance_arg_t nt) override
+      {
+        upload();
+        for (advance_arg_t t = 0; t < nt; ++t)
+        {
+          if (panic_) break;
+          check(mpdx_advance(h_, 1));
+        }
+        download();
+      }
+
+      // sharedmem.hpp:522-531: interior of the current time level, base 0
+      arr_t advectee(int e = 0) override
+      {
+        psi_dirty_.at(e) = true;
+        std::array<int, n_dims> lb{};
+        return arr_t(psi_host_[e].data(), lb, interior_extent());
+      }
+
+      const arr_t advectee_global(int e = 0) override { return advectee(e); }
+
+      void advectee_global_set(const arr_t arr, int e = 0) override { advectee(e).assign(arr); }
+
+      // sharedmem.hpp:543-577: faces -1..n_d-2 along d (index i <-> face i+1/2),
+      // x from -1 for the components d > 0 in 3D
+      arr_t advector(int d = 0) override
+      {
+        int ext3[3];
+        check(mpdx_advector_extents(h_, d, ext3));
+        std::array<int, n_dims> lb{}, ext{};
+        for (int q = 0; q < n_dims; ++q) ext[q] = ext3[q];
+        lb[0] = -1;
+        gc_dirty_.at(d) = true;
+        return arr_t(gc_host_[d].data(), lb, ext);
+      }
+
+      // sharedmem.hpp:579-591 (nug is not supported here)
+      arr_t g_factor() override
+      {
+        throw std::runtime_error("libmpdata++: g_factor() called with nug option unset?");
+      }
+
+      bool *panic_ptr() override { return &panic_; }
+
+      const real_t time() const override
+      {
+        double t;
+        mpdx_time(h_, &t);
+        return real_t(t);
+      }
+
+      const real_t min(int e = 0) const override
+      {
+        double v;
+        check(mpdx_reduce(h_, e, MPDX_RED_MIN, &v));
+        return real_t(v);
+      }
+
+      const real_t max(int e = 0) const override
+      {
+        double v;
+        check(mpdx_reduce(h_, e, MPDX_RED_MAX, &v));
+        return real_t(v);
+      }
+
+      // FP64 Kahan-compensated sum of the advectee (sharedmem.hpp:115-155 with sum_khn)
+      double mass(int e = 0) const
+      {
+        double v;
+        check(mpdx_reduce(h_, e, MPDX_RED_SUM, &v));
+        return v;
+      }
+
+      mpdx_solver *handle() const { return h_; }
+    };
+  } // namespace concurr
+} // namespace libmpdataxx

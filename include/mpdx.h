@@ -1,0 +1,148 @@
+/*
+ * mpdx.h -- C ABI of the MI355X MPDATA advection backend.
+ *
+ * This is the drop-in boundary behind libmpdata++'s concurrency-backend slot:
+ * the header-only C++ template libmpdataxx::concurr::hip<solver_t, bcond...>
+ * (include/libmpdata++/concurr/hip.hpp) calls only these functions, exactly
+ * where the reference's concurr::serial/openmp would run the solver objects.
+ * Plain pointers and sizes only; all calls return 0 on success and a nonzero
+ * MPDX_E* code on failure, with the message (libmpdata++'s own wording where
+ * the reference throws) available from mpdx_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   mpdx_create            concurr::serial ctor -> concurr_common ctor
+ *                          (concurr/serial.hpp:49-51,
+ *                           concurr/detail/concurr_common.hpp:176-187),
+ *                          solver_common ctor checks (solvers/detail/
+ *                          solver_common.hpp:246-248)
+ *   mpdx_advance           concurr_common::advance -> solver_common::solve
+ *                          (concurr_common.hpp:380-385, solver_common.hpp:270-358)
+ *   mpdx_set/get_advectee  advectee(e) / advectee_global(_set)
+ *                          (concurr_common.hpp:387-408, sharedmem.hpp:411-419,522-531)
+ *   mpdx_set/get_advector  advector(d) (concurr_common.hpp:410-413,
+ *                          sharedmem.hpp:431-456,543-577)
+ *   mpdx_reduce            min(e)/max(e) (concurr_common.hpp:445-453) and the
+ *                          double-precision Kahan sum of sharedmem::sum
+ *                          (sharedmem.hpp:115-155, kahan_reduction.hpp:15-58)
+ *   mpdx_time / mpdx_timestep   time() (concurr_common.hpp:440-443)
+ *   mpdx_panic_ptr         panic_ptr() (concurr_common.hpp:435-438)
+ */
+#ifndef MPDX_H
+#define MPDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPDX_MAX_EQNS 64
+
+/* option bits: libmpdata++/opts.hpp:34-49 (same values) */
+#define MPDX_OPT_FCT (1u << 0)
+#define MPDX_OPT_ABS (1u << 1)
+#define MPDX_OPT_TOT (1u << 2)
+#define MPDX_OPT_PFC (1u << 3)
+#define MPDX_OPT_NPA (1u << 4)
+#define MPDX_OPT_IGA (1u << 5)
+#define MPDX_OPT_NUG (1u << 6)
+#define MPDX_OPT_DFL (1u << 7)
+#define MPDX_OPT_KHN (1u << 8)
+
+/* boundary kinds: bcond::cyclic / bcond::open (bcond/detail/bcond_common.hpp:17) */
+#define MPDX_BC_CYCLIC 0
+#define MPDX_BC_OPEN 1
+
+/* precision of state, advector and fluxes */
+#define MPDX_REAL_F32 4
+#define MPDX_REAL_F64 8
+
+/* how the near-zero-denominator guards of the psi ratios and FCT betas are
+ * evaluated on the FP32 path: in FP64 with FP64 constants (default), or in
+ * FP32 with numeric_limits<float> constants as libmpdata++ does for
+ * real_t=float (formulae/mpdata/formulae_mpdata_common.hpp:61-98) */
+#define MPDX_GUARD_F64 0
+#define MPDX_GUARD_NATIVE 1
+
+/* execution path */
+#define MPDX_PATH_AUTO 0      /* fused kernels where available, else per-pass */
+#define MPDX_PATH_PERPASS 1   /* one kernel per reference pass, literal halo fills */
+#define MPDX_PATH_FUSED 2     /* fused stencil kernels (error if unsupported) */
+
+/* error codes */
+#define MPDX_OK 0
+#define MPDX_EINVAL 1
+#define MPDX_EDIVERGENT 2
+#define MPDX_EHIP 3
+#define MPDX_ENOMEM 4
+#define MPDX_EUNSUPPORTED 5
+
+/* reductions */
+#define MPDX_RED_MIN 0
+#define MPDX_RED_MAX 1
+#define MPDX_RED_SUM 2        /* FP64 Kahan-compensated sum over the interior */
+
+typedef struct mpdx_solver mpdx_solver;
+
+typedef struct mpdx_params {
+  int n_dims;                     /* 2 or 3 (ct_params_t::n_dims) */
+  int grid[3];                    /* rt_params_t::grid_size */
+  int real_bytes;                 /* MPDX_REAL_F32 or MPDX_REAL_F64 (ct_params_t::real_t) */
+  uint32_t opts;                  /* ct_params_t::opts */
+  int n_iters;                    /* rt_params_t::n_iters (mpdata_common.hpp:74-78) */
+  int n_eqns;                     /* ct_params_t::n_eqns: tracers sharing one advector */
+  int bc[3][2];                   /* [dim][left,right] MPDX_BC_* */
+  int upwind_filter_freq;         /* rt_params_t::upwind_filter_freq */
+  int hint_scale[MPDX_MAX_EQNS];  /* ct_params_t::hint_scale(e) */
+  double dt;                      /* rt_params_t::dt */
+  double max_abs_div_eps;         /* rt_params_t::max_abs_div_eps; < 0 = epsilon<real_t> */
+  int guard;                      /* MPDX_GUARD_* */
+  int path;                       /* MPDX_PATH_* */
+} mpdx_params;
+
+/* fills p with the reference defaults (opts = iga|fct, n_iters = 2, ...) */
+void mpdx_params_default(mpdx_params *p);
+
+int mpdx_create(const mpdx_params *p, mpdx_solver **out);
+int mpdx_destroy(mpdx_solver *h);
+/* message of the last failed call on h (h may be NULL for mpdx_create) */
+const char *mpdx_last_error(const mpdx_solver *h);
+
+/* hipStream_t to enqueue on (NULL = the library's own stream) */
+int mpdx_set_stream(mpdx_solver *h, void *stream);
+
+/* Advectee/advector copies.  buf holds real_bytes-wide elements; strides are
+ * in elements, for [x][y][z]; on_device != 0 means buf is device memory.
+ * The advectee covers the interior (grid); the advector(d) view covers the
+ * same indices as the reference's advector(d): faces -1..n_d-2 along d and,
+ * for 3D components d > 0, x = -1..n0-1 (buf points at that first element). */
+int mpdx_advector_extents(const mpdx_solver *h, int d, int ext[3]);
+int mpdx_set_advectee(mpdx_solver *h, int e, const void *buf, const int64_t strides[3], int on_device);
+int mpdx_get_advectee(mpdx_solver *h, int e, void *buf, const int64_t strides[3], int on_device);
+int mpdx_set_advector(mpdx_solver *h, int d, const void *buf, const int64_t strides[3], int on_device);
+int mpdx_get_advector(mpdx_solver *h, int d, void *buf, const int64_t strides[3], int on_device);
+
+/* advance by nt time steps (synchronous: returns when the work is done) */
+int mpdx_advance(mpdx_solver *h, int64_t nt);
+
+int mpdx_reduce(mpdx_solver *h, int e, int op, double *out);
+int mpdx_time(const mpdx_solver *h, double *t);
+int mpdx_timestep(const mpdx_solver *h, int64_t *ts);
+/* host flag polled once per step; set nonzero to make advance() return early */
+int *mpdx_panic_ptr(mpdx_solver *h);
+/* which path advance() uses for this configuration (MPDX_PATH_PERPASS/FUSED) */
+int mpdx_active_path(const mpdx_solver *h);
+
+/* Kernel timing with HIP events on the solver's stream (no reference
+ * counterpart; the reference prints a wall-clock timer, concurr/detail/
+ * timer.hpp:19-56).  mpdx_profile(h, 1) clears and starts recording;
+ * mpdx_profile_read returns, for the idx-th kernel class recorded, its name,
+ * summed device time and launch count (MPDX_EINVAL past the last class). */
+int mpdx_profile(mpdx_solver *h, int enable);
+int mpdx_profile_read(mpdx_solver *h, int idx, char *name, int name_len,
+                      double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPDX_H */

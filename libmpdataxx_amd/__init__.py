@@ -1,0 +1,250 @@
+"""MI355X-native MPDATA advection backend (Python binding of the C ABI).
+
+The product is the HIP library ``libmpdx.so`` (C ABI in ``include/mpdx.h``)
+and the header-only C++ API in ``include/libmpdata++/``.  This module is a
+thin ctypes binding used by the tests and the benchmark; it has no compute
+of its own and raises if the HIP library is missing.
+
+Names follow libmpdata++: ``Solver`` mirrors
+``concurr::<backend><solvers::mpdata<ct_params_t>, bcond...>`` with
+``advectee(e)``, ``advector(d)``, ``advance(nt)``, ``min/max(e)``, ``time``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmpdx.so")
+MAX_EQNS = 64
+
+# option bits (libmpdata++/opts.hpp:34-49)
+FCT, ABS, TOT, PFC, NPA, IGA, NUG, DFL, KHN = (1 << b for b in range(9))
+CYCLIC, OPEN = 0, 1
+GUARD_F64, GUARD_NATIVE = 0, 1
+PATH_AUTO, PATH_PERPASS, PATH_FUSED = 0, 1, 2
+RED_MIN, RED_MAX, RED_SUM = 0, 1, 2
+
+EXPORTED = [
+    "mpdx_params_default", "mpdx_create", "mpdx_destroy", "mpdx_last_error",
+    "mpdx_set_stream", "mpdx_advector_extents", "mpdx_set_advectee",
+    "mpdx_get_advectee", "mpdx_set_advector", "mpdx_get_advector",
+    "mpdx_advance", "mpdx_reduce", "mpdx_time", "mpdx_timestep",
+    "mpdx_panic_ptr", "mpdx_active_path", "mpdx_profile", "mpdx_profile_read",
+]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("n_dims", ctypes.c_int),
+        ("grid", ctypes.c_int * 3),
+        ("real_bytes", ctypes.c_int),
+        ("opts", ctypes.c_uint32),
+        ("n_iters", ctypes.c_int),
+        ("n_eqns", ctypes.c_int),
+        ("bc", (ctypes.c_int * 2) * 3),
+        ("upwind_filter_freq", ctypes.c_int),
+        ("hint_scale", ctypes.c_int * MAX_EQNS),
+        ("dt", ctypes.c_double),
+        ("max_abs_div_eps", ctypes.c_double),
+        ("guard", ctypes.c_int),
+        ("path", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libmpdx.so (build it first with ``python -m libmpdataxx_amd.build``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libmpdx.so not built ({LIB_PATH}); run python -m libmpdataxx_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.mpdx_params_default.argtypes = [ctypes.POINTER(Params)]
+        L.mpdx_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(P)]
+        L.mpdx_destroy.argtypes = [P]
+        L.mpdx_last_error.restype = ctypes.c_char_p
+        L.mpdx_last_error.argtypes = [P]
+        L.mpdx_set_stream.argtypes = [P, P]
+        L.mpdx_advector_extents.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        for name in ("mpdx_set_advectee", "mpdx_get_advectee", "mpdx_set_advector", "mpdx_get_advector"):
+            getattr(L, name).argtypes = [P, ctypes.c_int, P, i64p, ctypes.c_int]
+        L.mpdx_advance.argtypes = [P, ctypes.c_int64]
+        L.mpdx_reduce.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.mpdx_time.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
+        L.mpdx_timestep.argtypes = [P, i64p]
+        L.mpdx_panic_ptr.restype = ctypes.POINTER(ctypes.c_int)
+        L.mpdx_panic_ptr.argtypes = [P]
+        L.mpdx_active_path.argtypes = [P]
+        L.mpdx_profile.argtypes = [P, ctypes.c_int]
+        L.mpdx_profile_read.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double), i64p]
+        _lib = L
+    return _lib
+
+
+class MpdxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def _strides(a: np.ndarray):
+    st = [s // a.itemsize for s in a.strides] + [0] * (3 - a.ndim)
+    return (ctypes.c_int64 * 3)(*st)
+
+
+class Solver:
+    """One MPDATA solver on the GPU (n_eqns advectees sharing one advector)."""
+
+    def __init__(self, grid, opts=IGA | FCT, n_iters=2, n_eqns=1, bcs=None,
+                 dtype=np.float64, dt=0.0, upwind_filter_freq=0, hint_scale=None,
+                 max_abs_div_eps=-1.0, guard=GUARD_F64, path=PATH_AUTO):
+        L = lib()
+        p = Params()
+        L.mpdx_params_default(ctypes.byref(p))
+        nd = len(grid)
+        p.n_dims = nd
+        for d in range(nd):
+            p.grid[d] = int(grid[d])
+        if bcs is None:
+            bcs = [(OPEN, OPEN)] * nd
+        for d in range(nd):
+            p.bc[d][0], p.bc[d][1] = bcs[d]
+        self.dtype = np.dtype(dtype)
+        p.real_bytes = self.dtype.itemsize
+        p.opts = int(opts)
+        p.n_iters = int(n_iters)
+        p.n_eqns = int(n_eqns)
+        p.upwind_filter_freq = int(upwind_filter_freq)
+        for e, v in enumerate(hint_scale or []):
+            p.hint_scale[e] = int(v)
+        p.dt = float(dt)
+        p.max_abs_div_eps = float(max_abs_div_eps)
+        p.guard = int(guard)
+        p.path = int(path)
+        h = ctypes.c_void_p()
+        rc = L.mpdx_create(ctypes.byref(p), ctypes.byref(h))
+        if rc:
+            raise MpdxError(rc, L.mpdx_last_error(None).decode())
+        self._L = L
+        self._h = h
+        self.grid = tuple(int(g) for g in grid)
+        self.n_dims = nd
+        self.n_eqns = n_eqns
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.mpdx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise MpdxError(rc, self._L.mpdx_last_error(self._h).decode())
+
+    @property
+    def active_path(self):
+        return self._L.mpdx_active_path(self._h)
+
+    def advector_shape(self, d):
+        ext = (ctypes.c_int * 3)()
+        self._check(self._L.mpdx_advector_extents(self._h, d, ext))
+        return tuple(ext[:self.n_dims])
+
+    # host (numpy) copies -----------------------------------------------
+    def set_advectee(self, arr, e=0):
+        a = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert a.shape == self.grid, (a.shape, self.grid)
+        self._check(self._L.mpdx_set_advectee(self._h, e, a.ctypes.data, _strides(a), 0))
+
+    def advectee(self, e=0):
+        out = np.empty(self.grid, dtype=self.dtype)
+        self._check(self._L.mpdx_get_advectee(self._h, e, out.ctypes.data, _strides(out), 0))
+        return out
+
+    def set_advector(self, arr, d):
+        a = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert a.shape == self.advector_shape(d), (a.shape, self.advector_shape(d))
+        self._check(self._L.mpdx_set_advector(self._h, d, a.ctypes.data, _strides(a), 0))
+
+    def advector(self, d):
+        out = np.empty(self.advector_shape(d), dtype=self.dtype)
+        self._check(self._L.mpdx_get_advector(self._h, d, out.ctypes.data, _strides(out), 0))
+        return out
+
+    # device (torch) copies ----------------------------------------------
+    def set_advectee_device(self, t, e=0):
+        st = (ctypes.c_int64 * 3)(*(list(t.stride()) + [0] * (3 - t.dim())))
+        self._check(self._L.mpdx_set_advectee(self._h, e, t.data_ptr(), st, 1))
+
+    def get_advectee_device(self, t, e=0):
+        st = (ctypes.c_int64 * 3)(*(list(t.stride()) + [0] * (3 - t.dim())))
+        self._check(self._L.mpdx_get_advectee(self._h, e, t.data_ptr(), st, 1))
+
+    def set_advector_device(self, t, d):
+        st = (ctypes.c_int64 * 3)(*(list(t.stride()) + [0] * (3 - t.dim())))
+        self._check(self._L.mpdx_set_advector(self._h, d, t.data_ptr(), st, 1))
+
+    # stepping and diagnostics ---------------------------------------------
+    def set_stream(self, stream_ptr):
+        self._check(self._L.mpdx_set_stream(self._h, stream_ptr))
+
+    def advance(self, nt):
+        self._check(self._L.mpdx_advance(self._h, int(nt)))
+
+    def reduce(self, op, e=0):
+        out = ctypes.c_double()
+        self._check(self._L.mpdx_reduce(self._h, e, op, ctypes.byref(out)))
+        return out.value
+
+    def min(self, e=0):
+        return self.reduce(RED_MIN, e)
+
+    def max(self, e=0):
+        return self.reduce(RED_MAX, e)
+
+    def mass(self, e=0):
+        """FP64 Kahan-compensated sum of the advectee over the grid."""
+        return self.reduce(RED_SUM, e)
+
+    @property
+    def time(self):
+        t = ctypes.c_double()
+        self._L.mpdx_time(self._h, ctypes.byref(t))
+        return t.value
+
+    @property
+    def timestep(self):
+        t = ctypes.c_int64()
+        self._L.mpdx_timestep(self._h, ctypes.byref(t))
+        return t.value
+
+    def profile(self, enable=True):
+        """Start (clearing) or stop per-kernel-class HIP-event timing."""
+        self._check(self._L.mpdx_profile(self._h, int(bool(enable))))
+
+    def profile_read(self):
+        """{kernel class: (total device ms, launches)} recorded so far."""
+        out = {}
+        idx = 0
+        buf = ctypes.create_string_buffer(128)
+        while True:
+            ms = ctypes.c_double()
+            n = ctypes.c_int64()
+            if self._L.mpdx_profile_read(self._h, idx, buf, 128, ctypes.byref(ms), ctypes.byref(n)):
+                break
+            out[buf.value.decode()] = (ms.value, n.value)
+            idx += 1
+        return out
+
+    def set_panic(self, v=True):
+        self._L.mpdx_panic_ptr(self._h)[0] = int(bool(v))

@@ -1,0 +1,711 @@
+// C ABI of the MI355X MPDATA backend (include/mpdx.h).
+//
+// Host side: validates the configuration like the reference's constructors,
+// owns the device arrays, replays solver_common::solve (solver_common.hpp:
+// 270-358) and mpdata_osc::advop (mpdata_osc_3d.hpp:73-202) as a sequence of
+// kernel launches on one HIP stream, and exposes the advectee/advector views.
+#include "mpdx.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mpdx_solver.hpp"
+#include "mpdx_perpass.hpp"
+#include "mpdx_fused.hpp"
+
+using namespace mpdx;
+
+namespace {
+
+struct rng { int f, l; };
+inline rng R_(int f, int l) { return rng{f, l}; }
+inline rng ext_(rng r, int n) { return R_(r.f - n, r.l + n); }
+inline rng h_(rng r) { return R_(r.f - 1, r.l); }
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+
+#define HIPCHK(expr)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      s->err = std::string("libmpdata++: HIP error: ") + hipGetErrorString(e_) \
+               + " (" #expr ")";                                            \
+      return MPDX_EHIP;                                                     \
+    }                                                                       \
+  } while (0)
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRedBlocks = 1024;
+
+template <class K, class... A>
+void launch(hipStream_t st, long total, K kern, A... args)
+{
+  if (total <= 0) return;
+  const long nb = (total + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(kBlock), 0, st, args...);
+}
+
+// the same, timed under `name` when profiling is on
+template <class K, class... A>
+void plaunch(mpdx_solver *s, const char *name, long total, K kern, A... args)
+{
+  if (total <= 0) return;
+  ProfScope ps(s, name);
+  launch(s->stream, total, kern, args...);
+}
+
+rng interior(const mpdx_solver *s, int d) { return R_(0, s->L.n[d] - 1); }
+
+Box interior_box(const Layout &L)
+{
+  return make_box(0, L.n[0] - 1, 0, L.n[1] - 1, 0, L.nd == 3 ? L.n[2] - 1 : 0);
+}
+
+// ----------------------------------------------------------------- per-pass stepper
+
+template <class R, class G>
+struct PerPass {
+  mpdx_solver *s;
+  const Layout &L;
+  hipStream_t st;
+  unsigned o;
+
+  PerPass(mpdx_solver *s_) : s(s_), L(s_->L), st(s_->stream), o(s_->p.opts) {}
+
+  R *A(void *p) const { return static_cast<R *>(p); }
+  R **av_of(void **arr, R *out[3]) const
+  {
+    for (int d = 0; d < 3; ++d) out[d] = d < L.nd ? A(arr[d]) : A(arr[0]);
+    return out;
+  }
+
+  // scalar-style fill along d, mode 0/1 from the bcond kind, or 2 = zero
+  void fill(R *a, int d, int side, rng ra, rng rb, bool nrml)
+  {
+    const int kind = s->p.bc[d][side];
+    int mode = kind == MPDX_BC_CYCLIC ? 0 : (nrml ? 2 : 1);
+    if (L.nd == 2) rb = R_(0, 0);
+    Box bx = make_box(ra.f, ra.l, rb.f, rb.l, 1, L.hw);
+    plaunch(s, "pp_halo", bx.total, k_fill<R>, a, make_rot(L, d), bx, side, mode, L.n[d]);
+  }
+
+  void alng(R **av, int d, int side, rng ra, rng rb, int ad)
+  {
+    const int N = L.n[d], hw = L.hw, nd = L.nd;
+    Rot r = make_rot(L, d);
+    if (nd == 2) rb = R_(0, 0);
+    R *avd = av[d], *av1 = av[(d + 1) % nd], *av2 = nd == 3 ? av[(d + 2) % 3] : av[0];
+    if (s->p.bc[d][side] == MPDX_BC_CYCLIC) {
+      Box bx = make_box(1, hw, ra.f, ra.l, rb.f, rb.l);
+      plaunch(s, "pp_halo", bx.total, k_alng<R>, avd, (const R *)av1, (const R *)av2, r, bx, nd, side, 1, N);
+      return;
+    }
+    const int e = side == 0 ? 0 : N - 1;
+    const long nz = std::max(ra.l - ra.f + 1, rb.l - rb.f + 1);
+    plaunch(s, "pp_halo", nz, k_alng_zero<R>, av1, av2, r, nd, d, e, ra.f, ra.l, rb.f, rb.l);
+    const int lo = side == 0 ? -hw : N - 1 + (ad ? 1 : 0);
+    const int hi = side == 0 ? -1 - (ad ? 1 : 0) : N - 2 + hw;
+    Box bx = make_box(lo, hi, ra.f, ra.l, rb.f, rb.l);
+    plaunch(s, "pp_halo", bx.total, k_alng<R>, avd, (const R *)av1, (const R *)av2, r, bx, nd, side, 0, N);
+  }
+
+  // solver_3d.hpp:47-67, solver_2d.hpp:40-56
+  void xchng_sclr(R *a, int ext)
+  {
+    rng X = ext_(interior(s, 0), ext), Y = ext_(interior(s, 1), ext);
+    if (L.nd == 3) {
+      rng Z = ext_(interior(s, 2), ext);
+      for (int sd = 0; sd < 2; ++sd) fill(a, 1, sd, Z, X, false);
+      for (int sd = 0; sd < 2; ++sd) fill(a, 0, sd, Y, Z, false);
+      for (int sd = 0; sd < 2; ++sd) fill(a, 2, sd, X, Y, false);
+    } else {
+      for (int sd = 0; sd < 2; ++sd) fill(a, 0, sd, Y, Y, false);
+      for (int sd = 0; sd < 2; ++sd) fill(a, 1, sd, X, X, false);
+    }
+  }
+
+  // solver_3d.hpp:73-103, solver_2d.hpp:63-88
+  void xchng_vctr_alng(R **av, int ad)
+  {
+    rng I = interior(s, 0), J = interior(s, 1);
+    if (L.nd == 3) {
+      rng K = interior(s, 2);
+      for (int sd = 0; sd < 2; ++sd) alng(av, 0, sd, J, K, ad);
+      for (int sd = 0; sd < 2; ++sd) alng(av, 1, sd, K, I, ad);
+      for (int sd = 0; sd < 2; ++sd) alng(av, 2, sd, I, J, ad);
+    } else {
+      for (int sd = 0; sd < 2; ++sd) alng(av, 0, sd, J, J, ad);
+      for (int sd = 0; sd < 2; ++sd) alng(av, 1, sd, I, I, ad);
+    }
+  }
+
+  // solver_3d.hpp:207-255, solver_2d.hpp:171-190
+  void xchng_vctr_nrml(R **av, int ext)
+  {
+    rng I = interior(s, 0), J = interior(s, 1);
+    if (L.nd == 3) {
+      rng K = interior(s, 2);
+      rng Xh = h_(ext_(I, ext)), X1 = ext_(ext_(I, ext), 1);
+      rng Yh = h_(ext_(J, ext)), Y1 = ext_(ext_(J, ext), 1);
+      rng Zh = h_(ext_(K, ext)), Z1 = ext_(ext_(K, ext), 1);
+      for (int sd = 0; sd < 2; ++sd) fill(av[0], 1, sd, Z1, Xh, true);
+      for (int sd = 0; sd < 2; ++sd) fill(av[0], 2, sd, Xh, Y1, true);
+      for (int sd = 0; sd < 2; ++sd) fill(av[1], 0, sd, Yh, Z1, true);
+      for (int sd = 0; sd < 2; ++sd) fill(av[1], 2, sd, X1, Yh, true);
+      for (int sd = 0; sd < 2; ++sd) fill(av[2], 0, sd, Y1, Zh, true);
+      for (int sd = 0; sd < 2; ++sd) fill(av[2], 1, sd, Zh, X1, true);
+    } else {
+      rng Xh = h_(ext_(I, ext)), Yh = h_(ext_(J, ext));
+      for (int sd = 0; sd < 2; ++sd) fill(av[0], 1, sd, Xh, Xh, true);
+      for (int sd = 0; sd < 2; ++sd) fill(av[1], 0, sd, Yh, Yh, true);
+    }
+  }
+
+  void make_flux(R *psi, R **C, R **out, int e)
+  {
+    for (int d = 0; d < L.nd; ++d) {
+      const int d1 = (d + 1) % L.nd, d2 = L.nd == 3 ? (d + 2) % 3 : 0;
+      Box bx = make_box(-1 - e, L.n[d] - 1 + e, -e, L.n[d1] - 1 + e,
+                        L.nd == 3 ? -e : 0, L.nd == 3 ? L.n[d2] - 1 + e : 0);
+      plaunch(s, "pp_flux", bx.total, k_flux<R>, out[d], (const R *)psi, (const R *)C[d], make_rot(L, d), bx, o);
+    }
+  }
+
+  void antidiff(R **V, R *psi, R **U)
+  {
+    for (int d = 0; d < L.nd; ++d) {
+      const int d1 = (d + 1) % L.nd, d2 = L.nd == 3 ? (d + 2) % 3 : 0;
+      Box bx = make_box(-1, L.n[d] - 1, 0, L.n[d1] - 1, 0, L.nd == 3 ? L.n[d2] - 1 : 0);
+      plaunch(s, "pp_antidiff", bx.total, k_antidiff<R, G>, V[d], (const R *)psi, (const R *)U[d],
+             (const R *)U[d1], (const R *)U[L.nd == 3 ? d2 : 0], make_rot(L, d), bx, L.nd, o);
+    }
+  }
+
+  Box box_ext1() const
+  {
+    return make_box(-1, L.n[0], -1, L.n[1], L.nd == 3 ? -1 : 0, L.nd == 3 ? L.n[2] : 0);
+  }
+
+  R **unco(int iter, R *buf[3])
+  {
+    if (iter == 1) return av_of(s->GC, buf);
+    return av_of(iter % 2 ? s->tmp[1] : s->tmp[0], buf);
+  }
+  R **corr(int iter, R *buf[3]) { return av_of(iter % 2 ? s->tmp[0] : s->tmp[1], buf); }
+
+  // mpdata_fct_3d.hpp:57-110
+  void fct_adjust(R *psi, int iter)
+  {
+    R *Vb[3], *Fb[3], *Mb[3];
+    R **V = corr(iter, Vb);
+    xchng_vctr_alng(V, 1);
+    xchng_vctr_nrml(V, 0);
+    R **flx = V;
+    if (!(o & O_IGA)) { flx = av_of(s->flux, Fb); make_flux(psi, V, flx, 1); }
+    Box b1 = box_ext1();
+    plaunch(s, "pp_betas", b1.total, k_betas<R, G>, A(s->bup), A(s->bdn), (const R *)psi, (const R *)A(s->pmin),
+           (const R *)A(s->pmax), (const R *)flx[0], (const R *)flx[1],
+           (const R *)flx[L.nd == 3 ? 2 : 0], L, b1, o);
+    R **M = av_of(s->mono, Mb);
+    for (int d = 0; d < L.nd; ++d) {
+      const int d1 = (d + 1) % L.nd, d2 = L.nd == 3 ? (d + 2) % 3 : 0;
+      Box bx = make_box(-1, L.n[d] - 1, 0, L.n[d1] - 1, 0, L.nd == 3 ? L.n[d2] - 1 : 0);
+      plaunch(s, "pp_gc_mono", bx.total, k_gc_mono<R>, M[d], (const R *)V[d], (const R *)psi,
+             (const R *)A(s->bup), (const R *)A(s->bdn), make_rot(L, d), bx, o);
+    }
+  }
+
+  // mpdata_osc_3d.hpp:73-202
+  void advop(int e)
+  {
+    int &lev = s->lev[e];
+    if (o & O_FCT) {
+      Box b1 = box_ext1();
+      plaunch(s, "pp_fct_init", b1.total, k_fct_init<R>, A(s->pmin), A(s->pmax), (const R *)A(s->psi[e][lev]), L, b1);
+    }
+    for (int iter = 0; iter < s->p.n_iters; ++iter) {
+      R *Cb[3], *Fb[3], *Ub[3], *Vb[3];
+      if (iter != 0) {
+        lev ^= 1;
+        xchng_sclr(A(s->psi[e][lev]), L.hw);
+        antidiff(corr(iter, Vb), A(s->psi[e][lev]), unco(iter, Ub));
+        if (!(o & O_FCT) && iter != s->p.n_iters - 1) xchng_vctr_nrml(corr(iter, Vb), 0);
+        if (o & O_FCT) fct_adjust(A(s->psi[e][lev]), iter);
+      }
+      R **C = iter == 0 ? av_of(s->GC, Cb) : ((o & O_FCT) ? av_of(s->mono, Cb) : corr(iter, Cb));
+      R **flx = C;
+      if (!(o & O_IGA) || iter == 0) { flx = av_of(s->flux, Fb); make_flux(A(s->psi[e][lev]), C, flx, 0); }
+      Box bi = interior_box(L);
+      plaunch(s, "pp_donorcell", bi.total, k_donorcell<R>, A(s->psi[e][lev ^ 1]), (const R *)A(s->psi[e][lev]),
+             (const R *)flx[0], (const R *)flx[1], (const R *)flx[L.nd == 3 ? 2 : 0], L, bi, o);
+      if (s->p.upwind_filter_freq > 0 && s->timestep % s->p.upwind_filter_freq == 0) break;
+    }
+  }
+
+  void scale(int e, int ex)
+  {
+    if (ex == 0) return;
+    Box bi = interior_box(L);
+    const R f = ex > 0 ? R(1) / R(1 << ex) : R(1 << -ex);
+    plaunch(s, "pp_scale", bi.total, k_scale<R>, A(s->psi[e][s->lev[e]]), L, bi, f);
+  }
+
+  void step()
+  {
+    for (int e = 0; e < s->p.n_eqns; ++e) {
+      scale(e, s->p.hint_scale[e]);
+      xchng_sclr(A(s->psi[e][s->lev[e]]), L.hw);
+      advop(e);
+      s->lev[e] ^= 1;
+      scale(e, -s->p.hint_scale[e]);
+    }
+  }
+};
+
+// ----------------------------------------------------------------- helpers
+
+int fill_nan(mpdx_solver *s, void *p)
+{
+  if (s->esz == 8)
+    launch(s->stream, s->L.size, k_fill_value<double>, (double *)p, s->L.size, (double)NAN);
+  else
+    launch(s->stream, s->L.size, k_fill_value<float>, (float *)p, s->L.size, (float)NAN);
+  return 0;
+}
+
+int dev_alloc(mpdx_solver *s, void **out, size_t bytes)
+{
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    s->err = std::string("libmpdata++: device allocation failed: ") + hipGetErrorString(e);
+    return MPDX_ENOMEM;
+  }
+  s->allocs.push_back(p);
+  *out = p;
+  return 0;
+}
+
+int alloc_array(mpdx_solver *s, void **out)
+{
+  int rc = dev_alloc(s, out, (size_t)s->L.size * s->esz);
+  if (rc) return rc;
+  return fill_nan(s, *out);
+}
+
+int ensure_perpass(mpdx_solver *s)
+{
+  if (s->perpass_ready) return 0;
+  int rc = 0;
+  for (int d = 0; d < s->L.nd && !rc; ++d) {
+    rc = rc || alloc_array(s, &s->tmp[0][d]);
+    rc = rc || alloc_array(s, &s->tmp[1][d]);
+    rc = rc || alloc_array(s, &s->flux[d]);
+    rc = rc || alloc_array(s, &s->mono[d]);
+  }
+  rc = rc || alloc_array(s, &s->pmin);
+  rc = rc || alloc_array(s, &s->pmax);
+  rc = rc || alloc_array(s, &s->bup);
+  rc = rc || alloc_array(s, &s->bdn);
+  if (!rc) s->perpass_ready = true;
+  return rc;
+}
+
+int ensure_staging(mpdx_solver *s, size_t bytes)
+{
+  if (s->staging_bytes >= bytes) return 0;
+  if (s->staging) {
+    (void)hipFree(s->staging);
+    for (auto &p : s->allocs)
+      if (p == s->staging) p = nullptr;
+    s->staging = nullptr;
+  }
+  int rc = dev_alloc(s, &s->staging, bytes);
+  if (!rc) s->staging_bytes = bytes;
+  return rc;
+}
+
+// advector(d) view bounds: sharedmem.hpp:431-456 (2D), 543-577 (3D)
+void adv_bounds(const mpdx_solver *s, int d, int lo[3], int hi[3])
+{
+  for (int q = 0; q < 3; ++q) { lo[q] = 0; hi[q] = q < s->L.nd ? s->L.n[q] - 1 : 0; }
+  lo[0] = -1;
+  if (d == 0) hi[0] = s->L.n[0] - 2;
+  else hi[d] = s->L.n[d] - 2;
+}
+
+template <class R>
+void launch_copy(hipStream_t st, R *dst, long dorg, const long ds[3], const R *src, long sorg,
+                 const long ss[3], Box bx)
+{
+  launch(st, bx.total, k_copy_box<R>, dst, dorg, ds[0], ds[1], ds[2], src, sorg, ss[0], ss[1], ss[2], bx);
+}
+
+// copy between a user buffer and the box [lo, hi] of a padded device array
+int copy_view(mpdx_solver *s, void *arr, const int lo[3], const int hi[3], void *buf,
+              const int64_t strides[3], int on_device, bool to_dev)
+{
+  const int nd = s->L.nd;
+  int ext[3];
+  for (int q = 0; q < 3; ++q) ext[q] = hi[q] - lo[q] + 1;
+  Box bx = make_box(0, ext[0] - 1, 0, ext[1] - 1, 0, ext[2] - 1);
+  const long aorg = s->L.org + lo[0] * s->L.st[0] + lo[1] * s->L.st[1] + (nd == 3 ? lo[2] * s->L.st[2] : 0);
+  long ust[3] = {(long)strides[0], (long)strides[1], nd == 3 ? (long)strides[2] : 0};
+  const long cst[3] = {(long)ext[1] * ext[2], (long)ext[2], 1};
+  const size_t n = (size_t)bx.total, esz = (size_t)s->esz;
+  void *dbuf = buf;
+  const long *bst = ust;
+  std::vector<char> host;
+  if (!on_device) {
+    if (int rc = ensure_staging(s, n * esz)) return rc;
+    host.resize(n * esz);
+    if (to_dev) {
+      const char *src = static_cast<const char *>(buf);
+      size_t q = 0;
+      for (int i = 0; i < ext[0]; ++i)
+        for (int j = 0; j < ext[1]; ++j)
+          for (int k = 0; k < ext[2]; ++k, ++q)
+            std::memcpy(&host[q * esz], src + (i * ust[0] + j * ust[1] + k * ust[2]) * (long)esz, esz);
+      HIPCHK(hipMemcpyAsync(s->staging, host.data(), n * esz, hipMemcpyHostToDevice, s->stream));
+    }
+    dbuf = s->staging;
+    bst = cst;
+  }
+  if (s->esz == 8) {
+    if (to_dev) launch_copy<double>(s->stream, (double *)arr, aorg, s->L.st, (const double *)dbuf, 0, bst, bx);
+    else launch_copy<double>(s->stream, (double *)dbuf, 0, bst, (const double *)arr, aorg, s->L.st, bx);
+  } else {
+    if (to_dev) launch_copy<float>(s->stream, (float *)arr, aorg, s->L.st, (const float *)dbuf, 0, bst, bx);
+    else launch_copy<float>(s->stream, (float *)dbuf, 0, bst, (const float *)arr, aorg, s->L.st, bx);
+  }
+  HIPCHK(hipGetLastError());
+  if (!on_device && !to_dev) {
+    HIPCHK(hipMemcpyAsync(host.data(), s->staging, n * esz, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    char *dst = static_cast<char *>(buf);
+    size_t q = 0;
+    for (int i = 0; i < ext[0]; ++i)
+      for (int j = 0; j < ext[1]; ++j)
+        for (int k = 0; k < ext[2]; ++k, ++q)
+          std::memcpy(dst + (i * ust[0] + j * ust[1] + k * ust[2]) * (long)esz, &host[q * esz], esz);
+  } else {
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  return 0;
+}
+
+template <class R>
+int reduce_impl(mpdx_solver *s, const void *arr, int op, double *out)
+{
+  Box bi = interior_box(s->L);
+  hipLaunchKernelGGL(k_reduce<R>, dim3(kRedBlocks), dim3(kBlock), 0, s->stream, s->red_part,
+                     (const R *)arr, s->L, bi, op);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1), 0, s->stream, s->red_out, s->red_part,
+                     kRedBlocks, op);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->red_out, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+// hook_ante_loop chain (solver_common.hpp:191-211, solver_3d.hpp:375-388,
+// mpdata_osc_3d.hpp:40-50)
+template <class R, class G>
+int ante_loop(mpdx_solver *s)
+{
+  if (int rc = ensure_perpass(s)) return rc;
+  if (!s->stat) {
+    if (int rc = alloc_array(s, &s->stat)) return rc;
+  }
+  PerPass<R, G> pp(s);
+  R *Gb[3];
+  R **gc = pp.av_of(s->GC, Gb);
+  pp.xchng_vctr_alng(gc, 0);
+  Box bi = interior_box(s->L);
+  launch(s->stream, bi.total, k_div<R>, (R *)s->stat, (const R *)gc[0], (const R *)gc[1],
+         (const R *)gc[s->L.nd == 3 ? 2 : 0], s->L, bi);
+  double mx = 0;
+  if (int rc = reduce_impl<R>(s, s->stat, MPDX_RED_MAX, &mx)) return rc;
+  const double eps = s->p.max_abs_div_eps < 0 ? (double)lim<R>::eps() : s->p.max_abs_div_eps;
+  if ((R)mx > (R)eps) {
+    s->err = "libmpdata++: initial advector field is divergent";
+    return MPDX_EDIVERGENT;
+  }
+  pp.xchng_vctr_nrml(gc, s->L.hw - 1);
+  return 0;
+}
+
+template <class R, class G>
+int advance_impl(mpdx_solver *s, int64_t nt)
+{
+  nt += s->timestep;
+  if (s->timestep == 0) {
+    if (int rc = ante_loop<R, G>(s)) return rc;
+  }
+  const bool fused = s->active_path == MPDX_PATH_FUSED;
+  if (!fused)
+    if (int rc = ensure_perpass(s)) return rc;
+  PerPass<R, G> pp(s);
+  while (s->timestep < nt) {
+    if (s->panic) break;
+    if (fused) {
+      if (int rc = fused_step<R, G>(s)) return rc;
+    } else {
+      pp.step();
+    }
+    HIPCHK(hipGetLastError());
+    s->timestep++;
+    s->time = s->timestep * s->p.dt;
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- C ABI
+
+extern "C" {
+
+void mpdx_params_default(mpdx_params *p)
+{
+  std::memset(p, 0, sizeof(*p));
+  p->n_dims = 3;
+  p->real_bytes = MPDX_REAL_F64;
+  p->opts = MPDX_OPT_IGA | MPDX_OPT_FCT;   // ct_params_default_t (opts.hpp:83)
+  p->n_iters = 2;                           // mpdata_common.hpp:76
+  p->n_eqns = 1;
+  p->max_abs_div_eps = -1;
+  p->guard = MPDX_GUARD_F64;
+  p->path = MPDX_PATH_AUTO;
+}
+
+int mpdx_create(const mpdx_params *p, mpdx_solver **out)
+{
+  *out = nullptr;
+  const uint32_t unsupported = MPDX_OPT_TOT | MPDX_OPT_NUG | MPDX_OPT_DFL | ~0x1FFu;
+  auto fail = [&](int code, const char *msg) {
+    g_create_error = msg;
+    return code;
+  };
+  if (!p) return fail(MPDX_EINVAL, "libmpdata++: null parameters");
+  if (p->n_dims != 2 && p->n_dims != 3) return fail(MPDX_EINVAL, "libmpdata++: n_dims must be 2 or 3");
+  for (int d = 0; d < p->n_dims; ++d)
+    if (p->grid[d] < 1) return fail(MPDX_EINVAL, "libmpdata++: bogus grid size");
+  if (p->real_bytes != 4 && p->real_bytes != 8) return fail(MPDX_EINVAL, "libmpdata++: real_bytes must be 4 or 8");
+  if (p->opts & unsupported) return fail(MPDX_EUNSUPPORTED, "libmpdata++: option not supported by the HIP backend");
+  if ((p->opts & MPDX_OPT_ABS) && (p->opts & MPDX_OPT_IGA))
+    return fail(MPDX_EINVAL, "libmpdata++: abs & iga options are mutually exclusive");
+  if (p->n_iters < 1) return fail(MPDX_EINVAL, "libmpdata++: n_iters must be positive");
+  if ((p->opts & MPDX_OPT_IGA) && p->n_iters > 2)
+    return fail(MPDX_EINVAL, "libmpdata++: infinite gauge option uses just one corrective step");
+  if (p->n_eqns < 1 || p->n_eqns > MPDX_MAX_EQNS) return fail(MPDX_EINVAL, "libmpdata++: bad n_eqns");
+  for (int d = 0; d < p->n_dims; ++d)
+    for (int sd = 0; sd < 2; ++sd)
+      if (p->bc[d][sd] != MPDX_BC_CYCLIC && p->bc[d][sd] != MPDX_BC_OPEN)
+        return fail(MPDX_EUNSUPPORTED, "libmpdata++: boundary condition not supported by the HIP backend");
+  const int hw = (p->opts & MPDX_OPT_FCT) ? 2 : 1;
+  for (int d = 0; d < p->n_dims; ++d)
+    if (p->grid[d] < hw + 2) return fail(MPDX_EINVAL, "libmpdata++: grid too small for the halo");
+
+  mpdx_solver *s = new mpdx_solver();
+  s->p = *p;
+  s->esz = p->real_bytes;
+  Layout &L = s->L;
+  L.nd = p->n_dims;
+  L.hw = hw;
+  for (int d = 0; d < 3; ++d) L.n[d] = d < L.nd ? p->grid[d] : 1;
+  const int align = 128 / s->esz;            // elements per 128 bytes
+  const int zoff = align;                    // >= hw, keeps index 0 aligned
+  const int inner = L.n[L.nd - 1];
+  const long pitch = ((long)zoff + inner + hw + align - 1) / align * align;
+  if (L.nd == 3) {
+    L.st[2] = 1;
+    L.st[1] = pitch;
+    L.st[0] = pitch * (L.n[1] + 2 * hw);
+    L.org = hw * L.st[0] + hw * L.st[1] + zoff;
+  } else {
+    L.st[2] = 0;
+    L.st[1] = 1;
+    L.st[0] = pitch;
+    L.org = hw * L.st[0] + zoff;
+  }
+  L.size = L.st[0] * (L.n[0] + 2 * hw);
+
+  hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    g_create_error = std::string("libmpdata++: HIP error: ") + hipGetErrorString(he);
+    delete s;
+    return MPDX_EHIP;
+  }
+  s->own_stream = true;
+
+  int rc = 0;
+  for (int e = 0; e < p->n_eqns && !rc; ++e) {
+    rc = rc || alloc_array(s, &s->psi[e][0]);
+    rc = rc || alloc_array(s, &s->psi[e][1]);
+  }
+  for (int d = 0; d < L.nd && !rc; ++d) rc = alloc_array(s, &s->GC[d]);
+  if (!rc) rc = dev_alloc(s, (void **)&s->red_part, sizeof(double2) * kRedBlocks);
+  if (!rc) rc = dev_alloc(s, (void **)&s->red_out, sizeof(double));
+  if (!rc) {
+    const bool fused_ok = fused_supported(s->p, L);
+    if (p->path == MPDX_PATH_FUSED && !fused_ok) {
+      s->err = "libmpdata++: no fused kernel for this configuration";
+      rc = MPDX_EUNSUPPORTED;
+    }
+    s->active_path = (p->path != MPDX_PATH_PERPASS && fused_ok) ? MPDX_PATH_FUSED : MPDX_PATH_PERPASS;
+  }
+  if (!rc && s->active_path == MPDX_PATH_FUSED) rc = fused_alloc(s);
+  if (!rc) {
+    hipError_t e2 = hipStreamSynchronize(s->stream);
+    if (e2 != hipSuccess) {
+      s->err = std::string("libmpdata++: HIP error: ") + hipGetErrorString(e2);
+      rc = MPDX_EHIP;
+    }
+  }
+  if (rc) {
+    g_create_error = s->err;
+    mpdx_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return MPDX_OK;
+}
+
+int mpdx_destroy(mpdx_solver *s)
+{
+  if (!s) return MPDX_OK;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (void *p : s->allocs)
+    if (p) (void)hipFree(p);
+  fused_free(s);
+  if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return MPDX_OK;
+}
+
+const char *mpdx_last_error(const mpdx_solver *s)
+{
+  return s ? s->err.c_str() : g_create_error.c_str();
+}
+
+int mpdx_set_stream(mpdx_solver *s, void *stream)
+{
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->own_stream) HIPCHK(hipStreamDestroy(s->stream));
+  s->own_stream = false;
+  s->stream = static_cast<hipStream_t>(stream);
+  if (!s->stream) {
+    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    s->own_stream = true;
+  }
+  return MPDX_OK;
+}
+
+int mpdx_advector_extents(const mpdx_solver *s, int d, int ext[3])
+{
+  if (d < 0 || d >= s->L.nd) return MPDX_EINVAL;
+  int lo[3], hi[3];
+  adv_bounds(s, d, lo, hi);
+  for (int q = 0; q < 3; ++q) ext[q] = hi[q] - lo[q] + 1;
+  return MPDX_OK;
+}
+
+int mpdx_set_advectee(mpdx_solver *s, int e, const void *buf, const int64_t strides[3], int on_device)
+{
+  if (e < 0 || e >= s->p.n_eqns) { s->err = "libmpdata++: bad equation index"; return MPDX_EINVAL; }
+  int lo[3] = {0, 0, 0}, hi[3] = {s->L.n[0] - 1, s->L.n[1] - 1, s->L.nd == 3 ? s->L.n[2] - 1 : 0};
+  return copy_view(s, s->psi[e][s->lev[e]], lo, hi, const_cast<void *>(buf), strides, on_device, true);
+}
+
+int mpdx_get_advectee(mpdx_solver *s, int e, void *buf, const int64_t strides[3], int on_device)
+{
+  if (e < 0 || e >= s->p.n_eqns) { s->err = "libmpdata++: bad equation index"; return MPDX_EINVAL; }
+  int lo[3] = {0, 0, 0}, hi[3] = {s->L.n[0] - 1, s->L.n[1] - 1, s->L.nd == 3 ? s->L.n[2] - 1 : 0};
+  return copy_view(s, s->psi[e][s->lev[e]], lo, hi, buf, strides, on_device, false);
+}
+
+int mpdx_set_advector(mpdx_solver *s, int d, const void *buf, const int64_t strides[3], int on_device)
+{
+  if (d < 0 || d >= s->L.nd) { s->err = "libmpdata++: bad advector component"; return MPDX_EINVAL; }
+  int lo[3], hi[3];
+  adv_bounds(s, d, lo, hi);
+  return copy_view(s, s->GC[d], lo, hi, const_cast<void *>(buf), strides, on_device, true);
+}
+
+int mpdx_get_advector(mpdx_solver *s, int d, void *buf, const int64_t strides[3], int on_device)
+{
+  if (d < 0 || d >= s->L.nd) { s->err = "libmpdata++: bad advector component"; return MPDX_EINVAL; }
+  int lo[3], hi[3];
+  adv_bounds(s, d, lo, hi);
+  return copy_view(s, s->GC[d], lo, hi, buf, strides, on_device, false);
+}
+
+int mpdx_advance(mpdx_solver *s, int64_t nt)
+{
+  if (nt < 0) { s->err = "libmpdata++: negative number of steps"; return MPDX_EINVAL; }
+  if (s->esz == 8) return advance_impl<double, double>(s, nt);
+  if (s->p.guard == MPDX_GUARD_NATIVE) return advance_impl<float, float>(s, nt);
+  return advance_impl<float, double>(s, nt);
+}
+
+int mpdx_reduce(mpdx_solver *s, int e, int op, double *out)
+{
+  if (e < 0 || e >= s->p.n_eqns || op < 0 || op > 2) { s->err = "libmpdata++: bad reduce arguments"; return MPDX_EINVAL; }
+  if (s->esz == 8) return reduce_impl<double>(s, s->psi[e][s->lev[e]], op, out);
+  return reduce_impl<float>(s, s->psi[e][s->lev[e]], op, out);
+}
+
+int mpdx_time(const mpdx_solver *s, double *t)
+{
+  *t = s->time;
+  return MPDX_OK;
+}
+
+int mpdx_timestep(const mpdx_solver *s, int64_t *ts)
+{
+  *ts = s->timestep;
+  return MPDX_OK;
+}
+
+int *mpdx_panic_ptr(mpdx_solver *s) { return &s->panic; }
+
+int mpdx_active_path(const mpdx_solver *s) { return s->active_path; }
+
+int mpdx_profile(mpdx_solver *s, int enable)
+{
+  HIPCHK(hipStreamSynchronize(s->stream));
+  prof_collect(s);
+  s->prof.clear();
+  s->prof_order.clear();
+  s->prof_on = enable != 0;
+  return MPDX_OK;
+}
+
+int mpdx_profile_read(mpdx_solver *s, int idx, char *name, int name_len, double *total_ms,
+                      int64_t *launches)
+{
+  HIPCHK(hipStreamSynchronize(s->stream));
+  prof_collect(s);
+  if (idx < 0 || idx >= (int)s->prof_order.size()) return MPDX_EINVAL;
+  const std::string &n = s->prof_order[idx];
+  if (name && name_len > 0) {
+    std::strncpy(name, n.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  const mpdx_prof_entry &pe = s->prof[n];
+  if (total_ms) *total_ms = pe.ms;
+  if (launches) *launches = pe.launches;
+  return MPDX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// Shared device helpers for the MPDATA kernels (gfx950).
+//
+// Arrays live in HBM as halo-extended boxes: dimension d spans
+// [-hw, n_d-1+hw] (scalars) or the face range [-hw, n_d-2+hw] of a
+// staggered component (face index i <-> i+1/2, formulae/arakawa_c.hpp:17-41),
+// with the innermost dimension padded so that element (.., .., 0) of every
+// row sits on a 128-byte boundary.  Element (i,j,k) is at
+// org + i*st[0] + j*st[1] + k*st[2]; in 2D st[2] == 0.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace mpdx {
+
+enum : unsigned {
+  O_FCT = 1u << 0, O_ABS = 1u << 1, O_TOT = 1u << 2, O_PFC = 1u << 3,
+  O_NPA = 1u << 4, O_IGA = 1u << 5, O_NUG = 1u << 6, O_DFL = 1u << 7,
+  O_KHN = 1u << 8
+};
+enum { BC_CYCLIC = 0, BC_OPEN = 1 };
+
+struct Layout {
+  int nd, hw;
+  int n[3];
+  long st[3];
+  long org;
+  long size;      // elements per array, including padding
+};
+
+// index rotation idxperm::pi<d> (formulae/idxperm.hpp:96-125):
+// position[d] = a, position[d+1] = b, position[d+2] = c
+struct Rot {
+  long o, sA, sB, sC;
+};
+
+inline Rot make_rot(const Layout &L, int d)
+{
+  Rot r;
+  r.o = L.org;
+  r.sA = L.st[d];
+  r.sB = L.st[(d + 1) % L.nd];
+  r.sC = L.nd == 3 ? L.st[(d + 2) % 3] : 0;
+  return r;
+}
+
+__host__ __device__ inline long rr(const Rot &r, int a, int b, int c)
+{
+  return r.o + (long)a * r.sA + (long)b * r.sB + (long)c * r.sC;
+}
+
+// flat index box: thread t -> (a, b, c) with c fastest
+struct Box {
+  int lo[3];
+  int ext[3];
+  long total;
+};
+
+inline Box make_box(int a0, int a1, int b0, int b1, int c0, int c1)
+{
+  Box b;
+  b.lo[0] = a0; b.lo[1] = b0; b.lo[2] = c0;
+  b.ext[0] = a1 - a0 + 1; b.ext[1] = b1 - b0 + 1; b.ext[2] = c1 - c0 + 1;
+  if (b.ext[0] <= 0 || b.ext[1] <= 0 || b.ext[2] <= 0) b.total = 0;
+  else b.total = (long)b.ext[0] * b.ext[1] * b.ext[2];
+  return b;
+}
+
+__device__ inline bool box_idx(const Box &b, long t, int &a, int &bb, int &c)
+{
+  if (t >= b.total) return false;
+  c = (int)(t % b.ext[2]) + b.lo[2];
+  t /= b.ext[2];
+  bb = (int)(t % b.ext[1]) + b.lo[1];
+  a = (int)(t / b.ext[1]) + b.lo[0];
+  return true;
+}
+
+template <class G> struct lim;
+template <> struct lim<float> {
+  __host__ __device__ static constexpr float tiny() { return FLT_MIN; }
+  __host__ __device__ static constexpr float eps() { return FLT_EPSILON; }
+};
+template <> struct lim<double> {
+  __host__ __device__ static constexpr double tiny() { return DBL_MIN; }
+  __host__ __device__ static constexpr double eps() { return DBL_EPSILON; }
+};
+
+// std::max / std::min argument order (ties and NaNs resolve like the reference)
+template <class T> __host__ __device__ inline T smax(T a, T b) { return (a < b) ? b : a; }
+template <class T> __host__ __device__ inline T smin(T a, T b) { return (b < a) ? b : a; }
+
+// pospart / negpart: formulae/common.hpp:87-124
+template <bool NPA, class R> __device__ inline R pospart(R x)
+{
+  if (NPA) return (x + fabs(x)) / 2;
+  return smax(R(0), x);
+}
+template <bool NPA, class R> __device__ inline R negpart(R x)
+{
+  if (NPA) return (x - fabs(x)) / 2;
+  return smin(R(0), x);
+}
+
+// frac / fct_frac: formulae/mpdata/formulae_mpdata_common.hpp:61-98, with the
+// guard evaluated in precision G (G = real_t reproduces the reference)
+template <bool PFC, class G, class R> __device__ inline R frac(R nom, R den)
+{
+  if (PFC) return den != 0 ? R(G(nom) / G(den)) : R(0);
+  return R(G(nom) / (G(den) + lim<G>::tiny()));
+}
+template <class G, class R> __device__ inline R fct_frac(R nom, R den)
+{
+  return R(G(nom) / (G(den) + lim<G>::eps()));
+}
+
+// donor-cell flux: formulae/donorcell_formulae.hpp:25-34
+template <bool NPA, class R> __device__ inline R donor_flux(R psi_l, R psi_r, R C)
+{
+  return pospart<NPA>(C) * psi_l + negpart<NPA>(C) * psi_r;
+}
+
+}  // namespace mpdx

@@ -1,0 +1,379 @@
+// Per-pass MPDATA kernels: one kernel per reference pass over a full
+// halo-extended array, plus small kernels that replay the reference's halo
+// fills with the reference's ranges and in the reference's order.  This path
+// is exact for every input (including cyclic fields whose duplicated edge
+// cells disagree) and serves every option combination; the fused kernels
+// (mpdx_fused.hpp) are checked against it.
+//
+// Citations are path:line in the reference tree (libmpdata++/...).
+#pragma once
+
+#include "mpdx_common.hpp"
+
+namespace mpdx {
+
+template <class R> __device__ inline R pos_rt(unsigned o, R x)
+{
+  return (o & O_NPA) ? pospart<true>(x) : pospart<false>(x);
+}
+template <class R> __device__ inline R neg_rt(unsigned o, R x)
+{
+  return (o & O_NPA) ? negpart<true>(x) : negpart<false>(x);
+}
+template <class G, class R> __device__ inline R frac_rt(unsigned o, R n, R d)
+{
+  return (o & O_PFC) ? frac<true, G>(n, d) : frac<false, G>(n, d);
+}
+
+// ---------------------------------------------------------------- halos
+
+// scalar / tangential-advector halo along d over the rotated (b, c, m) box:
+// mode 0 cyclic copy with period N-1 (bcond/cyclic_3d.hpp:29-33,120-124),
+// mode 1 edge copy (bcond/open_3d.hpp:32-42,154-164), mode 2 zero
+// (open tangential advector, open_3d.hpp:127-133,248-254)
+template <class R>
+__global__ void k_fill(R *a, Rot r, Box bx, int side, int mode, int N)
+{
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int b, c, m;
+  if (!box_idx(bx, t, b, c, m)) return;
+  long dst, src;
+  if (side == 0) { dst = rr(r, -m, b, c); src = rr(r, mode == 0 ? N - 1 - m : 0, b, c); }
+  else           { dst = rr(r, N - 1 + m, b, c); src = rr(r, mode == 0 ? m : N - 1, b, c); }
+  a[dst] = mode == 2 ? R(0) : a[src];
+}
+
+// open-boundary zeroing of perpendicular advector components at the edge
+// (open_3d.hpp:79-103,201-225; open_2d.hpp fill_halos_vctr_alng)
+template <class R>
+__global__ void k_alng_zero(R *av1, R *av2, Rot r, int nd, int d, int e,
+                            int raf, int ral, int rbf, int rbl)
+{
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd == 3) {
+    if (d == 0 && t <= ral - raf) {
+      int b = raf + t;
+      av2[rr(r, e, b, rbf - 1)] = R(0);
+      av2[rr(r, e, b, rbl)] = R(0);
+    }
+    if ((d == 0 || d == 1) && t <= rbl - rbf) {
+      int c = rbf + t;
+      av1[rr(r, e, raf - 1, c)] = R(0);
+      av1[rr(r, e, ral, c)] = R(0);
+    }
+  } else if (d == 0 && t == 0) {
+    av1[rr(r, e, raf - 1, 0)] = R(0);
+    av1[rr(r, e, ral, 0)] = R(0);
+  }
+}
+
+// normal advector halo faces: cyclic copy (cyclic_3d.hpp:44-48,135-139) or
+// the open zero-divergence condition (open_3d.hpp:111-124,233-245); the box
+// is (ii or m, b, c)
+template <class R>
+__global__ void k_alng(R *avd, const R *av1, const R *av2, Rot r, Box bx, int nd,
+                       int side, int cyclic, int N)
+{
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int ii, b, c;
+  if (!box_idx(bx, t, ii, b, c)) return;
+  if (cyclic) {
+    const int m = ii;
+    if (side == 0) avd[rr(r, -m, b, c)] = avd[rr(r, N - 1 - m, b, c)];
+    else           avd[rr(r, N - 2 + m, b, c)] = avd[rr(r, m - 1, b, c)];
+    return;
+  }
+  const int e = side == 0 ? 0 : N - 1;
+  R v;
+  if (nd == 3) {
+    if (side == 0)
+      v = avd[rr(r, e, b, c)] - (av1[rr(r, e, b - 1, c)] - av1[rr(r, e, b, c)])
+                              - (av2[rr(r, e, b, c - 1)] - av2[rr(r, e, b, c)]);
+    else
+      v = avd[rr(r, e - 1, b, c)] + (av1[rr(r, e, b - 1, c)] - av1[rr(r, e, b, c)])
+                                  + (av2[rr(r, e, b, c - 1)] - av2[rr(r, e, b, c)]);
+  } else {
+    if (side == 0)
+      v = avd[rr(r, e, b, 0)] - (av1[rr(r, e, b - 1, 0)] - av1[rr(r, e, b, 0)]);
+    else
+      v = avd[rr(r, e - 1, b, 0)] + (av1[rr(r, e, b - 1, 0)] - av1[rr(r, e, b, 0)]);
+  }
+  avd[rr(r, ii, b, c)] = v;
+}
+
+// ---------------------------------------------------------------- passes
+
+// donor-cell fluxes along d (donorcell_formulae.hpp:50-79)
+template <class R>
+__global__ void k_flux(R *out, const R *psi, const R *C, Rot r, Box bx, unsigned o)
+{
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int a, b, c;
+  if (!box_idx(bx, t, a, b, c)) return;
+  const long p0 = rr(r, a, b, c);
+  const R cc = C[p0];
+  out[p0] = pos_rt(o, cc) * psi[p0] + neg_rt(o, cc) * psi[rr(r, a + 1, b, c)];
+}
+
+// donorcell_sum over the interior (donorcell_formulae.hpp:105-165)
+template <class R>
+__global__ void k_donorcell(R *psi_new, const R *psi_old, const R *f0, const R *f1,
+                            const R *f2, Layout L, Box bx, unsigned o)
+{
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_idx(bx, t, i, j, k)) return;
+  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
+  R f[6];
+  f[0] = f0[p]; f[1] = f0[p - L.st[0]];
+  f[2] = f1[p]; f[3] = f1[p - L.st[1]];
+  if (L.nd == 3) { f[4] = f2[p]; f[5] = f2[p - L.st[2]]; }
+  if (!(o & O_KHN)) {
+    if (L.nd == 3)
+      psi_new[p] = psi_old[p] + ((-f[0] + f[1]) + (-f[2] + f[3]) + (-f[4] + f[5])) / 1;
+    else
+      psi_new[p] = psi_old[p] + ((-f[0] + f[1]) + (-f[2] + f[3])) / 1;
+  } else {
+    // kahan_zro / kahan_add sequence (formulae/kahan_sum.hpp:13-27)
+    R sum = 0, cc = 0;
+    R in[7];
+    in[0] = psi_old[p];
+    const int nq = 2 * L.nd;
+    for (int q = 0; q < nq; ++q) in[1 + q] = (q % 2 == 0) ? -f[q] / 1 : f[q] / 1;
+    for (int q = 0; q <= nq; ++q) {
+      R y = in[q] - cc;
+      R tt = sum + y;
+      cc = (tt - sum) - y;
+      sum = tt;
+    }
+    psi_new[p] = sum;
+  }
+}
+
+// antidiffusive velocity along d, standard form: formulae_mpdata_3d.hpp:253-295
+// (2D: formulae_mpdata_2d.hpp:241-278) over faces -1..N_d-1 x interior
+template <class R, class G>
+__global__ void k_antidiff(R *V, const R *psi, const R *U0, const R *U1, const R *U2,
+                           Rot r, Box bx, int nd, unsigned o)
+{
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int a, b, c;
+  if (!box_idx(bx, t, a, b, c)) return;
+  const bool iga = o & O_IGA, ab = o & O_ABS;
+  auto P = [&](int x, int y, int z) -> R {
+    R v = psi[rr(r, x, y, z)];
+    return ab ? fabs(v) : v;
+  };
+  const long p0 = rr(r, a, b, c);
+  const R u = U0[p0];
+  R ndx, ndy;
+  if (iga) {
+    ndx = nd == 3 ? psi[rr(r, a + 1, b, c)] - psi[p0]
+                  : 2 * (psi[rr(r, a + 1, b, c)] - psi[p0]) / (1 + 1);
+    ndy = (psi[rr(r, a + 1, b + 1, c)] + psi[rr(r, a, b + 1, c)]
+           - psi[rr(r, a + 1, b - 1, c)] - psi[rr(r, a, b - 1, c)]) / 4;
+  } else {
+    ndx = 2 * frac_rt<G>(o, P(a + 1, b, c) - P(a, b, c), P(a + 1, b, c) + P(a, b, c));
+    ndy = frac_rt<G>(o, P(a + 1, b + 1, c) + P(a, b + 1, c) - P(a + 1, b - 1, c) - P(a, b - 1, c),
+                        P(a + 1, b + 1, c) + P(a, b + 1, c) + P(a + 1, b - 1, c) + P(a, b - 1, c));
+  }
+  const R g1 = (U1[rr(r, a + 1, b, c)] + U1[rr(r, a, b, c)]
+                + U1[rr(r, a + 1, b - 1, c)] + U1[rr(r, a, b - 1, c)]) / 4;
+  if (nd == 3) {
+    R ndz;
+    if (iga)
+      ndz = (psi[rr(r, a + 1, b, c + 1)] + psi[rr(r, a, b, c + 1)]
+             - psi[rr(r, a + 1, b, c - 1)] - psi[rr(r, a, b, c - 1)]) / 4;
+    else
+      ndz = frac_rt<G>(o, P(a + 1, b, c + 1) + P(a, b, c + 1) - P(a + 1, b, c - 1) - P(a, b, c - 1),
+                          P(a + 1, b, c + 1) + P(a, b, c + 1) + P(a + 1, b, c - 1) + P(a, b, c - 1));
+    const R g2 = (U2[rr(r, a + 1, b, c)] + U2[rr(r, a, b, c)]
+                  + U2[rr(r, a + 1, b, c - 1)] + U2[rr(r, a, b, c - 1)]) / 4;
+    V[p0] = fabs(u) / 2 * (1 - fabs(u) / 1) * ndx - u / 2 * (g1 * ndy + g2 * ndz) / 1;
+  } else {
+    V[p0] = fabs(u) / 2 * (1 - fabs(u) / 1) * ndx - u * g1 / (2 * 1) * ndy;
+  }
+}
+
+// psi_min / psi_max over interior^1 (mpdata_fct_3d.hpp:31-55, mpdata_fct_2d.hpp:31-46)
+template <class R>
+__global__ void k_fct_init(R *pmin, R *pmax, const R *psi, Layout L, Box bx)
+{
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_idx(bx, t, i, j, k)) return;
+  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
+  const long s0 = L.st[0], s1 = L.st[1], s2 = L.st[2];
+  if (L.nd == 3) {
+    pmin[p] = smin(smin(smin(smin(smin(smin(psi[p], psi[p + s0]), psi[p - s0]), psi[p + s1]),
+                              psi[p - s1]), psi[p + s2]), psi[p - s2]);
+    pmax[p] = smax(smax(smax(smax(smax(smax(psi[p], psi[p + s0]), psi[p - s0]), psi[p + s1]),
+                              psi[p - s1]), psi[p + s2]), psi[p - s2]);
+  } else {
+    pmin[p] = smin(smin(smin(smin(psi[p + s1], psi[p - s0]), psi[p]), psi[p + s0]), psi[p - s1]);
+    pmax[p] = smax(smax(smax(smax(psi[p + s1], psi[p - s0]), psi[p]), psi[p + s0]), psi[p - s1]);
+  }
+}
+
+// beta_up / beta_dn over interior^1: formulae_mpdata_fct_3d.hpp:21-145,
+// formulae_mpdata_fct_2d.hpp:21-120
+template <class R, class G>
+__global__ void k_betas(R *bup, R *bdn, const R *psi, const R *pmin, const R *pmax,
+                        const R *f0, const R *f1, const R *f2, Layout L, Box bx, unsigned o)
+{
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_idx(bx, t, i, j, k)) return;
+  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
+  const long s0 = L.st[0], s1 = L.st[1], s2 = L.st[2];
+  R up_nom, dn_nom, up_den, dn_den;
+  if (L.nd == 3) {
+    up_nom = (smax(pmax[p], smax(psi[p], smax(psi[p + s0], smax(psi[p - s0], smax(psi[p + s1],
+              smax(psi[p - s1], smax(psi[p + s2], psi[p - s2]))))))) - psi[p]) * 1;
+    dn_nom = (psi[p] - smin(pmin[p], smin(psi[p + s1], smin(psi[p - s0], smin(psi[p],
+              smin(psi[p + s0], smin(psi[p + s2], smin(psi[p - s2], psi[p - s1])))))))) * 1;
+    up_den = (pos_rt(o, f0[p - s0]) - neg_rt(o, f0[p]))
+           + (pos_rt(o, f1[p - s1]) - neg_rt(o, f1[p]))
+           + (pos_rt(o, f2[p - s2]) - neg_rt(o, f2[p]));
+    dn_den = (pos_rt(o, f0[p]) - neg_rt(o, f0[p - s0]))
+           + (pos_rt(o, f1[p]) - neg_rt(o, f1[p - s1]))
+           + (pos_rt(o, f2[p]) - neg_rt(o, f2[p - s2]));
+  } else {
+    up_nom = (smax(pmax[p], smax(psi[p + s1], smax(psi[p - s0], smax(psi[p],
+              smax(psi[p + s0], psi[p - s1]))))) - psi[p]) * 1;
+    dn_nom = (psi[p] - smin(pmin[p], smin(psi[p + s1], smin(psi[p - s0], smin(psi[p],
+              smin(psi[p + s0], psi[p - s1])))))) * 1;
+    up_den = (pos_rt(o, f0[p - s0]) - neg_rt(o, f0[p]))
+           + (pos_rt(o, f1[p - s1]) - neg_rt(o, f1[p]));
+    dn_den = (pos_rt(o, f0[p]) - neg_rt(o, f0[p - s0]))
+           + (pos_rt(o, f1[p]) - neg_rt(o, f1[p - s1]));
+  }
+  bup[p] = fct_frac<G>(up_nom, up_den);
+  bdn[p] = fct_frac<G>(dn_nom, dn_den);
+}
+
+// limited antidiffusive velocity: formulae_mpdata_fct_3d.hpp:147-247
+template <class R>
+__global__ void k_gc_mono(R *M, const R *V, const R *psi, const R *bup, const R *bdn,
+                          Rot r, Box bx, unsigned o)
+{
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int a, b, c;
+  if (!box_idx(bx, t, a, b, c)) return;
+  const long p0 = rr(r, a, b, c), p1 = rr(r, a + 1, b, c);
+  const R v = V[p0];
+  const R m_dn_up = smin(R(1), smin(bdn[p0], bup[p1]));
+  const R m_up_dn = smin(R(1), smin(bup[p0], bdn[p1]));
+  R m;
+  const bool absv = (o & O_ABS) && !(o & O_IGA);
+  if (!absv) m = v > 0 ? m_dn_up : m_up_dn;
+  else if (v > 0) m = psi[p0] > 0 ? m_dn_up : m_up_dn;
+  else m = psi[p1] > 0 ? m_up_dn : m_dn_up;
+  M[p0] = v * m;
+}
+
+// state scaling by an exact power of two (solver_common.hpp:374-379)
+template <class R>
+__global__ void k_scale(R *a, Layout L, Box bx, R factor)
+{
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_idx(bx, t, i, j, k)) return;
+  a[L.org + i * L.st[0] + j * L.st[1] + k * L.st[2]] *= factor;
+}
+
+// |div GC| per interior cell into stat (solver_3d.hpp:400-409, solver_2d.hpp:305-313)
+template <class R>
+__global__ void k_div(R *stat, const R *g0, const R *g1, const R *g2, Layout L, Box bx)
+{
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_idx(bx, t, i, j, k)) return;
+  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
+  R v;
+  if (L.nd == 3)
+    v = fabs((g0[p] - g0[p - L.st[0]]) + (g1[p] - g1[p - L.st[1]]) + (g2[p] - g2[p - L.st[2]])) / 1;
+  else
+    v = fabs((g0[p] - g0[p - L.st[0]]) + (g1[p] - g1[p - L.st[1]])) / 1;
+  stat[p] = v;
+}
+
+template <class R>
+__global__ void k_fill_value(R *a, long n, R v)
+{
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t < n) a[t] = v;
+}
+
+// ---------------------------------------------------------------- copies and reductions
+
+template <class R>
+__global__ void k_copy_box(R *dst, long dorg, long ds0, long ds1, long ds2,
+                           const R *src, long sorg, long ss0, long ss1, long ss2, Box bx)
+{
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (!box_idx(bx, t, i, j, k)) return;
+  dst[dorg + i * ds0 + j * ds1 + k * ds2] = src[sorg + i * ss0 + j * ss1 + k * ss2];
+}
+
+// block-partial min / max / Kahan sum over an interior box; partials are
+// (value, compensation) pairs in FP64, combined in a fixed order afterwards
+template <class R>
+__global__ void k_reduce(double2 *part, const R *a, Layout L, Box bx, int op)
+{
+  __shared__ double2 sh[256];
+  double acc = op == 0 ? INFINITY : (op == 1 ? -INFINITY : 0.0), cc = 0.0;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < bx.total;
+       t += (long)gridDim.x * blockDim.x) {
+    int i, j, k;
+    box_idx(bx, t, i, j, k);
+    const double v = double(a[L.org + i * L.st[0] + j * L.st[1] + k * L.st[2]]);
+    if (op == 0) acc = smin(acc, v);
+    else if (op == 1) acc = smax(acc, v);
+    else {
+      const double y = v - cc, s = acc + y;
+      cc = (s - acc) - y;
+      acc = s;
+    }
+  }
+  sh[threadIdx.x] = make_double2(acc, cc);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double A = op == 0 ? INFINITY : (op == 1 ? -INFINITY : 0.0), C = 0.0;
+    for (int q = 0; q < (int)blockDim.x; ++q) {
+      if (op == 0) A = smin(A, sh[q].x);
+      else if (op == 1) A = smax(A, sh[q].x);
+      else {
+        const double in2[2] = {sh[q].x, -sh[q].y};
+        for (int w = 0; w < 2; ++w) {
+          const double y = in2[w] - C, s = A + y;
+          C = (s - A) - y;
+          A = s;
+        }
+      }
+    }
+    part[blockIdx.x] = make_double2(A, C);
+  }
+}
+
+__global__ void k_reduce_final(double *out, const double2 *part, int n, int op)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double A = op == 0 ? INFINITY : (op == 1 ? -INFINITY : 0.0), C = 0.0;
+  for (int q = 0; q < n; ++q) {
+    if (op == 0) A = smin(A, part[q].x);
+    else if (op == 1) A = smax(A, part[q].x);
+    else {
+      const double in2[2] = {part[q].x, -part[q].y};
+      for (int w = 0; w < 2; ++w) {
+        const double y = in2[w] - C, s = A + y;
+        C = (s - A) - y;
+        A = s;
+      }
+    }
+  }
+  *out = A;
+}
+
+}  // namespace mpdx

@@ -1,0 +1,107 @@
+// Host-side state of one solver object (one advectee set sharing one advector).
+#pragma once
+
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "mpdx.h"
+#include "mpdx_common.hpp"
+
+struct mpdx_fused_state;
+
+// per-kernel-class timing with HIP events on the solver's stream
+struct mpdx_prof_entry {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double ms = 0;
+  int64_t launches = 0;
+};
+
+struct mpdx_solver {
+  mpdx_params p;
+  mpdx::Layout L;
+  int esz = 8;
+  std::vector<void *> allocs;
+  void *psi[MPDX_MAX_EQNS][2] = {};
+  int lev[MPDX_MAX_EQNS] = {};
+  void *GC[3] = {};
+  void *tmp[2][3] = {};
+  void *flux[3] = {};
+  void *mono[3] = {};
+  void *pmin = nullptr, *pmax = nullptr, *bup = nullptr, *bdn = nullptr, *stat = nullptr;
+  bool perpass_ready = false;
+  double2 *red_part = nullptr;
+  double *red_out = nullptr;
+  void *staging = nullptr;
+  size_t staging_bytes = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int64_t timestep = 0;
+  double time = 0;
+  int panic = 0;
+  int active_path = MPDX_PATH_PERPASS;
+  std::string err;
+  mpdx_fused_state *fz = nullptr;
+  bool prof_on = false;
+  std::vector<std::string> prof_order;
+  std::map<std::string, mpdx_prof_entry> prof;
+  std::vector<hipEvent_t> ev_pool;
+};
+
+namespace mpdx {
+
+inline hipEvent_t prof_event(mpdx_solver *s)
+{
+  hipEvent_t e = nullptr;
+  if (!s->ev_pool.empty()) {
+    e = s->ev_pool.back();
+    s->ev_pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    e = nullptr;
+  }
+  return e;
+}
+
+// records a start/stop event pair around the kernel launches in its scope
+struct ProfScope {
+  mpdx_solver *s;
+  const char *name;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(mpdx_solver *s_, const char *n) : s(s_), name(n)
+  {
+    if (!s->prof_on) return;
+    a = prof_event(s);
+    b = prof_event(s);
+    if (a) (void)hipEventRecord(a, s->stream);
+  }
+  ~ProfScope()
+  {
+    if (!a || !b) return;
+    (void)hipEventRecord(b, s->stream);
+    auto it = s->prof.find(name);
+    if (it == s->prof.end()) {
+      s->prof_order.push_back(name);
+      it = s->prof.emplace(name, mpdx_prof_entry()).first;
+    }
+    it->second.pending.emplace_back(a, b);
+  }
+};
+
+inline void prof_collect(mpdx_solver *s)
+{
+  for (auto &kv : s->prof) {
+    for (auto &pr : kv.second.pending) {
+      float ms = 0;
+      (void)hipEventSynchronize(pr.second);
+      if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) kv.second.ms += ms;
+      kv.second.launches += 1;
+      s->ev_pool.push_back(pr.first);
+      s->ev_pool.push_back(pr.second);
+    }
+    kv.second.pending.clear();
+  }
+}
+
+}  // namespace mpdx
+

@@ -161,13 +161,15 @@ def streamfunction_advector(grid, seed, max_courant=0.5, n_modes=4, periodic=Tru
     per z-level plus a w built from a second streamfunction in (x, z)."""
     rng = np.random.default_rng(seed)
     nd = len(grid)
+    if isinstance(periodic, bool):
+        periodic = (periodic,) * nd
 
-    def corner_field(nx, ny):
+    def corner_field(nx, ny, perx, pery):
         # corners at x = i+1/2 for i = -2..nx-1, same in y (covers all faces)
         xs = np.arange(-2, nx) + 0.5
         ys = np.arange(-2, ny) + 0.5
-        px = (nx - 1) if periodic else 2 * nx
-        py = (ny - 1) if periodic else 2 * ny
+        px = (nx - 1) if perx else 2 * nx
+        py = (ny - 1) if pery else 2 * ny
         f = np.zeros((xs.size, ys.size))
         for _ in range(n_modes):
             kx = rng.integers(1, 3)
@@ -179,7 +181,7 @@ def streamfunction_advector(grid, seed, max_courant=0.5, n_modes=4, periodic=Tru
 
     if nd == 2:
         nx, ny = grid
-        f = corner_field(nx, ny)
+        f = corner_field(nx, ny, periodic[0], periodic[1])
         # GC_x(face i+1/2, cell j) = f(i+1/2, j+1/2) - f(i+1/2, j-1/2)
         gx_full = f[:, 1:] - f[:, :-1]          # x corners -2..nx-1 -> faces; y cells -1..ny-1
         gy_full = -(f[1:, :] - f[:-1, :])       # x cells -1..nx-1; y faces -2..ny-1
@@ -189,11 +191,11 @@ def streamfunction_advector(grid, seed, max_courant=0.5, n_modes=4, periodic=Tru
         gy_full = -(f[1:, :] - f[:-1, :])
         # views: gx faces -1..nx-2 (corner index +1), y cells 0..ny-1 (index +1)
         gx = gx_full[1:nx + 1, 1:ny + 1]
-        gy = gy_full[0:nx + 1, 2:ny]
+        gy = gy_full[0:nx + 1, 2:ny + 1]
         return [np.ascontiguousarray(gx), np.ascontiguousarray(gy)]
     nx, ny, nz = grid
-    f = corner_field(nx, ny)
-    g = corner_field(nx, nz)
+    f = corner_field(nx, ny, periodic[0], periodic[1])
+    g = corner_field(nx, nz, periodic[0], periodic[2])
     gx_full = f[:, 1:] - f[:, :-1]
     gy_full = -(f[1:, :] - f[:-1, :])
     hx_full = g[:, 1:] - g[:, :-1]           # x faces, z cells
@@ -209,7 +211,7 @@ def streamfunction_advector(grid, seed, max_courant=0.5, n_modes=4, periodic=Tru
     gx = np.empty((nx, ny, nz))
     gx[:] = gx_full[1:nx + 1, 1:ny + 1][:, :, None] + hx_full[1:nx + 1, 1:nz + 1][:, None, :]
     gy = np.empty((nx + 1, ny - 1, nz))
-    gy[:] = gy_full[0:nx + 1, 2:ny][:, :, None]
+    gy[:] = gy_full[0:nx + 1, 2:ny + 1][:, :, None]
     gz = np.empty((nx + 1, ny, nz - 1))
-    gz[:] = hz_full[0:nx + 1, 2:nz][:, None, :]
+    gz[:] = hz_full[0:nx + 1, 2:nz + 1][:, None, :]
     return [gx, gy, gz]

@@ -1,0 +1,56 @@
+"""CPU-side checks of the C ABI: the library loads and exports every entry
+point include/mpdx.h declares, the parameter defaults match the reference's
+ct_params_default_t / rt_params_t, and the package refuses to run without the
+HIP library (no CPU fallback)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import libmpdataxx_amd as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "mpdx.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mpdx_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declares_expected_api():
+    assert declared_symbols() == sorted(M.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(M.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_params_default_matches_reference():
+    p = M.Params()
+    M.lib().mpdx_params_default(ctypes.byref(p))
+    assert p.opts == M.IGA | M.FCT          # opts.hpp:83
+    assert p.n_iters == 2                   # mpdata_common.hpp:76
+    assert p.n_eqns == 1 and p.max_abs_div_eps < 0
+    assert p.guard == M.GUARD_F64 and p.path == M.PATH_AUTO
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(grid=(0, 8)), "bogus grid size"),
+    (dict(grid=(8, 8), opts=M.ABS | M.IGA), "mutually exclusive"),
+    (dict(grid=(8, 8), opts=M.TOT), "not supported"),
+    (dict(grid=(8, 8), opts=M.IGA, n_iters=3), "one corrective step"),
+])
+def test_create_rejects_like_reference(kw, msg):
+    with pytest.raises(M.MpdxError, match=msg):
+        M.Solver(**kw)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(M, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(M, "_lib", None)
+    with pytest.raises(RuntimeError, match="not built"):
+        M.lib()
