@@ -57,12 +57,19 @@ extern "C" {
 #define MPDX_REAL_F32 4
 #define MPDX_REAL_F64 8
 
-/* how the near-zero-denominator guards of the psi ratios and FCT betas are
- * evaluated on the FP32 path: in FP64 with FP64 constants (default), or in
- * FP32 with numeric_limits<float> constants as libmpdata++ does for
- * real_t=float (formulae/mpdata/formulae_mpdata_common.hpp:61-98) */
+/* How the near-zero-denominator guards of the psi ratios (n/(d + tiny)) and
+ * FCT betas (n/(d + epsilon)) are evaluated on the FP32 path
+ * (formulae/mpdata/formulae_mpdata_common.hpp:61-98).  FP64 solvers always use
+ * the reference's FP64 semantics.
+ *   MPDX_GUARD_F64        FP64 guard constants, ratio = n * rcp(d + guard) in
+ *                         FP32 with the hardware reciprocal (default)
+ *   MPDX_GUARD_NATIVE     numeric_limits<float> constants and IEEE FP32
+ *                         division: libmpdata++ with real_t = float
+ *   MPDX_GUARD_F64_EXACT  operands promoted to FP64, FP64 constants, IEEE FP64
+ *                         division, result rounded to FP32 */
 #define MPDX_GUARD_F64 0
 #define MPDX_GUARD_NATIVE 1
+#define MPDX_GUARD_F64_EXACT 2
 
 /* execution path */
 #define MPDX_PATH_AUTO 0      /* fused kernels where available, else per-pass */

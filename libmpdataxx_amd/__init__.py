@@ -23,7 +23,7 @@ MAX_EQNS = 64
 # option bits (libmpdata++/opts.hpp:34-49)
 FCT, ABS, TOT, PFC, NPA, IGA, NUG, DFL, KHN = (1 << b for b in range(9))
 CYCLIC, OPEN = 0, 1
-GUARD_F64, GUARD_NATIVE = 0, 1
+GUARD_F64, GUARD_NATIVE, GUARD_F64_EXACT = 0, 1, 2
 PATH_AUTO, PATH_PERPASS, PATH_FUSED = 0, 1, 2
 RED_MIN, RED_MAX, RED_SUM = 0, 1, 2
 

@@ -301,22 +301,48 @@ int alloc_array(mpdx_solver *s, void **out)
   return fill_nan(s, *out);
 }
 
+int alloc_missing(mpdx_solver *s, void **slot)
+{
+  return *slot ? 0 : alloc_array(s, slot);
+}
+
 int ensure_perpass(mpdx_solver *s)
 {
   if (s->perpass_ready) return 0;
   int rc = 0;
   for (int d = 0; d < s->L.nd && !rc; ++d) {
-    rc = rc || alloc_array(s, &s->tmp[0][d]);
-    rc = rc || alloc_array(s, &s->tmp[1][d]);
-    rc = rc || alloc_array(s, &s->flux[d]);
-    rc = rc || alloc_array(s, &s->mono[d]);
+    rc = rc || alloc_missing(s, &s->tmp[0][d]);
+    rc = rc || alloc_missing(s, &s->tmp[1][d]);
+    rc = rc || alloc_missing(s, &s->flux[d]);
+    rc = rc || alloc_missing(s, &s->mono[d]);
   }
-  rc = rc || alloc_array(s, &s->pmin);
-  rc = rc || alloc_array(s, &s->pmax);
-  rc = rc || alloc_array(s, &s->bup);
-  rc = rc || alloc_array(s, &s->bdn);
+  rc = rc || alloc_missing(s, &s->pmin);
+  rc = rc || alloc_missing(s, &s->pmax);
+  rc = rc || alloc_missing(s, &s->bup);
+  rc = rc || alloc_missing(s, &s->bdn);
   if (!rc) s->perpass_ready = true;
   return rc;
+}
+
+// fused path: two advector scratch sets (GC_corr ping-pong) and one scalar
+// scratch per equation
+int fused_alloc(mpdx_solver *s)
+{
+  if (!s->fz) s->fz = new mpdx_fused_state();
+  int rc = 0;
+  for (int d = 0; d < s->L.nd && !rc; ++d) {
+    if (s->p.n_iters > 1) rc = rc || alloc_missing(s, &s->tmp[0][d]);
+    if (s->p.n_iters > 2) rc = rc || alloc_missing(s, &s->tmp[1][d]);
+  }
+  for (int e = 0; e < s->p.n_eqns && !rc; ++e)
+    if (s->p.n_iters > 1) rc = alloc_missing(s, &s->fz->psi_tmp[e]);
+  return rc;
+}
+
+void fused_free(mpdx_solver *s)
+{
+  delete s->fz;
+  s->fz = nullptr;
 }
 
 int ensure_staging(mpdx_solver *s, size_t bytes)
@@ -421,7 +447,6 @@ int reduce_impl(mpdx_solver *s, const void *arr, int op, double *out)
 template <class R, class G>
 int ante_loop(mpdx_solver *s)
 {
-  if (int rc = ensure_perpass(s)) return rc;
   if (!s->stat) {
     if (int rc = alloc_array(s, &s->stat)) return rc;
   }
@@ -440,6 +465,242 @@ int ante_loop(mpdx_solver *s)
     return MPDX_EDIVERGENT;
   }
   pp.xchng_vctr_nrml(gc, s->L.hw - 1);
+  return 0;
+}
+
+// ----------------------------------------------------------------- fused stepper
+
+template <class R, class G>
+struct Fused {
+  mpdx_solver *s;
+  PerPass<R, G> pp;
+  const Layout &L;
+  BcMap m;
+  unsigned o;
+
+  explicit Fused(mpdx_solver *s_) : s(s_), pp(s_), L(s_->L), m(make_bcmap(s_)), o(s_->p.opts) {}
+
+  R *A(void *p) const { return static_cast<R *>(p); }
+
+  dim3 grid_cells(int extra) const
+  {
+    const int inner = L.n[L.nd - 1] + extra;
+    const unsigned gx = (inner + 255) / 256;
+    if (L.nd == 3) return dim3(gx, L.n[1] + extra, L.n[0] + extra);
+    return dim3(gx, L.n[0] + extra, 1);
+  }
+
+  void donor_box(R *out, R *in, R **C, R in_scale, R out_scale, int flux_is_C, const Box &bx)
+  {
+    if (bx.total <= 0) return;
+    R *c2 = L.nd == 3 ? C[2] : C[0];
+    const int inner = bx.ext[L.nd - 1];
+    dim3 g((inner + 255) / 256, L.nd == 3 ? bx.ext[1] : bx.ext[0], L.nd == 3 ? bx.ext[0] : 1);
+    if (o & O_NPA)
+      hipLaunchKernelGGL((k_fz_donor<R, true>), g, dim3(256), 0, s->stream, out, (const R *)in,
+                         (const R *)C[0], (const R *)C[1], (const R *)c2, L, m, in_scale, out_scale, flux_is_C, bx);
+    else
+      hipLaunchKernelGGL((k_fz_donor<R, false>), g, dim3(256), 0, s->stream, out, (const R *)in,
+                         (const R *)C[0], (const R *)C[1], (const R *)c2, L, m, in_scale, out_scale, flux_is_C, bx);
+  }
+
+  void donor(R *out, R *in, R **C, R in_scale, R out_scale, int flux_is_C)
+  {
+    ProfScope ps(s, "fused_donor");
+    donor_box(out, in, C, in_scale, out_scale, flux_is_C, full_box(0));
+  }
+
+  // launch the antidiffusive velocity for owner cells in box bx
+  template <bool IGA, bool ABS, bool PFC>
+  void antidiff_t(R **V, R *psi, R **U, const Box &bx)
+  {
+    if (bx.total <= 0) return;
+    const int inner = bx.ext[L.nd - 1];
+    dim3 g((inner + 255) / 256, L.nd == 3 ? bx.ext[1] : bx.ext[0], L.nd == 3 ? bx.ext[0] : 1);
+    hipLaunchKernelGGL((k_fz_antidiff<R, G, IGA, ABS, PFC>), g, dim3(256), 0, s->stream,
+                       V[0], V[1], V[L.nd == 3 ? 2 : 0], (const R *)psi, (const R *)U[0], (const R *)U[1],
+                       (const R *)U[L.nd == 3 ? 2 : 0], L, m, bx);
+  }
+
+  void antidiff_box(R **V, R *psi, R **U, const Box &bx)
+  {
+    const bool iga = o & O_IGA, ab = o & O_ABS, pfc = o & O_PFC;
+    if (iga) antidiff_t<true, false, false>(V, psi, U, bx);
+    else if (ab && pfc) antidiff_t<false, true, true>(V, psi, U, bx);
+    else if (ab) antidiff_t<false, true, false>(V, psi, U, bx);
+    else if (pfc) antidiff_t<false, false, true>(V, psi, U, bx);
+    else antidiff_t<false, false, false>(V, psi, U, bx);
+  }
+
+  template <int ND, bool NPA, bool ABS, bool IGA>
+  void fct_t(R *out, R *psi, R *psin, R **V, R in_scale, R out_scale, const Box &bx)
+  {
+    if (bx.total <= 0) return;
+    constexpr int TX = FzBrick<ND>::TX, TY = FzBrick<ND>::TY, TZ = FzBrick<ND>::TZ;
+    dim3 g(ND == 3 ? (bx.ext[2] + TZ - 1) / TZ : 1, (bx.ext[1] + TY - 1) / TY, (bx.ext[0] + TX - 1) / TX);
+    hipLaunchKernelGGL((k_fz_fct<R, G, ND, NPA, ABS, IGA>), g, dim3(256), 0, s->stream, out,
+                       (const R *)psi, (const R *)psin, (const R *)V[0], (const R *)V[1],
+                       (const R *)V[ND == 3 ? 2 : 0], L, m, in_scale, out_scale, bx);
+  }
+
+  template <int ND>
+  void fct_nd(R *out, R *psi, R *psin, R **V, R in_scale, R out_scale, const Box &bx)
+  {
+    const bool npa = o & O_NPA, ab = o & O_ABS, iga = o & O_IGA;
+    if (iga) { npa ? fct_t<ND, true, false, true>(out, psi, psin, V, in_scale, out_scale, bx)
+                   : fct_t<ND, false, false, true>(out, psi, psin, V, in_scale, out_scale, bx); }
+    else if (ab) { npa ? fct_t<ND, true, true, false>(out, psi, psin, V, in_scale, out_scale, bx)
+                       : fct_t<ND, false, true, false>(out, psi, psin, V, in_scale, out_scale, bx); }
+    else { npa ? fct_t<ND, true, false, false>(out, psi, psin, V, in_scale, out_scale, bx)
+               : fct_t<ND, false, false, false>(out, psi, psin, V, in_scale, out_scale, bx); }
+  }
+
+  void fct_box(R *out, R *psi, R *psin, R **V, R in_scale, R out_scale, const Box &bx)
+  {
+    if (L.nd == 3) fct_nd<3>(out, psi, psin, V, in_scale, out_scale, bx);
+    else fct_nd<2>(out, psi, psin, V, in_scale, out_scale, bx);
+  }
+
+  // the boxes of [f, l]^nd minus [f+wl, l-wh]^nd (faces x, then y, then z slabs)
+  std::vector<Box> shell(int f, int wl, int wh) const
+  {
+    std::vector<Box> out;
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = d < L.nd ? f : 0;
+      hi[d] = d < L.nd ? L.n[d] - 1 : 0;
+    }
+    int ilo[3], ihi[3];
+    for (int d = 0; d < 3; ++d) {
+      ilo[d] = d < L.nd ? lo[d] + wl : 0;
+      ihi[d] = d < L.nd ? hi[d] - wh : 0;
+    }
+    bool empty = false;
+    for (int d = 0; d < L.nd; ++d)
+      if (ilo[d] > ihi[d]) empty = true;
+    if (empty) {
+      out.push_back(make_box(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]));
+      return out;
+    }
+    int cl[3] = {lo[0], lo[1], lo[2]}, ch[3] = {hi[0], hi[1], hi[2]};
+    for (int d = 0; d < L.nd; ++d) {
+      int a[3] = {cl[0], cl[1], cl[2]}, b[3] = {ch[0], ch[1], ch[2]};
+      b[d] = ilo[d] - 1;
+      out.push_back(make_box(a[0], b[0], a[1], b[1], a[2], b[2]));
+      a[d] = ihi[d] + 1;
+      b[d] = ch[d];
+      out.push_back(make_box(a[0], b[0], a[1], b[1], a[2], b[2]));
+      cl[d] = ilo[d];
+      ch[d] = ihi[d];
+    }
+    return out;
+  }
+
+  Box full_box(int f) const
+  {
+    return make_box(f, L.n[0] - 1, f, L.n[1] - 1, L.nd == 3 ? f : 0, L.nd == 3 ? L.n[2] - 1 : 0);
+  }
+
+  template <bool FCT, bool IGA, bool STOREV>
+  void interior_t(R *out, R *psi, R *psin, R **U, R **V, R in_scale, R out_scale)
+  {
+    InteriorArgs<R> a;
+    a.out = out; a.psi = psi; a.psin = psin;
+    a.U0 = U[0]; a.U1 = U[1]; a.U2 = U[2];
+    a.V0 = V[0]; a.V1 = V[1]; a.V2 = V[2];
+    a.L = L;
+    for (int d = 0; d < 3; ++d) { a.lo[d] = 2; a.hi[d] = L.n[d] - 3; }
+    a.in_scale = in_scale; a.out_scale = out_scale;
+    constexpr int TY = ITile<R>::Y, TZ = ITile<R>::Z;
+    dim3 g((L.n[2] - 4 + TZ - 5) / (TZ - 4), (L.n[1] - 4 + TY - 5) / (TY - 4),
+           (L.n[0] - 4 + IX_CHUNK - 1) / IX_CHUNK);
+    hipLaunchKernelGGL((k_fz_interior<R, G, FCT, IGA, STOREV>), g, dim3(TZ, TY), 0, s->stream, a);
+  }
+
+  void interior(R *out, R *psi, R *psin, R **U, R **V, bool storev, R in_scale, R out_scale)
+  {
+    ProfScope ps(s, (o & O_FCT) ? "fused_interior_fct" : "fused_interior");
+    const bool fct = o & O_FCT, iga = o & O_IGA;
+#define MPDX_INT(F, I, S) if (fct == F && iga == I && storev == S) interior_t<F, I, S>(out, psi, psin, U, V, in_scale, out_scale)
+    MPDX_INT(true, false, true); else MPDX_INT(true, false, false); else MPDX_INT(true, true, true);
+    else MPDX_INT(true, true, false); else MPDX_INT(false, false, true); else MPDX_INT(false, false, false);
+    else MPDX_INT(false, true, true); else MPDX_INT(false, true, false);
+#undef MPDX_INT
+  }
+
+  // one corrective pass (iteration it) writing psi^{k+1} into out
+  void corrective(R *out, R *psik, R *psin, int it, int passes, R in_scale, R os)
+  {
+    R *Ub[3], *Vb[3];
+    R **U = pp.unco(it, Ub);
+    R **V = pp.corr(it, Vb);
+    const bool fct = o & O_FCT;
+    const bool last = it == passes - 1;
+    const bool storev = !last;   // the next pass reads this velocity as GC_unco
+    const bool use_int = interior_supported(s->p, L);
+    // antidiffusive velocity on the boundary shell (or everywhere)
+    {
+      ProfScope ps(s, "fused_antidiff");
+      if (use_int)
+        for (const Box &b : shell(-1, 5, 4)) antidiff_box(V, psik, U, b);
+      else
+        antidiff_box(V, psik, U, full_box(-1));
+    }
+    if (fct) {
+      {
+        ProfScope ps(s, "fused_halo");
+        pp.xchng_vctr_alng(V, 1);
+        pp.xchng_vctr_nrml(V, 0);
+      }
+      ProfScope ps(s, "fused_fct");
+      if (use_int)
+        for (const Box &b : shell(0, 2, 2)) fct_box(out, psik, psin, V, in_scale, os, b);
+      else
+        fct_box(out, psik, psin, V, in_scale, os, full_box(0));
+    } else {
+      if (it != s->p.n_iters - 1) {
+        ProfScope ps(s, "fused_halo");
+        pp.xchng_vctr_nrml(V, 0);
+      }
+      if (use_int) {
+        ProfScope ps(s, "fused_donor_shell");
+        for (const Box &b : shell(0, 2, 2)) donor_box(out, psik, V, R(1), os, (o & O_IGA) ? 1 : 0, b);
+      } else {
+        donor(out, psik, V, R(1), os, (o & O_IGA) ? 1 : 0);
+      }
+    }
+    if (use_int) interior(out, psik, psin, U, V, storev, in_scale, os);
+  }
+
+  // one time step of equation e (solver_common.hpp:127-136 with advop inlined)
+  void step_eqn(int e)
+  {
+    const int n_iters = s->p.n_iters;
+    const int hs = s->p.hint_scale[e];
+    const R in_scale = hs > 0 ? R(1) / R(1 << hs) : (hs < 0 ? R(1 << -hs) : R(1));
+    const R out_scale = hs > 0 ? R(1 << hs) : (hs < 0 ? R(1) / R(1 << -hs) : R(1));
+    const int lev = s->lev[e];
+    R *psin = A(s->psi[e][lev]);
+    R *fin = A(s->psi[e][lev ^ 1]);
+    R *tmp = n_iters > 1 ? A(s->fz->psi_tmp[e]) : fin;
+    // upwind_filter_freq (mpdata_osc_3d.hpp:195-198): only the donor-cell pass
+    const bool upwind_only = s->p.upwind_filter_freq > 0 && s->timestep % s->p.upwind_filter_freq == 0;
+    const int passes = upwind_only ? 1 : n_iters;
+    auto out_of = [&](int it) { return ((passes - 1 - it) % 2 == 0) ? fin : tmp; };
+    R *Gb[3];
+    R **gc = pp.av_of(s->GC, Gb);
+    donor(out_of(0), psin, gc, in_scale, passes == 1 ? out_scale : R(1), 0);
+    for (int it = 1; it < passes; ++it)
+      corrective(out_of(it), out_of(it - 1), psin, it, passes, in_scale, it == passes - 1 ? out_scale : R(1));
+    s->lev[e] ^= 1;
+  }
+};
+
+template <class R, class G>
+int fused_step(mpdx_solver *s)
+{
+  Fused<R, G> fz(s);
+  for (int e = 0; e < s->p.n_eqns; ++e) fz.step_eqn(e);
   return 0;
 }
 
@@ -655,7 +916,8 @@ int mpdx_advance(mpdx_solver *s, int64_t nt)
   if (nt < 0) { s->err = "libmpdata++: negative number of steps"; return MPDX_EINVAL; }
   if (s->esz == 8) return advance_impl<double, double>(s, nt);
   if (s->p.guard == MPDX_GUARD_NATIVE) return advance_impl<float, float>(s, nt);
-  return advance_impl<float, double>(s, nt);
+  if (s->p.guard == MPDX_GUARD_F64_EXACT) return advance_impl<float, double>(s, nt);
+  return advance_impl<float, GuardFast>(s, nt);
 }
 
 int mpdx_reduce(mpdx_solver *s, int e, int op, double *out)

@@ -104,6 +104,12 @@ template <bool NPA, class R> __device__ inline R negpart(R x)
   return smin(R(0), x);
 }
 
+// Guard policy for FP32 state: the FP64 guard constants (epsilon<double>
+// for fct_frac; the smallest normal float stands in for tiny<double>, which
+// underflows in FP32) with the ratio formed by the hardware reciprocal
+// (v_rcp_f32, <= 1 ulp) times the numerator.
+struct GuardFast {};
+
 // frac / fct_frac: formulae/mpdata/formulae_mpdata_common.hpp:61-98, with the
 // guard evaluated in precision G (G = real_t reproduces the reference)
 template <bool PFC, class G, class R> __device__ inline R frac(R nom, R den)
@@ -114,6 +120,18 @@ template <bool PFC, class G, class R> __device__ inline R frac(R nom, R den)
 template <class G, class R> __device__ inline R fct_frac(R nom, R den)
 {
   return R(G(nom) / (G(den) + lim<G>::eps()));
+}
+template <> __device__ inline float frac<true, GuardFast, float>(float nom, float den)
+{
+  return den != 0 ? nom * __builtin_amdgcn_rcpf(den) : 0.f;
+}
+template <> __device__ inline float frac<false, GuardFast, float>(float nom, float den)
+{
+  return nom * __builtin_amdgcn_rcpf(den + FLT_MIN);
+}
+template <> __device__ inline float fct_frac<GuardFast, float>(float nom, float den)
+{
+  return nom * __builtin_amdgcn_rcpf(den + float(DBL_EPSILON));
 }
 
 // donor-cell flux: formulae/donorcell_formulae.hpp:25-34

@@ -1,13 +1,558 @@
-// Fused MPDATA step kernels (placeholder until the fused path lands).
+// Fused MPDATA kernels.
+//
+// One MPDATA pass costs three kernels instead of the per-pass path's ~15:
+//   k_fz_donor      donor-cell pass: fluxes and the update in one sweep
+//                   (donorcell_formulae.hpp:50-79,105-165); with iga the
+//                   corrective velocity itself is the flux (mpdata_osc_3d.hpp:164-168)
+//   k_fz_antidiff   all advector components of the antidiffusive velocity
+//                   over their reference ranges (formulae_mpdata_3d.hpp:253-295)
+//   k_fz_fct        FCT: psi_min/psi_max from psi^n, beta denominators from the
+//                   donor fluxes of the unlimited velocity, beta_up/beta_dn,
+//                   the limited velocity and the donor-cell update, all in
+//                   registers/LDS (mpdata_fct_3d.hpp:31-110,
+//                   formulae_mpdata_fct_3d.hpp:21-247)
+// Between k_fz_antidiff and the next kernel the reference's own advector halo
+// fills run on the boundary planes (same kernels as the per-pass path), so the
+// fused path reproduces the reference for every input.  Scalar halos never
+// exist in memory: psi is read through the per-dimension index map of the
+// scalar halo fill (cyclic: period N-1; open: clamp to the edge).
 #pragma once
 
+#include "mpdx_perpass.hpp"
 #include "mpdx_solver.hpp"
+
+struct mpdx_fused_state {
+  void *psi_tmp[MPDX_MAX_EQNS] = {};
+};
 
 namespace mpdx {
 
-inline bool fused_supported(const mpdx_params &, const Layout &) { return false; }
-inline int fused_alloc(mpdx_solver *) { return 0; }
-inline void fused_free(mpdx_solver *) {}
-template <class R, class G> int fused_step(mpdx_solver *) { return MPDX_EUNSUPPORTED; }
+// per-dimension index map of the scalar halo fill (bcond/cyclic_3d.hpp:29-33,
+// bcond/open_3d.hpp:32-42): cyclic copies with period N-1, open copies the edge
+struct BcMap {
+  int n[3];
+  unsigned char cyc[3][2];
+};
+
+__device__ inline int map_idx(const BcMap &m, int d, int i)
+{
+  const int N = m.n[d];
+  if (i < 0) return m.cyc[d][0] ? i + (N - 1) : 0;
+  if (i > N - 1) return m.cyc[d][1] ? i - (N - 1) : N - 1;
+  return i;
+}
+
+template <class R>
+__device__ inline R psi_at(const R *psi, const Layout &L, const BcMap &m, int i, int j, int k)
+{
+  i = map_idx(m, 0, i);
+  j = map_idx(m, 1, j);
+  if (L.nd == 3) k = map_idx(m, 2, k);
+  return psi[L.org + i * L.st[0] + j * L.st[1] + k * L.st[2]];
+}
+
+// ---------------------------------------------------------------- donor-cell pass
+
+template <class R, bool NPA>
+__global__ void __launch_bounds__(256)
+k_fz_donor(R *out, const R *in, const R *C0, const R *C1, const R *C2, Layout L, BcMap m,
+           R in_scale, R out_scale, int flux_is_C, Box bx)
+{
+  // cells of the box bx, innermost array dimension on threadIdx.x
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= bx.ext[L.nd - 1]) return;
+  const int i = bx.lo[0] + (L.nd == 3 ? (int)blockIdx.z : (int)blockIdx.y);
+  const int j = L.nd == 3 ? bx.lo[1] + (int)blockIdx.y : bx.lo[1] + t;
+  const int k = L.nd == 3 ? bx.lo[2] + t : 0;
+  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
+  const R pc = psi_at(in, L, m, i, j, k) * in_scale;
+  R f[6];
+  if (flux_is_C) {
+    f[0] = C0[p]; f[1] = C0[p - L.st[0]];
+    f[2] = C1[p]; f[3] = C1[p - L.st[1]];
+    if (L.nd == 3) { f[4] = C2[p]; f[5] = C2[p - L.st[2]]; }
+  } else {
+    const R xp = psi_at(in, L, m, i + 1, j, k) * in_scale, xm = psi_at(in, L, m, i - 1, j, k) * in_scale;
+    const R yp = psi_at(in, L, m, i, j + 1, k) * in_scale, ym = psi_at(in, L, m, i, j - 1, k) * in_scale;
+    f[0] = donor_flux<NPA>(pc, xp, C0[p]);
+    f[1] = donor_flux<NPA>(xm, pc, C0[p - L.st[0]]);
+    f[2] = donor_flux<NPA>(pc, yp, C1[p]);
+    f[3] = donor_flux<NPA>(ym, pc, C1[p - L.st[1]]);
+    if (L.nd == 3) {
+      const R zp = psi_at(in, L, m, i, j, k + 1) * in_scale, zm = psi_at(in, L, m, i, j, k - 1) * in_scale;
+      f[4] = donor_flux<NPA>(pc, zp, C2[p]);
+      f[5] = donor_flux<NPA>(zm, pc, C2[p - L.st[2]]);
+    }
+  }
+  R r;
+  if (L.nd == 3) r = pc + ((-f[0] + f[1]) + (-f[2] + f[3]) + (-f[4] + f[5])) / 1;
+  else r = pc + ((-f[0] + f[1]) + (-f[2] + f[3])) / 1;
+  out[p] = r * out_scale;
+}
+
+// ---------------------------------------------------------------- antidiffusive velocity
+
+// Antidiffusive velocity at one face from its psi stencil and the two
+// tangential advector averages (formulae_mpdata_3d.hpp:253-295,
+// formulae_mpdata_2d.hpp:241-278, psi_3d.hpp:261-464).  P(da, db, dc) returns
+// psi at the rotated offset (da along the face normal, db / dc along the
+// first / second tangential direction of idxperm::pi<d>).
+template <class R, class G, int ND, bool IGA, bool ABS, bool PFC, class PF>
+__device__ inline R antidiff_core(R u, R g1, R g2, PF P)
+{
+  auto Q = [&](int da, int db, int dc) -> R {
+    const R v = P(da, db, dc);
+    return ABS ? fabs(v) : v;
+  };
+  R ndx, ndy;
+  if (IGA) {
+    ndx = ND == 3 ? P(1, 0, 0) - P(0, 0, 0) : 2 * (P(1, 0, 0) - P(0, 0, 0)) / (1 + 1);
+    ndy = (P(1, 1, 0) + P(0, 1, 0) - P(1, -1, 0) - P(0, -1, 0)) / 4;
+  } else {
+    const R a1 = Q(1, 0, 0), a0 = Q(0, 0, 0);
+    ndx = 2 * frac<PFC, G>(a1 - a0, a1 + a0);
+    const R b11 = Q(1, 1, 0), b01 = Q(0, 1, 0), b1m = Q(1, -1, 0), b0m = Q(0, -1, 0);
+    ndy = frac<PFC, G>(b11 + b01 - b1m - b0m, b11 + b01 + b1m + b0m);
+  }
+  if (ND == 3) {
+    R ndz;
+    if (IGA) ndz = (P(1, 0, 1) + P(0, 0, 1) - P(1, 0, -1) - P(0, 0, -1)) / 4;
+    else {
+      const R c11 = Q(1, 0, 1), c01 = Q(0, 0, 1), c1m = Q(1, 0, -1), c0m = Q(0, 0, -1);
+      ndz = frac<PFC, G>(c11 + c01 - c1m - c0m, c11 + c01 + c1m + c0m);
+    }
+    return fabs(u) / 2 * (1 - fabs(u) / 1) * ndx - u / 2 * (g1 * ndy + g2 * ndz) / 1;
+  }
+  return fabs(u) / 2 * (1 - fabs(u) / 1) * ndx - u * g1 / (2 * 1) * ndy;
+}
+
+// V_d at face (a+1/2 along d) reading psi through the halo index map
+template <class R, class G, bool IGA, bool ABS, bool PFC>
+__device__ inline R antidiff_at(const R *psi, const R *Ud, const R *U1, const R *U2, const Layout &L,
+                                const BcMap &m, int d, int i, int j, int k)
+{
+  int e[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  const int *ea = e[d];
+  const int *eb = e[(d + 1) % L.nd];
+  const int *ec = e[L.nd == 3 ? (d + 2) % 3 : 2];
+  auto P = [&](int da, int db, int dc) -> R {
+    return psi_at(psi, L, m, i + da * ea[0] + db * eb[0] + dc * ec[0],
+                  j + da * ea[1] + db * eb[1] + dc * ec[1],
+                  k + da * ea[2] + db * eb[2] + dc * ec[2]);
+  };
+  const long sA = L.st[d], sB = L.st[(d + 1) % L.nd], sC = L.nd == 3 ? L.st[(d + 2) % 3] : 0;
+  const long p0 = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
+  const R g1 = (U1[p0 + sA] + U1[p0] + U1[p0 + sA - sB] + U1[p0 - sB]) / 4;
+  if (L.nd == 3) {
+    const R g2 = (U2[p0 + sA] + U2[p0] + U2[p0 + sA - sC] + U2[p0 - sC]) / 4;
+    return antidiff_core<R, G, 3, IGA, ABS, PFC>(Ud[p0], g1, g2, P);
+  }
+  return antidiff_core<R, G, 2, IGA, ABS, PFC>(Ud[p0], g1, R(0), P);
+}
+
+// all components over the box [-1, n-1]^nd; component d is written where the
+// other coordinates are interior (the reference's im x j x k ranges)
+template <class R, class G, bool IGA, bool ABS, bool PFC>
+__global__ void __launch_bounds__(256)
+k_fz_antidiff(R *V0, R *V1, R *V2, const R *psi, const R *U0, const R *U1, const R *U2,
+              Layout L, BcMap m, Box bx)
+{
+  // bx: sub-box of the owner cells [-1, n-1]^nd, innermost dimension on x
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= bx.ext[L.nd - 1]) return;
+  const int i = bx.lo[0] + (L.nd == 3 ? (int)blockIdx.z : (int)blockIdx.y);
+  const int j = L.nd == 3 ? bx.lo[1] + (int)blockIdx.y : bx.lo[1] + t;
+  const int k = L.nd == 3 ? bx.lo[2] + t : 0;
+  const bool in0 = i >= 0, in1 = j >= 0, in2 = L.nd == 2 || k >= 0;
+  const long p0 = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
+  const R *U[3] = {U0, U1, U2};
+  R *V[3] = {V0, V1, V2};
+  const bool inside[3] = {in0, in1, in2};
+  for (int d = 0; d < L.nd; ++d) {
+    bool ok = true;
+    for (int q = 0; q < L.nd; ++q)
+      if (q != d && !inside[q]) ok = false;
+    if (!ok) continue;
+    const int d1 = (d + 1) % L.nd, d2 = L.nd == 3 ? (d + 2) % 3 : 0;
+    V[d][p0] = antidiff_at<R, G, IGA, ABS, PFC>(psi, U[d], U[d1], U[d2], L, m, d, i, j, k);
+  }
+}
+
+// ---------------------------------------------------------------- FCT + donor-cell
+
+// One thread per interior cell; the block owns a brick of cells and stages
+// beta over the brick plus a one-cell frame in LDS, so each beta is computed
+// once per block.  3D brick 2 x 4 x 32 (x, y, z); 2D brick 8 x 32 (x, y).
+template <int ND> struct FzBrick;
+template <> struct FzBrick<3> { static constexpr int TX = 2, TY = 4, TZ = 32; };
+template <> struct FzBrick<2> { static constexpr int TX = 8, TY = 32, TZ = 1; };
+
+// beta_up / beta_dn of one cell from psi^k (centre + 6 neighbours), the
+// psi^n bounds and the six face fluxes of the unlimited velocity
+// (formulae_mpdata_fct_3d.hpp:21-145, formulae_mpdata_fct_2d.hpp:21-120)
+template <class R, class G, int ND, bool NPA>
+__device__ inline void beta_core(R &bup, R &bdn, R pc, R xp, R xm, R yp, R ym, R zp, R zm, R pmin,
+                                 R pmax, R fxp, R fxm, R fyp, R fym, R fzp, R fzm)
+{
+  R up_nom, dn_nom, up_den, dn_den;
+  if (ND == 3) {
+    up_nom = (smax(pmax, smax(pc, smax(xp, smax(xm, smax(yp, smax(ym, smax(zp, zm))))))) - pc) * 1;
+    dn_nom = (pc - smin(pmin, smin(yp, smin(xm, smin(pc, smin(xp, smin(zp, smin(zm, ym)))))))) * 1;
+    up_den = (pospart<NPA>(fxm) - negpart<NPA>(fxp)) + (pospart<NPA>(fym) - negpart<NPA>(fyp))
+           + (pospart<NPA>(fzm) - negpart<NPA>(fzp));
+    dn_den = (pospart<NPA>(fxp) - negpart<NPA>(fxm)) + (pospart<NPA>(fyp) - negpart<NPA>(fym))
+           + (pospart<NPA>(fzp) - negpart<NPA>(fzm));
+  } else {
+    up_nom = (smax(pmax, smax(yp, smax(xm, smax(pc, smax(xp, ym))))) - pc) * 1;
+    dn_nom = (pc - smin(pmin, smin(yp, smin(xm, smin(pc, smin(xp, ym)))))) * 1;
+    up_den = (pospart<NPA>(fxm) - negpart<NPA>(fxp)) + (pospart<NPA>(fym) - negpart<NPA>(fyp));
+    dn_den = (pospart<NPA>(fxp) - negpart<NPA>(fxm)) + (pospart<NPA>(fyp) - negpart<NPA>(fym));
+  }
+  bup = fct_frac<G>(up_nom, up_den);
+  bdn = fct_frac<G>(dn_nom, dn_den);
+}
+
+// psi_min / psi_max of psi^n over the 7-point (5-point) stencil
+// (mpdata_fct_3d.hpp:31-55, mpdata_fct_2d.hpp:31-46)
+template <class R, int ND>
+__device__ inline void bounds_core(R &pmin, R &pmax, R c, R a, R b, R d, R e, R f, R g)
+{
+  // a = x+1, b = x-1, d = y+1, e = y-1, f = z+1, g = z-1
+  if (ND == 3) {
+    pmin = smin(smin(smin(smin(smin(smin(c, a), b), d), e), f), g);
+    pmax = smax(smax(smax(smax(smax(smax(c, a), b), d), e), f), g);
+  } else {
+    pmin = smin(smin(smin(smin(d, b), c), a), e);
+    pmax = smax(smax(smax(smax(d, b), c), a), e);
+  }
+}
+
+// limited velocity (GC_mono) on a face between cells l and r
+// (formulae_mpdata_fct_3d.hpp:147-247)
+template <class R, bool ABS, bool IGA>
+__device__ inline R mono_core(R v, R bul, R bdl, R bur, R bdr, R psl, R psr)
+{
+  const R m_dn_up = smin(R(1), smin(bdl, bur));
+  const R m_up_dn = smin(R(1), smin(bul, bdr));
+  R lim;
+  if (!ABS || IGA) lim = v > 0 ? m_dn_up : m_up_dn;
+  else if (v > 0) lim = psl > 0 ? m_dn_up : m_up_dn;
+  else lim = psr > 0 ? m_up_dn : m_dn_up;
+  return v * lim;
+}
+
+template <class R, class G, bool NPA, bool ABS>
+__device__ inline void beta_at(R &bup, R &bdn, const R *psi, const R *psin, const R *V0, const R *V1,
+                               const R *V2, const Layout &L, const BcMap &m, int i, int j, int k,
+                               R in_scale, bool iga)
+{
+  const int nd = L.nd;
+  auto P = [&](int a, int b, int c) { return psi_at(psi, L, m, a, b, c); };
+  auto Q = [&](int a, int b, int c) { return psi_at(psin, L, m, a, b, c) * in_scale; };
+  const R pc = P(i, j, k);
+  const R xp = P(i + 1, j, k), xm = P(i - 1, j, k), yp = P(i, j + 1, k), ym = P(i, j - 1, k);
+  R zp = 0, zm = 0;
+  if (nd == 3) { zp = P(i, j, k + 1); zm = P(i, j, k - 1); }
+  R pmin, pmax;
+  if (nd == 3)
+    bounds_core<R, 3>(pmin, pmax, Q(i, j, k), Q(i + 1, j, k), Q(i - 1, j, k), Q(i, j + 1, k),
+                      Q(i, j - 1, k), Q(i, j, k + 1), Q(i, j, k - 1));
+  else
+    bounds_core<R, 2>(pmin, pmax, Q(i, j, k), Q(i + 1, j, k), Q(i - 1, j, k), Q(i, j + 1, k),
+                      Q(i, j - 1, k), R(0), R(0));
+  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
+  R fxp, fxm, fyp, fym, fzp = 0, fzm = 0;
+  if (iga) {
+    fxp = V0[p]; fxm = V0[p - L.st[0]]; fyp = V1[p]; fym = V1[p - L.st[1]];
+    if (nd == 3) { fzp = V2[p]; fzm = V2[p - L.st[2]]; }
+  } else {
+    fxp = donor_flux<NPA>(pc, xp, V0[p]);
+    fxm = donor_flux<NPA>(xm, pc, V0[p - L.st[0]]);
+    fyp = donor_flux<NPA>(pc, yp, V1[p]);
+    fym = donor_flux<NPA>(ym, pc, V1[p - L.st[1]]);
+    if (nd == 3) {
+      fzp = donor_flux<NPA>(pc, zp, V2[p]);
+      fzm = donor_flux<NPA>(zm, pc, V2[p - L.st[2]]);
+    }
+  }
+  if (nd == 3) beta_core<R, G, 3, NPA>(bup, bdn, pc, xp, xm, yp, ym, zp, zm, pmin, pmax, fxp, fxm, fyp, fym, fzp, fzm);
+  else beta_core<R, G, 2, NPA>(bup, bdn, pc, xp, xm, yp, ym, zp, zm, pmin, pmax, fxp, fxm, fyp, fym, fzp, fzm);
+}
+
+template <class R, class G, int ND, bool NPA, bool ABS, bool IGA>
+__global__ void __launch_bounds__(256)
+k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R *V2, Layout L,
+         BcMap m, R in_scale, R out_scale, Box bx)
+{
+  constexpr int TX = FzBrick<ND>::TX, TY = FzBrick<ND>::TY, TZ = FzBrick<ND>::TZ;
+  constexpr int BX = TX + 2, BY = TY + 2, BZ = ND == 3 ? TZ + 2 : 1;
+  __shared__ R sbu[BX * BY * BZ], sbd[BX * BY * BZ];
+  const int i0 = bx.lo[0] + blockIdx.z * TX, j0 = bx.lo[1] + blockIdx.y * TY;
+  const int k0 = ND == 3 ? bx.lo[2] + blockIdx.x * TZ : 0;
+  const int tid = threadIdx.x;
+  // beta over the brick plus a one-cell frame (cells -1..n of the domain)
+  for (int q = tid; q < BX * BY * BZ; q += blockDim.x) {
+    const int bz = q % BZ, by = (q / BZ) % BY, bx = q / (BZ * BY);
+    const int i = i0 - 1 + bx, j = j0 - 1 + by, k = ND == 3 ? k0 - 1 + bz : 0;
+    R bu = 0, bd = 0;
+    const bool inb = i >= -1 && i <= L.n[0] && j >= -1 && j <= L.n[1] &&
+                     (ND == 2 || (k >= -1 && k <= L.n[2]));
+    if (inb) beta_at<R, G, NPA, ABS>(bu, bd, psi, psin, V0, V1, V2, L, m, i, j, k, in_scale, IGA);
+    sbu[q] = bu;
+    sbd[q] = bd;
+  }
+  __syncthreads();
+  const int tz = tid % TZ, ty = (tid / TZ) % TY, tx = tid / (TZ * TY);
+  const int i = i0 + tx, j = j0 + ty, k = ND == 3 ? k0 + tz : 0;
+  if (i > bx.lo[0] + bx.ext[0] - 1 || j > bx.lo[1] + bx.ext[1] - 1 ||
+      (ND == 3 && k > bx.lo[2] + bx.ext[2] - 1)) return;
+  auto B = [&](const R *sb, int di, int dj, int dk) -> R {
+    return sb[((tx + 1 + di) * BY + (ty + 1 + dj)) * BZ + (ND == 3 ? tz + 1 + dk : 0)];
+  };
+  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
+  const R pc = psi_at(psi, L, m, i, j, k);
+  const R bu0 = B(sbu, 0, 0, 0), bd0 = B(sbd, 0, 0, 0);
+  auto mono = [&](R v, R bul, R bdl, R bur, R bdr, R psl, R psr) -> R {
+    return mono_core<R, ABS, IGA>(v, bul, bdl, bur, bdr, psl, psr);
+  };
+  R f[6];
+  {
+    const R xp = psi_at(psi, L, m, i + 1, j, k), xm = psi_at(psi, L, m, i - 1, j, k);
+    const R mp = mono(V0[p], bu0, bd0, B(sbu, 1, 0, 0), B(sbd, 1, 0, 0), pc, xp);
+    const R mm = mono(V0[p - L.st[0]], B(sbu, -1, 0, 0), B(sbd, -1, 0, 0), bu0, bd0, xm, pc);
+    f[0] = IGA ? mp : donor_flux<NPA>(pc, xp, mp);
+    f[1] = IGA ? mm : donor_flux<NPA>(xm, pc, mm);
+  }
+  {
+    const R yp = psi_at(psi, L, m, i, j + 1, k), ym = psi_at(psi, L, m, i, j - 1, k);
+    const R mp = mono(V1[p], bu0, bd0, B(sbu, 0, 1, 0), B(sbd, 0, 1, 0), pc, yp);
+    const R mm = mono(V1[p - L.st[1]], B(sbu, 0, -1, 0), B(sbd, 0, -1, 0), bu0, bd0, ym, pc);
+    f[2] = IGA ? mp : donor_flux<NPA>(pc, yp, mp);
+    f[3] = IGA ? mm : donor_flux<NPA>(ym, pc, mm);
+  }
+  R r;
+  if (ND == 3) {
+    const R zp = psi_at(psi, L, m, i, j, k + 1), zm = psi_at(psi, L, m, i, j, k - 1);
+    const R mp = mono(V2[p], bu0, bd0, B(sbu, 0, 0, 1), B(sbd, 0, 0, 1), pc, zp);
+    const R mm = mono(V2[p - L.st[2]], B(sbu, 0, 0, -1), B(sbd, 0, 0, -1), bu0, bd0, zm, pc);
+    f[4] = IGA ? mp : donor_flux<NPA>(pc, zp, mp);
+    f[5] = IGA ? mm : donor_flux<NPA>(zm, pc, mm);
+    r = pc + ((-f[0] + f[1]) + (-f[2] + f[3]) + (-f[4] + f[5])) / 1;
+  } else {
+    r = pc + ((-f[0] + f[1]) + (-f[2] + f[3])) / 1;
+  }
+  out[p] = r * out_scale;
+}
+
+// ---------------------------------------------------------------- interior corrective pass (3D)
+
+// One whole corrective pass (antidiffusive velocity, FCT limiter, donor-cell
+// update) for cells at distance >= 2 from every boundary, where no halo rule
+// touches any value the pass reads, so no advector halo fill is needed.
+// A block owns a (IT_Y-4) x (IT_Z-4) column of output cells and marches along
+// x through IX_CHUNK output planes; each thread owns one (y, z) column of the
+// (IT_Y x IT_Z) tile, which includes a 2-cell frame.  At step p (newest psi^k
+// plane p) it computes the velocity on plane p-1, beta on plane p-1, the
+// limited fluxes and writes psi^{k+1} on plane p-2.  psi^k and psi^n planes,
+// the in-plane velocities, beta and the y/z fluxes go through LDS rings;
+// everything along x stays in registers.
+template <class R> struct ITile {
+  static constexpr int Y = sizeof(R) == 8 ? 16 : 32, Z = 32;
+};
+constexpr int IX_CHUNK = 64;
+
+template <class R>
+struct InteriorArgs {
+  R *out;
+  const R *psi, *psin;
+  const R *U0, *U1, *U2;
+  R *V0, *V1, *V2;      // unlimited velocity out (next pass's GC_unco), may be null
+  Layout L;
+  int lo[3], hi[3];     // output cells, inclusive; lo >= 2, hi <= n-3
+  R in_scale, out_scale;
+};
+
+template <class R, class G, bool FCT, bool IGA, bool STOREV>
+__global__ void __launch_bounds__(ITile<R>::Y * ITile<R>::Z)
+k_fz_interior(InteriorArgs<R> a)
+{
+  constexpr int IT_Y = ITile<R>::Y, IT_Z = ITile<R>::Z;
+  __shared__ R s_psi[3][IT_Y][IT_Z];
+  __shared__ R s_pn[FCT ? 2 : 1][IT_Y][IT_Z];
+  __shared__ R s_vy[IT_Y][IT_Z], s_vz[IT_Y][IT_Z];
+  __shared__ R s_bu[FCT ? IT_Y : 1][IT_Z], s_bd[FCT ? IT_Y : 1][IT_Z];
+  __shared__ R s_fy[2][IT_Y][IT_Z], s_fz[2][IT_Y][IT_Z];
+
+  const Layout &L = a.L;
+  const int tz = threadIdx.x, ty = threadIdx.y;
+  const int y = a.lo[1] + (int)blockIdx.y * (IT_Y - 4) - 2 + ty;
+  const int z = a.lo[2] + (int)blockIdx.x * (IT_Z - 4) - 2 + tz;
+  const int x0 = a.lo[0] + (int)blockIdx.z * IX_CHUNK;
+  const int x1 = min(x0 + IX_CHUNK - 1, a.hi[0]);
+  // loads stay inside the interior: y, z >= lo-2 >= 0; clamp the far side
+  const int yl = min(y, L.n[1] - 1), zl = min(z, L.n[2] - 1);
+  const int ym1 = max(ty - 1, 0), yp1 = min(ty + 1, IT_Y - 1);
+  const int zm1 = max(tz - 1, 0), zp1 = min(tz + 1, IT_Z - 1);
+  const long sx = L.st[0], sy = L.st[1];
+  const long col = L.org + yl * sy + zl;                       // (x=0, y, z)
+  const long coly1 = L.org + min(yl + 1, L.n[1] - 1) * sy + zl;  // y+1
+  const long colym = L.org + max(yl - 1, 0) * sy + zl;           // y-1
+  const long colz1 = L.org + yl * sy + min(zl + 1, L.n[2] - 1);  // z+1
+  const long colzm = L.org + yl * sy + max(zl - 1, 0);           // z-1
+  const bool out_ok = ty >= 2 && ty < IT_Y - 2 && tz >= 2 && tz < IT_Z - 2 &&
+                      y <= a.hi[1] && z <= a.hi[2];
+
+  R ps_m2 = 0, ps_m1 = 0, ps_0 = 0;     // own psi^k at planes p-2, p-1, p
+  R pn_m2 = 0, pn_m1 = 0, pn_0 = 0;     // own psi^n
+  R vx_m2 = 0;                          // V_x on face owned by p-2
+  R fpx_m2 = 0;                         // unlimited donor flux on that face
+  R bu_m2 = 0, bd_m2 = 0;               // beta on plane p-2
+  R fx_m3 = 0;                          // limited flux on face owned by p-3
+
+  for (int p = x0 - 2; p <= x1 + 2; ++p) {
+    // ---- stage 1: new psi^k (and psi^n) plane
+    ps_m2 = ps_m1; ps_m1 = ps_0;
+    ps_0 = a.psi[col + p * sx];
+    s_psi[(p + 3) % 3][ty][tz] = ps_0;
+    if (FCT) {
+      pn_m2 = pn_m1; pn_m1 = pn_0;
+      pn_0 = a.psin[col + p * sx] * a.in_scale;
+      s_pn[(p + 2) % 2][ty][tz] = pn_0;
+    }
+    __syncthreads();
+    const int sm2 = (p + 1) % 3, sm1 = (p + 2) % 3, s0 = (p + 3) % 3;   // slots of p-2, p-1, p
+
+    // ---- stage 2: velocity on plane p-1 (V_x on the face owned by p-1)
+    R vx = 0, vy = 0, vz = 0;
+    if (p >= x0 - 1) {
+      const int q = p - 1;
+      const long c = col + q * sx;
+      {
+        // V_x at face q+1/2: normal x, tangential (y, z)
+        auto P = [&](int da, int db, int dc) -> R {
+          const int sl = da == 0 ? sm1 : s0;
+          const int yy = db > 0 ? yp1 : (db < 0 ? ym1 : ty);
+          const int zz = dc > 0 ? zp1 : (dc < 0 ? zm1 : tz);
+          return s_psi[sl][yy][zz];
+        };
+        const R g1 = (a.U1[c + sx] + a.U1[c] + a.U1[colym + (q + 1) * sx] + a.U1[colym + q * sx]) / 4;
+        const R g2 = (a.U2[c + sx] + a.U2[c] + a.U2[colzm + (q + 1) * sx] + a.U2[colzm + q * sx]) / 4;
+        vx = antidiff_core<R, G, 3, IGA, false, false>(a.U0[c], g1, g2, P);
+      }
+      if (p >= x0) {
+        {
+          // V_y at face y+1/2 of plane q: normal y, tangential (z, x)
+          auto P = [&](int da, int db, int dc) -> R {
+            const int yy = da > 0 ? yp1 : ty;
+            const int zz = db > 0 ? zp1 : (db < 0 ? zm1 : tz);
+            const int sl = dc > 0 ? s0 : (dc < 0 ? sm2 : sm1);
+            return s_psi[sl][yy][zz];
+          };
+          const R g1 = (a.U2[coly1 + q * sx] + a.U2[c] + a.U2[coly1 + q * sx - 1] + a.U2[c - 1]) / 4;
+          const R g2 = (a.U0[coly1 + q * sx] + a.U0[c] + a.U0[coly1 + (q - 1) * sx] + a.U0[c - sx]) / 4;
+          vy = antidiff_core<R, G, 3, IGA, false, false>(a.U1[c], g1, g2, P);
+        }
+        {
+          // V_z at face z+1/2 of plane q: normal z, tangential (x, y)
+          auto P = [&](int da, int db, int dc) -> R {
+            const int zz = da > 0 ? zp1 : tz;
+            const int sl = db > 0 ? s0 : (db < 0 ? sm2 : sm1);
+            const int yy = dc > 0 ? yp1 : (dc < 0 ? ym1 : ty);
+            return s_psi[sl][yy][zz];
+          };
+          const R g1 = (a.U0[colz1 + q * sx] + a.U0[c] + a.U0[colz1 + (q - 1) * sx] + a.U0[c - sx]) / 4;
+          const R g2 = (a.U1[colz1 + q * sx] + a.U1[c] + a.U1[colz1 + q * sx - sy] + a.U1[c - sy]) / 4;
+          vz = antidiff_core<R, G, 3, IGA, false, false>(a.U2[c], g1, g2, P);
+        }
+        s_vy[ty][tz] = vy;
+        s_vz[ty][tz] = vz;
+      }
+      if (STOREV && out_ok && q >= x0 && q <= x1) {
+        a.V0[c] = vx;
+        a.V1[c] = vy;
+        a.V2[c] = vz;
+      }
+    }
+    __syncthreads();
+
+    // ---- stage 3: beta on plane p-1 (FCT)
+    R bu = 0, bd = 0;
+    const R fpx = IGA ? vx : donor_flux<false>(ps_m1, ps_0, vx);   // unlimited flux, face owned by p-1
+    if (FCT && p >= x0) {
+      const R pc = ps_m1;
+      const R yp = s_psi[sm1][yp1][tz], ym = s_psi[sm1][ym1][tz];
+      const R zp = s_psi[sm1][ty][zp1], zm = s_psi[sm1][ty][zm1];
+      R pmin, pmax;
+      const int n1 = (p + 1) % 2;   // slot of psi^n plane p-1
+      bounds_core<R, 3>(pmin, pmax, pn_m1, pn_0, pn_m2, s_pn[n1][yp1][tz], s_pn[n1][ym1][tz],
+                        s_pn[n1][ty][zp1], s_pn[n1][ty][zm1]);
+      const R vym = s_vy[ym1][tz], vzm = s_vz[ty][zm1];
+      const R fyp = IGA ? vy : donor_flux<false>(pc, yp, vy);
+      const R fym = IGA ? vym : donor_flux<false>(ym, pc, vym);
+      const R fzp = IGA ? vz : donor_flux<false>(pc, zp, vz);
+      const R fzm = IGA ? vzm : donor_flux<false>(zm, pc, vzm);
+      beta_core<R, G, 3, false>(bu, bd, pc, ps_0, ps_m2, yp, ym, zp, zm, pmin, pmax, fpx, fpx_m2,
+                                fyp, fym, fzp, fzm);
+      s_bu[ty][tz] = bu;
+      s_bd[ty][tz] = bd;
+    }
+    if (FCT) __syncthreads();
+
+    // ---- stage 4: limited fluxes; psi^{k+1} on plane p-2
+    if (p >= x0 + 1) {
+      // x face owned by p-2 (between planes p-2 and p-1)
+      const R mx = FCT ? mono_core<R, false, IGA>(vx_m2, bu_m2, bd_m2, bu, bd, ps_m2, ps_m1) : vx_m2;
+      const R fx = IGA ? mx : donor_flux<false>(ps_m2, ps_m1, mx);
+      // y and z faces of plane p-1
+      const int f1 = (p + 1) % 2;   // slot of plane p-1
+      R my = vy, mz = vz;
+      if (FCT) {
+        my = mono_core<R, false, IGA>(vy, bu, bd, s_bu[yp1][tz], s_bd[yp1][tz], ps_m1, s_psi[sm1][yp1][tz]);
+        mz = mono_core<R, false, IGA>(vz, bu, bd, s_bu[ty][zp1], s_bd[ty][zp1], ps_m1, s_psi[sm1][ty][zp1]);
+      }
+      s_fy[f1][ty][tz] = IGA ? my : donor_flux<false>(ps_m1, s_psi[sm1][yp1][tz], my);
+      s_fz[f1][ty][tz] = IGA ? mz : donor_flux<false>(ps_m1, s_psi[sm1][ty][zp1], mz);
+      if (p >= x0 + 2 && out_ok) {
+        const int f2 = p % 2;       // slot of plane p-2
+        const R r = ps_m2 + ((-fx + fx_m3) + (-s_fy[f2][ty][tz] + s_fy[f2][ym1][tz])
+                             + (-s_fz[f2][ty][tz] + s_fz[f2][ty][zm1])) / 1;
+        a.out[col + (p - 2) * sx] = r * a.out_scale;
+      }
+      fx_m3 = fx;
+    }
+    vx_m2 = vx;
+    fpx_m2 = fpx;
+    bu_m2 = bu;
+    bd_m2 = bd;
+  }
+}
+
+// ---------------------------------------------------------------- host side
+
+inline bool fused_supported(const mpdx_params &p, const Layout &)
+{
+  // khn (element-wise Kahan donor sums) stays on the per-pass path
+  return !(p.opts & O_KHN);
+}
+
+// the interior kernel covers 3D, standard or iga antidiff without abs/npa/pfc
+inline bool interior_supported(const mpdx_params &p, const Layout &L)
+{
+  if (L.nd != 3) return false;
+  if (p.opts & (O_ABS | O_NPA | O_PFC | O_KHN)) return false;
+  for (int d = 0; d < 3; ++d)
+    if (L.n[d] < 12) return false;
+  return true;
+}
+
+inline BcMap make_bcmap(const mpdx_solver *s)
+{
+  BcMap m;
+  for (int d = 0; d < 3; ++d) {
+    m.n[d] = s->L.n[d];
+    for (int sd = 0; sd < 2; ++sd) m.cyc[d][sd] = d < s->L.nd && s->p.bc[d][sd] == MPDX_BC_CYCLIC;
+  }
+  return m;
+}
 
 }  // namespace mpdx

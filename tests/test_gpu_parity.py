@@ -37,38 +37,76 @@ def _case(nd, bcs, name, seed):
 @pytest.mark.parametrize("name", list(OPTS))
 @pytest.mark.parametrize("bi", range(4))
 @pytest.mark.parametrize("nd", [2, 3])
-def test_perpass_fp64_matches_oracle(nd, bi, name):
+@pytest.mark.parametrize("path", [1, 2])
+def test_fp64_matches_oracle(path, nd, bi, name):
+    """path 1 = per-pass kernels, path 2 = fused kernels (khn is per-pass only)."""
+    if path == 2 and name == "khn_fct":
+        pytest.skip("khn runs on the per-pass path")
     bcs = (BCS2 if nd == 2 else BCS3)[bi]
     grid, opts, iters, psi, gc = _case(nd, bcs, name, 10 * bi + nd)
     want = run_oracle(grid, bcs, opts, iters, psi, gc, 4)[0]
-    got = run_gpu(grid, bcs, opts, iters, psi, gc, 4, path=1)[0]
+    got = run_gpu(grid, bcs, opts, iters, psi, gc, 4, path=path)[0]
     assert not np.isnan(got).any()
     assert rel_err(got, want) <= 1e-12
 
 
 @pytest.mark.parametrize("name", ["basic", "fct", "fct3", "iga_fct"])
 @pytest.mark.parametrize("nd", [2, 3])
-def test_perpass_fp32_close_to_oracle(nd, name):
+@pytest.mark.parametrize("path", [1, 2])
+def test_fp32_close_to_oracle(path, nd, name):
     bcs = (BCS2 if nd == 2 else BCS3)[1]
     grid, opts, iters, psi, gc = _case(nd, bcs, name, 77)
     want = run_oracle(grid, bcs, opts, iters, psi, gc, 4)[0]
-    got = run_gpu(grid, bcs, opts, iters, psi, gc, 4, dtype=np.float32, path=1)[0]
+    got = run_gpu(grid, bcs, opts, iters, psi, gc, 4, dtype=np.float32, path=path)[0]
     assert rel_err(got, want) <= 1e-5
 
 
-def test_multi_equation_and_hint_scale():
+@pytest.mark.parametrize("path", [1, 2])
+def test_multi_equation_and_hint_scale(path):
     grid, bcs = (9, 8, 10), [(0, 0)] * 3
     psi, gc = setup_case(grid, bcs, 5)
     psi2 = psi[::-1].copy()
     want = run_oracle(grid, bcs, O.FCT, 3, [psi, psi2], gc, 3, n_eqns=2, hint_scale=[0, 3])
-    got = run_gpu(grid, bcs, O.FCT, 3, [psi, psi2], gc, 3, n_eqns=2, hint_scale=[0, 3], path=1)
+    got = run_gpu(grid, bcs, O.FCT, 3, [psi, psi2], gc, 3, n_eqns=2, hint_scale=[0, 3], path=path)
     for g, w in zip(got, want):
         assert rel_err(g, w) <= 1e-12
 
 
-def test_upwind_filter_freq():
+@pytest.mark.parametrize("path", [1, 2])
+def test_upwind_filter_freq(path):
     grid, bcs = (13, 11), [(1, 1)] * 2
     psi, gc = setup_case(grid, bcs, 9)
     want = run_oracle(grid, bcs, O.FCT, 3, psi, gc, 5, upwind_filter_freq=2)[0]
-    got = run_gpu(grid, bcs, O.FCT, 3, psi, gc, 5, upwind_filter_freq=2, path=1)[0]
+    got = run_gpu(grid, bcs, O.FCT, 3, psi, gc, 5, upwind_filter_freq=2, path=path)[0]
     assert rel_err(got, want) <= 1e-12
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_fused_larger_grid_fp64(nd):
+    """A grid spanning several kernel bricks in every direction."""
+    grid = (70, 45) if nd == 2 else (21, 19, 70)
+    bcs = [(1, 1)] * nd
+    psi, gc = setup_case(grid, bcs, 31)
+    want = run_oracle(grid, bcs, O.FCT, 3, psi, gc, 3)[0]
+    got = run_gpu(grid, bcs, O.FCT, 3, psi, gc, 3, path=2)[0]
+    assert rel_err(got, want) <= 1e-12
+
+
+INTERIOR_OPTS = ["basic", "iters3", "fct", "fct3", "iga", "iga_fct"]
+
+
+@pytest.mark.parametrize("name", INTERIOR_OPTS)
+@pytest.mark.parametrize("bi", range(4))
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_fused_interior_matches_oracle(dtype, bi, name):
+    """3D grids large enough for the x-marching interior kernel, with several
+    x chunks and y/z tiles, every boundary combination."""
+    opts, iters = OPTS[name]
+    bcs = BCS3[bi]
+    grid = [(140, 14, 36), (14, 70, 36), (30, 26, 75), (40, 33, 20)][bi]
+    psi, gc = setup_case(grid, bcs, 100 + bi)
+    print(f"interior case grid={grid} bcs={bcs} opts={name} dtype={np.dtype(dtype).name}", flush=True)
+    want = run_oracle(grid, bcs, opts, iters, psi, gc, 3)[0]
+    got = run_gpu(grid, bcs, opts, iters, psi, gc, 3, dtype=dtype, path=2)[0]
+    assert not np.isnan(got).any()
+    assert rel_err(got, want) <= (1e-12 if dtype == np.float64 else 1e-5)

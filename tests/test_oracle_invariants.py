@@ -1,0 +1,163 @@
+"""Invariants and closed forms that pin the oracle where no reference data
+exists (cyclic halos, n_iters >= 3, options), chosen so that a plausible
+mistake (dropped term, wrong sign, wrong index or iteration chaining) fails.
+
+* C = 0 is the identity; a constant field stays constant under a discretely
+  divergence-free advector (open and cyclic).
+* cyclic domains conserve mass over the N-1 unique cells (the last cell
+  duplicates the first, bcond/cyclic_3d.hpp:29-48).
+* Courant number 1 is an exact one-cell shift (antidiffusive velocity
+  |U|(1-|U|) ... vanishes).
+* 1D-like problems reduce to the textbook MPDATA recursion (Smolarkiewicz
+  1984): V_k = (|U_k| - U_k^2) (psi_R - psi_L) / (psi_R + psi_L) with
+  U_k = V_{k-1} -- pins the GC_unco / GC_corr chaining for n_iters = 2..4.
+* FCT creates no new extrema; standard MPDATA keeps positive fields positive.
+* basic MPDATA is second-order accurate (error ratio ~4 on grid doubling,
+  as in the reference's sandbox/convergence_2d_3d test).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from libmpdataxx_amd import workloads as W
+
+
+def run(grid, bcs, opts, iters, psi, gc, nt, **kw):
+    o = O.Oracle(grid, opts=opts, n_iters=iters, bcs=bcs, **kw)
+    o.set_advectee(psi)
+    for d in range(len(grid)):
+        o.set_advector(gc[d], d)
+    o.advance(nt)
+    return o.advectee()
+
+
+def periodic_consistent(a, axes):
+    a = a.copy()
+    for ax in axes:
+        sl_last = [slice(None)] * a.ndim
+        sl_first = [slice(None)] * a.ndim
+        sl_last[ax] = -1
+        sl_first[ax] = 0
+        a[tuple(sl_last)] = a[tuple(sl_first)]
+    return a
+
+
+@pytest.mark.parametrize("opts,iters", [(0, 2), (O.FCT, 3), (O.IGA | O.FCT, 2)])
+def test_zero_advector_is_identity(opts, iters):
+    grid = (9, 10, 11)
+    psi = W.random_field(grid, 3)
+    gc = [np.zeros(s) for s in [(9, 10, 11), (10, 9, 11), (10, 10, 10)]]
+    out = run(grid, [(O.OPEN, O.OPEN)] * 3, opts, iters, psi, gc, 3)
+    assert np.array_equal(out, psi)
+
+
+@pytest.mark.parametrize("bc", [O.CYCLIC, O.OPEN])
+@pytest.mark.parametrize("opts,iters", [(0, 3), (O.FCT, 3), (O.ABS | O.FCT, 2)])
+def test_constant_field_stays_constant(bc, opts, iters):
+    grid = (12, 10, 11)
+    gc = W.streamfunction_advector(grid, 4, periodic=bc == O.CYCLIC, max_courant=0.5)
+    out = run(grid, [(bc, bc)] * 3, opts, iters, np.full(grid, 2.5), gc, 5)
+    assert np.abs(out - 2.5).max() <= 1e-13
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("opts,iters", [(0, 2), (O.FCT, 3), (O.IGA, 2), (O.FCT | O.NPA | O.PFC, 3)])
+def test_cyclic_mass_conservation(nd, opts, iters):
+    grid = (14, 12) if nd == 2 else (10, 9, 12)
+    gc = W.streamfunction_advector(grid, 5, periodic=True, max_courant=0.5)
+    psi = periodic_consistent(W.random_field(grid, 6), range(nd))
+    out = run(grid, [(O.CYCLIC, O.CYCLIC)] * nd, opts, iters, psi, gc, 6)
+    unique = tuple(slice(0, n - 1) for n in grid)
+    m0, m1 = psi[unique].sum(), out[unique].sum()
+    assert abs(m1 - m0) / m0 <= 1e-14
+    # the duplicated edge stays a duplicate
+    assert np.array_equal(out, periodic_consistent(out, range(nd)))
+
+
+@pytest.mark.parametrize("iters", [1, 2, 3])
+def test_courant_one_is_exact_shift(iters):
+    n = 16
+    psi = W.random_field((n, n), 8)
+    psi = periodic_consistent(psi, [0, 1])
+    gc = [np.ones((n, n)), np.zeros((n + 1, n - 1))]
+    out = run((n, n), [(O.CYCLIC, O.CYCLIC)] * 2, 0, iters, psi, gc, 1)
+    u = psi[:-1]                      # the n-1 unique rows
+    want = np.roll(u, 1, axis=0)
+    assert np.allclose(out[:-1], want, rtol=0, atol=1e-15)
+
+
+def textbook_mpdata_1d(u, courant, iters, steps):
+    """Smolarkiewicz (1984) MPDATA on a periodic 1D grid of unique cells."""
+    psi = u.copy()
+    for _ in range(steps):
+        U = np.full(psi.size, courant)        # U[i] at face i+1/2
+        for k in range(iters):
+            right = np.roll(psi, -1)
+            F = np.maximum(U, 0) * psi + np.minimum(U, 0) * right
+            psi = psi - (F - np.roll(F, 1))
+            if k + 1 < iters:
+                # next pass: pseudo-velocity from the latest iterate and the
+                # previous pass's (pseudo-)velocity
+                right = np.roll(psi, -1)
+                U = (np.abs(U) - U * U) * (right - psi) / (right + psi)
+    return psi
+
+
+@pytest.mark.parametrize("iters", [2, 3, 4])
+@pytest.mark.parametrize("courant", [0.3, -0.45])
+def test_reduces_to_textbook_1d(iters, courant):
+    n, m = 24, 5
+    x = np.arange(n - 1)
+    u = 1.5 + np.sin(2 * np.pi * x / (n - 1)) + 0.3 * np.cos(6 * np.pi * x / (n - 1))
+    psi = np.empty((n, m))
+    psi[:-1] = u[:, None]
+    psi[-1] = psi[0]
+    gc = [np.full((n, m), courant), np.zeros((n + 1, m - 1))]
+    out = run((n, m), [(O.CYCLIC, O.CYCLIC), (O.OPEN, O.OPEN)], 0, iters, psi, gc, 7)
+    want = textbook_mpdata_1d(u, courant, iters, 7)
+    assert np.allclose(out[:-1, 2], want, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("opts,iters", [(O.FCT, 2), (O.FCT, 3), (O.IGA | O.FCT, 2)])
+def test_fct_no_new_extrema(opts, iters):
+    psi, gc, dt = W.revolving_sphere_3d(31)
+    out = run((31, 31, 31), [(O.OPEN, O.OPEN)] * 3, opts, iters, psi, gc, 40)
+    assert out.max() <= psi.max() * (1 + 1e-12)
+    assert out.min() >= -1e-12
+
+
+def test_basic_mpdata_overshoots_without_fct():
+    """The same discontinuity without FCT does create extrema (so the test
+    above is not vacuous)."""
+    psi, gc, dt = W.revolving_sphere_3d(31)
+    out = run((31, 31, 31), [(O.OPEN, O.OPEN)] * 3, 0, 2, psi, gc, 40)
+    assert out.max() > psi.max() * 1.01
+
+
+@pytest.mark.parametrize("iters", [2, 3])
+def test_positive_definite(iters):
+    grid = (16, 14)
+    gc = W.streamfunction_advector(grid, 11, periodic=True, max_courant=0.5)
+    psi = periodic_consistent(W.random_field(grid, 12, lo=0.0, hi=1.0) ** 4, [0, 1])
+    out = run(grid, [(O.CYCLIC, O.CYCLIC)] * 2, 0, iters, psi, gc, 20)
+    assert out.min() >= 0
+
+
+def _smooth_error(n, iters):
+    # one full period of uniform diagonal advection on a cyclic (n-1)-periodic grid
+    per = n - 1
+    x = np.arange(n) / per
+    psi = 2 + np.sin(2 * np.pi * x)[:, None] * np.cos(2 * np.pi * x)[None, :]
+    c = 0.25
+    steps = int(round(per / c))
+    gc = [np.full((n, n), c), np.full((n + 1, n - 1), c)]
+    out = run((n, n), [(O.CYCLIC, O.CYCLIC)] * 2, 0, iters, psi, gc, steps)
+    return np.sqrt(np.mean((out - psi) ** 2))
+
+
+def test_basic_mpdata_second_order():
+    e1, e2 = _smooth_error(33, 2), _smooth_error(65, 2)
+    order = np.log2(e1 / e2)
+    assert 1.7 < order < 2.5, order
+    u1, u2 = _smooth_error(33, 1), _smooth_error(65, 1)
+    assert np.log2(u1 / u2) < 1.3       # upwind is first order
