@@ -26,23 +26,22 @@ import time
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 METRIC = "Gcell-updates/s, 3D MPDATA-FCT iters=3, FP32 vs FP64, % of HBM peak"
 
-# Compulsory HBM words per interior cell for one launch of each kernel class
-# (reads of the arrays it consumes + writes of the arrays it produces; halo
-# re-reads, intermediates kept on chip and reductions excluded).
+# Compulsory HBM words per cell for one launch of each kernel class (reads
+# of the arrays it consumes + writes of the arrays it produces; halo re-reads,
+# values kept on chip and reductions excluded), and which cells one launch
+# covers: "all" = n^3, "interior" = (n-4)^3 (cells >= 2 from every boundary).
 ALG_WORDS = {
-    "fused_step": 5,        # psi^n, GC x3 in; psi^{n+1} out
-    "fused_pass0": 5,       # psi^n, GC x3 in; psi^1 out
-    "fused_corr_first": 9,  # psi^k, GC x3, psi^n in; psi^{k+1}, V x3 out
-    "fused_corr_mid": 9,    # psi^k, V^{k-1} x3, psi^n in; psi^{k+1}, V^k x3 out
-    "fused_corr_last": 6,   # psi^k, U x3, psi^n in; psi^{k+1} out
-    "pp_antidiff": 5,       # psi, U_d, U_d+1, U_d+2 in; V_d out (per component)
-    "pp_flux": 3,           # psi, C_d in; F_d out (per component)
-    "pp_donorcell": 5,      # psi, F x3 in; psi' out
-    "pp_betas": 10,         # psi, psi_min, psi_max, F' x3 in... (x6 faces); beta x2 out
-    "pp_gc_mono": 5,        # V_d, beta x2, psi in; M_d out
-    "pp_fct_init": 3,       # psi in; psi_min, psi_max out
-    "pp_halo": 0,
-    "pp_scale": 2,
+    "fused_interior_fct_v": (9, "interior"),  # psi^k, U x3, psi^n in; psi^k+1, V x3 out
+    "fused_interior_fct": (6, "interior"),    # psi^k, U x3, psi^n in; psi^k+1 out
+    "fused_interior_v": (8, "interior"),      # psi^k, U x3 in; psi^k+1, V x3 out
+    "fused_interior": (5, "interior"),        # psi^k, U x3 in; psi^k+1 out
+    "fused_donor": (5, "all"),                # psi^n, GC x3 in; psi^1 out
+    "pp_antidiff": (5, "all"),                # psi, U_d, U_d+1, U_d+2 in; V_d out
+    "pp_flux": (3, "all"),
+    "pp_donorcell": (5, "all"),
+    "pp_betas": (10, "all"),
+    "pp_gc_mono": (5, "all"),
+    "pp_fct_init": (3, "all"),
 }
 
 
@@ -145,8 +144,9 @@ def main():
     dom = max(prof32.items(), key=lambda kv: kv[1][0])
     dname, (dms, dlaunch) = dom
     avg_ms = dms / max(dlaunch, 1)
-    words = ALG_WORDS.get(dname, 0)
-    alg_bytes = words * 4 * cells
+    words, span = ALG_WORDS.get(dname, (0, "all"))
+    launch_cells = (n - 4) ** 3 if span == "interior" else cells
+    alg_bytes = words * 4 * launch_cells
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
 
     out = {

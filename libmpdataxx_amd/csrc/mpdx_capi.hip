@@ -494,8 +494,7 @@ struct Fused {
   {
     if (bx.total <= 0) return;
     R *c2 = L.nd == 3 ? C[2] : C[0];
-    const int inner = bx.ext[L.nd - 1];
-    dim3 g((inner + 255) / 256, L.nd == 3 ? bx.ext[1] : bx.ext[0], L.nd == 3 ? bx.ext[0] : 1);
+    dim3 g((unsigned)((bx.total + 255) / 256));
     if (o & O_NPA)
       hipLaunchKernelGGL((k_fz_donor<R, true>), g, dim3(256), 0, s->stream, out, (const R *)in,
                          (const R *)C[0], (const R *)C[1], (const R *)c2, L, m, in_scale, out_scale, flux_is_C, bx);
@@ -515,8 +514,7 @@ struct Fused {
   void antidiff_t(R **V, R *psi, R **U, const Box &bx)
   {
     if (bx.total <= 0) return;
-    const int inner = bx.ext[L.nd - 1];
-    dim3 g((inner + 255) / 256, L.nd == 3 ? bx.ext[1] : bx.ext[0], L.nd == 3 ? bx.ext[0] : 1);
+    dim3 g((unsigned)((bx.total + 255) / 256));
     hipLaunchKernelGGL((k_fz_antidiff<R, G, IGA, ABS, PFC>), g, dim3(256), 0, s->stream,
                        V[0], V[1], V[L.nd == 3 ? 2 : 0], (const R *)psi, (const R *)U[0], (const R *)U[1],
                        (const R *)U[L.nd == 3 ? 2 : 0], L, m, bx);
@@ -532,15 +530,27 @@ struct Fused {
     else antidiff_t<false, false, false>(V, psi, U, bx);
   }
 
+  template <int ND, int SHAPE, bool NPA, bool ABS, bool IGA>
+  void fct_s(R *out, R *psi, R *psin, R **V, R in_scale, R out_scale, const Box &bx)
+  {
+    constexpr int TX = FzBrick<ND, SHAPE>::TX, TY = FzBrick<ND, SHAPE>::TY, TZ = FzBrick<ND, SHAPE>::TZ;
+    dim3 g(ND == 3 ? (bx.ext[2] + TZ - 1) / TZ : 1, (bx.ext[1] + TY - 1) / TY, (bx.ext[0] + TX - 1) / TX);
+    hipLaunchKernelGGL((k_fz_fct<R, G, ND, SHAPE, NPA, ABS, IGA>), g, dim3(256), 0, s->stream, out,
+                       (const R *)psi, (const R *)psin, (const R *)V[0], (const R *)V[1],
+                       (const R *)V[ND == 3 ? 2 : 0], L, m, in_scale, out_scale, bx);
+  }
+
+  // brick oriented along the box's thinnest dimension
   template <int ND, bool NPA, bool ABS, bool IGA>
   void fct_t(R *out, R *psi, R *psin, R **V, R in_scale, R out_scale, const Box &bx)
   {
     if (bx.total <= 0) return;
-    constexpr int TX = FzBrick<ND>::TX, TY = FzBrick<ND>::TY, TZ = FzBrick<ND>::TZ;
-    dim3 g(ND == 3 ? (bx.ext[2] + TZ - 1) / TZ : 1, (bx.ext[1] + TY - 1) / TY, (bx.ext[0] + TX - 1) / TX);
-    hipLaunchKernelGGL((k_fz_fct<R, G, ND, NPA, ABS, IGA>), g, dim3(256), 0, s->stream, out,
-                       (const R *)psi, (const R *)psin, (const R *)V[0], (const R *)V[1],
-                       (const R *)V[ND == 3 ? 2 : 0], L, m, in_scale, out_scale, bx);
+    int thin = 0;
+    for (int d = 1; d < ND; ++d)
+      if (bx.ext[d] < bx.ext[thin]) thin = d;
+    if (thin == 0 || bx.ext[thin] > 8) fct_s<ND, 0, NPA, ABS, IGA>(out, psi, psin, V, in_scale, out_scale, bx);
+    else if (thin == 1) fct_s<ND, 1, NPA, ABS, IGA>(out, psi, psin, V, in_scale, out_scale, bx);
+    else fct_s<ND, 2, NPA, ABS, IGA>(out, psi, psin, V, in_scale, out_scale, bx);
   }
 
   template <int ND>
@@ -619,7 +629,8 @@ struct Fused {
 
   void interior(R *out, R *psi, R *psin, R **U, R **V, bool storev, R in_scale, R out_scale)
   {
-    ProfScope ps(s, (o & O_FCT) ? "fused_interior_fct" : "fused_interior");
+    ProfScope ps(s, (o & O_FCT) ? (storev ? "fused_interior_fct_v" : "fused_interior_fct")
+                                : (storev ? "fused_interior_v" : "fused_interior"));
     const bool fct = o & O_FCT, iga = o & O_IGA;
 #define MPDX_INT(F, I, S) if (fct == F && iga == I && storev == S) interior_t<F, I, S>(out, psi, psin, U, V, in_scale, out_scale)
     MPDX_INT(true, false, true); else MPDX_INT(true, false, false); else MPDX_INT(true, true, true);

@@ -58,12 +58,9 @@ __global__ void __launch_bounds__(256)
 k_fz_donor(R *out, const R *in, const R *C0, const R *C1, const R *C2, Layout L, BcMap m,
            R in_scale, R out_scale, int flux_is_C, Box bx)
 {
-  // cells of the box bx, innermost array dimension on threadIdx.x
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= bx.ext[L.nd - 1]) return;
-  const int i = bx.lo[0] + (L.nd == 3 ? (int)blockIdx.z : (int)blockIdx.y);
-  const int j = L.nd == 3 ? bx.lo[1] + (int)blockIdx.y : bx.lo[1] + t;
-  const int k = L.nd == 3 ? bx.lo[2] + t : 0;
+  // cells of the box bx (flattened, last coordinate fastest)
+  int i, j, k;
+  if (!box_idx(bx, blockIdx.x * (long)blockDim.x + threadIdx.x, i, j, k)) return;
   const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
   const R pc = psi_at(in, L, m, i, j, k) * in_scale;
   R f[6];
@@ -157,12 +154,9 @@ __global__ void __launch_bounds__(256)
 k_fz_antidiff(R *V0, R *V1, R *V2, const R *psi, const R *U0, const R *U1, const R *U2,
               Layout L, BcMap m, Box bx)
 {
-  // bx: sub-box of the owner cells [-1, n-1]^nd, innermost dimension on x
-  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (t >= bx.ext[L.nd - 1]) return;
-  const int i = bx.lo[0] + (L.nd == 3 ? (int)blockIdx.z : (int)blockIdx.y);
-  const int j = L.nd == 3 ? bx.lo[1] + (int)blockIdx.y : bx.lo[1] + t;
-  const int k = L.nd == 3 ? bx.lo[2] + t : 0;
+  // bx: sub-box of the owner cells [-1, n-1]^nd (flattened, last coordinate fastest)
+  int i, j, k;
+  if (!box_idx(bx, blockIdx.x * (long)blockDim.x + threadIdx.x, i, j, k)) return;
   const bool in0 = i >= 0, in1 = j >= 0, in2 = L.nd == 2 || k >= 0;
   const long p0 = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
   const R *U[3] = {U0, U1, U2};
@@ -183,9 +177,14 @@ k_fz_antidiff(R *V0, R *V1, R *V2, const R *psi, const R *U0, const R *U1, const
 // One thread per interior cell; the block owns a brick of cells and stages
 // beta over the brick plus a one-cell frame in LDS, so each beta is computed
 // once per block.  3D brick 2 x 4 x 32 (x, y, z); 2D brick 8 x 32 (x, y).
-template <int ND> struct FzBrick;
-template <> struct FzBrick<3> { static constexpr int TX = 2, TY = 4, TZ = 32; };
-template <> struct FzBrick<2> { static constexpr int TX = 8, TY = 32, TZ = 1; };
+// SHAPE picks the brick for a slab that is thin along x (0), y (1) or z (2)
+template <int ND, int SHAPE> struct FzBrick;
+template <> struct FzBrick<3, 0> { static constexpr int TX = 2, TY = 4, TZ = 32; };
+template <> struct FzBrick<3, 1> { static constexpr int TX = 8, TY = 2, TZ = 16; };
+template <> struct FzBrick<3, 2> { static constexpr int TX = 8, TY = 16, TZ = 2; };
+template <> struct FzBrick<2, 0> { static constexpr int TX = 8, TY = 32, TZ = 1; };
+template <> struct FzBrick<2, 1> { static constexpr int TX = 128, TY = 2, TZ = 1; };
+template <> struct FzBrick<2, 2> { static constexpr int TX = 8, TY = 32, TZ = 1; };
 
 // beta_up / beta_dn of one cell from psi^k (centre + 6 neighbours), the
 // psi^n bounds and the six face fluxes of the unlimited velocity
@@ -279,12 +278,12 @@ __device__ inline void beta_at(R &bup, R &bdn, const R *psi, const R *psin, cons
   else beta_core<R, G, 2, NPA>(bup, bdn, pc, xp, xm, yp, ym, zp, zm, pmin, pmax, fxp, fxm, fyp, fym, fzp, fzm);
 }
 
-template <class R, class G, int ND, bool NPA, bool ABS, bool IGA>
+template <class R, class G, int ND, int SHAPE, bool NPA, bool ABS, bool IGA>
 __global__ void __launch_bounds__(256)
 k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R *V2, Layout L,
          BcMap m, R in_scale, R out_scale, Box bx)
 {
-  constexpr int TX = FzBrick<ND>::TX, TY = FzBrick<ND>::TY, TZ = FzBrick<ND>::TZ;
+  constexpr int TX = FzBrick<ND, SHAPE>::TX, TY = FzBrick<ND, SHAPE>::TY, TZ = FzBrick<ND, SHAPE>::TZ;
   constexpr int BX = TX + 2, BY = TY + 2, BZ = ND == 3 ? TZ + 2 : 1;
   __shared__ R sbu[BX * BY * BZ], sbd[BX * BY * BZ];
   const int i0 = bx.lo[0] + blockIdx.z * TX, j0 = bx.lo[1] + blockIdx.y * TY;
@@ -377,8 +376,12 @@ __global__ void __launch_bounds__(ITile<R>::Y * ITile<R>::Z)
 k_fz_interior(InteriorArgs<R> a)
 {
   constexpr int IT_Y = ITile<R>::Y, IT_Z = ITile<R>::Z;
+  // LDS rings, slot = plane index modulo ring size
   __shared__ R s_psi[3][IT_Y][IT_Z];
   __shared__ R s_pn[FCT ? 2 : 1][IT_Y][IT_Z];
+  __shared__ R s_ux[3][IT_Y][IT_Z];     // U_x on the face owned by the plane
+  __shared__ R s_uy[2][IT_Y][IT_Z];
+  __shared__ R s_uz[2][IT_Y][IT_Z];
   __shared__ R s_vy[IT_Y][IT_Z], s_vz[IT_Y][IT_Z];
   __shared__ R s_bu[FCT ? IT_Y : 1][IT_Z], s_bd[FCT ? IT_Y : 1][IT_Z];
   __shared__ R s_fy[2][IT_Y][IT_Z], s_fz[2][IT_Y][IT_Z];
@@ -393,40 +396,48 @@ k_fz_interior(InteriorArgs<R> a)
   const int yl = min(y, L.n[1] - 1), zl = min(z, L.n[2] - 1);
   const int ym1 = max(ty - 1, 0), yp1 = min(ty + 1, IT_Y - 1);
   const int zm1 = max(tz - 1, 0), zp1 = min(tz + 1, IT_Z - 1);
-  const long sx = L.st[0], sy = L.st[1];
-  const long col = L.org + yl * sy + zl;                       // (x=0, y, z)
-  const long coly1 = L.org + min(yl + 1, L.n[1] - 1) * sy + zl;  // y+1
-  const long colym = L.org + max(yl - 1, 0) * sy + zl;           // y-1
-  const long colz1 = L.org + yl * sy + min(zl + 1, L.n[2] - 1);  // z+1
-  const long colzm = L.org + yl * sy + max(zl - 1, 0);           // z-1
+  const long sx = L.st[0];
+  const long col = L.org + yl * L.st[1] + zl;   // (x = 0, y, z)
   const bool out_ok = ty >= 2 && ty < IT_Y - 2 && tz >= 2 && tz < IT_Z - 2 &&
                       y <= a.hi[1] && z <= a.hi[2];
 
   R ps_m2 = 0, ps_m1 = 0, ps_0 = 0;     // own psi^k at planes p-2, p-1, p
   R pn_m2 = 0, pn_m1 = 0, pn_0 = 0;     // own psi^n
-  R vx_m2 = 0;                          // V_x on face owned by p-2
-  R fpx_m2 = 0;                         // unlimited donor flux on that face
+  R vx_m2 = 0;                          // V_x on the face owned by p-2
+  R fpx_m2 = 0;                         // its unlimited donor flux
   R bu_m2 = 0, bd_m2 = 0;               // beta on plane p-2
-  R fx_m3 = 0;                          // limited flux on face owned by p-3
+  R fx_m3 = 0;                          // limited flux on the face owned by p-3
+
+  // prefetched plane x0-2
+  long off = col + (long)(x0 - 2) * sx;
+  R n_ps = a.psi[off], n_pn = FCT ? a.psin[off] : R(0);
+  R n_ux = a.U0[off], n_uy = a.U1[off], n_uz = a.U2[off];
 
   for (int p = x0 - 2; p <= x1 + 2; ++p) {
-    // ---- stage 1: new psi^k (and psi^n) plane
-    ps_m2 = ps_m1; ps_m1 = ps_0;
-    ps_0 = a.psi[col + p * sx];
-    s_psi[(p + 3) % 3][ty][tz] = ps_0;
+    // ---- stage 1: publish plane p, prefetch plane p+1
+    ps_m2 = ps_m1; ps_m1 = ps_0; ps_0 = n_ps;
+    const int s0 = p % 3, sm1 = (p + 2) % 3, sm2 = (p + 1) % 3;
+    const int u0 = p & 1, um1 = (p + 1) & 1;
+    s_psi[s0][ty][tz] = ps_0;
+    s_ux[s0][ty][tz] = n_ux;
+    s_uy[u0][ty][tz] = n_uy;
+    s_uz[u0][ty][tz] = n_uz;
     if (FCT) {
-      pn_m2 = pn_m1; pn_m1 = pn_0;
-      pn_0 = a.psin[col + p * sx] * a.in_scale;
-      s_pn[(p + 2) % 2][ty][tz] = pn_0;
+      pn_m2 = pn_m1; pn_m1 = pn_0; pn_0 = n_pn * a.in_scale;
+      s_pn[u0][ty][tz] = pn_0;
+    }
+    if (p + 1 <= x1 + 2) {
+      off = col + (long)(p + 1) * sx;
+      n_ps = a.psi[off];
+      if (FCT) n_pn = a.psin[off];
+      n_ux = a.U0[off]; n_uy = a.U1[off]; n_uz = a.U2[off];
     }
     __syncthreads();
-    const int sm2 = (p + 1) % 3, sm1 = (p + 2) % 3, s0 = (p + 3) % 3;   // slots of p-2, p-1, p
 
     // ---- stage 2: velocity on plane p-1 (V_x on the face owned by p-1)
     R vx = 0, vy = 0, vz = 0;
     if (p >= x0 - 1) {
       const int q = p - 1;
-      const long c = col + q * sx;
       {
         // V_x at face q+1/2: normal x, tangential (y, z)
         auto P = [&](int da, int db, int dc) -> R {
@@ -435,9 +446,9 @@ k_fz_interior(InteriorArgs<R> a)
           const int zz = dc > 0 ? zp1 : (dc < 0 ? zm1 : tz);
           return s_psi[sl][yy][zz];
         };
-        const R g1 = (a.U1[c + sx] + a.U1[c] + a.U1[colym + (q + 1) * sx] + a.U1[colym + q * sx]) / 4;
-        const R g2 = (a.U2[c + sx] + a.U2[c] + a.U2[colzm + (q + 1) * sx] + a.U2[colzm + q * sx]) / 4;
-        vx = antidiff_core<R, G, 3, IGA, false, false>(a.U0[c], g1, g2, P);
+        const R g1 = (s_uy[u0][ty][tz] + s_uy[um1][ty][tz] + s_uy[u0][ym1][tz] + s_uy[um1][ym1][tz]) / 4;
+        const R g2 = (s_uz[u0][ty][tz] + s_uz[um1][ty][tz] + s_uz[u0][ty][zm1] + s_uz[um1][ty][zm1]) / 4;
+        vx = antidiff_core<R, G, 3, IGA, false, false>(s_ux[sm1][ty][tz], g1, g2, P);
       }
       if (p >= x0) {
         {
@@ -448,9 +459,9 @@ k_fz_interior(InteriorArgs<R> a)
             const int sl = dc > 0 ? s0 : (dc < 0 ? sm2 : sm1);
             return s_psi[sl][yy][zz];
           };
-          const R g1 = (a.U2[coly1 + q * sx] + a.U2[c] + a.U2[coly1 + q * sx - 1] + a.U2[c - 1]) / 4;
-          const R g2 = (a.U0[coly1 + q * sx] + a.U0[c] + a.U0[coly1 + (q - 1) * sx] + a.U0[c - sx]) / 4;
-          vy = antidiff_core<R, G, 3, IGA, false, false>(a.U1[c], g1, g2, P);
+          const R g1 = (s_uz[um1][yp1][tz] + s_uz[um1][ty][tz] + s_uz[um1][yp1][zm1] + s_uz[um1][ty][zm1]) / 4;
+          const R g2 = (s_ux[sm1][yp1][tz] + s_ux[sm1][ty][tz] + s_ux[sm2][yp1][tz] + s_ux[sm2][ty][tz]) / 4;
+          vy = antidiff_core<R, G, 3, IGA, false, false>(s_uy[um1][ty][tz], g1, g2, P);
         }
         {
           // V_z at face z+1/2 of plane q: normal z, tangential (x, y)
@@ -460,14 +471,15 @@ k_fz_interior(InteriorArgs<R> a)
             const int yy = dc > 0 ? yp1 : (dc < 0 ? ym1 : ty);
             return s_psi[sl][yy][zz];
           };
-          const R g1 = (a.U0[colz1 + q * sx] + a.U0[c] + a.U0[colz1 + (q - 1) * sx] + a.U0[c - sx]) / 4;
-          const R g2 = (a.U1[colz1 + q * sx] + a.U1[c] + a.U1[colz1 + q * sx - sy] + a.U1[c - sy]) / 4;
-          vz = antidiff_core<R, G, 3, IGA, false, false>(a.U2[c], g1, g2, P);
+          const R g1 = (s_ux[sm1][ty][zp1] + s_ux[sm1][ty][tz] + s_ux[sm2][ty][zp1] + s_ux[sm2][ty][tz]) / 4;
+          const R g2 = (s_uy[um1][ty][zp1] + s_uy[um1][ty][tz] + s_uy[um1][ym1][zp1] + s_uy[um1][ym1][tz]) / 4;
+          vz = antidiff_core<R, G, 3, IGA, false, false>(s_uz[um1][ty][tz], g1, g2, P);
         }
         s_vy[ty][tz] = vy;
         s_vz[ty][tz] = vz;
       }
       if (STOREV && out_ok && q >= x0 && q <= x1) {
+        const long c = col + (long)q * sx;
         a.V0[c] = vx;
         a.V1[c] = vy;
         a.V2[c] = vz;
@@ -483,9 +495,8 @@ k_fz_interior(InteriorArgs<R> a)
       const R yp = s_psi[sm1][yp1][tz], ym = s_psi[sm1][ym1][tz];
       const R zp = s_psi[sm1][ty][zp1], zm = s_psi[sm1][ty][zm1];
       R pmin, pmax;
-      const int n1 = (p + 1) % 2;   // slot of psi^n plane p-1
-      bounds_core<R, 3>(pmin, pmax, pn_m1, pn_0, pn_m2, s_pn[n1][yp1][tz], s_pn[n1][ym1][tz],
-                        s_pn[n1][ty][zp1], s_pn[n1][ty][zm1]);
+      bounds_core<R, 3>(pmin, pmax, pn_m1, pn_0, pn_m2, s_pn[um1][yp1][tz], s_pn[um1][ym1][tz],
+                        s_pn[um1][ty][zp1], s_pn[um1][ty][zm1]);
       const R vym = s_vy[ym1][tz], vzm = s_vz[ty][zm1];
       const R fyp = IGA ? vy : donor_flux<false>(pc, yp, vy);
       const R fym = IGA ? vym : donor_flux<false>(ym, pc, vym);
@@ -504,19 +515,20 @@ k_fz_interior(InteriorArgs<R> a)
       const R mx = FCT ? mono_core<R, false, IGA>(vx_m2, bu_m2, bd_m2, bu, bd, ps_m2, ps_m1) : vx_m2;
       const R fx = IGA ? mx : donor_flux<false>(ps_m2, ps_m1, mx);
       // y and z faces of plane p-1
-      const int f1 = (p + 1) % 2;   // slot of plane p-1
+      const int f1 = (p + 1) & 1;   // slot of plane p-1
+      const R pyp = s_psi[sm1][yp1][tz], pzp = s_psi[sm1][ty][zp1];
       R my = vy, mz = vz;
       if (FCT) {
-        my = mono_core<R, false, IGA>(vy, bu, bd, s_bu[yp1][tz], s_bd[yp1][tz], ps_m1, s_psi[sm1][yp1][tz]);
-        mz = mono_core<R, false, IGA>(vz, bu, bd, s_bu[ty][zp1], s_bd[ty][zp1], ps_m1, s_psi[sm1][ty][zp1]);
+        my = mono_core<R, false, IGA>(vy, bu, bd, s_bu[yp1][tz], s_bd[yp1][tz], ps_m1, pyp);
+        mz = mono_core<R, false, IGA>(vz, bu, bd, s_bu[ty][zp1], s_bd[ty][zp1], ps_m1, pzp);
       }
-      s_fy[f1][ty][tz] = IGA ? my : donor_flux<false>(ps_m1, s_psi[sm1][yp1][tz], my);
-      s_fz[f1][ty][tz] = IGA ? mz : donor_flux<false>(ps_m1, s_psi[sm1][ty][zp1], mz);
+      s_fy[f1][ty][tz] = IGA ? my : donor_flux<false>(ps_m1, pyp, my);
+      s_fz[f1][ty][tz] = IGA ? mz : donor_flux<false>(ps_m1, pzp, mz);
       if (p >= x0 + 2 && out_ok) {
-        const int f2 = p % 2;       // slot of plane p-2
+        const int f2 = p & 1;       // slot of plane p-2
         const R r = ps_m2 + ((-fx + fx_m3) + (-s_fy[f2][ty][tz] + s_fy[f2][ym1][tz])
                              + (-s_fz[f2][ty][tz] + s_fz[f2][ty][zm1])) / 1;
-        a.out[col + (p - 2) * sx] = r * a.out_scale;
+        a.out[col + (long)(p - 2) * sx] = r * a.out_scale;
       }
       fx_m3 = fx;
     }
