@@ -140,6 +140,34 @@ int *mpdx_panic_ptr(mpdx_solver *h);
 /* which path advance() uses for this configuration (MPDX_PATH_PERPASS/FUSED) */
 int mpdx_active_path(const mpdx_solver *h);
 
+/* Time-varying advector (the reference's calc_gc() hook, solver_common.hpp:
+ * 119-120,308, e.g. tests/mp3_paper_2018_JCP/reversing_deform/
+ * reversing_deform.hpp:64-105,172-179).  fn is called once per time step,
+ * before the first equation is advected, with the mid-step time t + dt/2; it
+ * may write the advector with mpdx_set_advector or through the pointers of
+ * mpdx_advector_device (kernels on the solver's stream).  A nonzero return
+ * means the advector changed: the library then refills its halos exactly as
+ * the reference's calc_gc implementations do (xchng_vctr_alng followed by
+ * xchng_vctr_nrml with ext = halo - 1).  fn = NULL removes the hook. */
+typedef int (*mpdx_calc_gc_fn)(mpdx_solver *h, double t_half, void *user);
+int mpdx_set_calc_gc(mpdx_solver *h, mpdx_calc_gc_fn fn, void *user);
+
+/* device address of the first element of the advector(d) view and its
+ * element strides (same indexing as mpdx_set_advector) */
+int mpdx_advector_device(mpdx_solver *h, int d, void **ptr, int64_t strides[3]);
+
+/* Built-in time-varying advector computed on the GPU at every step (no
+ * reference counterpart; it reproduces the calc_gc pattern above without a
+ * host round trip).  MPDX_GC_DEFORM: the reversing deformational flow of the
+ * streamfunction sin^2(pi x) sin^2(pi y) cos(pi t / T) / pi with
+ * x = (i + 1/2)/(n0 - 1), y = (j + 1/2)/(n1 - 1) at cell corners (period n-1,
+ * consistent with cyclic boundaries), GC = scaled discrete curl, z-independent
+ * in 3D (GC_z = 0).  params = {T, max_courant}; corner values are rounded to
+ * multiples of 2^-17 so the discrete divergence is exactly zero. */
+#define MPDX_GC_NONE 0
+#define MPDX_GC_DEFORM 1
+int mpdx_set_advector_generator(mpdx_solver *h, int kind, const double *params, int n_params);
+
 /* Kernel timing with HIP events on the solver's stream (no reference
  * counterpart; the reference prints a wall-clock timer, concurr/detail/
  * timer.hpp:19-56).  mpdx_profile(h, 1) clears and starts recording;

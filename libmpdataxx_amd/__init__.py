@@ -33,7 +33,10 @@ EXPORTED = [
     "mpdx_get_advectee", "mpdx_set_advector", "mpdx_get_advector",
     "mpdx_advance", "mpdx_reduce", "mpdx_time", "mpdx_timestep",
     "mpdx_panic_ptr", "mpdx_active_path", "mpdx_profile", "mpdx_profile_read",
+    "mpdx_set_calc_gc", "mpdx_advector_device", "mpdx_set_advector_generator",
 ]
+GC_NONE, GC_DEFORM = 0, 1
+CALC_GC_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p)
 
 
 class Params(ctypes.Structure):
@@ -82,6 +85,9 @@ def lib():
         L.mpdx_panic_ptr.restype = ctypes.POINTER(ctypes.c_int)
         L.mpdx_panic_ptr.argtypes = [P]
         L.mpdx_active_path.argtypes = [P]
+        L.mpdx_set_calc_gc.argtypes = [P, CALC_GC_FN, P]
+        L.mpdx_advector_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), i64p]
+        L.mpdx_set_advector_generator.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.mpdx_profile.argtypes = [P, ctypes.c_int]
         L.mpdx_profile_read.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_double), i64p]
@@ -245,6 +251,26 @@ class Solver:
             out[buf.value.decode()] = (ms.value, n.value)
             idx += 1
         return out
+
+    def set_calc_gc(self, fn):
+        """fn(t_half) -> bool, called once per step like the reference's
+        calc_gc(); it may call set_advector / set_advector_device."""
+        if fn is None:
+            self._calc_gc = None
+            self._check(self._L.mpdx_set_calc_gc(self._h, CALC_GC_FN(), None))
+            return
+
+        def tramp(h, t_half, user):
+            try:
+                return 1 if fn(t_half) else 0
+            except Exception:          # noqa: BLE001 - reported as an error code
+                return -1
+        self._calc_gc = CALC_GC_FN(tramp)
+        self._check(self._L.mpdx_set_calc_gc(self._h, self._calc_gc, None))
+
+    def set_advector_generator(self, kind, *params):
+        arr = (ctypes.c_double * max(len(params), 1))(*params)
+        self._check(self._L.mpdx_set_advector_generator(self._h, kind, arr, len(params)))
 
     def set_panic(self, v=True):
         self._L.mpdx_panic_ptr(self._h)[0] = int(bool(v))

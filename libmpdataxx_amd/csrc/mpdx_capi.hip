@@ -8,6 +8,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -62,6 +63,27 @@ void plaunch(mpdx_solver *s, const char *name, long total, K kern, A... args)
   launch(s->stream, total, kern, args...);
 }
 
+// grid of a Box kernel (box_idx): x covers the (b, c) plane, y the a extent
+inline dim3 box_grid(const Box &bx)
+{
+  return dim3((bx.plane + kBlock - 1) / kBlock, (unsigned)bx.ext[0]);
+}
+
+template <class K, class... A>
+void blaunch(hipStream_t st, const Box &bx, K kern, A... args)
+{
+  if (bx.total <= 0) return;
+  hipLaunchKernelGGL(kern, box_grid(bx), dim3(kBlock), 0, st, args...);
+}
+
+template <class K, class... A>
+void pblaunch(mpdx_solver *s, const char *name, const Box &bx, K kern, A... args)
+{
+  if (bx.total <= 0) return;
+  ProfScope ps(s, name);
+  blaunch(s->stream, bx, kern, args...);
+}
+
 rng interior(const mpdx_solver *s, int d) { return R_(0, s->L.n[d] - 1); }
 
 Box interior_box(const Layout &L)
@@ -94,7 +116,7 @@ struct PerPass {
     int mode = kind == MPDX_BC_CYCLIC ? 0 : (nrml ? 2 : 1);
     if (L.nd == 2) rb = R_(0, 0);
     Box bx = make_box(ra.f, ra.l, rb.f, rb.l, 1, L.hw);
-    plaunch(s, "pp_halo", bx.total, k_fill<R>, a, make_rot(L, d), bx, side, mode, L.n[d]);
+    pblaunch(s, "pp_halo", bx, k_fill<R>, a, make_rot(L, d), bx, side, mode, L.n[d]);
   }
 
   void alng(R **av, int d, int side, rng ra, rng rb, int ad)
@@ -105,7 +127,7 @@ struct PerPass {
     R *avd = av[d], *av1 = av[(d + 1) % nd], *av2 = nd == 3 ? av[(d + 2) % 3] : av[0];
     if (s->p.bc[d][side] == MPDX_BC_CYCLIC) {
       Box bx = make_box(1, hw, ra.f, ra.l, rb.f, rb.l);
-      plaunch(s, "pp_halo", bx.total, k_alng<R>, avd, (const R *)av1, (const R *)av2, r, bx, nd, side, 1, N);
+      pblaunch(s, "pp_halo", bx, k_alng<R>, avd, (const R *)av1, (const R *)av2, r, bx, nd, side, 1, N);
       return;
     }
     const int e = side == 0 ? 0 : N - 1;
@@ -114,7 +136,7 @@ struct PerPass {
     const int lo = side == 0 ? -hw : N - 1 + (ad ? 1 : 0);
     const int hi = side == 0 ? -1 - (ad ? 1 : 0) : N - 2 + hw;
     Box bx = make_box(lo, hi, ra.f, ra.l, rb.f, rb.l);
-    plaunch(s, "pp_halo", bx.total, k_alng<R>, avd, (const R *)av1, (const R *)av2, r, bx, nd, side, 0, N);
+    pblaunch(s, "pp_halo", bx, k_alng<R>, avd, (const R *)av1, (const R *)av2, r, bx, nd, side, 0, N);
   }
 
   // solver_3d.hpp:47-67, solver_2d.hpp:40-56
@@ -175,7 +197,7 @@ struct PerPass {
       const int d1 = (d + 1) % L.nd, d2 = L.nd == 3 ? (d + 2) % 3 : 0;
       Box bx = make_box(-1 - e, L.n[d] - 1 + e, -e, L.n[d1] - 1 + e,
                         L.nd == 3 ? -e : 0, L.nd == 3 ? L.n[d2] - 1 + e : 0);
-      plaunch(s, "pp_flux", bx.total, k_flux<R>, out[d], (const R *)psi, (const R *)C[d], make_rot(L, d), bx, o);
+      pblaunch(s, "pp_flux", bx, k_flux<R>, out[d], (const R *)psi, (const R *)C[d], make_rot(L, d), bx, o);
     }
   }
 
@@ -184,7 +206,7 @@ struct PerPass {
     for (int d = 0; d < L.nd; ++d) {
       const int d1 = (d + 1) % L.nd, d2 = L.nd == 3 ? (d + 2) % 3 : 0;
       Box bx = make_box(-1, L.n[d] - 1, 0, L.n[d1] - 1, 0, L.nd == 3 ? L.n[d2] - 1 : 0);
-      plaunch(s, "pp_antidiff", bx.total, k_antidiff<R, G>, V[d], (const R *)psi, (const R *)U[d],
+      pblaunch(s, "pp_antidiff", bx, k_antidiff<R, G>, V[d], (const R *)psi, (const R *)U[d],
              (const R *)U[d1], (const R *)U[L.nd == 3 ? d2 : 0], make_rot(L, d), bx, L.nd, o);
     }
   }
@@ -211,14 +233,14 @@ struct PerPass {
     R **flx = V;
     if (!(o & O_IGA)) { flx = av_of(s->flux, Fb); make_flux(psi, V, flx, 1); }
     Box b1 = box_ext1();
-    plaunch(s, "pp_betas", b1.total, k_betas<R, G>, A(s->bup), A(s->bdn), (const R *)psi, (const R *)A(s->pmin),
+    pblaunch(s, "pp_betas", b1, k_betas<R, G>, A(s->bup), A(s->bdn), (const R *)psi, (const R *)A(s->pmin),
            (const R *)A(s->pmax), (const R *)flx[0], (const R *)flx[1],
            (const R *)flx[L.nd == 3 ? 2 : 0], L, b1, o);
     R **M = av_of(s->mono, Mb);
     for (int d = 0; d < L.nd; ++d) {
       const int d1 = (d + 1) % L.nd, d2 = L.nd == 3 ? (d + 2) % 3 : 0;
       Box bx = make_box(-1, L.n[d] - 1, 0, L.n[d1] - 1, 0, L.nd == 3 ? L.n[d2] - 1 : 0);
-      plaunch(s, "pp_gc_mono", bx.total, k_gc_mono<R>, M[d], (const R *)V[d], (const R *)psi,
+      pblaunch(s, "pp_gc_mono", bx, k_gc_mono<R>, M[d], (const R *)V[d], (const R *)psi,
              (const R *)A(s->bup), (const R *)A(s->bdn), make_rot(L, d), bx, o);
     }
   }
@@ -229,7 +251,7 @@ struct PerPass {
     int &lev = s->lev[e];
     if (o & O_FCT) {
       Box b1 = box_ext1();
-      plaunch(s, "pp_fct_init", b1.total, k_fct_init<R>, A(s->pmin), A(s->pmax), (const R *)A(s->psi[e][lev]), L, b1);
+      pblaunch(s, "pp_fct_init", b1, k_fct_init<R>, A(s->pmin), A(s->pmax), (const R *)A(s->psi[e][lev]), L, b1);
     }
     for (int iter = 0; iter < s->p.n_iters; ++iter) {
       R *Cb[3], *Fb[3], *Ub[3], *Vb[3];
@@ -244,7 +266,7 @@ struct PerPass {
       R **flx = C;
       if (!(o & O_IGA) || iter == 0) { flx = av_of(s->flux, Fb); make_flux(A(s->psi[e][lev]), C, flx, 0); }
       Box bi = interior_box(L);
-      plaunch(s, "pp_donorcell", bi.total, k_donorcell<R>, A(s->psi[e][lev ^ 1]), (const R *)A(s->psi[e][lev]),
+      pblaunch(s, "pp_donorcell", bi, k_donorcell<R>, A(s->psi[e][lev ^ 1]), (const R *)A(s->psi[e][lev]),
              (const R *)flx[0], (const R *)flx[1], (const R *)flx[L.nd == 3 ? 2 : 0], L, bi, o);
       if (s->p.upwind_filter_freq > 0 && s->timestep % s->p.upwind_filter_freq == 0) break;
     }
@@ -255,7 +277,7 @@ struct PerPass {
     if (ex == 0) return;
     Box bi = interior_box(L);
     const R f = ex > 0 ? R(1) / R(1 << ex) : R(1 << -ex);
-    plaunch(s, "pp_scale", bi.total, k_scale<R>, A(s->psi[e][s->lev[e]]), L, bi, f);
+    pblaunch(s, "pp_scale", bi, k_scale<R>, A(s->psi[e][s->lev[e]]), L, bi, f);
   }
 
   void step()
@@ -372,7 +394,7 @@ template <class R>
 void launch_copy(hipStream_t st, R *dst, long dorg, const long ds[3], const R *src, long sorg,
                  const long ss[3], Box bx)
 {
-  launch(st, bx.total, k_copy_box<R>, dst, dorg, ds[0], ds[1], ds[2], src, sorg, ss[0], ss[1], ss[2], bx);
+  blaunch(st, bx, k_copy_box<R>, dst, dorg, ds[0], ds[1], ds[2], src, sorg, ss[0], ss[1], ss[2], bx);
 }
 
 // copy between a user buffer and the box [lo, hi] of a padded device array
@@ -455,7 +477,7 @@ int ante_loop(mpdx_solver *s)
   R **gc = pp.av_of(s->GC, Gb);
   pp.xchng_vctr_alng(gc, 0);
   Box bi = interior_box(s->L);
-  launch(s->stream, bi.total, k_div<R>, (R *)s->stat, (const R *)gc[0], (const R *)gc[1],
+  blaunch(s->stream, bi, k_div<R>, (R *)s->stat, (const R *)gc[0], (const R *)gc[1],
          (const R *)gc[s->L.nd == 3 ? 2 : 0], s->L, bi);
   double mx = 0;
   if (int rc = reduce_impl<R>(s, s->stat, MPDX_RED_MAX, &mx)) return rc;
@@ -494,7 +516,7 @@ struct Fused {
   {
     if (bx.total <= 0) return;
     R *c2 = L.nd == 3 ? C[2] : C[0];
-    dim3 g((unsigned)((bx.total + 255) / 256));
+    const dim3 g = box_grid(bx);
     if (o & O_NPA)
       hipLaunchKernelGGL((k_fz_donor<R, true>), g, dim3(256), 0, s->stream, out, (const R *)in,
                          (const R *)C[0], (const R *)C[1], (const R *)c2, L, m, in_scale, out_scale, flux_is_C, bx);
@@ -514,7 +536,7 @@ struct Fused {
   void antidiff_t(R **V, R *psi, R **U, const Box &bx)
   {
     if (bx.total <= 0) return;
-    dim3 g((unsigned)((bx.total + 255) / 256));
+    const dim3 g = box_grid(bx);
     hipLaunchKernelGGL((k_fz_antidiff<R, G, IGA, ABS, PFC>), g, dim3(256), 0, s->stream,
                        V[0], V[1], V[L.nd == 3 ? 2 : 0], (const R *)psi, (const R *)U[0], (const R *)U[1],
                        (const R *)U[L.nd == 3 ? 2 : 0], L, m, bx);
@@ -622,8 +644,11 @@ struct Fused {
     for (int d = 0; d < 3; ++d) { a.lo[d] = 2; a.hi[d] = L.n[d] - 3; }
     a.in_scale = in_scale; a.out_scale = out_scale;
     constexpr int TY = ITile<R>::Y, TZ = ITile<R>::Z;
-    dim3 g((L.n[2] - 4 + TZ - 5) / (TZ - 4), (L.n[1] - 4 + TY - 5) / (TY - 4),
-           (L.n[0] - 4 + IX_CHUNK - 1) / IX_CHUNK);
+    a.nb[0] = (L.n[2] - 4 + TZ - 5) / (TZ - 4);
+    a.nb[1] = (L.n[1] - 4 + TY - 5) / (TY - 4);
+    a.nb[2] = (L.n[0] - 4 + IX_CHUNK - 1) / IX_CHUNK;
+    const int ntile = a.nb[0] * a.nb[1] * a.nb[2];
+    const dim3 g(8 * ((ntile + 7) / 8));   // tiles are spread over the 8 XCDs in the kernel
     hipLaunchKernelGGL((k_fz_interior<R, G, FCT, IGA, STOREV>), g, dim3(TZ, TY), 0, s->stream, a);
   }
 
@@ -715,6 +740,45 @@ int fused_step(mpdx_solver *s)
   return 0;
 }
 
+// time-varying advector for this step (solver_common.hpp:306-308), then the
+// halo refill that the reference's calc_gc implementations perform
+template <class R, class G>
+int calc_gc_step(mpdx_solver *s)
+{
+  if (!s->calc_gc && s->gen_kind == MPDX_GC_NONE) return 0;
+  const double t_half = s->time + 0.5 * s->p.dt;
+  bool changed = false;
+  if (s->gen_kind == MPDX_GC_DEFORM) {
+    const double amp = s->gen_scale * std::cos(3.141592653589793 * t_half / s->gen_T);
+    // (same expression as the oracle's, so both see the same amplitude)
+    const Layout &L = s->L;
+    Box bx = make_box(-1, L.n[0] - 1, -1, L.n[1] - 1, L.nd == 3 ? 0 : 0, L.nd == 3 ? L.n[2] - 1 : 0);
+    ProfScope ps(s, "gen_deform");
+    blaunch(s->stream, bx, k_gen_deform<R>, (R *)s->GC[0], (R *)s->GC[1],
+           (R *)s->GC[L.nd == 3 ? 2 : 0], L, bx, amp, (const double *)s->gen_sx2,
+           (const double *)s->gen_sy2);
+    changed = true;
+  }
+  if (s->calc_gc) {
+    const int r = s->calc_gc(s, t_half, s->calc_gc_user);
+    if (r < 0) {
+      if (s->err.empty()) s->err = "libmpdata++: calc_gc callback failed";
+      return MPDX_EINVAL;
+    }
+    changed = changed || r != 0;
+  }
+  if (changed) {
+    ProfScope ps(s, "gc_halo");
+    PerPass<R, G> pp(s);
+    R *Gb[3];
+    R **gc = pp.av_of(s->GC, Gb);
+    pp.xchng_vctr_alng(gc, 0);
+    pp.xchng_vctr_nrml(gc, s->L.hw - 1);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 template <class R, class G>
 int advance_impl(mpdx_solver *s, int64_t nt)
 {
@@ -726,12 +790,40 @@ int advance_impl(mpdx_solver *s, int64_t nt)
   if (!fused)
     if (int rc = ensure_perpass(s)) return rc;
   PerPass<R, G> pp(s);
+  // the fused step is replayed from a captured hipGraph unless something in
+  // it changes from step to step (host calc_gc callback, upwind filter steps)
+  // or kernels are being timed individually
+  // (the legacy null stream cannot be captured)
+  const bool graphs = fused && s->use_graphs && !s->prof_on && !s->calc_gc && s->stream != nullptr &&
+                      s->gen_kind == MPDX_GC_NONE && s->p.upwind_filter_freq <= 0;
   while (s->timestep < nt) {
     if (s->panic) break;
-    if (fused) {
-      if (int rc = fused_step<R, G>(s)) return rc;
+    if (graphs) {
+      const int key = s->lev[0] & 1;
+      if (!s->graph[key]) {
+        int lev_save[MPDX_MAX_EQNS];
+        for (int e = 0; e < s->p.n_eqns; ++e) lev_save[e] = s->lev[e];
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        int rc = calc_gc_step<R, G>(s);
+        if (!rc) rc = fused_step<R, G>(s);
+        hipError_t ec = hipStreamEndCapture(s->stream, &g);
+        for (int e = 0; e < s->p.n_eqns; ++e) s->lev[e] = lev_save[e];
+        if (rc) return rc;
+        HIPCHK(ec);
+        hipError_t ei = hipGraphInstantiate(&s->graph[key], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(ei);
+      }
+      HIPCHK(hipGraphLaunch(s->graph[key], s->stream));
+      for (int e = 0; e < s->p.n_eqns; ++e) s->lev[e] ^= 1;
     } else {
-      pp.step();
+      if (int rc = calc_gc_step<R, G>(s)) return rc;
+      if (fused) {
+        if (int rc = fused_step<R, G>(s)) return rc;
+      } else {
+        pp.step();
+      }
     }
     HIPCHK(hipGetLastError());
     s->timestep++;
@@ -787,6 +879,13 @@ int mpdx_create(const mpdx_params *p, mpdx_solver **out)
   const int hw = (p->opts & MPDX_OPT_FCT) ? 2 : 1;
   for (int d = 0; d < p->n_dims; ++d)
     if (p->grid[d] < hw + 2) return fail(MPDX_EINVAL, "libmpdata++: grid too small for the halo");
+  {
+    // box kernels put the first index on the grid's y dimension and split
+    // the (second, third) plane with 32-bit arithmetic
+    const long n1 = p->grid[1] + 2 * hw, n2 = p->n_dims == 3 ? p->grid[2] + 2 * hw : 1;
+    if (p->grid[0] + 2 * hw > 65535 || n1 * n2 >= (1l << 31))
+      return fail(MPDX_EUNSUPPORTED, "libmpdata++: grid too large for the HIP backend's index space");
+  }
 
   mpdx_solver *s = new mpdx_solver();
   s->p = *p;
@@ -819,6 +918,7 @@ int mpdx_create(const mpdx_params *p, mpdx_solver **out)
     return MPDX_EHIP;
   }
   s->own_stream = true;
+  if (const char *ev = std::getenv("MPDX_NO_GRAPHS")) s->use_graphs = !(ev[0] && ev[0] != '0');
 
   int rc = 0;
   for (int e = 0; e < p->n_eqns && !rc; ++e) {
@@ -857,6 +957,8 @@ int mpdx_destroy(mpdx_solver *s)
 {
   if (!s) return MPDX_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (int q = 0; q < 2; ++q)
+    if (s->graph[q]) (void)hipGraphExecDestroy(s->graph[q]);
   for (void *p : s->allocs)
     if (p) (void)hipFree(p);
   fused_free(s);
@@ -953,6 +1055,62 @@ int mpdx_timestep(const mpdx_solver *s, int64_t *ts)
 int *mpdx_panic_ptr(mpdx_solver *s) { return &s->panic; }
 
 int mpdx_active_path(const mpdx_solver *s) { return s->active_path; }
+
+int mpdx_set_calc_gc(mpdx_solver *s, mpdx_calc_gc_fn fn, void *user)
+{
+  s->calc_gc = fn;
+  s->calc_gc_user = user;
+  return MPDX_OK;
+}
+
+int mpdx_advector_device(mpdx_solver *s, int d, void **ptr, int64_t strides[3])
+{
+  if (d < 0 || d >= s->L.nd) { s->err = "libmpdata++: bad advector component"; return MPDX_EINVAL; }
+  int lo[3], hi[3];
+  adv_bounds(s, d, lo, hi);
+  const long off = s->L.org + lo[0] * s->L.st[0] + lo[1] * s->L.st[1] + (s->L.nd == 3 ? lo[2] * s->L.st[2] : 0);
+  *ptr = static_cast<char *>(s->GC[d]) + off * s->esz;
+  for (int q = 0; q < 3; ++q) strides[q] = q < s->L.nd ? s->L.st[q] : 0;
+  return MPDX_OK;
+}
+
+int mpdx_set_advector_generator(mpdx_solver *s, int kind, const double *params, int n_params)
+{
+  if (kind == MPDX_GC_NONE) { s->gen_kind = kind; return MPDX_OK; }
+  if (kind != MPDX_GC_DEFORM || n_params < 2 || !(params[0] > 0) || !(params[1] > 0)) {
+    s->err = "libmpdata++: bad advector generator parameters";
+    return MPDX_EINVAL;
+  }
+  // sin^2 tables at corners a + 1/2, a = -2 .. n-1 (host libm, shared with
+  // the oracle), and the amplitude that makes the largest |GC| (at cos = 1)
+  // equal the Courant number
+  const int n0 = s->L.n[0], n1 = s->L.n[1];
+  const double pi = 3.141592653589793;
+  std::vector<double> sx2(n0 + 2), sy2(n1 + 2);
+  for (int a = -2; a < n0; ++a) { const double v = std::sin(pi * (a + 0.5) / (n0 - 1)); sx2[a + 2] = v * v; }
+  for (int b = -2; b < n1; ++b) { const double v = std::sin(pi * (b + 0.5) / (n1 - 1)); sy2[b + 2] = v * v; }
+  double mx = 0;
+  for (int a = -1; a < n0; ++a)
+    for (int b = -1; b < n1; ++b) {
+      const double c = sx2[a + 2] * sy2[b + 2];
+      mx = std::max(mx, std::fabs(c - sx2[a + 2] * sy2[b + 1]));
+      mx = std::max(mx, std::fabs(sx2[a + 1] * sy2[b + 2] - c));
+    }
+  if (!s->gen_sx2) {
+    void *px = nullptr, *py = nullptr;
+    if (int rc = dev_alloc(s, &px, sizeof(double) * (n0 + 2))) return rc;
+    if (int rc = dev_alloc(s, &py, sizeof(double) * (n1 + 2))) return rc;
+    s->gen_sx2 = (double *)px;
+    s->gen_sy2 = (double *)py;
+  }
+  HIPCHK(hipMemcpyAsync(s->gen_sx2, sx2.data(), sizeof(double) * (n0 + 2), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->gen_sy2, sy2.data(), sizeof(double) * (n1 + 2), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->gen_kind = kind;
+  s->gen_T = params[0];
+  s->gen_scale = mx > 0 ? params[1] / mx : 0;
+  return MPDX_OK;
+}
 
 int mpdx_profile(mpdx_solver *s, int enable)
 {

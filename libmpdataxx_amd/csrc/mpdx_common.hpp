@@ -13,6 +13,8 @@
 #include <cfloat>
 #include <cstdint>
 
+#include "mpdx_fastdiv.hpp"
+
 namespace mpdx {
 
 enum : unsigned {
@@ -51,11 +53,15 @@ __host__ __device__ inline long rr(const Rot &r, int a, int b, int c)
   return r.o + (long)a * r.sA + (long)b * r.sB + (long)c * r.sC;
 }
 
-// flat index box: thread t -> (a, b, c) with c fastest
+// index box (a, b, c), c fastest.  Box kernels run on a 2D grid: blockIdx.y
+// selects a, blockIdx.x * blockDim.x + threadIdx.x the (b, c) pair within
+// the plane, split by one 32-bit multiply-high.
 struct Box {
   int lo[3];
   int ext[3];
   long total;
+  unsigned plane;   // ext[1] * ext[2]
+  FastDiv d1, d2;   // division by ext[1], ext[2]
 };
 
 inline Box make_box(int a0, int a1, int b0, int b1, int c0, int c1)
@@ -63,18 +69,28 @@ inline Box make_box(int a0, int a1, int b0, int b1, int c0, int c1)
   Box b;
   b.lo[0] = a0; b.lo[1] = b0; b.lo[2] = c0;
   b.ext[0] = a1 - a0 + 1; b.ext[1] = b1 - b0 + 1; b.ext[2] = c1 - c0 + 1;
-  if (b.ext[0] <= 0 || b.ext[1] <= 0 || b.ext[2] <= 0) b.total = 0;
-  else b.total = (long)b.ext[0] * b.ext[1] * b.ext[2];
+  if (b.ext[0] <= 0 || b.ext[1] <= 0 || b.ext[2] <= 0) {
+    b.total = 0;
+    b.plane = 0;
+    b.d1 = make_fastdiv(1);
+    b.d2 = make_fastdiv(1);
+  } else {
+    b.total = (long)b.ext[0] * b.ext[1] * b.ext[2];
+    b.plane = (unsigned)((long)b.ext[1] * b.ext[2]);
+    b.d1 = make_fastdiv((unsigned)b.ext[1]);
+    b.d2 = make_fastdiv((unsigned)b.ext[2]);
+  }
   return b;
 }
 
-__device__ inline bool box_idx(const Box &b, long t, int &a, int &bb, int &c)
+__device__ inline bool box_idx(const Box &b, int &a, int &bb, int &c)
 {
-  if (t >= b.total) return false;
-  c = (int)(t % b.ext[2]) + b.lo[2];
-  t /= b.ext[2];
-  bb = (int)(t % b.ext[1]) + b.lo[1];
-  a = (int)(t / b.ext[1]) + b.lo[0];
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.plane) return false;
+  const unsigned q = fdiv(t, b.d2);
+  a = (int)blockIdx.y + b.lo[0];
+  bb = (int)q + b.lo[1];
+  c = (int)(t - q * (unsigned)b.ext[2]) + b.lo[2];
   return true;
 }
 

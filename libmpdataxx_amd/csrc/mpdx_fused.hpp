@@ -18,6 +18,8 @@
 // scalar halo fill (cyclic: period N-1; open: clamp to the edge).
 #pragma once
 
+#include <type_traits>
+
 #include "mpdx_perpass.hpp"
 #include "mpdx_solver.hpp"
 
@@ -58,9 +60,9 @@ __global__ void __launch_bounds__(256)
 k_fz_donor(R *out, const R *in, const R *C0, const R *C1, const R *C2, Layout L, BcMap m,
            R in_scale, R out_scale, int flux_is_C, Box bx)
 {
-  // cells of the box bx (flattened, last coordinate fastest)
+  // cells of the box bx (last coordinate fastest)
   int i, j, k;
-  if (!box_idx(bx, blockIdx.x * (long)blockDim.x + threadIdx.x, i, j, k)) return;
+  if (!box_idx(bx, i, j, k)) return;
   const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
   const R pc = psi_at(in, L, m, i, j, k) * in_scale;
   R f[6];
@@ -154,9 +156,9 @@ __global__ void __launch_bounds__(256)
 k_fz_antidiff(R *V0, R *V1, R *V2, const R *psi, const R *U0, const R *U1, const R *U2,
               Layout L, BcMap m, Box bx)
 {
-  // bx: sub-box of the owner cells [-1, n-1]^nd (flattened, last coordinate fastest)
+  // bx: sub-box of the owner cells [-1, n-1]^nd (last coordinate fastest)
   int i, j, k;
-  if (!box_idx(bx, blockIdx.x * (long)blockDim.x + threadIdx.x, i, j, k)) return;
+  if (!box_idx(bx, i, j, k)) return;
   const bool in0 = i >= 0, in1 = j >= 0, in2 = L.nd == 2 || k >= 0;
   const long p0 = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
   const R *U[3] = {U0, U1, U2};
@@ -348,17 +350,38 @@ k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R 
 // One whole corrective pass (antidiffusive velocity, FCT limiter, donor-cell
 // update) for cells at distance >= 2 from every boundary, where no halo rule
 // touches any value the pass reads, so no advector halo fill is needed.
-// A block owns a (IT_Y-4) x (IT_Z-4) column of output cells and marches along
-// x through IX_CHUNK output planes; each thread owns one (y, z) column of the
-// (IT_Y x IT_Z) tile, which includes a 2-cell frame.  At step p (newest psi^k
-// plane p) it computes the velocity on plane p-1, beta on plane p-1, the
-// limited fluxes and writes psi^{k+1} on plane p-2.  psi^k and psi^n planes,
-// the in-plane velocities, beta and the y/z fluxes go through LDS rings;
-// everything along x stays in registers.
+// A block owns a (Y-4) x (Z-4) column of output cells and marches along x
+// through IX_CHUNK output planes; each thread owns one (y, z) column of the
+// (Y x Z) tile, which includes a 2-cell frame.  At step q (newest psi^k plane
+// p = x0 - 2 + q) it computes the velocity on plane p-1, beta on plane p-1,
+// the limited fluxes and writes psi^{k+1} on plane p-2.
+//
+// psi^k and psi^n planes, the in-plane velocities, beta and the y/z fluxes go
+// through LDS rings; a thread's own column stays in register rings.  The
+// march is unrolled by 6 (the period of the 3- and 2-plane rings) so every
+// ring slot is a compile-time constant, and each LDS plane carries a padding
+// row above and below the tile, so every neighbour read is the thread's own
+// LDS address plus an immediate offset (a z neighbour past the tile edge
+// reads the adjacent row or the padding; such values reach only frame
+// cells, whose results are discarded).
 template <class R> struct ITile {
   static constexpr int Y = sizeof(R) == 8 ? 16 : 32, Z = 32;
+  static constexpr int PLANE = (Y + 2) * Z;
 };
 constexpr int IX_CHUNK = 64;
+
+// LDS planes of the interior kernel
+enum {
+  IL_PSI = 0,     // psi^k ring, 3 planes
+  IL_UX = 3,      // U_x ring, 3
+  IL_UY = 6,      // U_y ring, 2
+  IL_UZ = 8,      // U_z ring, 2
+  IL_PN = 10,     // psi^n ring, 2
+  IL_FY = 12,     // limited y flux ring, 2
+  IL_FZ = 14,     // limited z flux ring, 2
+  IL_VY = 16, IL_VZ = 17, IL_BU = 18, IL_BD = 19,
+  IL_N = 20
+};
 
 template <class R>
 struct InteriorArgs {
@@ -368,174 +391,241 @@ struct InteriorArgs {
   R *V0, *V1, *V2;      // unlimited velocity out (next pass's GC_unco), may be null
   Layout L;
   int lo[3], hi[3];     // output cells, inclusive; lo >= 2, hi <= n-3
+  int nb[3];            // tiles along z, y and x chunks
   R in_scale, out_scale;
 };
+
+// max / min without the std::max argument-order semantics (they differ only
+// for NaNs and the sign of zero, which no result of this pass depends on);
+// FP32 uses IEEE 754-2019 maximum / minimum, which gfx950 executes as one
+// three-operand v_maximum3 / v_minimum3 and needs no NaN quieting of inputs
+__device__ __forceinline__ float fmx(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+__device__ __forceinline__ float fmn(float a, float b) { return __builtin_elementwise_minimum(a, b); }
+__device__ __forceinline__ double fmx(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ double fmn(double a, double b) { return fmin(a, b); }
+
+template <class R, class G>
+__device__ __forceinline__ void ibeta(R &bup, R &bdn, R pc, R xp, R xm, R yp, R ym, R zp, R zm, R pmin,
+                                      R pmax, R fxp, R fxm, R fyp, R fym, R fzp, R fzm)
+{
+  const R mx = fmx(fmx(fmx(pmax, pc), fmx(xp, xm)), fmx(fmx(yp, ym), fmx(zp, zm)));
+  const R mn = fmn(fmn(fmn(pmin, pc), fmn(xp, xm)), fmn(fmn(yp, ym), fmn(zp, zm)));
+  const R z = 0;
+  const R up_den = (fmx(z, fxm) - fmn(z, fxp)) + (fmx(z, fym) - fmn(z, fyp)) + (fmx(z, fzm) - fmn(z, fzp));
+  const R dn_den = (fmx(z, fxp) - fmn(z, fxm)) + (fmx(z, fyp) - fmn(z, fym)) + (fmx(z, fzp) - fmn(z, fzm));
+  bup = fct_frac<G>(mx - pc, up_den);
+  bdn = fct_frac<G>(pc - mn, dn_den);
+}
+
+template <class R>
+__device__ __forceinline__ R imono(R v, R bul, R bdl, R bur, R bdr)
+{
+  const R m_dn_up = fmn(R(1), fmn(bdl, bur));
+  const R m_up_dn = fmn(R(1), fmn(bul, bdr));
+  return v * (v > 0 ? m_dn_up : m_up_dn);
+}
+
+// donor-cell flux pospart(c) l + negpart(c) r: the same value (up to the
+// sign of a zero) as c times the upwind psi
+template <class R> __device__ __forceinline__ R iflux(R l, R r, R c)
+{
+  return c * (c > 0 ? l : r);
+}
+
+template <int N> using ic = std::integral_constant<int, N>;
 
 template <class R, class G, bool FCT, bool IGA, bool STOREV>
 __global__ void __launch_bounds__(ITile<R>::Y * ITile<R>::Z)
 k_fz_interior(InteriorArgs<R> a)
 {
-  constexpr int IT_Y = ITile<R>::Y, IT_Z = ITile<R>::Z;
-  // LDS rings, slot = plane index modulo ring size
-  __shared__ R s_psi[3][IT_Y][IT_Z];
-  __shared__ R s_pn[FCT ? 2 : 1][IT_Y][IT_Z];
-  __shared__ R s_ux[3][IT_Y][IT_Z];     // U_x on the face owned by the plane
-  __shared__ R s_uy[2][IT_Y][IT_Z];
-  __shared__ R s_uz[2][IT_Y][IT_Z];
-  __shared__ R s_vy[IT_Y][IT_Z], s_vz[IT_Y][IT_Z];
-  __shared__ R s_bu[FCT ? IT_Y : 1][IT_Z], s_bd[FCT ? IT_Y : 1][IT_Z];
-  __shared__ R s_fy[2][IT_Y][IT_Z], s_fz[2][IT_Y][IT_Z];
+  constexpr int TY = ITile<R>::Y, TZ = ITile<R>::Z, PL = ITile<R>::PLANE;
+  __shared__ R lds[IL_N * PL];
+
+  // XCD-aware tile order: workgroup w runs on XCD w % 8, and XCD x takes the
+  // x-th contiguous run of tiles (z fastest), so neighbouring tiles that
+  // share halo cache lines run on the same XCD at the same time
+  const int nbz = a.nb[0], nby = a.nb[1];
+  const int ntile = nbz * nby * a.nb[2];
+  const int per = (ntile + 7) / 8;
+  const int t = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (t >= ntile) return;
+  const int bz = t % nbz, by = (t / nbz) % nby, bxc = t / (nbz * nby);
 
   const Layout &L = a.L;
   const int tz = threadIdx.x, ty = threadIdx.y;
-  const int y = a.lo[1] + (int)blockIdx.y * (IT_Y - 4) - 2 + ty;
-  const int z = a.lo[2] + (int)blockIdx.x * (IT_Z - 4) - 2 + tz;
-  const int x0 = a.lo[0] + (int)blockIdx.z * IX_CHUNK;
+  const int ylo = a.lo[1] - 2 + by * (TY - 4), zlo = a.lo[2] - 2 + bz * (TZ - 4);
+  const int y = ylo + ty, z = zlo + tz;
+  const int x0 = a.lo[0] + bxc * IX_CHUNK;
   const int x1 = min(x0 + IX_CHUNK - 1, a.hi[0]);
-  // loads stay inside the interior: y, z >= lo-2 >= 0; clamp the far side
+  const int nsteps = x1 - x0 + 5;   // psi^k planes x0-2 .. x1+2
+  // loads stay inside the domain: y, z >= lo-2 >= 0; clamp the far side
   const int yl = min(y, L.n[1] - 1), zl = min(z, L.n[2] - 1);
-  const int ym1 = max(ty - 1, 0), yp1 = min(ty + 1, IT_Y - 1);
-  const int zm1 = max(tz - 1, 0), zp1 = min(tz + 1, IT_Z - 1);
   const long sx = L.st[0];
-  const long col = L.org + yl * L.st[1] + zl;   // (x = 0, y, z)
-  const bool out_ok = ty >= 2 && ty < IT_Y - 2 && tz >= 2 && tz < IT_Z - 2 &&
-                      y <= a.hi[1] && z <= a.hi[2];
+  const long base0 = L.org + (long)(x0 - 2) * sx + (long)ylo * L.st[1] + zlo;
+  const unsigned toff = (unsigned)(((long)(yl - ylo) * L.st[1] + (zl - zlo)) * (long)sizeof(R));
+  const bool out_ok = ty >= 2 && ty < TY - 2 && tz >= 2 && tz < TZ - 2 && y <= a.hi[1] && z <= a.hi[2];
+  // running byte addresses of the tile corner: loads at the prefetched plane,
+  // V stores at plane q-1, psi^{k+1} stores at plane q-2 (plane 0 = x0-2)
+  const unsigned long long sxb = (unsigned long long)sx * sizeof(R);
+  auto at = [&](const R *arr) { return (unsigned long long)(arr + base0); };
+  unsigned long long g_ps = at(a.psi), g_pn = FCT ? at(a.psin) : 0ull, g_ux = at(a.U0), g_uy = at(a.U1),
+                     g_uz = at(a.U2);
+  unsigned long long g_out = at(a.out) - 2 * sxb;
+  unsigned long long g_v0 = STOREV ? at(a.V0) - sxb : 0ull, g_v1 = STOREV ? at(a.V1) - sxb : 0ull,
+                     g_v2 = STOREV ? at(a.V2) - sxb : 0ull;
+  auto LD = [&](unsigned long long g) -> R { return *reinterpret_cast<const R *>(g + toff); };
+  auto ST = [&](unsigned long long g, R v) { *reinterpret_cast<R *>(g + toff) = v; };
 
-  R ps_m2 = 0, ps_m1 = 0, ps_0 = 0;     // own psi^k at planes p-2, p-1, p
-  R pn_m2 = 0, pn_m1 = 0, pn_0 = 0;     // own psi^n
-  R vx_m2 = 0;                          // V_x on the face owned by p-2
-  R fpx_m2 = 0;                         // its unlimited donor flux
-  R bu_m2 = 0, bd_m2 = 0;               // beta on plane p-2
-  R fx_m3 = 0;                          // limited flux on the face owned by p-3
+  // zero the padding rows (read only on behalf of frame cells)
+  const int tid = ty * TZ + tz;
+  for (int i = tid; i < IL_N * 2 * TZ; i += TY * TZ) {
+    const int pl = i / (2 * TZ), r = (i / TZ) & 1, c = i % TZ;
+    lds[pl * PL + r * (TY + 1) * TZ + c] = R(0);
+  }
+  const int ti = (ty + 1) * TZ + tz;
+  auto S = [&](int plane, int dy, int dz) -> R & { return lds[plane * PL + ti + dy * TZ + dz]; };
 
-  // prefetched plane x0-2
-  long off = col + (long)(x0 - 2) * sx;
-  R n_ps = a.psi[off], n_pn = FCT ? a.psin[off] : R(0);
-  R n_ux = a.U0[off], n_uy = a.U1[off], n_uz = a.U2[off];
+  R ps[3] = {0, 0, 0}, pn[3] = {0, 0, 0}, ux[3] = {0, 0, 0}, uy[2] = {0, 0}, uz[2] = {0, 0};
+  R vx_[2] = {0, 0}, fpx_[2] = {0, 0}, bu_[2] = {0, 0}, bd_[2] = {0, 0}, fx_[2] = {0, 0};
+  R n_ps = LD(g_ps), n_pn = FCT ? LD(g_pn) : R(0);
+  R n_ux = LD(g_ux), n_uy = LD(g_uy), n_uz = LD(g_uz);
 
-  for (int p = x0 - 2; p <= x1 + 2; ++p) {
+  auto step = [&](auto PHc, const int q) __attribute__((always_inline)) {
+    constexpr int PH = decltype(PHc)::value;
+    constexpr int s0 = PH % 3, sm1 = (PH + 2) % 3, sm2 = (PH + 1) % 3;
+    constexpr int u0 = PH & 1, um1 = u0 ^ 1;
+
     // ---- stage 1: publish plane p, prefetch plane p+1
-    ps_m2 = ps_m1; ps_m1 = ps_0; ps_0 = n_ps;
-    const int s0 = p % 3, sm1 = (p + 2) % 3, sm2 = (p + 1) % 3;
-    const int u0 = p & 1, um1 = (p + 1) & 1;
-    s_psi[s0][ty][tz] = ps_0;
-    s_ux[s0][ty][tz] = n_ux;
-    s_uy[u0][ty][tz] = n_uy;
-    s_uz[u0][ty][tz] = n_uz;
+    ps[s0] = n_ps; ux[s0] = n_ux; uy[u0] = n_uy; uz[u0] = n_uz;
+    S(IL_PSI + s0, 0, 0) = n_ps;
+    S(IL_UX + s0, 0, 0) = n_ux;
+    S(IL_UY + u0, 0, 0) = n_uy;
+    S(IL_UZ + u0, 0, 0) = n_uz;
     if (FCT) {
-      pn_m2 = pn_m1; pn_m1 = pn_0; pn_0 = n_pn * a.in_scale;
-      s_pn[u0][ty][tz] = pn_0;
+      pn[s0] = n_pn * a.in_scale;
+      S(IL_PN + u0, 0, 0) = pn[s0];
     }
-    if (p + 1 <= x1 + 2) {
-      off = col + (long)(p + 1) * sx;
-      n_ps = a.psi[off];
-      if (FCT) n_pn = a.psin[off];
-      n_ux = a.U0[off]; n_uy = a.U1[off]; n_uz = a.U2[off];
+    if (q + 1 < nsteps) {
+      g_ps += sxb; g_ux += sxb; g_uy += sxb; g_uz += sxb;
+      n_ps = LD(g_ps);
+      if (FCT) { g_pn += sxb; n_pn = LD(g_pn); }
+      n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
     }
     __syncthreads();
 
     // ---- stage 2: velocity on plane p-1 (V_x on the face owned by p-1)
     R vx = 0, vy = 0, vz = 0;
-    if (p >= x0 - 1) {
-      const int q = p - 1;
+    if (q >= 1) {
       {
-        // V_x at face q+1/2: normal x, tangential (y, z)
+        // V_x at face x+1/2: normal x (planes p-1, p), tangential (y, z)
         auto P = [&](int da, int db, int dc) -> R {
-          const int sl = da == 0 ? sm1 : s0;
-          const int yy = db > 0 ? yp1 : (db < 0 ? ym1 : ty);
-          const int zz = dc > 0 ? zp1 : (dc < 0 ? zm1 : tz);
-          return s_psi[sl][yy][zz];
+          if (db == 0 && dc == 0) return da == 0 ? ps[sm1] : ps[s0];
+          return da == 0 ? S(IL_PSI + sm1, db, dc) : S(IL_PSI + s0, db, dc);
         };
-        const R g1 = (s_uy[u0][ty][tz] + s_uy[um1][ty][tz] + s_uy[u0][ym1][tz] + s_uy[um1][ym1][tz]) / 4;
-        const R g2 = (s_uz[u0][ty][tz] + s_uz[um1][ty][tz] + s_uz[u0][ty][zm1] + s_uz[um1][ty][zm1]) / 4;
-        vx = antidiff_core<R, G, 3, IGA, false, false>(s_ux[sm1][ty][tz], g1, g2, P);
+        const R g1 = (uy[u0] + uy[um1] + S(IL_UY + u0, -1, 0) + S(IL_UY + um1, -1, 0)) / 4;
+        const R g2 = (uz[u0] + uz[um1] + S(IL_UZ + u0, 0, -1) + S(IL_UZ + um1, 0, -1)) / 4;
+        vx = antidiff_core<R, G, 3, IGA, false, false>(ux[sm1], g1, g2, P);
       }
-      if (p >= x0) {
+      if (q >= 2) {
         {
-          // V_y at face y+1/2 of plane q: normal y, tangential (z, x)
+          // V_y at face y+1/2 of plane p-1: normal y, tangential (z, x)
           auto P = [&](int da, int db, int dc) -> R {
-            const int yy = da > 0 ? yp1 : ty;
-            const int zz = db > 0 ? zp1 : (db < 0 ? zm1 : tz);
-            const int sl = dc > 0 ? s0 : (dc < 0 ? sm2 : sm1);
-            return s_psi[sl][yy][zz];
+            if (da == 0 && db == 0) return dc > 0 ? ps[s0] : (dc < 0 ? ps[sm2] : ps[sm1]);
+            const int pl = dc > 0 ? s0 : (dc < 0 ? sm2 : sm1);
+            return S(IL_PSI + pl, da, db);
           };
-          const R g1 = (s_uz[um1][yp1][tz] + s_uz[um1][ty][tz] + s_uz[um1][yp1][zm1] + s_uz[um1][ty][zm1]) / 4;
-          const R g2 = (s_ux[sm1][yp1][tz] + s_ux[sm1][ty][tz] + s_ux[sm2][yp1][tz] + s_ux[sm2][ty][tz]) / 4;
-          vy = antidiff_core<R, G, 3, IGA, false, false>(s_uy[um1][ty][tz], g1, g2, P);
+          const R g1 = (S(IL_UZ + um1, 1, 0) + uz[um1] + S(IL_UZ + um1, 1, -1) + S(IL_UZ + um1, 0, -1)) / 4;
+          const R g2 = (S(IL_UX + sm1, 1, 0) + ux[sm1] + S(IL_UX + sm2, 1, 0) + ux[sm2]) / 4;
+          vy = antidiff_core<R, G, 3, IGA, false, false>(uy[um1], g1, g2, P);
         }
         {
-          // V_z at face z+1/2 of plane q: normal z, tangential (x, y)
+          // V_z at face z+1/2 of plane p-1: normal z, tangential (x, y)
           auto P = [&](int da, int db, int dc) -> R {
-            const int zz = da > 0 ? zp1 : tz;
-            const int sl = db > 0 ? s0 : (db < 0 ? sm2 : sm1);
-            const int yy = dc > 0 ? yp1 : (dc < 0 ? ym1 : ty);
-            return s_psi[sl][yy][zz];
+            if (da == 0 && dc == 0) return db > 0 ? ps[s0] : (db < 0 ? ps[sm2] : ps[sm1]);
+            const int pl = db > 0 ? s0 : (db < 0 ? sm2 : sm1);
+            return S(IL_PSI + pl, dc, da);
           };
-          const R g1 = (s_ux[sm1][ty][zp1] + s_ux[sm1][ty][tz] + s_ux[sm2][ty][zp1] + s_ux[sm2][ty][tz]) / 4;
-          const R g2 = (s_uy[um1][ty][zp1] + s_uy[um1][ty][tz] + s_uy[um1][ym1][zp1] + s_uy[um1][ym1][tz]) / 4;
-          vz = antidiff_core<R, G, 3, IGA, false, false>(s_uz[um1][ty][tz], g1, g2, P);
+          const R g1 = (S(IL_UX + sm1, 0, 1) + ux[sm1] + S(IL_UX + sm2, 0, 1) + ux[sm2]) / 4;
+          const R g2 = (S(IL_UY + um1, 0, 1) + uy[um1] + S(IL_UY + um1, -1, 1) + S(IL_UY + um1, -1, 0)) / 4;
+          vz = antidiff_core<R, G, 3, IGA, false, false>(uz[um1], g1, g2, P);
         }
-        s_vy[ty][tz] = vy;
-        s_vz[ty][tz] = vz;
+        S(IL_VY, 0, 0) = vy;
+        S(IL_VZ, 0, 0) = vz;
       }
-      if (STOREV && out_ok && q >= x0 && q <= x1) {
-        const long c = col + (long)q * sx;
-        a.V0[c] = vx;
-        a.V1[c] = vy;
-        a.V2[c] = vz;
+      if (STOREV && out_ok && q - 1 >= 2 && q - 1 <= nsteps - 3) {
+        ST(g_v0, vx);
+        ST(g_v1, vy);
+        ST(g_v2, vz);
       }
     }
+    if (STOREV) { g_v0 += sxb; g_v1 += sxb; g_v2 += sxb; }
     __syncthreads();
 
     // ---- stage 3: beta on plane p-1 (FCT)
     R bu = 0, bd = 0;
-    const R fpx = IGA ? vx : donor_flux<false>(ps_m1, ps_0, vx);   // unlimited flux, face owned by p-1
-    if (FCT && p >= x0) {
-      const R pc = ps_m1;
-      const R yp = s_psi[sm1][yp1][tz], ym = s_psi[sm1][ym1][tz];
-      const R zp = s_psi[sm1][ty][zp1], zm = s_psi[sm1][ty][zm1];
-      R pmin, pmax;
-      bounds_core<R, 3>(pmin, pmax, pn_m1, pn_0, pn_m2, s_pn[um1][yp1][tz], s_pn[um1][ym1][tz],
-                        s_pn[um1][ty][zp1], s_pn[um1][ty][zm1]);
-      const R vym = s_vy[ym1][tz], vzm = s_vz[ty][zm1];
-      const R fyp = IGA ? vy : donor_flux<false>(pc, yp, vy);
-      const R fym = IGA ? vym : donor_flux<false>(ym, pc, vym);
-      const R fzp = IGA ? vz : donor_flux<false>(pc, zp, vz);
-      const R fzm = IGA ? vzm : donor_flux<false>(zm, pc, vzm);
-      beta_core<R, G, 3, false>(bu, bd, pc, ps_0, ps_m2, yp, ym, zp, zm, pmin, pmax, fpx, fpx_m2,
-                                fyp, fym, fzp, fzm);
-      s_bu[ty][tz] = bu;
-      s_bd[ty][tz] = bd;
+    const R fpx = IGA ? vx : iflux(ps[sm1], ps[s0], vx);   // unlimited flux, face owned by p-1
+    R yp = 0, zp = 0;
+    if (FCT && q >= 2) {
+      const R pc = ps[sm1];
+      yp = S(IL_PSI + sm1, 1, 0);
+      zp = S(IL_PSI + sm1, 0, 1);
+      const R ym = S(IL_PSI + sm1, -1, 0), zm = S(IL_PSI + sm1, 0, -1);
+      const R pmax = fmx(fmx(fmx(pn[sm1], pn[s0]), fmx(pn[sm2], S(IL_PN + um1, 1, 0))),
+                         fmx(S(IL_PN + um1, -1, 0), fmx(S(IL_PN + um1, 0, 1), S(IL_PN + um1, 0, -1))));
+      const R pmin = fmn(fmn(fmn(pn[sm1], pn[s0]), fmn(pn[sm2], S(IL_PN + um1, 1, 0))),
+                         fmn(S(IL_PN + um1, -1, 0), fmn(S(IL_PN + um1, 0, 1), S(IL_PN + um1, 0, -1))));
+      const R vym = S(IL_VY, -1, 0), vzm = S(IL_VZ, 0, -1);
+      const R fyp = IGA ? vy : iflux(pc, yp, vy);
+      const R fym = IGA ? vym : iflux(ym, pc, vym);
+      const R fzp = IGA ? vz : iflux(pc, zp, vz);
+      const R fzm = IGA ? vzm : iflux(zm, pc, vzm);
+      ibeta<R, G>(bu, bd, pc, ps[s0], ps[sm2], yp, ym, zp, zm, pmin, pmax, fpx, fpx_[um1],
+                  fyp, fym, fzp, fzm);
+      S(IL_BU, 0, 0) = bu;
+      S(IL_BD, 0, 0) = bd;
     }
     if (FCT) __syncthreads();
 
     // ---- stage 4: limited fluxes; psi^{k+1} on plane p-2
-    if (p >= x0 + 1) {
+    if (q >= 3) {
       // x face owned by p-2 (between planes p-2 and p-1)
-      const R mx = FCT ? mono_core<R, false, IGA>(vx_m2, bu_m2, bd_m2, bu, bd, ps_m2, ps_m1) : vx_m2;
-      const R fx = IGA ? mx : donor_flux<false>(ps_m2, ps_m1, mx);
-      // y and z faces of plane p-1
-      const int f1 = (p + 1) & 1;   // slot of plane p-1
-      const R pyp = s_psi[sm1][yp1][tz], pzp = s_psi[sm1][ty][zp1];
-      R my = vy, mz = vz;
-      if (FCT) {
-        my = mono_core<R, false, IGA>(vy, bu, bd, s_bu[yp1][tz], s_bd[yp1][tz], ps_m1, pyp);
-        mz = mono_core<R, false, IGA>(vz, bu, bd, s_bu[ty][zp1], s_bd[ty][zp1], ps_m1, pzp);
+      const R mx = FCT ? imono(vx_[um1], bu_[um1], bd_[um1], bu, bd) : vx_[um1];
+      const R fx = IGA ? mx : iflux(ps[sm2], ps[sm1], mx);
+      // y and z faces of plane p-1 (ring slot um1); plane p-2 is slot u0
+      if (!FCT) {
+        yp = S(IL_PSI + sm1, 1, 0);
+        zp = S(IL_PSI + sm1, 0, 1);
       }
-      s_fy[f1][ty][tz] = IGA ? my : donor_flux<false>(ps_m1, pyp, my);
-      s_fz[f1][ty][tz] = IGA ? mz : donor_flux<false>(ps_m1, pzp, mz);
-      if (p >= x0 + 2 && out_ok) {
-        const int f2 = p & 1;       // slot of plane p-2
-        const R r = ps_m2 + ((-fx + fx_m3) + (-s_fy[f2][ty][tz] + s_fy[f2][ym1][tz])
-                             + (-s_fz[f2][ty][tz] + s_fz[f2][ty][zm1])) / 1;
-        a.out[col + (long)(p - 2) * sx] = r * a.out_scale;
+      const R my = FCT ? imono(vy, bu, bd, S(IL_BU, 1, 0), S(IL_BD, 1, 0)) : vy;
+      const R mz = FCT ? imono(vz, bu, bd, S(IL_BU, 0, 1), S(IL_BD, 0, 1)) : vz;
+      S(IL_FY + um1, 0, 0) = IGA ? my : iflux(ps[sm1], yp, my);
+      S(IL_FZ + um1, 0, 0) = IGA ? mz : iflux(ps[sm1], zp, mz);
+      if (q >= 4 && out_ok) {
+        const R r = ps[sm2] + ((-fx + fx_[um1]) + (-S(IL_FY + u0, 0, 0) + S(IL_FY + u0, -1, 0))
+                               + (-S(IL_FZ + u0, 0, 0) + S(IL_FZ + u0, 0, -1))) / 1;
+        ST(g_out, r * a.out_scale);
       }
-      fx_m3 = fx;
+      fx_[u0] = fx;
     }
-    vx_m2 = vx;
-    fpx_m2 = fpx;
-    bu_m2 = bu;
-    bd_m2 = bd;
+    g_out += sxb;
+    vx_[u0] = vx;
+    fpx_[u0] = fpx;
+    bu_[u0] = bu;
+    bd_[u0] = bd;
+  };
+
+  for (int q = 0; q < nsteps; q += 6) {
+    step(ic<0>(), q);
+    if (q + 1 >= nsteps) break;
+    step(ic<1>(), q + 1);
+    if (q + 2 >= nsteps) break;
+    step(ic<2>(), q + 2);
+    if (q + 3 >= nsteps) break;
+    step(ic<3>(), q + 3);
+    if (q + 4 >= nsteps) break;
+    step(ic<4>(), q + 4);
+    if (q + 5 >= nsteps) break;
+    step(ic<5>(), q + 5);
   }
 }
 

@@ -34,9 +34,8 @@ template <class G, class R> __device__ inline R frac_rt(unsigned o, R n, R d)
 template <class R>
 __global__ void k_fill(R *a, Rot r, Box bx, int side, int mode, int N)
 {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int b, c, m;
-  if (!box_idx(bx, t, b, c, m)) return;
+  if (!box_idx(bx, b, c, m)) return;
   long dst, src;
   if (side == 0) { dst = rr(r, -m, b, c); src = rr(r, mode == 0 ? N - 1 - m : 0, b, c); }
   else           { dst = rr(r, N - 1 + m, b, c); src = rr(r, mode == 0 ? m : N - 1, b, c); }
@@ -74,9 +73,8 @@ template <class R>
 __global__ void k_alng(R *avd, const R *av1, const R *av2, Rot r, Box bx, int nd,
                        int side, int cyclic, int N)
 {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int ii, b, c;
-  if (!box_idx(bx, t, ii, b, c)) return;
+  if (!box_idx(bx, ii, b, c)) return;
   if (cyclic) {
     const int m = ii;
     if (side == 0) avd[rr(r, -m, b, c)] = avd[rr(r, N - 1 - m, b, c)];
@@ -107,9 +105,8 @@ __global__ void k_alng(R *avd, const R *av1, const R *av2, Rot r, Box bx, int nd
 template <class R>
 __global__ void k_flux(R *out, const R *psi, const R *C, Rot r, Box bx, unsigned o)
 {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int a, b, c;
-  if (!box_idx(bx, t, a, b, c)) return;
+  if (!box_idx(bx, a, b, c)) return;
   const long p0 = rr(r, a, b, c);
   const R cc = C[p0];
   out[p0] = pos_rt(o, cc) * psi[p0] + neg_rt(o, cc) * psi[rr(r, a + 1, b, c)];
@@ -120,9 +117,8 @@ template <class R>
 __global__ void k_donorcell(R *psi_new, const R *psi_old, const R *f0, const R *f1,
                             const R *f2, Layout L, Box bx, unsigned o)
 {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int i, j, k;
-  if (!box_idx(bx, t, i, j, k)) return;
+  if (!box_idx(bx, i, j, k)) return;
   const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
   R f[6];
   f[0] = f0[p]; f[1] = f0[p - L.st[0]];
@@ -156,9 +152,8 @@ template <class R, class G>
 __global__ void k_antidiff(R *V, const R *psi, const R *U0, const R *U1, const R *U2,
                            Rot r, Box bx, int nd, unsigned o)
 {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int a, b, c;
-  if (!box_idx(bx, t, a, b, c)) return;
+  if (!box_idx(bx, a, b, c)) return;
   const bool iga = o & O_IGA, ab = o & O_ABS;
   auto P = [&](int x, int y, int z) -> R {
     R v = psi[rr(r, x, y, z)];
@@ -199,9 +194,8 @@ __global__ void k_antidiff(R *V, const R *psi, const R *U0, const R *U1, const R
 template <class R>
 __global__ void k_fct_init(R *pmin, R *pmax, const R *psi, Layout L, Box bx)
 {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int i, j, k;
-  if (!box_idx(bx, t, i, j, k)) return;
+  if (!box_idx(bx, i, j, k)) return;
   const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
   const long s0 = L.st[0], s1 = L.st[1], s2 = L.st[2];
   if (L.nd == 3) {
@@ -221,9 +215,8 @@ template <class R, class G>
 __global__ void k_betas(R *bup, R *bdn, const R *psi, const R *pmin, const R *pmax,
                         const R *f0, const R *f1, const R *f2, Layout L, Box bx, unsigned o)
 {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int i, j, k;
-  if (!box_idx(bx, t, i, j, k)) return;
+  if (!box_idx(bx, i, j, k)) return;
   const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
   const long s0 = L.st[0], s1 = L.st[1], s2 = L.st[2];
   R up_nom, dn_nom, up_den, dn_den;
@@ -257,9 +250,8 @@ template <class R>
 __global__ void k_gc_mono(R *M, const R *V, const R *psi, const R *bup, const R *bdn,
                           Rot r, Box bx, unsigned o)
 {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int a, b, c;
-  if (!box_idx(bx, t, a, b, c)) return;
+  if (!box_idx(bx, a, b, c)) return;
   const long p0 = rr(r, a, b, c), p1 = rr(r, a + 1, b, c);
   const R v = V[p0];
   const R m_dn_up = smin(R(1), smin(bdn[p0], bup[p1]));
@@ -276,9 +268,8 @@ __global__ void k_gc_mono(R *M, const R *V, const R *psi, const R *bup, const R 
 template <class R>
 __global__ void k_scale(R *a, Layout L, Box bx, R factor)
 {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int i, j, k;
-  if (!box_idx(bx, t, i, j, k)) return;
+  if (!box_idx(bx, i, j, k)) return;
   a[L.org + i * L.st[0] + j * L.st[1] + k * L.st[2]] *= factor;
 }
 
@@ -286,9 +277,8 @@ __global__ void k_scale(R *a, Layout L, Box bx, R factor)
 template <class R>
 __global__ void k_div(R *stat, const R *g0, const R *g1, const R *g2, Layout L, Box bx)
 {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int i, j, k;
-  if (!box_idx(bx, t, i, j, k)) return;
+  if (!box_idx(bx, i, j, k)) return;
   const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
   R v;
   if (L.nd == 3)
@@ -296,6 +286,31 @@ __global__ void k_div(R *stat, const R *g0, const R *g1, const R *g2, Layout L, 
   else
     v = fabs((g0[p] - g0[p - L.st[0]]) + (g1[p] - g1[p - L.st[1]])) / 1;
   stat[p] = v;
+}
+
+// reversing deformational flow from a streamfunction on cell corners
+// (the analytic advector of the reference's reversing_deform test family,
+// tests/mp3_paper_2018_JCP/reversing_deform/reversing_deform.hpp:30-105):
+// corner value psi(a, b) = Q(amp * sx2[a] * sy2[b]) with sx2[a] =
+// sin^2(pi (a + 1/2)/(n0 - 1)) (sy2 likewise along y, both tabulated on the
+// host from a = -2) and Q rounding to a multiple of 2^-17;
+// GC_x = psi(i, j) - psi(i, j-1), GC_y = psi(i-1, j) - psi(i, j), GC_z = 0
+__device__ inline double deform_corner(int a, int b, const double *sx2, const double *sy2, double amp)
+{
+  return rint(amp * sx2[a + 2] * sy2[b + 2] * 131072.0) / 131072.0;
+}
+
+template <class R>
+__global__ void k_gen_deform(R *g0, R *g1, R *g2, Layout L, Box bx, double amp, const double *sx2,
+                             const double *sy2)
+{
+  int i, j, k;
+  if (!box_idx(bx, i, j, k)) return;
+  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
+  const double c = deform_corner(i, j, sx2, sy2, amp);
+  g0[p] = R(c - deform_corner(i, j - 1, sx2, sy2, amp));
+  g1[p] = R(deform_corner(i - 1, j, sx2, sy2, amp) - c);
+  if (L.nd == 3) g2[p] = R(0);
 }
 
 template <class R>
@@ -311,9 +326,8 @@ template <class R>
 __global__ void k_copy_box(R *dst, long dorg, long ds0, long ds1, long ds2,
                            const R *src, long sorg, long ss0, long ss1, long ss2, Box bx)
 {
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int i, j, k;
-  if (!box_idx(bx, t, i, j, k)) return;
+  if (!box_idx(bx, i, j, k)) return;
   dst[dorg + i * ds0 + j * ds1 + k * ds2] = src[sorg + i * ss0 + j * ss1 + k * ss2];
 }
 
@@ -324,17 +338,21 @@ __global__ void k_reduce(double2 *part, const R *a, Layout L, Box bx, int op)
 {
   __shared__ double2 sh[256];
   double acc = op == 0 ? INFINITY : (op == 1 ? -INFINITY : 0.0), cc = 0.0;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < bx.total;
-       t += (long)gridDim.x * blockDim.x) {
-    int i, j, k;
-    box_idx(bx, t, i, j, k);
-    const double v = double(a[L.org + i * L.st[0] + j * L.st[1] + k * L.st[2]]);
-    if (op == 0) acc = smin(acc, v);
-    else if (op == 1) acc = smax(acc, v);
-    else {
-      const double y = v - cc, s = acc + y;
-      cc = (s - acc) - y;
-      acc = s;
+  // rows (a, b) block-cyclic over the grid, cells c strided over the block
+  const unsigned rows = (unsigned)bx.ext[0] * (unsigned)bx.ext[1];
+  for (unsigned row = blockIdx.x; row < rows; row += gridDim.x) {
+    const unsigned qa = fdiv(row, bx.d1);
+    const int i = (int)qa + bx.lo[0], j = (int)(row - qa * (unsigned)bx.ext[1]) + bx.lo[1];
+    const long base = L.org + i * L.st[0] + j * L.st[1];
+    for (int kk = threadIdx.x; kk < bx.ext[2]; kk += blockDim.x) {
+      const double v = double(a[base + (kk + bx.lo[2]) * L.st[2]]);
+      if (op == 0) acc = smin(acc, v);
+      else if (op == 1) acc = smax(acc, v);
+      else {
+        const double y = v - cc, s = acc + y;
+        cc = (s - acc) - y;
+        acc = s;
+      }
     }
   }
   sh[threadIdx.x] = make_double2(acc, cc);

@@ -43,6 +43,14 @@ struct mpdx_solver {
   int active_path = MPDX_PATH_PERPASS;
   std::string err;
   mpdx_fused_state *fz = nullptr;
+  // one captured time step per time-level parity (fused path)
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  bool use_graphs = true;
+  mpdx_calc_gc_fn calc_gc = nullptr;
+  void *calc_gc_user = nullptr;
+  int gen_kind = MPDX_GC_NONE;
+  double gen_T = 1, gen_scale = 0;
+  double *gen_sx2 = nullptr, *gen_sy2 = nullptr;   // device sin^2 tables from index -2
   bool prof_on = false;
   std::vector<std::string> prof_order;
   std::map<std::string, mpdx_prof_entry> prof;

@@ -74,6 +74,7 @@ def _lib(fast: bool = False):
         lib.orc_panic_ptr.argtypes = [P]
         lib.orc_full_shape.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         lib.orc_get_full.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
+        lib.orc_set_generator.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         _libs[name] = lib
     return _libs[name]
 
@@ -150,6 +151,11 @@ class Oracle:
         out = np.empty(self.advector_shape(d), dtype=np.float64)
         self._lib.orc_get_advector(self._h, d, _dp(out))
         return out
+
+    def set_generator(self, kind, T=1.0, max_courant=0.5):
+        """kind 1: reversing deformational flow recomputed every step (0: off)."""
+        if self._lib.orc_set_generator(self._h, int(kind), float(T), float(max_courant)):
+            raise ValueError("bad advector generator parameters")
 
     def advance(self, nt):
         err = ctypes.create_string_buffer(256)

@@ -45,6 +45,9 @@ struct orc {
   long timestep;
   double time;
   int panic;
+  int gen_kind;               /* 0 none, 1 reversing deformational flow */
+  double gen_T, gen_scale;
+  double *gen_sx2, *gen_sy2;  /* sin^2 at corners a + 1/2 from a = -2 */
 };
 
 static inline rng R_(int f, int l) { rng r = {f, l}; return r; }
@@ -550,6 +553,62 @@ static void seterr(char *err, int len, const char *msg)
   if (err && len > 0) { strncpy(err, msg, len - 1); err[len - 1] = 0; }
 }
 
+/* Time-varying advector of the reversing_deform test family
+ * (tests/mp3_paper_2018_JCP/reversing_deform/reversing_deform.hpp:64-105):
+ * the discrete curl of the streamfunction sin^2(pi x) sin^2(pi y) cos(pi t/T)
+ * sampled at cell corners x = (a + 1/2)/(n0 - 1), y = (b + 1/2)/(n1 - 1),
+ * each corner value rounded to a multiple of 2^-17 (so the discrete
+ * divergence is exactly zero and the same values come out of any correctly
+ * rounded evaluation), written over i, j = -1 .. n-1 at t + dt/2 and
+ * followed by the advector halo refill of calc_gc_staggered (:98-100). */
+static double deform_corner(const orc_t *s, int a, int b, double amp)
+{
+  return rint(amp * s->gen_sx2[a + 2] * s->gen_sy2[b + 2] * 131072.0) / 131072.0;
+}
+
+static void calc_gc_deform(orc_t *s)
+{
+  const double t_half = s->time + 0.5 * s->p.dt;
+  const double amp = s->gen_scale * cos(3.141592653589793 * t_half / s->gen_T);
+  const int k0 = 0, k1 = s->nd == 3 ? s->n[2] - 1 : 0;
+  for (int i = -1; i <= s->n[0] - 1; ++i)
+    for (int j = -1; j <= s->n[1] - 1; ++j)
+      for (int k = k0; k <= k1; ++k) {
+        const long p = ix(s, i, j, k);
+        const double c = deform_corner(s, i, j, amp);
+        s->GC[0][p] = c - deform_corner(s, i, j - 1, amp);
+        s->GC[1][p] = deform_corner(s, i - 1, j, amp) - c;
+        if (s->nd == 3) s->GC[2][p] = 0;
+      }
+  xchng_vctr_alng(s, s->GC, 0);
+  xchng_vctr_nrml(s, s->GC, s->hw - 1);
+}
+
+int orc_set_generator(orc_t *s, int kind, double T, double max_courant)
+{
+  if (kind == 0) { s->gen_kind = 0; return 0; }
+  if (kind != 1 || !(T > 0) || !(max_courant > 0)) return 1;
+  const int n0 = s->n[0], n1 = s->n[1];
+  const double pi = 3.141592653589793;
+  free(s->gen_sx2);
+  free(s->gen_sy2);
+  s->gen_sx2 = (double *)malloc(sizeof(double) * (size_t)(n0 + 2));
+  s->gen_sy2 = (double *)malloc(sizeof(double) * (size_t)(n1 + 2));
+  for (int a = -2; a < n0; ++a) { const double v = sin(pi * (a + 0.5) / (n0 - 1)); s->gen_sx2[a + 2] = v * v; }
+  for (int b = -2; b < n1; ++b) { const double v = sin(pi * (b + 0.5) / (n1 - 1)); s->gen_sy2[b + 2] = v * v; }
+  double mx = 0;
+  for (int a = -1; a < n0; ++a)
+    for (int b = -1; b < n1; ++b) {
+      const double c = s->gen_sx2[a + 2] * s->gen_sy2[b + 2];
+      mx = dmax(mx, fabs(c - s->gen_sx2[a + 2] * s->gen_sy2[b + 1]));
+      mx = dmax(mx, fabs(s->gen_sx2[a + 1] * s->gen_sy2[b + 2] - c));
+    }
+  s->gen_kind = kind;
+  s->gen_T = T;
+  s->gen_scale = mx > 0 ? max_courant / mx : 0;
+  return 0;
+}
+
 /* solve(): solver_common.hpp:270-358 */
 int orc_advance(orc_t *s, long nt, char *err, int errlen)
 {
@@ -568,6 +627,8 @@ int orc_advance(orc_t *s, long nt, char *err, int errlen)
   }
   while (s->timestep < nt) {
     if (s->panic) break;
+    /* calc_gc() at the top of each step (solver_common.hpp:306-308) */
+    if (s->gen_kind == 1) calc_gc_deform(s);
     for (int e = 0; e < s->p.n_eqns; ++e) {
       scale(s, e, s->p.hint_scale[e]);
       xchng_sclr(s, s->psi[e][s->lev[e]], s->hw);
@@ -642,6 +703,7 @@ void orc_destroy(orc_t *s)
     free(s->GC[d]); free(s->tmp[0][d]); free(s->tmp[1][d]); free(s->flux[d]); free(s->mono[d]);
   }
   free(s->psi_min); free(s->psi_max); free(s->beta_up); free(s->beta_dn);
+  free(s->gen_sx2); free(s->gen_sy2);
   free(s);
 }
 

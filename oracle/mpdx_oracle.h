@@ -54,6 +54,11 @@ int orc_get_advector(const orc_t *s, int d, double *dense);
 /* runs nt more time steps; nonzero return = error (message in err) */
 int orc_advance(orc_t *s, long nt, char *err, int errlen);
 
+/* time-varying advector recomputed at the start of every step (the
+ * reversing_deform calc_gc pattern): kind 1 = reversing deformational flow
+ * with period parameter T and largest |GC| = max_courant; kind 0 = none */
+int orc_set_generator(orc_t *s, int kind, double T, double max_courant);
+
 long orc_timestep(const orc_t *s);
 double orc_time(const orc_t *s);
 int *orc_panic_ptr(orc_t *s);

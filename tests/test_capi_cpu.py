@@ -54,3 +54,17 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(M, "_lib", None)
     with pytest.raises(RuntimeError, match="not built"):
         M.lib()
+
+
+def test_box_index_fast_division(tmp_path):
+    """The box kernels split flat plane indices with a multiply-high division
+    by the box's run-time extent; its host half is compiled with g++ here and
+    checked against exact integer division over divisors 1..4096, powers of
+    two and their neighbours and large odd values, for dividends up to 2^31-1."""
+    import subprocess
+    exe = str(tmp_path / "fastdiv_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "libmpdataxx_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "cpp", "fastdiv_check.cpp"), "-o", exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
+    assert "0 mismatches" in r.stdout

@@ -161,3 +161,65 @@ def test_basic_mpdata_second_order():
     assert 1.7 < order < 2.5, order
     u1, u2 = _smooth_error(33, 1), _smooth_error(65, 1)
     assert np.log2(u1 / u2) < 1.3       # upwind is first order
+
+
+def _deform_oracle(grid, bcs, opts, iters, psi, T, courant, dt=1.0):
+    o = O.Oracle(grid, opts=opts, n_iters=iters, bcs=bcs, dt=dt)
+    o.set_advectee(psi)
+    for d in range(len(grid)):
+        o.set_advector(np.zeros(o.advector_shape(d)), d)
+    o.set_generator(1, T, courant)
+    return o
+
+
+def test_deform_generator_divergence_free_and_scaled():
+    """The generated advector is exactly divergence-free over the unique
+    cells of a cyclic grid, and its largest |GC| at t + dt/2 is
+    max_courant * cos(pi (t + dt/2) / T) up to the 2^-17 corner rounding."""
+    grid = (25, 21, 6)
+    bcs = [(O.CYCLIC, O.CYCLIC)] * 3
+    T, c = 30.0, 0.4
+    o = _deform_oracle(grid, bcs, O.FCT, 3, np.ones(grid), T, c)
+    for step in range(3):
+        o.advance(1)
+        g = [o.advector(d) for d in range(3)]
+        n0, n1, n2 = grid
+        # advector(0) index (i, j) <-> (x face i - 1/2, y cell j); advector(1)
+        # index (i, j) <-> (x cell i - 1, y face j + 1/2), the view extents of
+        # the reference's advector(); cells i = 0..n0-2, j = 1..n1-2
+        div = ((g[0][1:n0, 1:n1 - 1, :] - g[0][:n0 - 1, 1:n1 - 1, :])
+               + (g[1][1:n0, 1:n1 - 1, :] - g[1][1:n0, :n1 - 2, :]))
+        assert np.array_equal(div, np.zeros_like(div))
+        assert np.array_equal(g[2], np.zeros_like(g[2]))
+        want = c * np.cos(np.pi * (step + 0.5) / T)
+        got = max(np.abs(g[0]).max(), np.abs(g[1]).max())
+        assert abs(got - want) <= 2.0 ** -15, (got, want)
+
+
+def test_deform_generator_keeps_constant_tracer():
+    """A constant tracer stays constant under the time-varying flow (the
+    constant-tracer invariant of the multi-tracer workload)."""
+    grid = (25, 21, 6)
+    bcs = [(O.CYCLIC, O.CYCLIC)] * 3
+    o = _deform_oracle(grid, bcs, O.FCT, 3, np.ones(grid), 30.0, 0.45)
+    o.advance(12)
+    assert np.abs(o.advectee() - 1).max() <= 1e-13
+
+
+def test_deform_generator_reverses():
+    """cos(pi t / T) reverses the deformation: after one full period T the
+    field is much closer to the initial one than at T/2."""
+    n = 49
+    grid = (n, n)
+    bcs = [(O.CYCLIC, O.CYCLIC)] * 2
+    x = np.arange(n) / (n - 1)
+    psi = 1 + np.exp(-((x[:, None] - 0.35) ** 2 + (x[None, :] - 0.5) ** 2) / (2 * 0.08 ** 2))
+    psi = periodic_consistent(psi, [0, 1])
+    T = 48.0
+    o = _deform_oracle(grid, bcs, O.FCT, 3, psi, T, 0.45)
+    o.advance(24)
+    half = np.abs(o.advectee() - psi).max()
+    o.advance(24)
+    full = np.abs(o.advectee() - psi).max()
+    assert half > 0.3
+    assert full < 0.3 * half, (full, half)
