@@ -36,6 +36,7 @@ ALG_WORDS = {
     "fused_interior_v": (8, "interior"),      # psi^k, U x3 in; psi^k+1, V x3 out
     "fused_interior": (5, "interior"),        # psi^k, U x3 in; psi^k+1 out
     "fused_donor": (5, "all"),                # psi^n, GC x3 in; psi^1 out
+    "fused_psi_halo": (0, "all"),             # halo copies only
     "pp_antidiff": (5, "all"),                # psi, U_d, U_d+1, U_d+2 in; V_d out
     "pp_flux": (3, "all"),
     "pp_donorcell": (5, "all"),

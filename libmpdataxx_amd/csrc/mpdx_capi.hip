@@ -6,6 +6,7 @@
 // kernel launches on one HIP stream, and exposes the advectee/advector views.
 #include "mpdx.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
@@ -512,6 +513,27 @@ struct Fused {
     return dim3(gx, L.n[0] + extra, 1);
   }
 
+  // materialize the scalar halo of psi (the reference's xchng_sclr before
+  // every pass, mpdata_osc_3d.hpp:73-202 via solver_common.hpp:127-136)
+  void psi_halo(R *a)
+  {
+    ProfScope ps(s, "fused_psi_halo");
+    const int hw = L.hw;
+    const int n0 = L.n[0], n1 = L.n[1], n2 = L.nd == 3 ? L.n[2] : 1;
+    const int zl = L.nd == 3 ? -hw : 0, zh = L.nd == 3 ? n2 - 1 + hw : 0;
+    struct { Box b[6]; int nb; } hb;
+    hb.nb = 0;
+    hb.b[hb.nb++] = make_box(-hw, -1, -hw, n1 - 1 + hw, zl, zh);
+    hb.b[hb.nb++] = make_box(n0, n0 - 1 + hw, -hw, n1 - 1 + hw, zl, zh);
+    hb.b[hb.nb++] = make_box(0, n0 - 1, -hw, -1, zl, zh);
+    hb.b[hb.nb++] = make_box(0, n0 - 1, n1, n1 - 1 + hw, zl, zh);
+    if (L.nd == 3) {
+      hb.b[hb.nb++] = make_box(0, n0 - 1, 0, n1 - 1, -hw, -1);
+      hb.b[hb.nb++] = make_box(0, n0 - 1, 0, n1 - 1, n2, n2 - 1 + hw);
+    }
+    for (int q = 0; q < hb.nb; ++q) blaunch(s->stream, hb.b[q], k_fz_halo<R>, a, L, m, hb.b[q]);
+  }
+
   void donor_box(R *out, R *in, R **C, R in_scale, R out_scale, int flux_is_C, const Box &bx)
   {
     if (bx.total <= 0) return;
@@ -519,10 +541,10 @@ struct Fused {
     const dim3 g = box_grid(bx);
     if (o & O_NPA)
       hipLaunchKernelGGL((k_fz_donor<R, true>), g, dim3(256), 0, s->stream, out, (const R *)in,
-                         (const R *)C[0], (const R *)C[1], (const R *)c2, L, m, in_scale, out_scale, flux_is_C, bx);
+                         (const R *)C[0], (const R *)C[1], (const R *)c2, L, in_scale, out_scale, flux_is_C, bx);
     else
       hipLaunchKernelGGL((k_fz_donor<R, false>), g, dim3(256), 0, s->stream, out, (const R *)in,
-                         (const R *)C[0], (const R *)C[1], (const R *)c2, L, m, in_scale, out_scale, flux_is_C, bx);
+                         (const R *)C[0], (const R *)C[1], (const R *)c2, L, in_scale, out_scale, flux_is_C, bx);
   }
 
   void donor(R *out, R *in, R **C, R in_scale, R out_scale, int flux_is_C)
@@ -537,9 +559,14 @@ struct Fused {
   {
     if (bx.total <= 0) return;
     const dim3 g = box_grid(bx);
-    hipLaunchKernelGGL((k_fz_antidiff<R, G, IGA, ABS, PFC>), g, dim3(256), 0, s->stream,
-                       V[0], V[1], V[L.nd == 3 ? 2 : 0], (const R *)psi, (const R *)U[0], (const R *)U[1],
-                       (const R *)U[L.nd == 3 ? 2 : 0], L, m, bx);
+    if (L.nd == 3)
+      hipLaunchKernelGGL((k_fz_antidiff<R, G, 3, IGA, ABS, PFC>), g, dim3(256), 0, s->stream,
+                         V[0], V[1], V[2], (const R *)psi, (const R *)U[0], (const R *)U[1],
+                         (const R *)U[2], L, bx);
+    else
+      hipLaunchKernelGGL((k_fz_antidiff<R, G, 2, IGA, ABS, PFC>), g, dim3(256), 0, s->stream,
+                         V[0], V[1], V[0], (const R *)psi, (const R *)U[0], (const R *)U[1],
+                         (const R *)U[0], L, bx);
   }
 
   void antidiff_box(R **V, R *psi, R **U, const Box &bx)
@@ -559,7 +586,7 @@ struct Fused {
     dim3 g(ND == 3 ? (bx.ext[2] + TZ - 1) / TZ : 1, (bx.ext[1] + TY - 1) / TY, (bx.ext[0] + TX - 1) / TX);
     hipLaunchKernelGGL((k_fz_fct<R, G, ND, SHAPE, NPA, ABS, IGA>), g, dim3(256), 0, s->stream, out,
                        (const R *)psi, (const R *)psin, (const R *)V[0], (const R *)V[1],
-                       (const R *)V[ND == 3 ? 2 : 0], L, m, in_scale, out_scale, bx);
+                       (const R *)V[ND == 3 ? 2 : 0], L, in_scale, out_scale, bx);
   }
 
   // brick oriented along the box's thinnest dimension
@@ -674,6 +701,7 @@ struct Fused {
     const bool last = it == passes - 1;
     const bool storev = !last;   // the next pass reads this velocity as GC_unco
     const bool use_int = interior_supported(s->p, L);
+    psi_halo(psik);
     // antidiffusive velocity on the boundary shell (or everywhere)
     {
       ProfScope ps(s, "fused_antidiff");
@@ -725,6 +753,7 @@ struct Fused {
     auto out_of = [&](int it) { return ((passes - 1 - it) % 2 == 0) ? fin : tmp; };
     R *Gb[3];
     R **gc = pp.av_of(s->GC, Gb);
+    psi_halo(psin);
     donor(out_of(0), psin, gc, in_scale, passes == 1 ? out_scale : R(1), 0);
     for (int it = 1; it < passes; ++it)
       corrective(out_of(it), out_of(it - 1), psin, it, passes, in_scale, it == passes - 1 ? out_scale : R(1));
@@ -895,8 +924,18 @@ int mpdx_create(const mpdx_params *p, mpdx_solver **out)
   L.hw = hw;
   for (int d = 0; d < 3; ++d) L.n[d] = d < L.nd ? p->grid[d] : 1;
   const int align = 128 / s->esz;            // elements per 128 bytes
-  const int zoff = align;                    // >= hw, keeps index 0 aligned
   const int inner = L.n[L.nd - 1];
+  // Offset of index 0 within a row: the boundary kernels touch the strips
+  // [-2, 4] and [n-5, n+1] of the contiguous dimension on every row, so
+  // place both inside single 128-byte lines where the row length allows it
+  // (halves their line traffic); index 0 must stay >= hw from the row start.
+  int zoff = align;
+  for (int u = 0; u + 7 <= align; ++u) {
+    if ((u + inner - 3) % align + 7 <= align) {
+      zoff = u + 2;
+      break;
+    }
+  }
   const long pitch = ((long)zoff + inner + hw + align - 1) / align * align;
   if (L.nd == 3) {
     L.st[2] = 1;

@@ -2,9 +2,10 @@
 //
 // Arrays live in HBM as halo-extended boxes: dimension d spans
 // [-hw, n_d-1+hw] (scalars) or the face range [-hw, n_d-2+hw] of a
-// staggered component (face index i <-> i+1/2, formulae/arakawa_c.hpp:17-41),
-// with the innermost dimension padded so that element (.., .., 0) of every
-// row sits on a 128-byte boundary.  Element (i,j,k) is at
+// staggered component (face index i <-> i+1/2, formulae/arakawa_c.hpp:17-41).
+// Rows of the innermost dimension start on 128-byte boundaries, with index 0
+// placed so that the boundary strips [-2, 4] and [n-5, n+1] each fall in one
+// 128-byte line where possible.  Element (i,j,k) is at
 // org + i*st[0] + j*st[1] + k*st[2]; in 2D st[2] == 0.
 #pragma once
 

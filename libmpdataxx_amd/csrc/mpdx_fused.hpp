@@ -44,43 +44,68 @@ __device__ inline int map_idx(const BcMap &m, int d, int i)
   return i;
 }
 
+// Scalar halo of width hw, materialized in one launch: every halo cell takes
+// the value of the interior cell that the reference's xchng_sclr
+// (solver_3d.hpp:47-67, solver_2d.hpp:40-56) leaves there.  Its fills run
+// dimension by dimension over ranges widened by the halo, so a halo cell
+// ends up with the composition of the per-dimension rules: cyclic wrap by
+// N-1 (bcond/cyclic_3d.hpp:29-33) or open edge copy (bcond/open_3d.hpp:
+// 32-42).  Sources are interior cells, so the launch has no ordering hazard.
+// The host launches one box per slab: x slabs over the full extended (y, z)
+// range, y slabs over interior x, z slabs over interior (x, y).
 template <class R>
-__device__ inline R psi_at(const R *psi, const Layout &L, const BcMap &m, int i, int j, int k)
+__global__ void __launch_bounds__(256) k_fz_halo(R *a, Layout L, BcMap m, Box bx)
 {
-  i = map_idx(m, 0, i);
-  j = map_idx(m, 1, j);
-  if (L.nd == 3) k = map_idx(m, 2, k);
-  return psi[L.org + i * L.st[0] + j * L.st[1] + k * L.st[2]];
+  int i, j, k;
+  if (!box_idx(bx, i, j, k)) return;
+  const int si = map_idx(m, 0, i), sj = map_idx(m, 1, j), sk = L.nd == 3 ? map_idx(m, 2, k) : 0;
+  a[L.org + i * L.st[0] + j * L.st[1] + k * L.st[2]] = a[L.org + si * L.st[0] + sj * L.st[1] + sk * L.st[2]];
+}
+
+// element index of cell (i, j, k); the (j, k) part in 32-bit arithmetic
+// (a plane of the halo-extended array has fewer than 2^31 elements)
+__device__ __forceinline__ long cell_index(const Layout &L, int i, int j, int k)
+{
+  return L.org + (long)i * L.st[0] + (long)(j * (int)L.st[1] + k * (int)L.st[2]);
+}
+
+// A at the cell (dx, dy, dz) away from element p (constant offsets fold into
+// stride multiples); cells up to hw outside the domain come from the
+// materialized halo
+template <class R>
+__device__ __forceinline__ R rel(const R *A, long p, const Layout &L, int dx, int dy, int dz)
+{
+  return A[p + dx * L.st[0] + dy * L.st[1] + dz * L.st[2]];
 }
 
 // ---------------------------------------------------------------- donor-cell pass
 
 template <class R, bool NPA>
 __global__ void __launch_bounds__(256)
-k_fz_donor(R *out, const R *in, const R *C0, const R *C1, const R *C2, Layout L, BcMap m,
+k_fz_donor(R *out, const R *in, const R *C0, const R *C1, const R *C2, Layout L,
            R in_scale, R out_scale, int flux_is_C, Box bx)
 {
   // cells of the box bx (last coordinate fastest)
   int i, j, k;
   if (!box_idx(bx, i, j, k)) return;
-  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
-  const R pc = psi_at(in, L, m, i, j, k) * in_scale;
+  const long p = cell_index(L, i, j, k);
+  const R pc = in[p] * in_scale;
   R f[6];
   if (flux_is_C) {
-    f[0] = C0[p]; f[1] = C0[p - L.st[0]];
-    f[2] = C1[p]; f[3] = C1[p - L.st[1]];
-    if (L.nd == 3) { f[4] = C2[p]; f[5] = C2[p - L.st[2]]; }
+    f[0] = C0[p]; f[1] = rel(C0, p, L, -1, 0, 0);
+    f[2] = C1[p]; f[3] = rel(C1, p, L, 0, -1, 0);
+    if (L.nd == 3) { f[4] = C2[p]; f[5] = rel(C2, p, L, 0, 0, -1); }
   } else {
-    const R xp = psi_at(in, L, m, i + 1, j, k) * in_scale, xm = psi_at(in, L, m, i - 1, j, k) * in_scale;
-    const R yp = psi_at(in, L, m, i, j + 1, k) * in_scale, ym = psi_at(in, L, m, i, j - 1, k) * in_scale;
+    const R xp = rel(in, p, L, 1, 0, 0) * in_scale, xm = rel(in, p, L, -1, 0, 0) * in_scale;
+    const R yp = rel(in, p, L, 0, 1, 0) * in_scale, ym = rel(in, p, L, 0, -1, 0) * in_scale;
     f[0] = donor_flux<NPA>(pc, xp, C0[p]);
-    f[1] = donor_flux<NPA>(xm, pc, C0[p - L.st[0]]);
+    f[1] = donor_flux<NPA>(xm, pc, rel(C0, p, L, -1, 0, 0));
     f[2] = donor_flux<NPA>(pc, yp, C1[p]);
-    f[3] = donor_flux<NPA>(ym, pc, C1[p - L.st[1]]);
+    f[3] = donor_flux<NPA>(ym, pc, rel(C1, p, L, 0, -1, 0));
     if (L.nd == 3) {
-      const R zp = psi_at(in, L, m, i, j, k + 1) * in_scale, zm = psi_at(in, L, m, i, j, k - 1) * in_scale;
+      const R zp = rel(in, p, L, 0, 0, 1) * in_scale, zm = rel(in, p, L, 0, 0, -1) * in_scale;
       f[4] = donor_flux<NPA>(pc, zp, C2[p]);
-      f[5] = donor_flux<NPA>(zm, pc, C2[p - L.st[2]]);
+      f[5] = donor_flux<NPA>(zm, pc, rel(C2, p, L, 0, 0, -1));
     }
   }
   R r;
@@ -125,52 +150,49 @@ __device__ inline R antidiff_core(R u, R g1, R g2, PF P)
   return fabs(u) / 2 * (1 - fabs(u) / 1) * ndx - u * g1 / (2 * 1) * ndy;
 }
 
-// V_d at face (a+1/2 along d) reading psi through the halo index map
-template <class R, class G, bool IGA, bool ABS, bool PFC>
+// V_D at face (a+1/2 along D) of the cell at element p: the rotation
+// idxperm::pi<D> (formulae/idxperm.hpp:96-125) maps (a, b, c) to
+// (x, y, z), (y, z, x), (z, x, y) in 3D and (x, y), (y, x) in 2D; Ud, U1, U2
+// are GC_unco[D], [D+1], [D+2]
+template <class R, class G, int ND, int D, bool IGA, bool ABS, bool PFC>
 __device__ inline R antidiff_at(const R *psi, const R *Ud, const R *U1, const R *U2, const Layout &L,
-                                const BcMap &m, int d, int i, int j, int k)
+                                long p)
 {
-  int e[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-  const int *ea = e[d];
-  const int *eb = e[(d + 1) % L.nd];
-  const int *ec = e[L.nd == 3 ? (d + 2) % 3 : 2];
-  auto P = [&](int da, int db, int dc) -> R {
-    return psi_at(psi, L, m, i + da * ea[0] + db * eb[0] + dc * ec[0],
-                  j + da * ea[1] + db * eb[1] + dc * ec[1],
-                  k + da * ea[2] + db * eb[2] + dc * ec[2]);
+  // cell offset of the rotated (a, b, c) offset
+  auto off = [&](const R *A, int da, int db, int dc) -> R {
+    if (ND == 2) return D == 0 ? rel(A, p, L, da, db, 0) : rel(A, p, L, db, da, 0);
+    if (D == 0) return rel(A, p, L, da, db, dc);
+    if (D == 1) return rel(A, p, L, dc, da, db);
+    return rel(A, p, L, db, dc, da);
   };
-  const long sA = L.st[d], sB = L.st[(d + 1) % L.nd], sC = L.nd == 3 ? L.st[(d + 2) % 3] : 0;
-  const long p0 = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
-  const R g1 = (U1[p0 + sA] + U1[p0] + U1[p0 + sA - sB] + U1[p0 - sB]) / 4;
-  if (L.nd == 3) {
-    const R g2 = (U2[p0 + sA] + U2[p0] + U2[p0 + sA - sC] + U2[p0 - sC]) / 4;
-    return antidiff_core<R, G, 3, IGA, ABS, PFC>(Ud[p0], g1, g2, P);
+  auto P = [&](int da, int db, int dc) -> R { return off(psi, da, db, dc); };
+  const R g1 = (off(U1, 1, 0, 0) + U1[p] + off(U1, 1, -1, 0) + off(U1, 0, -1, 0)) / 4;
+  if (ND == 3) {
+    const R g2 = (off(U2, 1, 0, 0) + U2[p] + off(U2, 1, 0, -1) + off(U2, 0, 0, -1)) / 4;
+    return antidiff_core<R, G, 3, IGA, ABS, PFC>(Ud[p], g1, g2, P);
   }
-  return antidiff_core<R, G, 2, IGA, ABS, PFC>(Ud[p0], g1, R(0), P);
+  return antidiff_core<R, G, 2, IGA, ABS, PFC>(Ud[p], g1, R(0), P);
 }
 
 // all components over the box [-1, n-1]^nd; component d is written where the
 // other coordinates are interior (the reference's im x j x k ranges)
-template <class R, class G, bool IGA, bool ABS, bool PFC>
+template <class R, class G, int ND, bool IGA, bool ABS, bool PFC>
 __global__ void __launch_bounds__(256)
 k_fz_antidiff(R *V0, R *V1, R *V2, const R *psi, const R *U0, const R *U1, const R *U2,
-              Layout L, BcMap m, Box bx)
+              Layout L, Box bx)
 {
   // bx: sub-box of the owner cells [-1, n-1]^nd (last coordinate fastest)
   int i, j, k;
   if (!box_idx(bx, i, j, k)) return;
-  const bool in0 = i >= 0, in1 = j >= 0, in2 = L.nd == 2 || k >= 0;
-  const long p0 = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
-  const R *U[3] = {U0, U1, U2};
-  R *V[3] = {V0, V1, V2};
-  const bool inside[3] = {in0, in1, in2};
-  for (int d = 0; d < L.nd; ++d) {
-    bool ok = true;
-    for (int q = 0; q < L.nd; ++q)
-      if (q != d && !inside[q]) ok = false;
-    if (!ok) continue;
-    const int d1 = (d + 1) % L.nd, d2 = L.nd == 3 ? (d + 2) % 3 : 0;
-    V[d][p0] = antidiff_at<R, G, IGA, ABS, PFC>(psi, U[d], U[d1], U[d2], L, m, d, i, j, k);
+  const bool in0 = i >= 0, in1 = j >= 0, in2 = ND == 2 || k >= 0;
+  const long p = cell_index(L, i, j, k);
+  if (ND == 3) {
+    if (in1 && in2) V0[p] = antidiff_at<R, G, 3, 0, IGA, ABS, PFC>(psi, U0, U1, U2, L, p);
+    if (in2 && in0) V1[p] = antidiff_at<R, G, 3, 1, IGA, ABS, PFC>(psi, U1, U2, U0, L, p);
+    if (in0 && in1) V2[p] = antidiff_at<R, G, 3, 2, IGA, ABS, PFC>(psi, U2, U0, U1, L, p);
+  } else {
+    if (in1) V0[p] = antidiff_at<R, G, 2, 0, IGA, ABS, PFC>(psi, U0, U1, U1, L, p);
+    if (in0) V1[p] = antidiff_at<R, G, 2, 1, IGA, ABS, PFC>(psi, U1, U0, U0, L, p);
   }
 }
 
@@ -244,36 +266,35 @@ __device__ inline R mono_core(R v, R bul, R bdl, R bur, R bdr, R psl, R psr)
 
 template <class R, class G, bool NPA, bool ABS>
 __device__ inline void beta_at(R &bup, R &bdn, const R *psi, const R *psin, const R *V0, const R *V1,
-                               const R *V2, const Layout &L, const BcMap &m, int i, int j, int k,
-                               R in_scale, bool iga)
+                               const R *V2, const Layout &L, int i, int j, int k, R in_scale, bool iga)
 {
   const int nd = L.nd;
-  auto P = [&](int a, int b, int c) { return psi_at(psi, L, m, a, b, c); };
-  auto Q = [&](int a, int b, int c) { return psi_at(psin, L, m, a, b, c) * in_scale; };
-  const R pc = P(i, j, k);
-  const R xp = P(i + 1, j, k), xm = P(i - 1, j, k), yp = P(i, j + 1, k), ym = P(i, j - 1, k);
+  const long p = cell_index(L, i, j, k);
+  auto P = [&](int a, int b, int c) { return rel(psi, p, L, a, b, c); };
+  auto Q = [&](int a, int b, int c) { return rel(psin, p, L, a, b, c) * in_scale; };
+  const R pc = psi[p];
+  const R xp = P(1, 0, 0), xm = P(-1, 0, 0), yp = P(0, 1, 0), ym = P(0, -1, 0);
   R zp = 0, zm = 0;
-  if (nd == 3) { zp = P(i, j, k + 1); zm = P(i, j, k - 1); }
+  if (nd == 3) { zp = P(0, 0, 1); zm = P(0, 0, -1); }
   R pmin, pmax;
   if (nd == 3)
-    bounds_core<R, 3>(pmin, pmax, Q(i, j, k), Q(i + 1, j, k), Q(i - 1, j, k), Q(i, j + 1, k),
-                      Q(i, j - 1, k), Q(i, j, k + 1), Q(i, j, k - 1));
+    bounds_core<R, 3>(pmin, pmax, Q(0, 0, 0), Q(1, 0, 0), Q(-1, 0, 0), Q(0, 1, 0),
+                      Q(0, -1, 0), Q(0, 0, 1), Q(0, 0, -1));
   else
-    bounds_core<R, 2>(pmin, pmax, Q(i, j, k), Q(i + 1, j, k), Q(i - 1, j, k), Q(i, j + 1, k),
-                      Q(i, j - 1, k), R(0), R(0));
-  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
+    bounds_core<R, 2>(pmin, pmax, Q(0, 0, 0), Q(1, 0, 0), Q(-1, 0, 0), Q(0, 1, 0),
+                      Q(0, -1, 0), R(0), R(0));
   R fxp, fxm, fyp, fym, fzp = 0, fzm = 0;
   if (iga) {
-    fxp = V0[p]; fxm = V0[p - L.st[0]]; fyp = V1[p]; fym = V1[p - L.st[1]];
-    if (nd == 3) { fzp = V2[p]; fzm = V2[p - L.st[2]]; }
+    fxp = V0[p]; fxm = rel(V0, p, L, -1, 0, 0); fyp = V1[p]; fym = rel(V1, p, L, 0, -1, 0);
+    if (nd == 3) { fzp = V2[p]; fzm = rel(V2, p, L, 0, 0, -1); }
   } else {
     fxp = donor_flux<NPA>(pc, xp, V0[p]);
-    fxm = donor_flux<NPA>(xm, pc, V0[p - L.st[0]]);
+    fxm = donor_flux<NPA>(xm, pc, rel(V0, p, L, -1, 0, 0));
     fyp = donor_flux<NPA>(pc, yp, V1[p]);
-    fym = donor_flux<NPA>(ym, pc, V1[p - L.st[1]]);
+    fym = donor_flux<NPA>(ym, pc, rel(V1, p, L, 0, -1, 0));
     if (nd == 3) {
       fzp = donor_flux<NPA>(pc, zp, V2[p]);
-      fzm = donor_flux<NPA>(zm, pc, V2[p - L.st[2]]);
+      fzm = donor_flux<NPA>(zm, pc, rel(V2, p, L, 0, 0, -1));
     }
   }
   if (nd == 3) beta_core<R, G, 3, NPA>(bup, bdn, pc, xp, xm, yp, ym, zp, zm, pmin, pmax, fxp, fxm, fyp, fym, fzp, fzm);
@@ -283,7 +304,7 @@ __device__ inline void beta_at(R &bup, R &bdn, const R *psi, const R *psin, cons
 template <class R, class G, int ND, int SHAPE, bool NPA, bool ABS, bool IGA>
 __global__ void __launch_bounds__(256)
 k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R *V2, Layout L,
-         BcMap m, R in_scale, R out_scale, Box bx)
+         R in_scale, R out_scale, Box bx)
 {
   constexpr int TX = FzBrick<ND, SHAPE>::TX, TY = FzBrick<ND, SHAPE>::TY, TZ = FzBrick<ND, SHAPE>::TZ;
   constexpr int BX = TX + 2, BY = TY + 2, BZ = ND == 3 ? TZ + 2 : 1;
@@ -298,7 +319,7 @@ k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R 
     R bu = 0, bd = 0;
     const bool inb = i >= -1 && i <= L.n[0] && j >= -1 && j <= L.n[1] &&
                      (ND == 2 || (k >= -1 && k <= L.n[2]));
-    if (inb) beta_at<R, G, NPA, ABS>(bu, bd, psi, psin, V0, V1, V2, L, m, i, j, k, in_scale, IGA);
+    if (inb) beta_at<R, G, NPA, ABS>(bu, bd, psi, psin, V0, V1, V2, L, i, j, k, in_scale, IGA);
     sbu[q] = bu;
     sbd[q] = bd;
   }
@@ -310,32 +331,32 @@ k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R 
   auto B = [&](const R *sb, int di, int dj, int dk) -> R {
     return sb[((tx + 1 + di) * BY + (ty + 1 + dj)) * BZ + (ND == 3 ? tz + 1 + dk : 0)];
   };
-  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
-  const R pc = psi_at(psi, L, m, i, j, k);
+  const long p = cell_index(L, i, j, k);
+  const R pc = psi[p];
   const R bu0 = B(sbu, 0, 0, 0), bd0 = B(sbd, 0, 0, 0);
   auto mono = [&](R v, R bul, R bdl, R bur, R bdr, R psl, R psr) -> R {
     return mono_core<R, ABS, IGA>(v, bul, bdl, bur, bdr, psl, psr);
   };
   R f[6];
   {
-    const R xp = psi_at(psi, L, m, i + 1, j, k), xm = psi_at(psi, L, m, i - 1, j, k);
+    const R xp = rel(psi, p, L, 1, 0, 0), xm = rel(psi, p, L, -1, 0, 0);
     const R mp = mono(V0[p], bu0, bd0, B(sbu, 1, 0, 0), B(sbd, 1, 0, 0), pc, xp);
-    const R mm = mono(V0[p - L.st[0]], B(sbu, -1, 0, 0), B(sbd, -1, 0, 0), bu0, bd0, xm, pc);
+    const R mm = mono(rel(V0, p, L, -1, 0, 0), B(sbu, -1, 0, 0), B(sbd, -1, 0, 0), bu0, bd0, xm, pc);
     f[0] = IGA ? mp : donor_flux<NPA>(pc, xp, mp);
     f[1] = IGA ? mm : donor_flux<NPA>(xm, pc, mm);
   }
   {
-    const R yp = psi_at(psi, L, m, i, j + 1, k), ym = psi_at(psi, L, m, i, j - 1, k);
+    const R yp = rel(psi, p, L, 0, 1, 0), ym = rel(psi, p, L, 0, -1, 0);
     const R mp = mono(V1[p], bu0, bd0, B(sbu, 0, 1, 0), B(sbd, 0, 1, 0), pc, yp);
-    const R mm = mono(V1[p - L.st[1]], B(sbu, 0, -1, 0), B(sbd, 0, -1, 0), bu0, bd0, ym, pc);
+    const R mm = mono(rel(V1, p, L, 0, -1, 0), B(sbu, 0, -1, 0), B(sbd, 0, -1, 0), bu0, bd0, ym, pc);
     f[2] = IGA ? mp : donor_flux<NPA>(pc, yp, mp);
     f[3] = IGA ? mm : donor_flux<NPA>(ym, pc, mm);
   }
   R r;
   if (ND == 3) {
-    const R zp = psi_at(psi, L, m, i, j, k + 1), zm = psi_at(psi, L, m, i, j, k - 1);
+    const R zp = rel(psi, p, L, 0, 0, 1), zm = rel(psi, p, L, 0, 0, -1);
     const R mp = mono(V2[p], bu0, bd0, B(sbu, 0, 0, 1), B(sbd, 0, 0, 1), pc, zp);
-    const R mm = mono(V2[p - L.st[2]], B(sbu, 0, 0, -1), B(sbd, 0, 0, -1), bu0, bd0, zm, pc);
+    const R mm = mono(rel(V2, p, L, 0, 0, -1), B(sbu, 0, 0, -1), B(sbd, 0, 0, -1), bu0, bd0, zm, pc);
     f[4] = IGA ? mp : donor_flux<NPA>(pc, zp, mp);
     f[5] = IGA ? mm : donor_flux<NPA>(zm, pc, mm);
     r = pc + ((-f[0] + f[1]) + (-f[2] + f[3]) + (-f[4] + f[5])) / 1;
