@@ -35,6 +35,10 @@ ALG_WORDS = {
     "fused_interior_fct": (6, "interior"),    # psi^k, U x3, psi^n in; psi^k+1 out
     "fused_interior_v": (8, "interior"),      # psi^k, U x3 in; psi^k+1, V x3 out
     "fused_interior": (5, "interior"),        # psi^k, U x3 in; psi^k+1 out
+    "fused_first_fct_v": (8, "interior"),     # psi^n, GC x3 in; psi^2, V x3 out (psi^1 on chip)
+    "fused_first_fct": (5, "interior"),       # psi^n, GC x3 in; psi^2 out
+    "fused_first_v": (8, "interior"),
+    "fused_first": (5, "interior"),
     "fused_donor": (5, "all"),                # psi^n, GC x3 in; psi^1 out
     "fused_psi_halo": (0, "all"),             # halo copies only
     "pp_antidiff": (5, "all"),                # psi, U_d, U_d+1, U_d+2 in; V_d out

@@ -660,7 +660,7 @@ struct Fused {
     return make_box(f, L.n[0] - 1, f, L.n[1] - 1, L.nd == 3 ? f : 0, L.nd == 3 ? L.n[2] - 1 : 0);
   }
 
-  template <bool FCT, bool IGA, bool STOREV>
+  template <bool FCT, bool IGA, bool STOREV, bool FUSE0>
   void interior_t(R *out, R *psi, R *psin, R **U, R **V, R in_scale, R out_scale)
   {
     InteriorArgs<R> a;
@@ -676,23 +676,31 @@ struct Fused {
     a.nb[2] = (L.n[0] - 4 + IX_CHUNK - 1) / IX_CHUNK;
     const int ntile = a.nb[0] * a.nb[1] * a.nb[2];
     const dim3 g(8 * ((ntile + 7) / 8));   // tiles are spread over the 8 XCDs in the kernel
-    hipLaunchKernelGGL((k_fz_interior<R, G, FCT, IGA, STOREV>), g, dim3(TZ, TY), 0, s->stream, a);
+    hipLaunchKernelGGL((k_fz_interior<R, G, FCT, IGA, STOREV, FUSE0>), g, dim3(TZ, TY), 0, s->stream, a);
   }
 
-  void interior(R *out, R *psi, R *psin, R **U, R **V, bool storev, R in_scale, R out_scale)
+  // psi: psi^k, or unused with fuse0 (psi^1 computed from psin and U = GC)
+  void interior(R *out, R *psi, R *psin, R **U, R **V, bool storev, bool fuse0, R in_scale, R out_scale)
   {
-    ProfScope ps(s, (o & O_FCT) ? (storev ? "fused_interior_fct_v" : "fused_interior_fct")
-                                : (storev ? "fused_interior_v" : "fused_interior"));
+    ProfScope ps(s, fuse0 ? ((o & O_FCT) ? (storev ? "fused_first_fct_v" : "fused_first_fct")
+                                          : (storev ? "fused_first_v" : "fused_first"))
+                          : ((o & O_FCT) ? (storev ? "fused_interior_fct_v" : "fused_interior_fct")
+                                         : (storev ? "fused_interior_v" : "fused_interior")));
     const bool fct = o & O_FCT, iga = o & O_IGA;
-#define MPDX_INT(F, I, S) if (fct == F && iga == I && storev == S) interior_t<F, I, S>(out, psi, psin, U, V, in_scale, out_scale)
-    MPDX_INT(true, false, true); else MPDX_INT(true, false, false); else MPDX_INT(true, true, true);
-    else MPDX_INT(true, true, false); else MPDX_INT(false, false, true); else MPDX_INT(false, false, false);
-    else MPDX_INT(false, true, true); else MPDX_INT(false, true, false);
+#define MPDX_INT(F, I, S, Z) \
+  if (fct == F && iga == I && storev == S && fuse0 == Z) interior_t<F, I, S, Z>(out, psi, psin, U, V, in_scale, out_scale)
+#define MPDX_INT2(F, I, S) MPDX_INT(F, I, S, false); else MPDX_INT(F, I, S, true)
+    MPDX_INT2(true, false, true); else MPDX_INT2(true, false, false); else MPDX_INT2(true, true, true);
+    else MPDX_INT2(true, true, false); else MPDX_INT2(false, false, true); else MPDX_INT2(false, false, false);
+    else MPDX_INT2(false, true, true); else MPDX_INT2(false, true, false);
+#undef MPDX_INT2
 #undef MPDX_INT
   }
 
-  // one corrective pass (iteration it) writing psi^{k+1} into out
-  void corrective(R *out, R *psik, R *psin, int it, int passes, R in_scale, R os)
+  // one corrective pass (iteration it) writing psi^{k+1} into out; with
+  // fuse0 the interior kernel recomputes psi^1 (= psik, present only on the
+  // boundary shell) from psin and GC
+  void corrective(R *out, R *psik, R *psin, int it, int passes, R in_scale, R os, bool fuse0)
   {
     R *Ub[3], *Vb[3];
     R **U = pp.unco(it, Ub);
@@ -733,7 +741,7 @@ struct Fused {
         donor(out, psik, V, R(1), os, (o & O_IGA) ? 1 : 0);
       }
     }
-    if (use_int) interior(out, psik, psin, U, V, storev, in_scale, os);
+    if (use_int) interior(out, psik, psin, U, V, storev, fuse0, in_scale, os);
   }
 
   // one time step of equation e (solver_common.hpp:127-136 with advop inlined)
@@ -754,9 +762,19 @@ struct Fused {
     R *Gb[3];
     R **gc = pp.av_of(s->GC, Gb);
     psi_halo(psin);
-    donor(out_of(0), psin, gc, in_scale, passes == 1 ? out_scale : R(1), 0);
+    // pass 0: with an interior kernel, psi^1 is needed only on the boundary
+    // shell (cells < 5 from a boundary); the first corrective pass's interior
+    // kernel computes its own psi^1
+    const bool fuse0 = passes > 1 && interior_supported(s->p, L);
+    if (fuse0) {
+      ProfScope ps(s, "fused_donor_shell");
+      for (const Box &b : shell(0, 5, 5)) donor_box(out_of(0), psin, gc, in_scale, R(1), 0, b);
+    } else {
+      donor(out_of(0), psin, gc, in_scale, passes == 1 ? out_scale : R(1), 0);
+    }
     for (int it = 1; it < passes; ++it)
-      corrective(out_of(it), out_of(it - 1), psin, it, passes, in_scale, it == passes - 1 ? out_scale : R(1));
+      corrective(out_of(it), out_of(it - 1), psin, it, passes, in_scale, it == passes - 1 ? out_scale : R(1),
+                 fuse0 && it == 1);
     s->lev[e] ^= 1;
   }
 };

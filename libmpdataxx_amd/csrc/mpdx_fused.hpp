@@ -377,17 +377,24 @@ k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R 
 // p = x0 - 2 + q) it computes the velocity on plane p-1, beta on plane p-1,
 // the limited fluxes and writes psi^{k+1} on plane p-2.
 //
-// psi^k and psi^n planes, the in-plane velocities, beta and the y/z fluxes go
-// through LDS rings; a thread's own column stays in register rings.  The
-// march is unrolled by 6 (the period of the 3- and 2-plane rings) so every
-// ring slot is a compile-time constant, and each LDS plane carries a padding
-// row above and below the tile, so every neighbour read is the thread's own
-// LDS address plus an immediate offset (a z neighbour past the tile edge
-// reads the adjacent row or the padding; such values reach only frame
-// cells, whose results are discarded).
+// With FUSE0 (the first corrective pass) the kernel reads psi^n and GC
+// instead of psi^1 and computes psi^1 itself with the donor-cell formula
+// (donorcell_formulae.hpp:105-165): psi^n and GC run one plane ahead, and
+// step q produces psi^1 on plane p+1 after its own output.  The donor needs
+// psi^n and GC_y / GC_z one cell outside the tile, which a few threads load
+// into the LDS frame.
+//
+// psi and psi^n planes, the in-plane advector and velocity components, beta
+// and the y/z fluxes go through LDS rings; a thread's own column stays in
+// registers.  The march is unrolled by 6 (the period of the 3- and 2-plane
+// rings) so every ring slot is a compile-time constant.  Each LDS plane is
+// the tile plus a one-cell frame, so every neighbour read is the thread's
+// own LDS address plus an immediate offset; frame cells that nobody loads
+// hold zeros and reach only frame results, which are discarded.
 template <class R> struct ITile {
   static constexpr int Y = sizeof(R) == 8 ? 16 : 32, Z = 32;
-  static constexpr int PLANE = (Y + 2) * Z;
+  static constexpr int PW = Z + 2, PLANE = (Y + 2) * PW;
+  static constexpr int NFRAME = 3 * Z + 3 * Y;   // FUSE0 frame loads per plane
 };
 constexpr int IX_CHUNK = 64;
 
@@ -395,19 +402,19 @@ constexpr int IX_CHUNK = 64;
 enum {
   IL_PSI = 0,     // psi^k ring, 3 planes
   IL_UX = 3,      // U_x ring, 3
-  IL_UY = 6,      // U_y ring, 2
-  IL_UZ = 8,      // U_z ring, 2
-  IL_PN = 10,     // psi^n ring, 2
-  IL_FY = 12,     // limited y flux ring, 2
-  IL_FZ = 14,     // limited z flux ring, 2
-  IL_VY = 16, IL_VZ = 17, IL_BU = 18, IL_BD = 19,
-  IL_N = 20
+  IL_UY = 6,      // U_y ring, 2 (3 with FUSE0)
+  IL_UZ = 9,      // U_z ring, 2 (3 with FUSE0)
+  IL_PN = 12,     // psi^n ring, 2 (3 with FUSE0)
+  IL_FY = 15,     // limited y flux ring, 2
+  IL_FZ = 17,     // limited z flux ring, 2
+  IL_VY = 19, IL_VZ = 20, IL_BU = 21, IL_BD = 22,
+  IL_N = 23
 };
 
 template <class R>
 struct InteriorArgs {
   R *out;
-  const R *psi, *psin;
+  const R *psi, *psin;  // psi^k (unused with FUSE0) and psi^n
   const R *U0, *U1, *U2;
   R *V0, *V1, *V2;      // unlimited velocity out (next pass's GC_unco), may be null
   Layout L;
@@ -455,12 +462,12 @@ template <class R> __device__ __forceinline__ R iflux(R l, R r, R c)
 
 template <int N> using ic = std::integral_constant<int, N>;
 
-template <class R, class G, bool FCT, bool IGA, bool STOREV>
+template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0>
 __global__ void __launch_bounds__(ITile<R>::Y * ITile<R>::Z)
 k_fz_interior(InteriorArgs<R> a)
 {
-  constexpr int TY = ITile<R>::Y, TZ = ITile<R>::Z, PL = ITile<R>::PLANE;
-  __shared__ R lds[IL_N * PL];
+  constexpr int TY = ITile<R>::Y, TZ = ITile<R>::Z, PW = ITile<R>::PW, PL = ITile<R>::PLANE;
+  __shared__ R lds[IL_N * PL + (FUSE0 ? TY * TZ : 0)];
 
   // XCD-aware tile order: workgroup w runs on XCD w % 8, and XCD x takes the
   // x-th contiguous run of tiles (z fastest), so neighbouring tiles that
@@ -474,63 +481,150 @@ k_fz_interior(InteriorArgs<R> a)
 
   const Layout &L = a.L;
   const int tz = threadIdx.x, ty = threadIdx.y;
+  const int tid = ty * TZ + tz;
   const int ylo = a.lo[1] - 2 + by * (TY - 4), zlo = a.lo[2] - 2 + bz * (TZ - 4);
   const int y = ylo + ty, z = zlo + tz;
   const int x0 = a.lo[0] + bxc * IX_CHUNK;
   const int x1 = min(x0 + IX_CHUNK - 1, a.hi[0]);
   const int nsteps = x1 - x0 + 5;   // psi^k planes x0-2 .. x1+2
-  // loads stay inside the domain: y, z >= lo-2 >= 0; clamp the far side
-  const int yl = min(y, L.n[1] - 1), zl = min(z, L.n[2] - 1);
+  // loads: y, z >= lo-2 >= 0; on the far side rows and columns past the
+  // tile's last output cell are clamped to the first halo row (cell n),
+  // which the FUSE0 donor needs for psi^1 at cell n-1
+  const int yl = min(y, L.n[1]), zl = min(z, L.n[2]);
   const long sx = L.st[0];
   const long base0 = L.org + (long)(x0 - 2) * sx + (long)ylo * L.st[1] + zlo;
   const unsigned toff = (unsigned)(((long)(yl - ylo) * L.st[1] + (zl - zlo)) * (long)sizeof(R));
   const bool out_ok = ty >= 2 && ty < TY - 2 && tz >= 2 && tz < TZ - 2 && y <= a.hi[1] && z <= a.hi[2];
-  // running byte addresses of the tile corner: loads at the prefetched plane,
-  // V stores at plane q-1, psi^{k+1} stores at plane q-2 (plane 0 = x0-2)
+
+  // running byte addresses of the tile corner at the plane each stream reads
+  // next (plane 0 = x0-2); V stores at plane q-1, psi^{k+1} at plane q-2
   const unsigned long long sxb = (unsigned long long)sx * sizeof(R);
-  auto at = [&](const R *arr) { return (unsigned long long)(arr + base0); };
-  unsigned long long g_ps = at(a.psi), g_pn = FCT ? at(a.psin) : 0ull, g_ux = at(a.U0), g_uy = at(a.U1),
-                     g_uz = at(a.U2);
-  unsigned long long g_out = at(a.out) - 2 * sxb;
-  unsigned long long g_v0 = STOREV ? at(a.V0) - sxb : 0ull, g_v1 = STOREV ? at(a.V1) - sxb : 0ull,
-                     g_v2 = STOREV ? at(a.V2) - sxb : 0ull;
+  auto at = [&](const R *arr, int plane) { return (unsigned long long)(arr + base0) + plane * (long long)sxb; };
+  constexpr int AHEAD = FUSE0 ? 1 : 0;   // plane of the first "next" register set
+  unsigned long long g_ps = FUSE0 ? 0ull : at(a.psi, 0), g_pn = FCT || FUSE0 ? at(a.psin, AHEAD) : 0ull;
+  unsigned long long g_ux = at(a.U0, AHEAD), g_uy = at(a.U1, AHEAD), g_uz = at(a.U2, AHEAD);
+  unsigned long long g_out = at(a.out, -2);
+  unsigned long long g_v0 = STOREV ? at(a.V0, -1) : 0ull, g_v1 = STOREV ? at(a.V1, -1) : 0ull,
+                     g_v2 = STOREV ? at(a.V2, -1) : 0ull;
   auto LD = [&](unsigned long long g) -> R { return *reinterpret_cast<const R *>(g + toff); };
   auto ST = [&](unsigned long long g, R v) { *reinterpret_cast<R *>(g + toff) = v; };
 
-  // zero the padding rows (read only on behalf of frame cells)
-  const int tid = ty * TZ + tz;
-  for (int i = tid; i < IL_N * 2 * TZ; i += TY * TZ) {
-    const int pl = i / (2 * TZ), r = (i / TZ) & 1, c = i % TZ;
-    lds[pl * PL + r * (TY + 1) * TZ + c] = R(0);
+  // zero the frame of every LDS plane
+  for (int i = tid; i < IL_N * 2 * (PW + TY); i += TY * TZ) {
+    const int pl = i / (2 * (PW + TY)), r = i % (2 * (PW + TY));
+    int c;
+    if (r < 2 * PW) c = (r < PW ? 0 : (TY + 1) * PW) + r % PW;
+    else c = (1 + (r - 2 * PW) / 2) * PW + ((r - 2 * PW) % 2 ? PW - 1 : 0);
+    lds[pl * PL + c] = R(0);
   }
-  const int ti = (ty + 1) * TZ + tz;
-  auto S = [&](int plane, int dy, int dz) -> R & { return lds[plane * PL + ti + dy * TZ + dz]; };
+  __syncthreads();
+  const int ti = (ty + 1) * PW + tz + 1;
+  auto S = [&](int plane, int dy, int dz) -> R & { return lds[plane * PL + ti + dy * PW + dz]; };
 
-  R ps[3] = {0, 0, 0}, pn[3] = {0, 0, 0}, ux[3] = {0, 0, 0}, uy[2] = {0, 0}, uz[2] = {0, 0};
+  // FUSE0 frame job of this thread: one psi^n, GC_y or GC_z value outside
+  // the tile per plane (rows y = -1 and Y, columns z = -1 and Z of psi^n;
+  // row y = -1 of GC_y; column z = -1 of GC_z).  Threads without a job
+  // reload their own psi^n cell into a spare LDS slot, so the job needs no
+  // branch.
+  R fr_sc = 1;
+  int fr_lds = IL_N * PL + tid;
+  unsigned long long fr_g = at(a.psin, 0) + toff;
+  if (FUSE0 && tid < ITile<R>::NFRAME) {
+    int fy, fz, kind;
+    if (tid < 2 * TZ) { kind = 0; fy = tid < TZ ? -1 : TY; fz = tid % TZ; }
+    else if (tid < 2 * TZ + 2 * TY) { kind = 0; const int r = tid - 2 * TZ; fy = r % TY; fz = r < TY ? -1 : TZ; }
+    else if (tid < 3 * TZ + 2 * TY) { kind = 1; fy = -1; fz = tid - 2 * TZ - 2 * TY; }
+    else { kind = 2; fy = tid - 3 * TZ - 2 * TY; fz = -1; }
+    const int gy = min(ylo + fy, L.n[1]), gz = min(zlo + fz, L.n[2]);
+    const long off = ((long)(gy - ylo) * L.st[1] + (gz - zlo)) * (long)sizeof(R);
+    const R *arr = kind == 0 ? a.psin : (kind == 1 ? a.U1 : a.U2);
+    fr_g = (unsigned long long)((long long)at(arr, 0) + off);
+    fr_lds = (kind == 0 ? IL_PN : (kind == 1 ? IL_UY : IL_UZ)) * PL + (fy + 1) * PW + fz + 1;
+    fr_sc = kind == 0 ? a.in_scale : R(1);
+  }
+  auto FLD = [&](int plane) -> R {
+    return *reinterpret_cast<const R *>(fr_g + plane * (long long)sxb);
+  };
+
+  // register rings (slot = plane index mod 6)
+  R ps[6], pn[6], ux[6], uy[6], uz[6];
   R vx_[2] = {0, 0}, fpx_[2] = {0, 0}, bu_[2] = {0, 0}, bd_[2] = {0, 0}, fx_[2] = {0, 0};
-  R n_ps = LD(g_ps), n_pn = FCT ? LD(g_pn) : R(0);
-  R n_ux = LD(g_ux), n_uy = LD(g_uy), n_uz = LD(g_uz);
+  for (int i = 0; i < 6; ++i) ps[i] = pn[i] = ux[i] = uy[i] = uz[i] = R(0);
+  R n_ps = 0, n_pn = 0, n_ux = 0, n_uy = 0, n_uz = 0, n_fr = 0;
+  const R sc = a.in_scale;
+
+  if (!FUSE0) {
+    n_ps = LD(g_ps);
+    if (FCT) n_pn = LD(g_pn);
+    n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
+  } else {
+    // prologue: psi^1 on plane 0 from psi^n planes -1..1 and GC planes -1, 0
+    const unsigned long long gp0 = at(a.psin, 0);
+    pn[5] = LD(gp0 - sxb) * sc;
+    pn[0] = LD(gp0) * sc;
+    pn[1] = LD(gp0 + sxb) * sc;
+    ux[5] = LD(at(a.U0, -1));
+    ux[0] = LD(at(a.U0, 0));
+    uy[0] = LD(at(a.U1, 0));
+    uz[0] = LD(at(a.U2, 0));
+    S(IL_PN + 0, 0, 0) = pn[0];
+    S(IL_UX + 0, 0, 0) = ux[0];
+    S(IL_UY + 0, 0, 0) = uy[0];
+    S(IL_UZ + 0, 0, 0) = uz[0];
+    lds[fr_lds] = FLD(0) * fr_sc;
+    // plane 1 for the first step's publish
+    n_pn = LD(g_pn); n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
+    fr_g += sxb;
+    n_fr = *reinterpret_cast<const R *>(fr_g);
+    __syncthreads();
+    const R pc = pn[0];
+    const R f0 = iflux(pc, pn[1], ux[0]), f1 = iflux(pn[5], pc, ux[5]);
+    const R f2 = iflux(pc, S(IL_PN, 1, 0), uy[0]), f3 = iflux(S(IL_PN, -1, 0), pc, S(IL_UY, -1, 0));
+    const R f4 = iflux(pc, S(IL_PN, 0, 1), uz[0]), f5 = iflux(S(IL_PN, 0, -1), pc, S(IL_UZ, 0, -1));
+    ps[0] = pc + ((-f0 + f1) + (-f2 + f3) + (-f4 + f5)) / 1;
+    S(IL_PSI + 0, 0, 0) = ps[0];
+  }
 
   auto step = [&](auto PHc, const int q) __attribute__((always_inline)) {
     constexpr int PH = decltype(PHc)::value;
-    constexpr int s0 = PH % 3, sm1 = (PH + 2) % 3, sm2 = (PH + 1) % 3;
-    constexpr int u0 = PH & 1, um1 = u0 ^ 1;
+    // ring slots of plane p + k
+    auto s6 = [](int k) constexpr { return ((PH + k) % 6 + 6) % 6; };
+    auto s3 = [](int k) constexpr { return ((PH + k) % 3 + 3) % 3; };
+    auto s2 = [](int k) constexpr { return ((PH + k) % 2 + 2) % 2; };
+    // uy / uz / psi^n LDS rings: 2 planes, 3 with FUSE0
+    auto su = [](int k) constexpr { return FUSE0 ? ((PH + k) % 3 + 3) % 3 : ((PH + k) % 2 + 2) % 2; };
 
-    // ---- stage 1: publish plane p, prefetch plane p+1
-    ps[s0] = n_ps; ux[s0] = n_ux; uy[u0] = n_uy; uz[u0] = n_uz;
-    S(IL_PSI + s0, 0, 0) = n_ps;
-    S(IL_UX + s0, 0, 0) = n_ux;
-    S(IL_UY + u0, 0, 0) = n_uy;
-    S(IL_UZ + u0, 0, 0) = n_uz;
-    if (FCT) {
-      pn[s0] = n_pn * a.in_scale;
-      S(IL_PN + u0, 0, 0) = pn[s0];
-    }
-    if (q + 1 < nsteps) {
-      g_ps += sxb; g_ux += sxb; g_uy += sxb; g_uz += sxb;
-      n_ps = LD(g_ps);
-      if (FCT) { g_pn += sxb; n_pn = LD(g_pn); }
-      n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
+    // ---- stage 1: publish plane p (p+1 with FUSE0), prefetch the next one
+    if (!FUSE0) {
+      ps[s6(0)] = n_ps; ux[s6(0)] = n_ux; uy[s6(0)] = n_uy; uz[s6(0)] = n_uz;
+      S(IL_PSI + s3(0), 0, 0) = n_ps;
+      S(IL_UX + s3(0), 0, 0) = n_ux;
+      S(IL_UY + su(0), 0, 0) = n_uy;
+      S(IL_UZ + su(0), 0, 0) = n_uz;
+      if (FCT) {
+        pn[s6(0)] = n_pn * sc;
+        S(IL_PN + su(0), 0, 0) = pn[s6(0)];
+      }
+      if (q + 1 < nsteps) {
+        g_ps += sxb; g_ux += sxb; g_uy += sxb; g_uz += sxb;
+        n_ps = LD(g_ps);
+        if (FCT) { g_pn += sxb; n_pn = LD(g_pn); }
+        n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
+      }
+    } else {
+      ux[s6(1)] = n_ux; uy[s6(1)] = n_uy; uz[s6(1)] = n_uz;
+      pn[s6(1)] = n_pn * sc;
+      S(IL_UY + su(1), 0, 0) = n_uy;
+      S(IL_UZ + su(1), 0, 0) = n_uz;
+      S(IL_PN + su(1), 0, 0) = pn[s6(1)];
+      lds[fr_lds + (fr_lds < IL_N * PL ? su(1) * PL : 0)] = n_fr * fr_sc;
+      if (q + 1 < nsteps) { g_pn += sxb; n_pn = LD(g_pn); }
+      if (q + 2 < nsteps) {
+        g_ux += sxb; g_uy += sxb; g_uz += sxb;
+        n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
+        fr_g += sxb;
+        n_fr = *reinterpret_cast<const R *>(fr_g);
+      }
     }
     __syncthreads();
 
@@ -540,35 +634,33 @@ k_fz_interior(InteriorArgs<R> a)
       {
         // V_x at face x+1/2: normal x (planes p-1, p), tangential (y, z)
         auto P = [&](int da, int db, int dc) -> R {
-          if (db == 0 && dc == 0) return da == 0 ? ps[sm1] : ps[s0];
-          return da == 0 ? S(IL_PSI + sm1, db, dc) : S(IL_PSI + s0, db, dc);
+          if (db == 0 && dc == 0) return da == 0 ? ps[s6(-1)] : ps[s6(0)];
+          return da == 0 ? S(IL_PSI + s3(-1), db, dc) : S(IL_PSI + s3(0), db, dc);
         };
-        const R g1 = (uy[u0] + uy[um1] + S(IL_UY + u0, -1, 0) + S(IL_UY + um1, -1, 0)) / 4;
-        const R g2 = (uz[u0] + uz[um1] + S(IL_UZ + u0, 0, -1) + S(IL_UZ + um1, 0, -1)) / 4;
-        vx = antidiff_core<R, G, 3, IGA, false, false>(ux[sm1], g1, g2, P);
+        const R g1 = (uy[s6(0)] + uy[s6(-1)] + S(IL_UY + su(0), -1, 0) + S(IL_UY + su(-1), -1, 0)) / 4;
+        const R g2 = (uz[s6(0)] + uz[s6(-1)] + S(IL_UZ + su(0), 0, -1) + S(IL_UZ + su(-1), 0, -1)) / 4;
+        vx = antidiff_core<R, G, 3, IGA, false, false>(ux[s6(-1)], g1, g2, P);
       }
       if (q >= 2) {
         {
           // V_y at face y+1/2 of plane p-1: normal y, tangential (z, x)
           auto P = [&](int da, int db, int dc) -> R {
-            if (da == 0 && db == 0) return dc > 0 ? ps[s0] : (dc < 0 ? ps[sm2] : ps[sm1]);
-            const int pl = dc > 0 ? s0 : (dc < 0 ? sm2 : sm1);
-            return S(IL_PSI + pl, da, db);
+            if (da == 0 && db == 0) return ps[s6(dc - 1)];
+            return S(IL_PSI + s3(dc - 1), da, db);
           };
-          const R g1 = (S(IL_UZ + um1, 1, 0) + uz[um1] + S(IL_UZ + um1, 1, -1) + S(IL_UZ + um1, 0, -1)) / 4;
-          const R g2 = (S(IL_UX + sm1, 1, 0) + ux[sm1] + S(IL_UX + sm2, 1, 0) + ux[sm2]) / 4;
-          vy = antidiff_core<R, G, 3, IGA, false, false>(uy[um1], g1, g2, P);
+          const R g1 = (S(IL_UZ + su(-1), 1, 0) + uz[s6(-1)] + S(IL_UZ + su(-1), 1, -1) + S(IL_UZ + su(-1), 0, -1)) / 4;
+          const R g2 = (S(IL_UX + s3(-1), 1, 0) + ux[s6(-1)] + S(IL_UX + s3(-2), 1, 0) + ux[s6(-2)]) / 4;
+          vy = antidiff_core<R, G, 3, IGA, false, false>(uy[s6(-1)], g1, g2, P);
         }
         {
           // V_z at face z+1/2 of plane p-1: normal z, tangential (x, y)
           auto P = [&](int da, int db, int dc) -> R {
-            if (da == 0 && dc == 0) return db > 0 ? ps[s0] : (db < 0 ? ps[sm2] : ps[sm1]);
-            const int pl = db > 0 ? s0 : (db < 0 ? sm2 : sm1);
-            return S(IL_PSI + pl, dc, da);
+            if (da == 0 && dc == 0) return ps[s6(db - 1)];
+            return S(IL_PSI + s3(db - 1), dc, da);
           };
-          const R g1 = (S(IL_UX + sm1, 0, 1) + ux[sm1] + S(IL_UX + sm2, 0, 1) + ux[sm2]) / 4;
-          const R g2 = (S(IL_UY + um1, 0, 1) + uy[um1] + S(IL_UY + um1, -1, 1) + S(IL_UY + um1, -1, 0)) / 4;
-          vz = antidiff_core<R, G, 3, IGA, false, false>(uz[um1], g1, g2, P);
+          const R g1 = (S(IL_UX + s3(-1), 0, 1) + ux[s6(-1)] + S(IL_UX + s3(-2), 0, 1) + ux[s6(-2)]) / 4;
+          const R g2 = (S(IL_UY + su(-1), 0, 1) + uy[s6(-1)] + S(IL_UY + su(-1), -1, 1) + S(IL_UY + su(-1), -1, 0)) / 4;
+          vz = antidiff_core<R, G, 3, IGA, false, false>(uz[s6(-1)], g1, g2, P);
         }
         S(IL_VY, 0, 0) = vy;
         S(IL_VZ, 0, 0) = vz;
@@ -584,23 +676,24 @@ k_fz_interior(InteriorArgs<R> a)
 
     // ---- stage 3: beta on plane p-1 (FCT)
     R bu = 0, bd = 0;
-    const R fpx = IGA ? vx : iflux(ps[sm1], ps[s0], vx);   // unlimited flux, face owned by p-1
+    const R fpx = IGA ? vx : iflux(ps[s6(-1)], ps[s6(0)], vx);   // unlimited flux, face owned by p-1
     R yp = 0, zp = 0;
     if (FCT && q >= 2) {
-      const R pc = ps[sm1];
-      yp = S(IL_PSI + sm1, 1, 0);
-      zp = S(IL_PSI + sm1, 0, 1);
-      const R ym = S(IL_PSI + sm1, -1, 0), zm = S(IL_PSI + sm1, 0, -1);
-      const R pmax = fmx(fmx(fmx(pn[sm1], pn[s0]), fmx(pn[sm2], S(IL_PN + um1, 1, 0))),
-                         fmx(S(IL_PN + um1, -1, 0), fmx(S(IL_PN + um1, 0, 1), S(IL_PN + um1, 0, -1))));
-      const R pmin = fmn(fmn(fmn(pn[sm1], pn[s0]), fmn(pn[sm2], S(IL_PN + um1, 1, 0))),
-                         fmn(S(IL_PN + um1, -1, 0), fmn(S(IL_PN + um1, 0, 1), S(IL_PN + um1, 0, -1))));
+      const R pc = ps[s6(-1)];
+      yp = S(IL_PSI + s3(-1), 1, 0);
+      zp = S(IL_PSI + s3(-1), 0, 1);
+      const R ym = S(IL_PSI + s3(-1), -1, 0), zm = S(IL_PSI + s3(-1), 0, -1);
+      const int pm = IL_PN + su(-1);
+      const R pmax = fmx(fmx(fmx(pn[s6(-1)], pn[s6(0)]), fmx(pn[s6(-2)], S(pm, 1, 0))),
+                         fmx(S(pm, -1, 0), fmx(S(pm, 0, 1), S(pm, 0, -1))));
+      const R pmin = fmn(fmn(fmn(pn[s6(-1)], pn[s6(0)]), fmn(pn[s6(-2)], S(pm, 1, 0))),
+                         fmn(S(pm, -1, 0), fmn(S(pm, 0, 1), S(pm, 0, -1))));
       const R vym = S(IL_VY, -1, 0), vzm = S(IL_VZ, 0, -1);
       const R fyp = IGA ? vy : iflux(pc, yp, vy);
       const R fym = IGA ? vym : iflux(ym, pc, vym);
       const R fzp = IGA ? vz : iflux(pc, zp, vz);
       const R fzm = IGA ? vzm : iflux(zm, pc, vzm);
-      ibeta<R, G>(bu, bd, pc, ps[s0], ps[sm2], yp, ym, zp, zm, pmin, pmax, fpx, fpx_[um1],
+      ibeta<R, G>(bu, bd, pc, ps[s6(0)], ps[s6(-2)], yp, ym, zp, zm, pmin, pmax, fpx, fpx_[s2(-1)],
                   fyp, fym, fzp, fzm);
       S(IL_BU, 0, 0) = bu;
       S(IL_BD, 0, 0) = bd;
@@ -610,29 +703,43 @@ k_fz_interior(InteriorArgs<R> a)
     // ---- stage 4: limited fluxes; psi^{k+1} on plane p-2
     if (q >= 3) {
       // x face owned by p-2 (between planes p-2 and p-1)
-      const R mx = FCT ? imono(vx_[um1], bu_[um1], bd_[um1], bu, bd) : vx_[um1];
-      const R fx = IGA ? mx : iflux(ps[sm2], ps[sm1], mx);
-      // y and z faces of plane p-1 (ring slot um1); plane p-2 is slot u0
+      const R mx = FCT ? imono(vx_[s2(-1)], bu_[s2(-1)], bd_[s2(-1)], bu, bd) : vx_[s2(-1)];
+      const R fx = IGA ? mx : iflux(ps[s6(-2)], ps[s6(-1)], mx);
+      // y and z faces of plane p-1 (flux ring slot s2(-1)); plane p-2 is s2(-2)
       if (!FCT) {
-        yp = S(IL_PSI + sm1, 1, 0);
-        zp = S(IL_PSI + sm1, 0, 1);
+        yp = S(IL_PSI + s3(-1), 1, 0);
+        zp = S(IL_PSI + s3(-1), 0, 1);
       }
       const R my = FCT ? imono(vy, bu, bd, S(IL_BU, 1, 0), S(IL_BD, 1, 0)) : vy;
       const R mz = FCT ? imono(vz, bu, bd, S(IL_BU, 0, 1), S(IL_BD, 0, 1)) : vz;
-      S(IL_FY + um1, 0, 0) = IGA ? my : iflux(ps[sm1], yp, my);
-      S(IL_FZ + um1, 0, 0) = IGA ? mz : iflux(ps[sm1], zp, mz);
+      S(IL_FY + s2(-1), 0, 0) = IGA ? my : iflux(ps[s6(-1)], yp, my);
+      S(IL_FZ + s2(-1), 0, 0) = IGA ? mz : iflux(ps[s6(-1)], zp, mz);
       if (q >= 4 && out_ok) {
-        const R r = ps[sm2] + ((-fx + fx_[um1]) + (-S(IL_FY + u0, 0, 0) + S(IL_FY + u0, -1, 0))
-                               + (-S(IL_FZ + u0, 0, 0) + S(IL_FZ + u0, 0, -1))) / 1;
+        const R r = ps[s6(-2)] + ((-fx + fx_[s2(-1)]) + (-S(IL_FY + s2(-2), 0, 0) + S(IL_FY + s2(-2), -1, 0))
+                                  + (-S(IL_FZ + s2(-2), 0, 0) + S(IL_FZ + s2(-2), 0, -1))) / 1;
         ST(g_out, r * a.out_scale);
       }
-      fx_[u0] = fx;
+      fx_[s2(0)] = fx;
     }
     g_out += sxb;
-    vx_[u0] = vx;
-    fpx_[u0] = fpx;
-    bu_[u0] = bu;
-    bd_[u0] = bd;
+    vx_[s2(0)] = vx;
+    fpx_[s2(0)] = fpx;
+    bu_[s2(0)] = bu;
+    bd_[s2(0)] = bd;
+
+    // ---- FUSE0: psi^1 on plane p+1 (donor cell from psi^n, GC)
+    if (FUSE0) {
+      S(IL_UX + s3(1), 0, 0) = ux[s6(1)];
+      if (q + 1 < nsteps) {
+        const int pp = IL_PN + su(1);
+        const R pc = pn[s6(1)];
+        const R f0 = iflux(pc, n_pn * sc, ux[s6(1)]), f1 = iflux(pn[s6(0)], pc, ux[s6(0)]);
+        const R f2 = iflux(pc, S(pp, 1, 0), uy[s6(1)]), f3 = iflux(S(pp, -1, 0), pc, S(IL_UY + su(1), -1, 0));
+        const R f4 = iflux(pc, S(pp, 0, 1), uz[s6(1)]), f5 = iflux(S(pp, 0, -1), pc, S(IL_UZ + su(1), 0, -1));
+        ps[s6(1)] = pc + ((-f0 + f1) + (-f2 + f3) + (-f4 + f5)) / 1;
+        S(IL_PSI + s3(1), 0, 0) = ps[s6(1)];
+      }
+    }
   };
 
   for (int q = 0; q < nsteps; q += 6) {
