@@ -50,6 +50,23 @@ ALG_WORDS = {
 }
 
 
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+
+
+def profile_figure(fname, key, kernel):
+    """A per-launch figure recorded by rocprofv3 counter runs of this workload
+    (machine-readable lines '<key> <timer name> <value>' in profiles/)."""
+    try:
+        with open(os.path.join(PROFILES, fname)) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == key and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def rotation_sphere_device(torch, n, dtype, dev):
     """rotation_sphere_3d of libmpdataxx_amd.workloads, generated on the GPU."""
     c = (n - 1) / 2.0
@@ -118,7 +135,7 @@ def cpu_baseline(seconds=10.0):
         o.advance(1)
         steps += 1
         el = time.perf_counter() - t0
-        if el >= seconds or steps >= 200:
+        if el >= seconds or steps >= 5000:
             break
     return {"value": n ** 3 * steps / el / 1e9, "unit": "Gcell-updates/s", "cores": threads,
             "kind": "port",
@@ -160,6 +177,12 @@ def main():
     launch_cells = (n - 4) ** 3 if span == "interior" else cells
     alg_bytes = words * 4 * launch_cells
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    # HBM bytes per launch (FETCH_SIZE / WRITE_SIZE, copy-calibrated) and VALU
+    # issue utilisation, recorded for the default 512^3 workload only
+    default_wl = n == 512 and s32.active_path == 2
+    traffic = profile_figure("pmc_traffic_fct3_512cube.md", "traffic_bytes_per_launch", dname) \
+        if default_wl else None
+    valu = profile_figure("pmc_sq_interior_fct3_512cube.md", "valu_issue_util", dname) if default_wl else None
 
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "Gcell-updates/s", "n_gpus": 1,
@@ -170,8 +193,9 @@ def main():
                    "grid": [n, n, n], "n_iters": 3, "opts": "fct", "bcond": "open", "path": path_name},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_launch_ms": round(avg_ms, 4), "launches": dlaunch,
-                     "alg_bytes_per_launch": alg_bytes},
+                     "traffic": int(traffic) if traffic else None, "avg_launch_ms": round(avg_ms, 4),
+                     "launches": dlaunch, "alg_bytes_per_launch": alg_bytes,
+                     "valu_issue_util_pmc": valu},
         "step_roofline": {"alg_bytes_per_cell": 20, "achieved_GBs": round(value * 20, 2),
                           "frac": round(value * 20 / HBM_PEAK_GBS, 4)},
         "kernels_ms": {k: round(v[0], 3) for k, v in prof32.items()},
