@@ -43,13 +43,47 @@ def build_cpp_tests(verbose: bool = False) -> None:
         subprocess.run(cmd, check=True)
 
 
+RESOURCES = os.path.join(HERE, "kernel_resources.txt")
+
+
+def _resource_report(stderr: str) -> str:
+    """One line per kernel from the compiler's kernel-resource-usage remarks:
+    name, VGPRs, AGPRs, scratch bytes per lane, VGPR and SGPR spills, LDS."""
+    keys = {"VGPRs": "vgpr", "AGPRs": "agpr", "ScratchSize [bytes/lane]": "scratch",
+            "VGPRs Spill": "vspill", "SGPRs Spill": "sspill", "LDS Size [bytes/block]": "lds"}
+    rows, cur = [], None
+    for line in stderr.splitlines():
+        if "remark:" not in line or "kernel-resource-usage" not in line:
+            continue
+        body = line.split("remark:", 1)[1].split("[-Rpass", 1)[0].strip()
+        if body.startswith("Function Name:"):
+            cur = {"name": body.split(":", 1)[1].strip()}
+            rows.append(cur)
+        elif cur is not None and ":" in body:
+            k, v = body.rsplit(":", 1)
+            if k.strip() in keys:
+                cur[keys[k.strip()]] = v.strip()
+    out = []
+    for r in rows:
+        out.append(" ".join([r["name"]] + [f"{k}={r.get(k, '?')}" for k in keys.values()]))
+    return "\n".join(out) + "\n"
+
+
 def build(verbose: bool = False) -> str:
-    """Compile every HIP source into libmpdx.so; returns the library path."""
+    """Compile every HIP source into libmpdx.so; returns the library path.
+    The compiler's per-kernel register / scratch / LDS usage is written to
+    kernel_resources.txt next to the library."""
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]
+    cmd = [HIPCC, *FLAGS, "-Rpass-analysis=kernel-resource-usage",
+           *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr)
+        raise subprocess.CalledProcessError(r.returncode, cmd)
+    with open(RESOURCES, "w") as f:
+        f.write(_resource_report(r.stderr))
     os.replace(tmp, LIB)
     return LIB
 

@@ -110,34 +110,72 @@ struct PerPass {
     return out;
   }
 
-  // scalar-style fill along d, mode 0/1 from the bcond kind, or 2 = zero
+  // Scalar-style fills along d (mode 0 cyclic / 1 edge copy from the bcond
+  // kind, or 2 = zero).  Fills queue up in a batch that runs as one launch;
+  // callers flush wherever a later fill reads what an earlier one wrote.
+  FillBatch<R> fb{};
+
   void fill(R *a, int d, int side, rng ra, rng rb, bool nrml)
   {
     const int kind = s->p.bc[d][side];
     int mode = kind == MPDX_BC_CYCLIC ? 0 : (nrml ? 2 : 1);
     if (L.nd == 2) rb = R_(0, 0);
     Box bx = make_box(ra.f, ra.l, rb.f, rb.l, 1, L.hw);
-    pblaunch(s, "pp_halo", bx, k_fill<R>, a, make_rot(L, d), bx, side, mode, L.n[d]);
+    if (bx.total <= 0) return;
+    if (fb.n == 6) flush_fills();
+    const int q = fb.n++;
+    fb.a[q] = a; fb.r[q] = make_rot(L, d); fb.bx[q] = bx;
+    fb.side[q] = side; fb.mode[q] = mode; fb.N[q] = L.n[d];
   }
 
-  void alng(R **av, int d, int side, rng ra, rng rb, int ad)
+  void flush_fills()
+  {
+    if (fb.n == 0) return;
+    unsigned gx = 0, gy = 0;
+    for (int q = 0; q < fb.n; ++q) {
+      gx = std::max(gx, box_grid(fb.bx[q]).x);
+      gy = std::max(gy, (unsigned)fb.bx[q].ext[0]);
+    }
+    {
+      ProfScope ps(s, "pp_halo");
+      hipLaunchKernelGGL(k_fill_batch<R>, dim3(gx, gy, fb.n), dim3(kBlock), 0, s->stream, fb);
+    }
+    fb.n = 0;
+  }
+
+  // both sides of the normal-component halo along d (open: edge zeroing of
+  // the perpendicular components first, then the zero-divergence faces);
+  // the two sides touch disjoint cells, so each step covers both at once
+  void alng(R **av, int d, rng ra, rng rb, int ad)
   {
     const int N = L.n[d], hw = L.hw, nd = L.nd;
     Rot r = make_rot(L, d);
     if (nd == 2) rb = R_(0, 0);
     R *avd = av[d], *av1 = av[(d + 1) % nd], *av2 = nd == 3 ? av[(d + 2) % 3] : av[0];
-    if (s->p.bc[d][side] == MPDX_BC_CYCLIC) {
-      Box bx = make_box(1, hw, ra.f, ra.l, rb.f, rb.l);
-      pblaunch(s, "pp_halo", bx, k_alng<R>, avd, (const R *)av1, (const R *)av2, r, bx, nd, side, 1, N);
-      return;
+    int cyc[2], eopen[2], nopen = 0;
+    Box bx[2];
+    for (int side = 0; side < 2; ++side) {
+      cyc[side] = s->p.bc[d][side] == MPDX_BC_CYCLIC;
+      if (cyc[side]) {
+        bx[side] = make_box(1, hw, ra.f, ra.l, rb.f, rb.l);
+      } else {
+        eopen[nopen++] = side == 0 ? 0 : N - 1;
+        const int lo = side == 0 ? -hw : N - 1 + (ad ? 1 : 0);
+        const int hi = side == 0 ? -1 - (ad ? 1 : 0) : N - 2 + hw;
+        bx[side] = make_box(lo, hi, ra.f, ra.l, rb.f, rb.l);
+      }
     }
-    const int e = side == 0 ? 0 : N - 1;
-    const long nz = std::max(ra.l - ra.f + 1, rb.l - rb.f + 1);
-    plaunch(s, "pp_halo", nz, k_alng_zero<R>, av1, av2, r, nd, d, e, ra.f, ra.l, rb.f, rb.l);
-    const int lo = side == 0 ? -hw : N - 1 + (ad ? 1 : 0);
-    const int hi = side == 0 ? -1 - (ad ? 1 : 0) : N - 2 + hw;
-    Box bx = make_box(lo, hi, ra.f, ra.l, rb.f, rb.l);
-    pblaunch(s, "pp_halo", bx, k_alng<R>, avd, (const R *)av1, (const R *)av2, r, bx, nd, side, 0, N);
+    ProfScope ps(s, "pp_halo");
+    if (nopen > 0) {
+      const long nz = std::max(ra.l - ra.f + 1, rb.l - rb.f + 1);
+      hipLaunchKernelGGL(k_alng_zero<R>, dim3((unsigned)((nz + kBlock - 1) / kBlock), nopen), dim3(kBlock), 0,
+                         s->stream, av1, av2, r, nd, d, eopen[0], eopen[nopen - 1], ra.f, ra.l, rb.f, rb.l);
+    }
+    const unsigned gx = std::max(box_grid(bx[0]).x, box_grid(bx[1]).x);
+    const unsigned gy = (unsigned)std::max(bx[0].ext[0], bx[1].ext[0]);
+    if (bx[0].total > 0 || bx[1].total > 0)
+      hipLaunchKernelGGL(k_alng2<R>, dim3(gx, gy, 2), dim3(kBlock), 0, s->stream, avd, (const R *)av1,
+                         (const R *)av2, r, bx[0], bx[1], nd, cyc[0], cyc[1], N);
   }
 
   // solver_3d.hpp:47-67, solver_2d.hpp:40-56
@@ -146,12 +184,18 @@ struct PerPass {
     rng X = ext_(interior(s, 0), ext), Y = ext_(interior(s, 1), ext);
     if (L.nd == 3) {
       rng Z = ext_(interior(s, 2), ext);
+      // each dimension's fill reads the halos the previous one wrote
       for (int sd = 0; sd < 2; ++sd) fill(a, 1, sd, Z, X, false);
+      flush_fills();
       for (int sd = 0; sd < 2; ++sd) fill(a, 0, sd, Y, Z, false);
+      flush_fills();
       for (int sd = 0; sd < 2; ++sd) fill(a, 2, sd, X, Y, false);
+      flush_fills();
     } else {
       for (int sd = 0; sd < 2; ++sd) fill(a, 0, sd, Y, Y, false);
+      flush_fills();
       for (int sd = 0; sd < 2; ++sd) fill(a, 1, sd, X, X, false);
+      flush_fills();
     }
   }
 
@@ -161,12 +205,12 @@ struct PerPass {
     rng I = interior(s, 0), J = interior(s, 1);
     if (L.nd == 3) {
       rng K = interior(s, 2);
-      for (int sd = 0; sd < 2; ++sd) alng(av, 0, sd, J, K, ad);
-      for (int sd = 0; sd < 2; ++sd) alng(av, 1, sd, K, I, ad);
-      for (int sd = 0; sd < 2; ++sd) alng(av, 2, sd, I, J, ad);
+      alng(av, 0, J, K, ad);
+      alng(av, 1, K, I, ad);
+      alng(av, 2, I, J, ad);
     } else {
-      for (int sd = 0; sd < 2; ++sd) alng(av, 0, sd, J, J, ad);
-      for (int sd = 0; sd < 2; ++sd) alng(av, 1, sd, I, I, ad);
+      alng(av, 0, J, J, ad);
+      alng(av, 1, I, I, ad);
     }
   }
 
@@ -179,16 +223,22 @@ struct PerPass {
       rng Xh = h_(ext_(I, ext)), X1 = ext_(ext_(I, ext), 1);
       rng Yh = h_(ext_(J, ext)), Y1 = ext_(ext_(J, ext), 1);
       rng Zh = h_(ext_(K, ext)), Z1 = ext_(ext_(K, ext), 1);
+      // a component's second fill reads the halo its first one wrote;
+      // different components are independent, so the first fills of all
+      // three run together, then the second ones
       for (int sd = 0; sd < 2; ++sd) fill(av[0], 1, sd, Z1, Xh, true);
-      for (int sd = 0; sd < 2; ++sd) fill(av[0], 2, sd, Xh, Y1, true);
       for (int sd = 0; sd < 2; ++sd) fill(av[1], 0, sd, Yh, Z1, true);
-      for (int sd = 0; sd < 2; ++sd) fill(av[1], 2, sd, X1, Yh, true);
       for (int sd = 0; sd < 2; ++sd) fill(av[2], 0, sd, Y1, Zh, true);
+      flush_fills();
+      for (int sd = 0; sd < 2; ++sd) fill(av[0], 2, sd, Xh, Y1, true);
+      for (int sd = 0; sd < 2; ++sd) fill(av[1], 2, sd, X1, Yh, true);
       for (int sd = 0; sd < 2; ++sd) fill(av[2], 1, sd, Zh, X1, true);
+      flush_fills();
     } else {
       rng Xh = h_(ext_(I, ext)), Yh = h_(ext_(J, ext));
       for (int sd = 0; sd < 2; ++sd) fill(av[0], 1, sd, Xh, Xh, true);
       for (int sd = 0; sd < 2; ++sd) fill(av[1], 0, sd, Yh, Yh, true);
+      flush_fills();
     }
   }
 
@@ -668,12 +718,27 @@ struct Fused {
     a.U0 = U[0]; a.U1 = U[1]; a.U2 = U[2];
     a.V0 = V[0]; a.V1 = V[1]; a.V2 = V[2];
     a.L = L;
-    for (int d = 0; d < 3; ++d) { a.lo[d] = 2; a.hi[d] = L.n[d] - 3; }
+    // velocity of cells 2..3 and n-4..n-3 comes from the boundary shell's
+    // antidiff (which may run at the same time), so the interior kernel
+    // stores only cells 4..n-5
+    for (int d = 0; d < 3; ++d) {
+      a.lo[d] = 2; a.hi[d] = L.n[d] - 3;
+      a.vlo[d] = 4; a.vhi[d] = L.n[d] - 5;
+    }
     a.in_scale = in_scale; a.out_scale = out_scale;
     constexpr int TY = ITile<R>::Y, TZ = ITile<R>::Z;
     a.nb[0] = (L.n[2] - 4 + TZ - 5) / (TZ - 4);
     a.nb[1] = (L.n[1] - 4 + TY - 5) / (TY - 4);
-    a.nb[2] = (L.n[0] - 4 + IX_CHUNK - 1) / IX_CHUNK;
+    // x chunk: as long as possible (the 4-plane ramp of each chunk is
+    // overhead) while leaving at least two tiles per CU
+    int chunk = s->ix_chunk;
+    if (chunk <= 0) {
+      chunk = 256;
+      while (chunk > 16 && (long)a.nb[0] * a.nb[1] * ((L.n[0] - 4 + chunk - 1) / chunk) < 2L * s->n_cu)
+        chunk /= 2;
+    }
+    a.chunk = chunk;
+    a.nb[2] = (L.n[0] - 4 + a.chunk - 1) / a.chunk;
     const int ntile = a.nb[0] * a.nb[1] * a.nb[2];
     const dim3 g(8 * ((ntile + 7) / 8));   // tiles are spread over the 8 XCDs in the kernel
     hipLaunchKernelGGL((k_fz_interior<R, G, FCT, IGA, STOREV, FUSE0>), g, dim3(TZ, TY), 0, s->stream, a);
@@ -697,20 +762,42 @@ struct Fused {
 #undef MPDX_INT
   }
 
-  // one corrective pass (iteration it) writing psi^{k+1} into out; with
-  // fuse0 the interior kernel recomputes psi^1 (= psik, present only on the
-  // boundary shell) from psin and GC
-  void corrective(R *out, R *psik, R *psin, int it, int passes, R in_scale, R os, bool fuse0)
+  // The boundary shell of a pass (halo fills, shell kernels) and the
+  // interior kernel touch disjoint output cells and read nothing the other
+  // writes, so with an interior kernel the shell runs on the auxiliary
+  // stream next to it (fork / join through events; a captured step keeps
+  // the same dependencies).  Kernel timing runs everything on one stream.
+  bool concurrent() const { return s->aux && s->use_concurrency && !s->prof_on; }
+
+  void fork()
+  {
+    (void)hipEventRecord(s->ev_fork, s->stream);
+    (void)hipStreamWaitEvent(s->aux, s->ev_fork, 0);
+  }
+  void join(hipStream_t main)
+  {
+    (void)hipEventRecord(s->ev_join, s->aux);
+    (void)hipStreamWaitEvent(main, s->ev_join, 0);
+  }
+  // launches inside the scope go to the auxiliary stream
+  struct OnAux {
+    mpdx_solver *s;
+    hipStream_t main;
+    bool on;
+    OnAux(mpdx_solver *s_, bool on_) : s(s_), main(s_->stream), on(on_) { if (on) s->stream = s->aux; }
+    ~OnAux() { s->stream = main; }
+  };
+
+  // boundary part of corrective pass it (or the whole pass without an
+  // interior kernel): psi^k halo, antidiffusive velocity, its halos, FCT /
+  // donor cell
+  void corrective_shell(R *out, R *psik, R *psin, int it, int passes, R in_scale, R os, bool use_int)
   {
     R *Ub[3], *Vb[3];
     R **U = pp.unco(it, Ub);
     R **V = pp.corr(it, Vb);
     const bool fct = o & O_FCT;
-    const bool last = it == passes - 1;
-    const bool storev = !last;   // the next pass reads this velocity as GC_unco
-    const bool use_int = interior_supported(s->p, L);
     psi_halo(psik);
-    // antidiffusive velocity on the boundary shell (or everywhere)
     {
       ProfScope ps(s, "fused_antidiff");
       if (use_int)
@@ -730,7 +817,7 @@ struct Fused {
       else
         fct_box(out, psik, psin, V, in_scale, os, full_box(0));
     } else {
-      if (it != s->p.n_iters - 1) {
+      if (it != passes - 1) {
         ProfScope ps(s, "fused_halo");
         pp.xchng_vctr_nrml(V, 0);
       }
@@ -741,7 +828,17 @@ struct Fused {
         donor(out, psik, V, R(1), os, (o & O_IGA) ? 1 : 0);
       }
     }
-    if (use_int) interior(out, psik, psin, U, V, storev, fuse0, in_scale, os);
+  }
+
+  // interior part of corrective pass it; with fuse0 the kernel recomputes
+  // psi^1 (= psik, present only on the boundary shell) from psin and GC
+  void corrective_interior(R *out, R *psik, R *psin, int it, int passes, R in_scale, R os, bool fuse0)
+  {
+    R *Ub[3], *Vb[3];
+    R **U = pp.unco(it, Ub);
+    R **V = pp.corr(it, Vb);
+    const bool storev = it != passes - 1;   // the next pass reads this velocity as GC_unco
+    interior(out, psik, psin, U, V, storev, fuse0, in_scale, os);
   }
 
   // one time step of equation e (solver_common.hpp:127-136 with advop inlined)
@@ -761,20 +858,34 @@ struct Fused {
     auto out_of = [&](int it) { return ((passes - 1 - it) % 2 == 0) ? fin : tmp; };
     R *Gb[3];
     R **gc = pp.av_of(s->GC, Gb);
+    const bool use_int = interior_supported(s->p, L);
     psi_halo(psin);
-    // pass 0: with an interior kernel, psi^1 is needed only on the boundary
-    // shell (cells < 5 from a boundary); the first corrective pass's interior
-    // kernel computes its own psi^1
-    const bool fuse0 = passes > 1 && interior_supported(s->p, L);
-    if (fuse0) {
-      ProfScope ps(s, "fused_donor_shell");
-      for (const Box &b : shell(0, 5, 5)) donor_box(out_of(0), psin, gc, in_scale, R(1), 0, b);
-    } else {
+    if (passes == 1 || !use_int) {
       donor(out_of(0), psin, gc, in_scale, passes == 1 ? out_scale : R(1), 0);
+      for (int it = 1; it < passes; ++it)
+        corrective_shell(out_of(it), out_of(it - 1), psin, it, passes, in_scale,
+                         it == passes - 1 ? out_scale : R(1), false);
+    } else {
+      // pass 0 is needed only on the boundary shell (cells < 5 from a
+      // boundary): the first corrective pass's interior kernel computes its
+      // own psi^1
+      const bool conc = concurrent();
+      const hipStream_t main = s->stream;
+      for (int it = 1; it < passes; ++it) {
+        const R os = it == passes - 1 ? out_scale : R(1);
+        if (conc) fork();
+        {
+          OnAux g(s, conc);
+          if (it == 1) {
+            ProfScope ps(s, "fused_donor_shell");
+            for (const Box &b : shell(0, 5, 5)) donor_box(out_of(0), psin, gc, in_scale, R(1), 0, b);
+          }
+          corrective_shell(out_of(it), out_of(it - 1), psin, it, passes, in_scale, os, true);
+        }
+        corrective_interior(out_of(it), out_of(it - 1), psin, it, passes, in_scale, os, it == 1);
+        if (conc) join(main);
+      }
     }
-    for (int it = 1; it < passes; ++it)
-      corrective(out_of(it), out_of(it - 1), psin, it, passes, in_scale, it == passes - 1 ? out_scale : R(1),
-                 fuse0 && it == 1);
     s->lev[e] ^= 1;
   }
 };
@@ -975,7 +1086,26 @@ int mpdx_create(const mpdx_params *p, mpdx_solver **out)
     return MPDX_EHIP;
   }
   s->own_stream = true;
+  // auxiliary stream (default priority) and fork / join events; without them
+  // every launch stays on the main stream
+  if (hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    s->use_concurrency = false;
+  }
   if (const char *ev = std::getenv("MPDX_NO_GRAPHS")) s->use_graphs = !(ev[0] && ev[0] != '0');
+  if (const char *ev = std::getenv("MPDX_NO_CONCURRENCY")) s->use_concurrency = !(ev[0] && ev[0] != '0');
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+      s->n_cu = ncu;
+  }
+  if (const char *ev = std::getenv("MPDX_IX_CHUNK")) {
+    const int c = std::atoi(ev);
+    if (c >= 8 && c <= 4096) s->ix_chunk = c;
+  }
 
   int rc = 0;
   for (int e = 0; e < p->n_eqns && !rc; ++e) {
@@ -1019,6 +1149,12 @@ int mpdx_destroy(mpdx_solver *s)
   for (void *p : s->allocs)
     if (p) (void)hipFree(p);
   fused_free(s);
+  if (s->aux) {
+    (void)hipStreamSynchronize(s->aux);
+    (void)hipStreamDestroy(s->aux);
+  }
+  if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+  if (s->ev_join) (void)hipEventDestroy(s->ev_join);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return MPDX_OK;

@@ -372,7 +372,7 @@ k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R 
 // update) for cells at distance >= 2 from every boundary, where no halo rule
 // touches any value the pass reads, so no advector halo fill is needed.
 // A block owns a (Y-4) x (Z-4) column of output cells and marches along x
-// through IX_CHUNK output planes; each thread owns one (y, z) column of the
+// through a chunk of output planes; each thread owns one (y, z) column of the
 // (Y x Z) tile, which includes a 2-cell frame.  At step q (newest psi^k plane
 // p = x0 - 2 + q) it computes the velocity on plane p-1, beta on plane p-1,
 // the limited fluxes and writes psi^{k+1} on plane p-2.
@@ -396,7 +396,6 @@ template <class R> struct ITile {
   static constexpr int PW = Z + 2, PLANE = (Y + 2) * PW;
   static constexpr int NFRAME = 3 * Z + 3 * Y;   // FUSE0 frame loads per plane
 };
-constexpr int IX_CHUNK = 64;
 
 // LDS planes of the interior kernel
 enum {
@@ -419,7 +418,9 @@ struct InteriorArgs {
   R *V0, *V1, *V2;      // unlimited velocity out (next pass's GC_unco), may be null
   Layout L;
   int lo[3], hi[3];     // output cells, inclusive; lo >= 2, hi <= n-3
+  int vlo[3], vhi[3];   // cells whose velocity is stored (STOREV); the rest is the boundary shell's
   int nb[3];            // tiles along z, y and x chunks
+  int chunk;            // output planes per x chunk
   R in_scale, out_scale;
 };
 
@@ -484,8 +485,8 @@ k_fz_interior(InteriorArgs<R> a)
   const int tid = ty * TZ + tz;
   const int ylo = a.lo[1] - 2 + by * (TY - 4), zlo = a.lo[2] - 2 + bz * (TZ - 4);
   const int y = ylo + ty, z = zlo + tz;
-  const int x0 = a.lo[0] + bxc * IX_CHUNK;
-  const int x1 = min(x0 + IX_CHUNK - 1, a.hi[0]);
+  const int x0 = a.lo[0] + bxc * a.chunk;
+  const int x1 = min(x0 + a.chunk - 1, a.hi[0]);
   const int nsteps = x1 - x0 + 5;   // psi^k planes x0-2 .. x1+2
   // loads: y, z >= lo-2 >= 0; on the far side rows and columns past the
   // tile's last output cell are clamped to the first halo row (cell n),
@@ -495,6 +496,9 @@ k_fz_interior(InteriorArgs<R> a)
   const long base0 = L.org + (long)(x0 - 2) * sx + (long)ylo * L.st[1] + zlo;
   const unsigned toff = (unsigned)(((long)(yl - ylo) * L.st[1] + (zl - zlo)) * (long)sizeof(R));
   const bool out_ok = ty >= 2 && ty < TY - 2 && tz >= 2 && tz < TZ - 2 && y <= a.hi[1] && z <= a.hi[2];
+  const bool v_ok = out_ok && y >= a.vlo[1] && y <= a.vhi[1] && z >= a.vlo[2] && z <= a.vhi[2];
+  // steps whose velocity plane (p-1) is stored: planes x0..x1 within vlo..vhi
+  const int vq0 = max(3, a.vlo[0] - x0 + 3), vq1 = min(nsteps - 2, a.vhi[0] - x0 + 3);
 
   // running byte addresses of the tile corner at the plane each stream reads
   // next (plane 0 = x0-2); V stores at plane q-1, psi^{k+1} at plane q-2
@@ -665,7 +669,7 @@ k_fz_interior(InteriorArgs<R> a)
         S(IL_VY, 0, 0) = vy;
         S(IL_VZ, 0, 0) = vz;
       }
-      if (STOREV && out_ok && q - 1 >= 2 && q - 1 <= nsteps - 3) {
+      if (STOREV && v_ok && q >= vq0 && q <= vq1) {
         ST(g_v0, vx);
         ST(g_v1, vy);
         ST(g_v2, vz);

@@ -32,22 +32,52 @@ template <class G, class R> __device__ inline R frac_rt(unsigned o, R n, R d)
 // mode 1 edge copy (bcond/open_3d.hpp:32-42,154-164), mode 2 zero
 // (open tangential advector, open_3d.hpp:127-133,248-254)
 template <class R>
-__global__ void k_fill(R *a, Rot r, Box bx, int side, int mode, int N)
+__device__ inline void fill_one(R *a, const Rot &r, int b, int c, int m, int side, int mode, int N)
 {
-  int b, c, m;
-  if (!box_idx(bx, b, c, m)) return;
   long dst, src;
   if (side == 0) { dst = rr(r, -m, b, c); src = rr(r, mode == 0 ? N - 1 - m : 0, b, c); }
   else           { dst = rr(r, N - 1 + m, b, c); src = rr(r, mode == 0 ? m : N - 1, b, c); }
   a[dst] = mode == 2 ? R(0) : a[src];
 }
 
+template <class R>
+__global__ void k_fill(R *a, Rot r, Box bx, int side, int mode, int N)
+{
+  int b, c, m;
+  if (!box_idx(bx, b, c, m)) return;
+  fill_one(a, r, b, c, m, side, mode, N);
+}
+
+// up to 6 fills that touch disjoint cells and read none of each other's
+// outputs (both sides of one fill; fills of different arrays), one per
+// grid z index
+template <class R> struct FillBatch {
+  R *a[6];
+  Rot r[6];
+  Box bx[6];
+  int side[6], mode[6], N[6];
+  int n;
+};
+
+template <class R>
+__global__ void k_fill_batch(FillBatch<R> fb)
+{
+  const int q = blockIdx.z;
+  if ((int)blockIdx.y >= fb.bx[q].ext[0]) return;
+  int b, c, m;
+  if (!box_idx(fb.bx[q], b, c, m)) return;
+  fill_one(fb.a[q], fb.r[q], b, c, m, fb.side[q], fb.mode[q], fb.N[q]);
+}
+
 // open-boundary zeroing of perpendicular advector components at the edge
 // (open_3d.hpp:79-103,201-225; open_2d.hpp fill_halos_vctr_alng)
 template <class R>
-__global__ void k_alng_zero(R *av1, R *av2, Rot r, int nd, int d, int e,
+__global__ void k_alng_zero(R *av1, R *av2, Rot r, int nd, int d, int e0, int e1,
                             int raf, int ral, int rbf, int rbl)
 {
+  // grid y: the edge e0 (first side) or e1 (second side); the two touch
+  // different cells
+  const int e = blockIdx.y == 0 ? e0 : e1;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (nd == 3) {
     if (d == 0 && t <= ral - raf) {
@@ -70,11 +100,9 @@ __global__ void k_alng_zero(R *av1, R *av2, Rot r, int nd, int d, int e,
 // the open zero-divergence condition (open_3d.hpp:111-124,233-245); the box
 // is (ii or m, b, c)
 template <class R>
-__global__ void k_alng(R *avd, const R *av1, const R *av2, Rot r, Box bx, int nd,
-                       int side, int cyclic, int N)
+__device__ inline void alng_one(R *avd, const R *av1, const R *av2, const Rot &r, int ii, int b, int c,
+                                int nd, int side, int cyclic, int N)
 {
-  int ii, b, c;
-  if (!box_idx(bx, ii, b, c)) return;
   if (cyclic) {
     const int m = ii;
     if (side == 0) avd[rr(r, -m, b, c)] = avd[rr(r, N - 1 - m, b, c)];
@@ -97,6 +125,30 @@ __global__ void k_alng(R *avd, const R *av1, const R *av2, Rot r, Box bx, int nd
       v = avd[rr(r, e - 1, b, 0)] + (av1[rr(r, e, b - 1, 0)] - av1[rr(r, e, b, 0)]);
   }
   avd[rr(r, ii, b, c)] = v;
+}
+
+template <class R>
+__global__ void k_alng(R *avd, const R *av1, const R *av2, Rot r, Box bx, int nd,
+                       int side, int cyclic, int N)
+{
+  int ii, b, c;
+  if (!box_idx(bx, ii, b, c)) return;
+  alng_one(avd, av1, av2, r, ii, b, c, nd, side, cyclic, N);
+}
+
+// both sides of one dimension's normal-component fill in one launch (grid z =
+// side): each side writes its own halo faces and reads only cells the other
+// side does not write
+template <class R>
+__global__ void k_alng2(R *avd, const R *av1, const R *av2, Rot r, Box bx0, Box bx1, int nd,
+                        int cyc0, int cyc1, int N)
+{
+  const int side = blockIdx.z;
+  const Box &bx = side == 0 ? bx0 : bx1;
+  if ((int)blockIdx.y >= bx.ext[0]) return;
+  int ii, b, c;
+  if (!box_idx(bx, ii, b, c)) return;
+  alng_one(avd, av1, av2, r, ii, b, c, nd, side, side == 0 ? cyc0 : cyc1, N);
 }
 
 // ---------------------------------------------------------------- passes

@@ -46,6 +46,13 @@ struct mpdx_solver {
   // one captured time step per time-level parity (fused path)
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool use_graphs = true;
+  // auxiliary stream (boundary shell next to the interior kernel) and its
+  // fork / join events
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool use_concurrency = true;
+  int ix_chunk = 0;         // output planes per x chunk of the interior kernel (0: automatic)
+  int n_cu = 256;           // compute units of the device
   mpdx_calc_gc_fn calc_gc = nullptr;
   void *calc_gc_user = nullptr;
   int gen_kind = MPDX_GC_NONE;

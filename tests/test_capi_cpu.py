@@ -68,3 +68,19 @@ def test_box_index_fast_division(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout
     assert "0 mismatches" in r.stdout
+
+
+def test_interior_kernels_do_not_spill():
+    """The x-marching interior kernels run one 1024-thread block per CU with
+    at most 128 VGPRs per thread; a register spill to scratch memory costs
+    them up to 2x.  The build records the compiler's per-kernel resource
+    usage; every interior kernel variant must have no VGPR spills and no
+    scratch."""
+    path = os.path.join(ROOT, "libmpdataxx_amd", "kernel_resources.txt")
+    if not os.path.exists(path):
+        pytest.skip("library not built here")
+    rows = [l.split() for l in open(path) if "k_fz_interior" in l]
+    assert rows, "no interior kernels in the resource report"
+    for r in rows:
+        f = dict(kv.split("=") for kv in r[1:])
+        assert f["vspill"] == "0" and f["scratch"] == "0", r

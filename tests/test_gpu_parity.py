@@ -118,7 +118,8 @@ def test_graph_replay_matches_eager(dtype, monkeypatch):
     """Time steps replayed from the captured hipGraphs (one per time-level
     parity) give bit-identical fields to eagerly launched steps, over several
     replays of each graph with host reads, reductions and re-uploads of the
-    advectee between them."""
+    advectee between them; running the boundary shell on the auxiliary stream
+    or on the main stream gives the same fields."""
     import libmpdataxx_amd as M
     grid, bcs = (30, 26, 75), BCS3[2]
     psi, gc = setup_case(grid, bcs, 123)
@@ -142,6 +143,14 @@ def test_graph_replay_matches_eager(dtype, monkeypatch):
     eager.advance(7)
     want = [eager.advectee(e) for e in range(2)]
     eager.close()
+    # the boundary shell on the auxiliary stream or on the main stream
+    monkeypatch.setenv("MPDX_NO_CONCURRENCY", "1")
+    serial = make(True)
+    serial.advance(7)
+    for e in range(2):
+        assert np.array_equal(serial.advectee(e), want[e])
+    serial.close()
+    monkeypatch.delenv("MPDX_NO_CONCURRENCY")
 
     g = make(False)
     masses = []
