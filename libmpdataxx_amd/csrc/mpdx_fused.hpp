@@ -117,37 +117,45 @@ k_fz_donor(R *out, const R *in, const R *C0, const R *C1, const R *C2, Layout L,
 // ---------------------------------------------------------------- antidiffusive velocity
 
 // Antidiffusive velocity at one face from its psi stencil and the two
-// tangential advector averages (formulae_mpdata_3d.hpp:253-295,
+// tangential advector sums (formulae_mpdata_3d.hpp:253-295,
 // formulae_mpdata_2d.hpp:241-278, psi_3d.hpp:261-464).  P(da, db, dc) returns
 // psi at the rotated offset (da along the face normal, db / dc along the
-// first / second tangential direction of idxperm::pi<d>).
+// first / second tangential direction of idxperm::pi<d>); s1, s2 are the
+// four-point sums whose quarter is the reference's GC1_bar / GC2_bar.
+// The reference's factors 1/2 (|u|/2, u/2), 1/4 (the averages) and 2 (ndx)
+// are powers of two, so they are applied once at the end: |u|(1-|u|) frac
+// - u (s1 ndy + s2 ndz) / 8 (2D: - u s1 ndy / 8), with |u|(1-|u|) as one
+// fused multiply-add.
 template <class R, class G, int ND, bool IGA, bool ABS, bool PFC, class PF>
-__device__ inline R antidiff_core(R u, R g1, R g2, PF P)
+__device__ inline R antidiff_core(R u, R s1, R s2, PF P)
 {
   auto Q = [&](int da, int db, int dc) -> R {
     const R v = P(da, db, dc);
     return ABS ? fabs(v) : v;
   };
-  R ndx, ndy;
+  const R au = fabs(u);
   if (IGA) {
-    ndx = ND == 3 ? P(1, 0, 0) - P(0, 0, 0) : 2 * (P(1, 0, 0) - P(0, 0, 0)) / (1 + 1);
-    ndy = (P(1, 1, 0) + P(0, 1, 0) - P(1, -1, 0) - P(0, -1, 0)) / 4;
-  } else {
-    const R a1 = Q(1, 0, 0), a0 = Q(0, 0, 0);
-    ndx = 2 * frac<PFC, G>(a1 - a0, a1 + a0);
-    const R b11 = Q(1, 1, 0), b01 = Q(0, 1, 0), b1m = Q(1, -1, 0), b0m = Q(0, -1, 0);
-    ndy = frac<PFC, G>(b11 + b01 - b1m - b0m, b11 + b01 + b1m + b0m);
-  }
-  if (ND == 3) {
-    R ndz;
-    if (IGA) ndz = (P(1, 0, 1) + P(0, 0, 1) - P(1, 0, -1) - P(0, 0, -1)) / 4;
-    else {
-      const R c11 = Q(1, 0, 1), c01 = Q(0, 0, 1), c1m = Q(1, 0, -1), c0m = Q(0, 0, -1);
-      ndz = frac<PFC, G>(c11 + c01 - c1m - c0m, c11 + c01 + c1m + c0m);
+    const R ndx = ND == 3 ? P(1, 0, 0) - P(0, 0, 0) : 2 * (P(1, 0, 0) - P(0, 0, 0)) / (1 + 1);
+    const R ndy = (P(1, 1, 0) + P(0, 1, 0) - P(1, -1, 0) - P(0, -1, 0)) / 4;
+    const R g1 = s1 / 4;
+    if (ND == 3) {
+      const R g2 = s2 / 4;
+      const R ndz = (P(1, 0, 1) + P(0, 0, 1) - P(1, 0, -1) - P(0, 0, -1)) / 4;
+      return au / 2 * (1 - au / 1) * ndx - u / 2 * (g1 * ndy + g2 * ndz) / 1;
     }
-    return fabs(u) / 2 * (1 - fabs(u) / 1) * ndx - u / 2 * (g1 * ndy + g2 * ndz) / 1;
+    return au / 2 * (1 - au / 1) * ndx - u * g1 / (2 * 1) * ndy;
   }
-  return fabs(u) / 2 * (1 - fabs(u) / 1) * ndx - u * g1 / (2 * 1) * ndy;
+  const R a1 = Q(1, 0, 0), a0 = Q(0, 0, 0);
+  const R fx = frac<PFC, G>(a1 - a0, a1 + a0);
+  const R b11 = Q(1, 1, 0), b01 = Q(0, 1, 0), b1m = Q(1, -1, 0), b0m = Q(0, -1, 0);
+  const R ndy = frac<PFC, G>(b11 + b01 - b1m - b0m, b11 + b01 + b1m + b0m);
+  const R w = fma(-au, au, au) * fx;
+  if (ND == 3) {
+    const R c11 = Q(1, 0, 1), c01 = Q(0, 0, 1), c1m = Q(1, 0, -1), c0m = Q(0, 0, -1);
+    const R ndz = frac<PFC, G>(c11 + c01 - c1m - c0m, c11 + c01 + c1m + c0m);
+    return fma(-(u * R(0.125)), fma(s2, ndz, s1 * ndy), w);
+  }
+  return fma(-(u * R(0.125)), s1 * ndy, w);
 }
 
 // V_D at face (a+1/2 along D) of the cell at element p: the rotation
@@ -166,12 +174,12 @@ __device__ inline R antidiff_at(const R *psi, const R *Ud, const R *U1, const R 
     return rel(A, p, L, db, dc, da);
   };
   auto P = [&](int da, int db, int dc) -> R { return off(psi, da, db, dc); };
-  const R g1 = (off(U1, 1, 0, 0) + U1[p] + off(U1, 1, -1, 0) + off(U1, 0, -1, 0)) / 4;
+  const R s1 = off(U1, 1, 0, 0) + U1[p] + off(U1, 1, -1, 0) + off(U1, 0, -1, 0);
   if (ND == 3) {
-    const R g2 = (off(U2, 1, 0, 0) + U2[p] + off(U2, 1, 0, -1) + off(U2, 0, 0, -1)) / 4;
-    return antidiff_core<R, G, 3, IGA, ABS, PFC>(Ud[p], g1, g2, P);
+    const R s2 = off(U2, 1, 0, 0) + U2[p] + off(U2, 1, 0, -1) + off(U2, 0, 0, -1);
+    return antidiff_core<R, G, 3, IGA, ABS, PFC>(Ud[p], s1, s2, P);
   }
-  return antidiff_core<R, G, 2, IGA, ABS, PFC>(Ud[p], g1, R(0), P);
+  return antidiff_core<R, G, 2, IGA, ABS, PFC>(Ud[p], s1, R(0), P);
 }
 
 // all components over the box [-1, n-1]^nd; component d is written where the
@@ -641,9 +649,9 @@ k_fz_interior(InteriorArgs<R> a)
           if (db == 0 && dc == 0) return da == 0 ? ps[s6(-1)] : ps[s6(0)];
           return da == 0 ? S(IL_PSI + s3(-1), db, dc) : S(IL_PSI + s3(0), db, dc);
         };
-        const R g1 = (uy[s6(0)] + uy[s6(-1)] + S(IL_UY + su(0), -1, 0) + S(IL_UY + su(-1), -1, 0)) / 4;
-        const R g2 = (uz[s6(0)] + uz[s6(-1)] + S(IL_UZ + su(0), 0, -1) + S(IL_UZ + su(-1), 0, -1)) / 4;
-        vx = antidiff_core<R, G, 3, IGA, false, false>(ux[s6(-1)], g1, g2, P);
+        const R s1 = (uy[s6(0)] + uy[s6(-1)] + S(IL_UY + su(0), -1, 0) + S(IL_UY + su(-1), -1, 0));
+        const R s2 = (uz[s6(0)] + uz[s6(-1)] + S(IL_UZ + su(0), 0, -1) + S(IL_UZ + su(-1), 0, -1));
+        vx = antidiff_core<R, G, 3, IGA, false, false>(ux[s6(-1)], s1, s2, P);
       }
       if (q >= 2) {
         {
@@ -652,9 +660,9 @@ k_fz_interior(InteriorArgs<R> a)
             if (da == 0 && db == 0) return ps[s6(dc - 1)];
             return S(IL_PSI + s3(dc - 1), da, db);
           };
-          const R g1 = (S(IL_UZ + su(-1), 1, 0) + uz[s6(-1)] + S(IL_UZ + su(-1), 1, -1) + S(IL_UZ + su(-1), 0, -1)) / 4;
-          const R g2 = (S(IL_UX + s3(-1), 1, 0) + ux[s6(-1)] + S(IL_UX + s3(-2), 1, 0) + ux[s6(-2)]) / 4;
-          vy = antidiff_core<R, G, 3, IGA, false, false>(uy[s6(-1)], g1, g2, P);
+          const R s1 = (S(IL_UZ + su(-1), 1, 0) + uz[s6(-1)] + S(IL_UZ + su(-1), 1, -1) + S(IL_UZ + su(-1), 0, -1));
+          const R s2 = (S(IL_UX + s3(-1), 1, 0) + ux[s6(-1)] + S(IL_UX + s3(-2), 1, 0) + ux[s6(-2)]);
+          vy = antidiff_core<R, G, 3, IGA, false, false>(uy[s6(-1)], s1, s2, P);
         }
         {
           // V_z at face z+1/2 of plane p-1: normal z, tangential (x, y)
@@ -662,9 +670,9 @@ k_fz_interior(InteriorArgs<R> a)
             if (da == 0 && dc == 0) return ps[s6(db - 1)];
             return S(IL_PSI + s3(db - 1), dc, da);
           };
-          const R g1 = (S(IL_UX + s3(-1), 0, 1) + ux[s6(-1)] + S(IL_UX + s3(-2), 0, 1) + ux[s6(-2)]) / 4;
-          const R g2 = (S(IL_UY + su(-1), 0, 1) + uy[s6(-1)] + S(IL_UY + su(-1), -1, 1) + S(IL_UY + su(-1), -1, 0)) / 4;
-          vz = antidiff_core<R, G, 3, IGA, false, false>(uz[s6(-1)], g1, g2, P);
+          const R s1 = (S(IL_UX + s3(-1), 0, 1) + ux[s6(-1)] + S(IL_UX + s3(-2), 0, 1) + ux[s6(-2)]);
+          const R s2 = (S(IL_UY + su(-1), 0, 1) + uy[s6(-1)] + S(IL_UY + su(-1), -1, 1) + S(IL_UY + su(-1), -1, 0));
+          vz = antidiff_core<R, G, 3, IGA, false, false>(uz[s6(-1)], s1, s2, P);
         }
         S(IL_VY, 0, 0) = vy;
         S(IL_VZ, 0, 0) = vz;
