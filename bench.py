@@ -1,14 +1,15 @@
 #!/usr/bin/env python3
 """Benchmark of the MPDATA advection step on one MI355X.
 
-Workload (the north-star configuration): 3D MPDATA with FCT, n_iters = 3,
-open boundaries, 512^3 grid, FP32 state/advector/fluxes with FP64 guards and
-FP64 Kahan mass sums; solid-body rotation about z (max Courant 0.5) moving a
-discontinuous sphere.  A "step" is one time step of the whole grid.  The
-same problem is also run in FP64 for the FP32/FP64 ratio and as the
-accuracy reference (the FP64 GPU path agrees with the FP64 CPU oracle to
-<= 1e-12).  Inputs are generated on the device and resident in HBM before
-the timed region.
+Headline workload: 3D MPDATA with FCT, n_iters = 3, open boundaries, 512^3
+grid, FP32 state/advector/fluxes with FP64 guards and FP64 Kahan mass sums;
+solid-body rotation about z (max Courant 0.5) moving a discontinuous sphere.
+A "step" is one time step of the whole grid.  The same problem is also run
+in FP64 for the FP32/FP64 ratio and as the accuracy reference (the FP64 GPU
+path agrees with the FP64 CPU oracle to <= 1e-12).  A second leg runs the
+multi-tracer workload (512x512x128 cyclic, 8 tracers, time-varying
+deformational advector generated on the device each step).  Inputs are
+generated on the device and resident in HBM before the timed regions.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -114,6 +115,71 @@ def run_gpu(torch, M, n, dtype, steps, warmup, path):
     return s, (t1 - t0), prof, mass0, (t3 - t2)
 
 
+def multi_tracer_fields(torch, shape, dev):
+    """The eight tracers of the multi-tracer workload on a cyclic grid of
+    period n-1 (cell n-1 duplicates cell 0): 0-3 lognormal exp(g/2) with g a
+    unit-variance sum of 8 random Fourier modes, 4-6 slotted cylinders along
+    z, 7 the constant 1."""
+    n0, n1, n2 = shape
+    out = []
+    ix = [torch.arange(n, dtype=torch.float64, device=dev) for n in shape]
+    X, Y, Z = ix[0][:, None, None], ix[1][None, :, None], ix[2][None, None, :]
+    for t in range(8):
+        g = torch.Generator().manual_seed(4000 + t)
+        if t < 4:
+            f = torch.zeros(shape, dtype=torch.float64, device=dev)
+            for _ in range(8):
+                k = torch.randint(-4, 5, (3,), generator=g).tolist()
+                ph = float(torch.rand(1, generator=g)) * 2 * math.pi
+                f += 0.5 * torch.cos(2 * math.pi * (k[0] * X / (n0 - 1) + k[1] * Y / (n1 - 1)
+                                                    + k[2] * Z / (n2 - 1)) + ph)
+            f = torch.exp(0.5 * f)
+        elif t < 7:
+            c = (torch.rand(2, generator=g) * 0.5 + 0.25).tolist()
+            cx, cy, r = c[0] * (n0 - 1), c[1] * (n1 - 1), 0.15 * (n0 - 1)
+            inside = ((X - cx) ** 2 + (Y - cy) ** 2 <= r * r) & ~((abs(X - cx) <= r / 6) & (Y - cy >= -r / 3))
+            f = torch.where(inside, 1.0, 0.1).expand(shape).clone()
+        else:
+            f = torch.ones(shape, dtype=torch.float64, device=dev)
+        out.append(f.to(torch.float32).contiguous())
+    return out
+
+
+def run_multi_tracer(torch, M, steps, warmup, shape=(512, 512, 128), n_tr=8):
+    """3D MPDATA-FCT n_iters=3, cyclic, 8 tracers sharing one time-varying
+    (reversing deformational) advector generated on the device every step."""
+    dev = torch.device("cuda:0")
+    s = M.Solver(shape, opts=M.FCT, n_iters=3, n_eqns=n_tr, dtype="float32", dt=1.0,
+                 bcs=[(M.CYCLIC, M.CYCLIC)] * 3)
+    for e, f in enumerate(multi_tracer_fields(torch, shape, dev)):
+        s.set_advectee_device(f, e)
+    del f
+    for d in range(3):
+        ext = s.advector_shape(d)
+        s.set_advector_device(torch.zeros(ext, dtype=torch.float32, device=dev), d)
+    s.set_advector_generator(M.GC_DEFORM, 200.0, 0.4)
+    torch.cuda.synchronize()
+    s.advance(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.advance(steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    const = torch.empty(shape, dtype=torch.float32, device=dev)
+    s.get_advectee_device(const, n_tr - 1)
+    dev_const = float((const - 1).abs().max())
+    cells = shape[0] * shape[1] * shape[2]
+    res = {"value": round(cells * n_tr * steps / el / 1e9, 4), "unit": "Gcell-updates/s",
+           "ms_per_step": round(el * 1e3 / steps, 4), "steps": steps, "warmup": warmup,
+           "config": {"workload": f"3D MPDATA-FCT n_iters=3, {shape[0]}x{shape[1]}x{shape[2]} cyclic, "
+                                  f"{n_tr} tracers, reversing deformational advector generated each step",
+                      "grid": list(shape), "tracers": n_tr},
+           "alg_bytes_per_cell_step": 2 * n_tr * 4 + 3 * 4,
+           "constant_tracer_max_dev": dev_const}
+    s.close()
+    return res
+
+
 def cpu_baseline(seconds=10.0):
     """FP64 oracle (reference release-style flags, OpenMP) on a 128^3 sample
     of the same workload; FP64 like the reference's double-precision runs."""
@@ -153,6 +219,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-multi", action="store_true", help="skip the multi-tracer leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharding design is "
@@ -224,6 +291,8 @@ def main():
         s64.close()
     s32.close()
 
+    if not args.no_multi:
+        out["multi_tracer"] = run_multi_tracer(torch, M, max(K // 2, 2), min(W, 2))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -907,14 +907,13 @@ int calc_gc_step(mpdx_solver *s)
   const double t_half = s->time + 0.5 * s->p.dt;
   bool changed = false;
   if (s->gen_kind == MPDX_GC_DEFORM) {
-    const double amp = s->gen_scale * std::cos(3.141592653589793 * t_half / s->gen_T);
-    // (same expression as the oracle's, so both see the same amplitude)
+    // amplitude from the table advance_impl uploaded for this call
     const Layout &L = s->L;
     Box bx = make_box(-1, L.n[0] - 1, -1, L.n[1] - 1, L.nd == 3 ? 0 : 0, L.nd == 3 ? L.n[2] - 1 : 0);
     ProfScope ps(s, "gen_deform");
     blaunch(s->stream, bx, k_gen_deform<R>, (R *)s->GC[0], (R *)s->GC[1],
-           (R *)s->GC[L.nd == 3 ? 2 : 0], L, bx, amp, (const double *)s->gen_sx2,
-           (const double *)s->gen_sy2);
+           (R *)s->GC[L.nd == 3 ? 2 : 0], L, bx, (const double *)s->gen_amp, (const int *)s->gen_step,
+           (const double *)s->gen_sx2, (const double *)s->gen_sy2);
     changed = true;
   }
   if (s->calc_gc) {
@@ -948,12 +947,37 @@ int advance_impl(mpdx_solver *s, int64_t nt)
   if (!fused)
     if (int rc = ensure_perpass(s)) return rc;
   PerPass<R, G> pp(s);
+  // on-device advector generator: the amplitude of every step of this call,
+  // evaluated on the host like the oracle's, and a device step index
+  std::vector<double> amps;
+  const bool gen = s->gen_kind == MPDX_GC_DEFORM;
+  if (gen) {
+    const long n = nt - s->timestep;
+    if (n > s->gen_amp_cap) {
+      void *pa = nullptr;
+      if (int rc = dev_alloc(s, &pa, sizeof(double) * (size_t)n)) return rc;
+      s->gen_amp = (double *)pa;
+      s->gen_amp_cap = n;
+    }
+    if (!s->gen_step) {
+      void *pc = nullptr;
+      if (int rc = dev_alloc(s, &pc, sizeof(int))) return rc;
+      s->gen_step = (int *)pc;
+    }
+    amps.resize(n > 0 ? n : 1);
+    for (long q = 0; q < n; ++q) {
+      const double t_half = (s->timestep + q) * s->p.dt + 0.5 * s->p.dt;
+      amps[q] = s->gen_scale * std::cos(3.141592653589793 * t_half / s->gen_T);
+    }
+    if (n > 0) HIPCHK(hipMemcpyAsync(s->gen_amp, amps.data(), sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, s->stream, s->gen_step, 0);
+  }
   // the fused step is replayed from a captured hipGraph unless something in
-  // it changes from step to step (host calc_gc callback, upwind filter steps)
-  // or kernels are being timed individually
-  // (the legacy null stream cannot be captured)
+  // it changes from step to step in a way a graph cannot follow (host
+  // calc_gc callback, upwind filter steps) or kernels are being timed
+  // individually (the legacy null stream cannot be captured)
   const bool graphs = fused && s->use_graphs && !s->prof_on && !s->calc_gc && s->stream != nullptr &&
-                      s->gen_kind == MPDX_GC_NONE && s->p.upwind_filter_freq <= 0;
+                      s->p.upwind_filter_freq <= 0;
   while (s->timestep < nt) {
     if (s->panic) break;
     if (graphs) {
@@ -965,6 +989,7 @@ int advance_impl(mpdx_solver *s, int64_t nt)
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         int rc = calc_gc_step<R, G>(s);
         if (!rc) rc = fused_step<R, G>(s);
+        if (gen) hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, s->stream, s->gen_step, -1);
         hipError_t ec = hipStreamEndCapture(s->stream, &g);
         for (int e = 0; e < s->p.n_eqns; ++e) s->lev[e] = lev_save[e];
         if (rc) return rc;
@@ -982,6 +1007,7 @@ int advance_impl(mpdx_solver *s, int64_t nt)
       } else {
         pp.step();
       }
+      if (gen) hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, s->stream, s->gen_step, -1);
     }
     HIPCHK(hipGetLastError());
     s->timestep++;

@@ -352,17 +352,26 @@ __device__ inline double deform_corner(int a, int b, const double *sx2, const do
   return rint(amp * sx2[a + 2] * sy2[b + 2] * 131072.0) / 131072.0;
 }
 
+// amp_tab[*step] is the amplitude of the current step (a table uploaded per
+// advance call, the step counter advanced on the device by k_tick), so a
+// captured time step can be replayed unchanged
 template <class R>
-__global__ void k_gen_deform(R *g0, R *g1, R *g2, Layout L, Box bx, double amp, const double *sx2,
-                             const double *sy2)
+__global__ void k_gen_deform(R *g0, R *g1, R *g2, Layout L, Box bx, const double *amp_tab, const int *step,
+                             const double *sx2, const double *sy2)
 {
   int i, j, k;
   if (!box_idx(bx, i, j, k)) return;
+  const double amp = amp_tab[*step];
   const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
   const double c = deform_corner(i, j, sx2, sy2, amp);
   g0[p] = R(c - deform_corner(i, j - 1, sx2, sy2, amp));
   g1[p] = R(deform_corner(i - 1, j, sx2, sy2, amp) - c);
   if (L.nd == 3) g2[p] = R(0);
+}
+
+__global__ void k_tick(int *step, int set)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step = set >= 0 ? set : *step + 1;
 }
 
 template <class R>

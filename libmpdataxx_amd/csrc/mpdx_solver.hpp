@@ -58,6 +58,9 @@ struct mpdx_solver {
   int gen_kind = MPDX_GC_NONE;
   double gen_T = 1, gen_scale = 0;
   double *gen_sx2 = nullptr, *gen_sy2 = nullptr;   // device sin^2 tables from index -2
+  double *gen_amp = nullptr;   // device amplitude per step of the current advance call
+  long gen_amp_cap = 0;
+  int *gen_step = nullptr;     // device index into gen_amp
   bool prof_on = false;
   std::vector<std::string> prof_order;
   std::map<std::string, mpdx_prof_entry> prof;
