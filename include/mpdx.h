@@ -168,6 +168,30 @@ int mpdx_advector_device(mpdx_solver *h, int d, void **ptr, int64_t strides[3]);
 #define MPDX_GC_DEFORM 1
 int mpdx_set_advector_generator(mpdx_solver *h, int kind, const double *params, int n_params);
 
+/* Explicit forcing: the rhs schemes of solvers::mpdata_rhs
+ * (solvers/mpdata_rhs.hpp:20-25,116-195): euler_a psi^{n+1} = ADV(psi^n) +
+ * dt R^n; euler_b psi^{n+1} = ADV(psi^n + dt R^n); trapez psi^{n+1} =
+ * ADV(psi^n + dt/2 R^n) + dt/2 R^{n+1}.  fn plays the user's update_rhs(rhs,
+ * dt, at) (at = 0: R^n, at = 1: R^{n+1}, trapez only): before each call the
+ * forcing arrays are zeroed over the interior, as the reference's base
+ * update_rhs does (mpdata_rhs.hpp:52-70); fn fills them with mpdx_set_rhs or
+ * through mpdx_rhs_device, reading the state with mpdx_get_advectee or
+ * mpdx_advectee_device.  Forcing is added over the interior (apply_rhs,
+ * mpdata_rhs.hpp:72-84).  The `mixed` scheme and hint_norhs are not
+ * supported. */
+#define MPDX_RHS_NONE 0
+#define MPDX_RHS_EULER_A 1
+#define MPDX_RHS_EULER_B 2
+#define MPDX_RHS_TRAPEZ 3
+typedef int (*mpdx_update_rhs_fn)(mpdx_solver *h, double dt, int at, void *user);
+int mpdx_set_rhs_scheme(mpdx_solver *h, int scheme, mpdx_update_rhs_fn fn, void *user);
+/* copy the forcing of equation e (interior, like mpdx_set_advectee) */
+int mpdx_set_rhs(mpdx_solver *h, int e, const void *buf, const int64_t strides[3], int on_device);
+/* device address of element (0, 0, 0) of the forcing / the current state of
+ * equation e, and the element strides */
+int mpdx_rhs_device(mpdx_solver *h, int e, void **ptr, int64_t strides[3]);
+int mpdx_advectee_device(mpdx_solver *h, int e, void **ptr, int64_t strides[3]);
+
 /* Kernel timing with HIP events on the solver's stream (no reference
  * counterpart; the reference prints a wall-clock timer, concurr/detail/
  * timer.hpp:19-56).  mpdx_profile(h, 1) clears and starts recording;

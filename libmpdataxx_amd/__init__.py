@@ -34,9 +34,12 @@ EXPORTED = [
     "mpdx_advance", "mpdx_reduce", "mpdx_time", "mpdx_timestep",
     "mpdx_panic_ptr", "mpdx_active_path", "mpdx_profile", "mpdx_profile_read",
     "mpdx_set_calc_gc", "mpdx_advector_device", "mpdx_set_advector_generator",
+    "mpdx_set_rhs_scheme", "mpdx_set_rhs", "mpdx_rhs_device", "mpdx_advectee_device",
 ]
 GC_NONE, GC_DEFORM = 0, 1
 CALC_GC_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p)
+RHS_NONE, RHS_EULER_A, RHS_EULER_B, RHS_TRAPEZ = 0, 1, 2, 3
+UPDATE_RHS_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_void_p)
 
 
 class Params(ctypes.Structure):
@@ -76,7 +79,8 @@ def lib():
         L.mpdx_last_error.argtypes = [P]
         L.mpdx_set_stream.argtypes = [P, P]
         L.mpdx_advector_extents.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
-        for name in ("mpdx_set_advectee", "mpdx_get_advectee", "mpdx_set_advector", "mpdx_get_advector"):
+        for name in ("mpdx_set_advectee", "mpdx_get_advectee", "mpdx_set_advector", "mpdx_get_advector",
+                     "mpdx_set_rhs"):
             getattr(L, name).argtypes = [P, ctypes.c_int, P, i64p, ctypes.c_int]
         L.mpdx_advance.argtypes = [P, ctypes.c_int64]
         L.mpdx_reduce.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
@@ -87,6 +91,9 @@ def lib():
         L.mpdx_active_path.argtypes = [P]
         L.mpdx_set_calc_gc.argtypes = [P, CALC_GC_FN, P]
         L.mpdx_advector_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), i64p]
+        L.mpdx_rhs_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), i64p]
+        L.mpdx_advectee_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), i64p]
+        L.mpdx_set_rhs_scheme.argtypes = [P, ctypes.c_int, UPDATE_RHS_FN, P]
         L.mpdx_set_advector_generator.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.mpdx_profile.argtypes = [P, ctypes.c_int]
         L.mpdx_profile_read.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
@@ -267,6 +274,26 @@ class Solver:
                 return -1
         self._calc_gc = CALC_GC_FN(tramp)
         self._check(self._L.mpdx_set_calc_gc(self._h, self._calc_gc, None))
+
+    def set_rhs_scheme(self, scheme, fn=None):
+        """Explicit forcing (solvers::mpdata_rhs): scheme RHS_EULER_A /
+        RHS_EULER_B / RHS_TRAPEZ (RHS_NONE = off); fn(dt, at) plays
+        update_rhs and sets each equation's forcing with set_rhs (the forcing
+        is zeroed before each call)."""
+        def tramp(h, dt, at, user):
+            try:
+                if fn is not None:
+                    fn(dt, at)
+                return 0
+            except Exception:          # noqa: BLE001 - reported as an error code
+                return -1
+        self._rhs_cb = UPDATE_RHS_FN(tramp)
+        self._check(self._L.mpdx_set_rhs_scheme(self._h, int(scheme), self._rhs_cb, None))
+
+    def set_rhs(self, e, arr):
+        a = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert a.shape == tuple(self.grid), (a.shape, self.grid)
+        self._check(self._L.mpdx_set_rhs(self._h, e, a.ctypes.data, _strides(a), 0))
 
     def set_advector_generator(self, kind, *params):
         arr = (ctypes.c_double * max(len(params), 1))(*params)

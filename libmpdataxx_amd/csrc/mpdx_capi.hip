@@ -936,13 +936,43 @@ int calc_gc_step(mpdx_solver *s)
   return 0;
 }
 
+// mpdata_rhs::update_rhs: zero the forcing over the interior, then the
+// user's fn fills it (solvers/mpdata_rhs.hpp:52-70)
+template <class R>
+int update_rhs(mpdx_solver *s, double dt, int at)
+{
+  Box bi = interior_box(s->L);
+  for (int e = 0; e < s->p.n_eqns; ++e)
+    blaunch(s->stream, bi, k_set_box<R>, (R *)s->rhs[e], s->L, bi, R(0));
+  HIPCHK(hipGetLastError());
+  if (s->rhs_fn && s->rhs_fn(s, dt, at, s->rhs_user) < 0) {
+    if (s->err.empty()) s->err = "libmpdata++: update_rhs callback failed";
+    return MPDX_EINVAL;
+  }
+  return 0;
+}
+
+// mpdata_rhs::apply_rhs: state += dt_arg * R over the interior (:72-84)
+template <class R>
+void apply_rhs(mpdx_solver *s, double dt_arg)
+{
+  Box bi = interior_box(s->L);
+  ProfScope ps(s, "rhs_apply");
+  for (int e = 0; e < s->p.n_eqns; ++e)
+    blaunch(s->stream, bi, k_axpy_box<R>, (R *)s->psi[e][s->lev[e]], (const R *)s->rhs[e], s->L, bi, R(dt_arg));
+}
+
 template <class R, class G>
 int advance_impl(mpdx_solver *s, int64_t nt)
 {
   nt += s->timestep;
   if (s->timestep == 0) {
     if (int rc = ante_loop<R, G>(s)) return rc;
+    // mpdata_rhs::hook_ante_loop (solvers/mpdata_rhs.hpp:116-133)
+    if (s->rhs_scheme == MPDX_RHS_TRAPEZ)
+      if (int rc = update_rhs<R>(s, s->p.dt / 2, 0)) return rc;
   }
+  const int rs = s->rhs_scheme;
   const bool fused = s->active_path == MPDX_PATH_FUSED;
   if (!fused)
     if (int rc = ensure_perpass(s)) return rc;
@@ -977,7 +1007,7 @@ int advance_impl(mpdx_solver *s, int64_t nt)
   // calc_gc callback, upwind filter steps) or kernels are being timed
   // individually (the legacy null stream cannot be captured)
   const bool graphs = fused && s->use_graphs && !s->prof_on && !s->calc_gc && s->stream != nullptr &&
-                      s->p.upwind_filter_freq <= 0;
+                      s->p.upwind_filter_freq <= 0 && rs == MPDX_RHS_NONE;
   while (s->timestep < nt) {
     if (s->panic) break;
     if (graphs) {
@@ -1002,6 +1032,13 @@ int advance_impl(mpdx_solver *s, int64_t nt)
       for (int e = 0; e < s->p.n_eqns; ++e) s->lev[e] ^= 1;
     } else {
       if (int rc = calc_gc_step<R, G>(s)) return rc;
+      // mpdata_rhs::hook_ante_step (solvers/mpdata_rhs.hpp:135-160)
+      if (rs == MPDX_RHS_EULER_A || rs == MPDX_RHS_EULER_B) {
+        if (int rc = update_rhs<R>(s, s->p.dt, 0)) return rc;
+        if (rs == MPDX_RHS_EULER_B) apply_rhs<R>(s, s->p.dt);
+      } else if (rs == MPDX_RHS_TRAPEZ) {
+        apply_rhs<R>(s, s->p.dt / 2);
+      }
       if (fused) {
         if (int rc = fused_step<R, G>(s)) return rc;
       } else {
@@ -1012,6 +1049,13 @@ int advance_impl(mpdx_solver *s, int64_t nt)
     HIPCHK(hipGetLastError());
     s->timestep++;
     s->time = s->timestep * s->p.dt;
+    // mpdata_rhs::hook_post_step (solvers/mpdata_rhs.hpp:162-183)
+    if (rs == MPDX_RHS_EULER_A) {
+      apply_rhs<R>(s, s->p.dt);
+    } else if (rs == MPDX_RHS_TRAPEZ) {
+      if (int rc = update_rhs<R>(s, s->p.dt / 2, 1)) return rc;
+      apply_rhs<R>(s, s->p.dt / 2);
+    }
   }
   HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
@@ -1280,6 +1324,53 @@ int mpdx_set_calc_gc(mpdx_solver *s, mpdx_calc_gc_fn fn, void *user)
   s->calc_gc = fn;
   s->calc_gc_user = user;
   return MPDX_OK;
+}
+
+int mpdx_set_rhs_scheme(mpdx_solver *s, int scheme, mpdx_update_rhs_fn fn, void *user)
+{
+  if (scheme < MPDX_RHS_NONE || scheme > MPDX_RHS_TRAPEZ) {
+    s->err = "libmpdata++: unsupported rhs scheme";
+    return MPDX_EUNSUPPORTED;
+  }
+  if (scheme != MPDX_RHS_NONE)
+    for (int e = 0; e < s->p.n_eqns; ++e)
+      if (int rc = alloc_missing(s, &s->rhs[e])) return rc;
+  s->rhs_scheme = scheme;
+  s->rhs_fn = fn;
+  s->rhs_user = user;
+  return MPDX_OK;
+}
+
+int mpdx_set_rhs(mpdx_solver *s, int e, const void *buf, const int64_t strides[3], int on_device)
+{
+  if (e < 0 || e >= s->p.n_eqns || !s->rhs[e]) {
+    s->err = "libmpdata++: no forcing array for this equation (set an rhs scheme first)";
+    return MPDX_EINVAL;
+  }
+  int lo[3] = {0, 0, 0}, hi[3] = {s->L.n[0] - 1, s->L.n[1] - 1, s->L.nd == 3 ? s->L.n[2] - 1 : 0};
+  return copy_view(s, s->rhs[e], lo, hi, const_cast<void *>(buf), strides, on_device, true);
+}
+
+static int interior_device(mpdx_solver *s, void *arr, void **ptr, int64_t strides[3])
+{
+  *ptr = static_cast<char *>(arr) + s->L.org * s->esz;
+  for (int q = 0; q < 3; ++q) strides[q] = q < s->L.nd ? s->L.st[q] : 0;
+  return MPDX_OK;
+}
+
+int mpdx_rhs_device(mpdx_solver *s, int e, void **ptr, int64_t strides[3])
+{
+  if (e < 0 || e >= s->p.n_eqns || !s->rhs[e]) {
+    s->err = "libmpdata++: no forcing array for this equation (set an rhs scheme first)";
+    return MPDX_EINVAL;
+  }
+  return interior_device(s, s->rhs[e], ptr, strides);
+}
+
+int mpdx_advectee_device(mpdx_solver *s, int e, void **ptr, int64_t strides[3])
+{
+  if (e < 0 || e >= s->p.n_eqns) { s->err = "libmpdata++: bad equation index"; return MPDX_EINVAL; }
+  return interior_device(s, s->psi[e][s->lev[e]], ptr, strides);
 }
 
 int mpdx_advector_device(mpdx_solver *s, int d, void **ptr, int64_t strides[3])

@@ -369,6 +369,25 @@ __global__ void k_gen_deform(R *g0, R *g1, R *g2, Layout L, Box bx, const double
   if (L.nd == 3) g2[p] = R(0);
 }
 
+// explicit forcing (solvers/mpdata_rhs.hpp:52-93): zero the forcing array
+// over a box, and psi += dt_arg * R over a box
+template <class R>
+__global__ void k_set_box(R *a, Layout L, Box bx, R v)
+{
+  int i, j, k;
+  if (!box_idx(bx, i, j, k)) return;
+  a[L.org + i * L.st[0] + j * L.st[1] + k * L.st[2]] = v;
+}
+
+template <class R>
+__global__ void k_axpy_box(R *y, const R *x, Layout L, Box bx, R alpha)
+{
+  int i, j, k;
+  if (!box_idx(bx, i, j, k)) return;
+  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
+  y[p] += alpha * x[p];
+}
+
 __global__ void k_tick(int *step, int set)
 {
   if (threadIdx.x == 0 && blockIdx.x == 0) *step = set >= 0 ? set : *step + 1;

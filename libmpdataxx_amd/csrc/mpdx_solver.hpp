@@ -53,6 +53,11 @@ struct mpdx_solver {
   bool use_concurrency = true;
   int ix_chunk = 0;         // output planes per x chunk of the interior kernel (0: automatic)
   int n_cu = 256;           // compute units of the device
+  // explicit forcing (mpdata_rhs): scheme, user update_rhs, forcing arrays
+  int rhs_scheme = MPDX_RHS_NONE;
+  mpdx_update_rhs_fn rhs_fn = nullptr;
+  void *rhs_user = nullptr;
+  void *rhs[MPDX_MAX_EQNS] = {};
   mpdx_calc_gc_fn calc_gc = nullptr;
   void *calc_gc_user = nullptr;
   int gen_kind = MPDX_GC_NONE;

@@ -215,3 +215,61 @@ def streamfunction_advector(grid, seed, max_courant=0.5, n_modes=4, periodic=Tru
     gz = np.empty((nx + 1, ny, nz - 1))
     gz[:] = hz_full[0:nx + 1, 2:nz + 1][:, None, :]
     return [gx, gy, gz]
+
+
+def coupled_harmosc(n=1001, ny=4):
+    """Coupled harmonic oscillator advected at Courant 0.5 on a cyclic grid
+    (tests/paper_2015_GMD/6_coupled_harmosc/coupled_harmosc.cpp:40-70): psi a
+    cosine bump on cells 51..149, phi = 0, omega = 2 pi / 400 (dt = 1).  The
+    reference's 1D problem is embedded in a y-uniform 2D grid with zero
+    y-advector (the 2D formulae reduce to the 1D ones on it).  Returns
+    (psi, phi, gc, omega)."""
+    i = np.arange(n)
+    psi1 = np.where((i <= 50) | (i >= 150), 0.0, 0.5 * (1 + np.cos(2 * np.pi * i / 100)))
+    psi = np.repeat(psi1[:, None], ny, 1)
+    phi = np.zeros((n, ny))
+    gc = [np.full((n, ny), 0.5), np.zeros((n + 1, ny - 1))]
+    return psi, phi, gc, 2 * np.pi / 400
+
+
+def coupled_harmosc_rhs(psi, phi, dt, at, omega):
+    """update_rhs of tests/paper_2015_GMD/6_coupled_harmosc/coupled_harmosc.hpp:
+    26-56: explicit R^n (at = 0), implicit R^{n+1} (at = 1)."""
+    if at == 0:
+        return omega * phi, -omega * psi
+    d = 1 + (dt * omega) ** 2
+    return ((psi + dt * omega * phi) / d - psi) / dt, ((phi - dt * omega * psi) / d - phi) / dt
+
+
+def coupled_harmosc_stats(psi, phi, timestep, dt=1.0, vel=0.5):
+    """min / max with first index and the rms error against the advected
+    bump (coupled_harmosc_stats.hpp:24-72), on 1D rows."""
+    n = psi.size
+    t = timestep * dt
+    i = np.arange(n)
+    sol = np.where((i <= 50 + vel * t) | (i >= 150 + vel * t), 0.0, 0.5 * (1 + np.cos(2 * np.pi * i / 100.)))
+    tmp = (psi ** 2 + phi ** 2 - sol ** 2) ** 2
+    return {"timestep": timestep,
+            "min_psi": (psi.min(), int(psi.argmin())), "max_psi": (psi.max(), int(psi.argmax())),
+            "min_phi": (phi.min(), int(phi.argmin())), "max_phi": (phi.max(), int(phi.argmax())),
+            "rms": np.sqrt((tmp.sum() - tmp[-1]) / (n - 1)) / timestep}
+
+
+def parse_coupled_harmosc_stats(text):
+    """The refdata stats.txt of the coupled oscillator as a list of dicts."""
+    import re
+    out, cur = [], None
+    for line in text.splitlines():
+        m = re.match(r"timestep = (\d+)", line)
+        if m:
+            cur = {"timestep": int(m.group(1))}
+            out.append(cur)
+            continue
+        m = re.match(r"(min|max)\((psi|phi)\) =\s*(\S+) @ index \((\d+)\)", line)
+        if m and cur is not None:
+            cur[f"{m.group(1)}_{m.group(2)}"] = (float(m.group(3)), int(m.group(4)))
+            continue
+        m = re.match(r"rms = (\S+)", line)
+        if m and cur is not None:
+            cur["rms"] = float(m.group(1))
+    return out

@@ -22,6 +22,8 @@ MAX_EQNS = 64
 # libmpdata++/opts.hpp:34-49
 FCT, ABS, TOT, PFC, NPA, IGA, NUG, DFL, KHN = (1 << b for b in range(9))
 CYCLIC, OPEN = 0, 1
+RHS_NONE, RHS_EULER_A, RHS_EULER_B, RHS_TRAPEZ = 0, 1, 2, 3
+RHS_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_void_p)
 
 
 class _Params(ctypes.Structure):
@@ -75,6 +77,8 @@ def _lib(fast: bool = False):
         lib.orc_full_shape.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         lib.orc_get_full.argtypes = [P, ctypes.c_int, ctypes.c_int, dp]
         lib.orc_set_generator.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        lib.orc_set_rhs_scheme.argtypes = [P, ctypes.c_int, RHS_FN, ctypes.c_void_p]
+        lib.orc_set_rhs.argtypes = [P, ctypes.c_int, dp]
         _libs[name] = lib
     return _libs[name]
 
@@ -156,6 +160,27 @@ class Oracle:
         """kind 1: reversing deformational flow recomputed every step (0: off)."""
         if self._lib.orc_set_generator(self._h, int(kind), float(T), float(max_courant)):
             raise ValueError("bad advector generator parameters")
+
+    def set_rhs_scheme(self, scheme, fn=None):
+        """scheme: RHS_EULER_A / RHS_EULER_B / RHS_TRAPEZ (0 = off); fn(dt, at)
+        plays update_rhs: it sets the forcing of every equation with set_rhs
+        (the arrays are zeroed before each call)."""
+        def tramp(h, dt, at, user):
+            try:
+                if fn is not None:
+                    fn(dt, at)
+                return 0
+            except Exception:          # noqa: BLE001 - reported as an error code
+                return -1
+        self._rhs_cb = RHS_FN(tramp)
+        if self._lib.orc_set_rhs_scheme(self._h, int(scheme), self._rhs_cb, None):
+            raise ValueError("bad rhs scheme")
+
+    def set_rhs(self, e, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == tuple(self.grid), (a.shape, self.grid)
+        if self._lib.orc_set_rhs(self._h, e, _dp(a)):
+            raise ValueError("rhs not enabled or bad equation index")
 
     def advance(self, nt):
         err = ctypes.create_string_buffer(256)

@@ -45,6 +45,10 @@ struct orc {
   long timestep;
   double time;
   int panic;
+  int rhs_scheme;             /* 0 none, 1 euler_a, 2 euler_b, 3 trapez */
+  orc_rhs_fn rhs_fn;
+  void *rhs_user;
+  double *rhs[ORC_MAX_EQNS];
   int gen_kind;               /* 0 none, 1 reversing deformational flow */
   double gen_T, gen_scale;
   double *gen_sx2, *gen_sy2;  /* sin^2 at corners a + 1/2 from a = -2 */
@@ -609,6 +613,57 @@ int orc_set_generator(orc_t *s, int kind, double T, double max_courant)
   return 0;
 }
 
+/* mpdata_rhs (solvers/mpdata_rhs.hpp:52-195): update_rhs zeroes the
+ * forcing and lets the user add R; apply_rhs adds dt_arg * R to the state
+ * over the interior */
+static int update_rhs(orc_t *s, double dt, int at)
+{
+  const int n0 = s->n[0], n1 = s->n[1], n2 = s->nd == 3 ? s->n[2] : 1;
+  for (int e = 0; e < s->p.n_eqns; ++e)
+    for (int i = 0; i < n0; ++i)
+      for (int j = 0; j < n1; ++j)
+        for (int k = 0; k < n2; ++k) s->rhs[e][ix(s, i, j, k)] = 0;
+  return s->rhs_fn ? s->rhs_fn(s, dt, at, s->rhs_user) : 0;
+}
+
+static void apply_rhs(orc_t *s, double dt_arg)
+{
+  const int n0 = s->n[0], n1 = s->n[1], n2 = s->nd == 3 ? s->n[2] : 1;
+  for (int e = 0; e < s->p.n_eqns; ++e) {
+    double *a = s->psi[e][s->lev[e]];
+    for (int i = 0; i < n0; ++i)
+      for (int j = 0; j < n1; ++j)
+        for (int k = 0; k < n2; ++k) {
+          const long p = ix(s, i, j, k);
+          a[p] += dt_arg * s->rhs[e][p];
+        }
+  }
+}
+
+static double *alloc_nan(long n);
+
+int orc_set_rhs_scheme(orc_t *s, int scheme, orc_rhs_fn fn, void *user)
+{
+  if (scheme < 0 || scheme > 3) return 1;
+  for (int e = 0; e < s->p.n_eqns && scheme; ++e)
+    if (!s->rhs[e]) s->rhs[e] = alloc_nan(s->size);
+  s->rhs_scheme = scheme;
+  s->rhs_fn = fn;
+  s->rhs_user = user;
+  return 0;
+}
+
+int orc_set_rhs(orc_t *s, int e, const double *dense)
+{
+  if (e < 0 || e >= s->p.n_eqns || !s->rhs[e]) return 1;
+  const int n0 = s->n[0], n1 = s->n[1], n2 = s->nd == 3 ? s->n[2] : 1;
+  long q = 0;
+  for (int i = 0; i < n0; ++i)
+    for (int j = 0; j < n1; ++j)
+      for (int k = 0; k < n2; ++k, ++q) s->rhs[e][ix(s, i, j, k)] = dense[q];
+  return 0;
+}
+
 /* solve(): solver_common.hpp:270-358 */
 int orc_advance(orc_t *s, long nt, char *err, int errlen)
 {
@@ -624,11 +679,23 @@ int orc_advance(orc_t *s, long nt, char *err, int errlen)
       return 1;
     }
     xchng_vctr_nrml(s, s->GC, s->hw - 1);
+    /* mpdata_rhs::hook_ante_loop (mpdata_rhs.hpp:116-133) */
+    if (s->rhs_scheme == 3 && update_rhs(s, s->p.dt / 2, 0)) {
+      seterr(err, errlen, "libmpdata++: update_rhs failed");
+      return 1;
+    }
   }
   while (s->timestep < nt) {
     if (s->panic) break;
     /* calc_gc() at the top of each step (solver_common.hpp:306-308) */
     if (s->gen_kind == 1) calc_gc_deform(s);
+    /* mpdata_rhs::hook_ante_step (mpdata_rhs.hpp:135-160) */
+    if (s->rhs_scheme == 1 || s->rhs_scheme == 2) {
+      if (update_rhs(s, s->p.dt, 0)) { seterr(err, errlen, "libmpdata++: update_rhs failed"); return 1; }
+      if (s->rhs_scheme == 2) apply_rhs(s, s->p.dt);
+    } else if (s->rhs_scheme == 3) {
+      apply_rhs(s, s->p.dt / 2);
+    }
     for (int e = 0; e < s->p.n_eqns; ++e) {
       scale(s, e, s->p.hint_scale[e]);
       xchng_sclr(s, s->psi[e][s->lev[e]], s->hw);
@@ -638,6 +705,13 @@ int orc_advance(orc_t *s, long nt, char *err, int errlen)
     }
     s->timestep++;
     s->time = s->timestep * s->p.dt;
+    /* mpdata_rhs::hook_post_step (mpdata_rhs.hpp:162-183) */
+    if (s->rhs_scheme == 1) {
+      apply_rhs(s, s->p.dt);
+    } else if (s->rhs_scheme == 3) {
+      if (update_rhs(s, s->p.dt / 2, 1)) { seterr(err, errlen, "libmpdata++: update_rhs failed"); return 1; }
+      apply_rhs(s, s->p.dt / 2);
+    }
   }
   return 0;
 }
@@ -704,6 +778,7 @@ void orc_destroy(orc_t *s)
   }
   free(s->psi_min); free(s->psi_max); free(s->beta_up); free(s->beta_dn);
   free(s->gen_sx2); free(s->gen_sy2);
+  for (int e = 0; e < s->p.n_eqns; ++e) free(s->rhs[e]);
   free(s);
 }
 

@@ -59,6 +59,14 @@ int orc_advance(orc_t *s, long nt, char *err, int errlen);
  * with period parameter T and largest |GC| = max_courant; kind 0 = none */
 int orc_set_generator(orc_t *s, int kind, double T, double max_courant);
 
+/* Explicit forcing of solvers::mpdata_rhs (solvers/mpdata_rhs.hpp:20-25):
+ * scheme 0 none, 1 euler_a, 2 euler_b, 3 trapez.  fn plays update_rhs(dt,
+ * at): it is called with the forcing arrays zeroed and writes R for every
+ * equation with orc_set_rhs (reading the state with orc_get_advectee). */
+typedef int (*orc_rhs_fn)(orc_t *s, double dt, int at, void *user);
+int orc_set_rhs_scheme(orc_t *s, int scheme, orc_rhs_fn fn, void *user);
+int orc_set_rhs(orc_t *s, int e, const double *dense);
+
 long orc_timestep(const orc_t *s);
 double orc_time(const orc_t *s);
 int *orc_panic_ptr(orc_t *s);

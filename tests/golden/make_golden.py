@@ -8,6 +8,7 @@ Fixtures written (all data, no code):
   revolving_sphere_3d/initial_0.f32.gz       the t=0 field (refdata/common/...)
   revolving_sphere_3d/stats_<variant>.txt    refdata/stats_<variant>.txt.gz
   rotating_cone_2d/stats_<variant>_.txt      refdata/stats_<variant>_.txt.gz
+  coupled_harmosc/stats.txt                  6_coupled_harmosc/refdata/stats.txt.gz
   SHA256SUMS                                 checksums of the above
 
 The HDF5 datasets are float32 little-endian, C order (output/hdf5.hpp:56,
@@ -58,8 +59,13 @@ def main():
         with gzip.open(os.path.join(rc, f"stats_{v}_.txt.gz"), "rb") as f, \
                 open(os.path.join(cone, f"stats_{v}_.txt"), "wb") as g:
             shutil.copyfileobj(f, g)
+    osc = os.path.join(HERE, "coupled_harmosc")
+    os.makedirs(osc, exist_ok=True)
+    with gzip.open(os.path.join(REF, "6_coupled_harmosc", "refdata", "stats.txt.gz"), "rb") as f, \
+            open(os.path.join(osc, "stats.txt"), "wb") as g:
+        shutil.copyfileobj(f, g)
     lines = []
-    for sub in ("revolving_sphere_3d", "rotating_cone_2d"):
+    for sub in ("revolving_sphere_3d", "rotating_cone_2d", "coupled_harmosc"):
         for name in sorted(os.listdir(os.path.join(HERE, sub))):
             path = os.path.join(HERE, sub, name)
             digest = hashlib.sha256(open(path, "rb").read()).hexdigest()
