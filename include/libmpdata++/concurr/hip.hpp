@@ -132,6 +132,8 @@ namespace libmpdataxx
         for (int e = 0; e < n_eqns && e < MPDX_MAX_EQNS; ++e) mp.hint_scale[e] = ct_params_t::hint_scale(e);
         mp.dt = p.dt;
         mp.max_abs_div_eps = p.max_abs_div_eps;
+        mp.var_dt = ct_params_t::var_dt ? 1 : 0;
+        mp.max_courant = p.max_courant;
         if (mpdx_create(&mp, &h_) != MPDX_OK) throw std::runtime_error(mpdx_last_error(nullptr));
         std::size_t n = 1;
         for (int d = 0; d < n_dims; ++d) n *= grid_[d];
@@ -153,13 +155,21 @@ namespace libmpdataxx
 
       // concurr_common::advance (concurr_common.hpp:380-385); the panic flag is
       // polled between time steps like solver_common::solve (solver_common.hpp:301-302)
+      // with var_dt the argument is a duration (solver_common.hpp:270-274)
       void advance(advance_arg_t nt) override
       {
         upload();
-        for (advance_arg_t t = 0; t < nt; ++t)
+        if constexpr (ct_params_t::var_dt)
         {
-          if (panic_) break;
-          check(mpdx_advance(h_, 1));
+          if (!panic_) check(mpdx_advance_time(h_, nt));
+        }
+        else
+        {
+          for (advance_arg_t t = 0; t < nt; ++t)
+          {
+            if (panic_) break;
+            check(mpdx_advance(h_, 1));
+          }
         }
         download();
       }

@@ -42,7 +42,6 @@ namespace libmpdataxx
       static_assert(n_dims == 2 || n_dims == 3, "the HIP backend implements 2D and 3D");
       static_assert(std::is_same<real_t, float>::value || std::is_same<real_t, double>::value,
                     "real_t must be float or double");
-      static_assert(!ct_params_t::var_dt, "var_dt is not supported by the HIP backend");
       static_assert(!(opts::isset(opts, opts::abs) && opts::isset(opts, opts::iga)),
                     "abs & iga options are mutually exclusive");
 

@@ -105,6 +105,8 @@ typedef struct mpdx_params {
   double max_abs_div_eps;         /* rt_params_t::max_abs_div_eps; < 0 = epsilon<real_t> */
   int guard;                      /* MPDX_GUARD_* */
   int path;                       /* MPDX_PATH_* */
+  int var_dt;                     /* ct_params_t::var_dt: adaptive time step */
+  double max_courant;             /* rt_params_t::max_courant (solver_common.hpp:216-220) */
 } mpdx_params;
 
 /* fills p with the reference defaults (opts = iga|fct, n_iters = 2, ...) */
@@ -131,6 +133,17 @@ int mpdx_get_advector(mpdx_solver *h, int d, void *buf, const int64_t strides[3]
 
 /* advance by nt time steps (synchronous: returns when the work is done) */
 int mpdx_advance(mpdx_solver *h, int64_t nt);
+
+/* var_dt: advance by a duration (advance(real_t) of a var_dt solver,
+ * solver_common.hpp:270-274,312-336).  On the first call dt and the
+ * advector are rescaled so that the Courant number (solver_3d.hpp:390-398)
+ * equals max_courant (hook_ante_loop, solver_common.hpp:199-210); with a
+ * calc_gc hook dt is re-adjusted every step until the Courant number is at
+ * most max_courant.  Steps are taken while time < time at the call +
+ * duration.  mpdx_advance is refused for var_dt solvers and this call for
+ * the others.  mpdx_dt returns the current dt. */
+int mpdx_advance_time(mpdx_solver *h, double duration);
+int mpdx_dt(const mpdx_solver *h, double *dt);
 
 int mpdx_reduce(mpdx_solver *h, int e, int op, double *out);
 int mpdx_time(const mpdx_solver *h, double *t);

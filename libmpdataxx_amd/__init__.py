@@ -35,6 +35,7 @@ EXPORTED = [
     "mpdx_panic_ptr", "mpdx_active_path", "mpdx_profile", "mpdx_profile_read",
     "mpdx_set_calc_gc", "mpdx_advector_device", "mpdx_set_advector_generator",
     "mpdx_set_rhs_scheme", "mpdx_set_rhs", "mpdx_rhs_device", "mpdx_advectee_device",
+    "mpdx_advance_time", "mpdx_dt",
 ]
 GC_NONE, GC_DEFORM = 0, 1
 CALC_GC_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p)
@@ -57,6 +58,8 @@ class Params(ctypes.Structure):
         ("max_abs_div_eps", ctypes.c_double),
         ("guard", ctypes.c_int),
         ("path", ctypes.c_int),
+        ("var_dt", ctypes.c_int),
+        ("max_courant", ctypes.c_double),
     ]
 
 
@@ -83,6 +86,8 @@ def lib():
                      "mpdx_set_rhs"):
             getattr(L, name).argtypes = [P, ctypes.c_int, P, i64p, ctypes.c_int]
         L.mpdx_advance.argtypes = [P, ctypes.c_int64]
+        L.mpdx_advance_time.argtypes = [P, ctypes.c_double]
+        L.mpdx_dt.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.mpdx_reduce.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.mpdx_time.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.mpdx_timestep.argtypes = [P, i64p]
@@ -118,7 +123,7 @@ class Solver:
 
     def __init__(self, grid, opts=IGA | FCT, n_iters=2, n_eqns=1, bcs=None,
                  dtype=np.float64, dt=0.0, upwind_filter_freq=0, hint_scale=None,
-                 max_abs_div_eps=-1.0, guard=GUARD_F64, path=PATH_AUTO):
+                 max_abs_div_eps=-1.0, guard=GUARD_F64, path=PATH_AUTO, var_dt=False, max_courant=0.5):
         L = lib()
         p = Params()
         L.mpdx_params_default(ctypes.byref(p))
@@ -141,6 +146,8 @@ class Solver:
         p.dt = float(dt)
         p.max_abs_div_eps = float(max_abs_div_eps)
         p.guard = int(guard)
+        p.var_dt = int(bool(var_dt))
+        p.max_courant = float(max_courant)
         p.path = int(path)
         h = ctypes.c_void_p()
         rc = L.mpdx_create(ctypes.byref(p), ctypes.byref(h))
@@ -233,6 +240,16 @@ class Solver:
     def time(self):
         t = ctypes.c_double()
         self._L.mpdx_time(self._h, ctypes.byref(t))
+        return t.value
+
+    def advance_time(self, duration):
+        """var_dt solvers: advance by a duration with the adaptive step."""
+        self._check(self._L.mpdx_advance_time(self._h, float(duration)))
+
+    @property
+    def dt(self):
+        t = ctypes.c_double()
+        self._L.mpdx_dt(self._h, ctypes.byref(t))
         return t.value
 
     @property

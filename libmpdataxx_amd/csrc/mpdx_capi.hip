@@ -515,6 +515,40 @@ int reduce_impl(mpdx_solver *s, const void *arr, int op, double *out)
   return 0;
 }
 
+// max Courant number of the advector (solver_3d.hpp:390-398)
+template <class R>
+int courant_impl(mpdx_solver *s, double *cfl)
+{
+  if (!s->stat)
+    if (int rc = alloc_array(s, &s->stat)) return rc;
+  Box bi = interior_box(s->L);
+  blaunch(s->stream, bi, k_courant<R>, (R *)s->stat, (const R *)s->GC[0], (const R *)s->GC[1],
+          (const R *)s->GC[s->L.nd == 3 ? 2 : 0], s->L, bi);
+  return reduce_impl<R>(s, s->stat, MPDX_RED_MAX, cfl);
+}
+
+// scale_gc (solver_3d.hpp:411-421, solver_2d.hpp:315-324): faces -1/2 ..
+// n-3/2 along d times the interior across, then the advector halos
+template <class R, class G>
+int scale_gc(mpdx_solver *s, double cur_dt, double old_dt)
+{
+  const Layout &L = s->L;
+  for (int d = 0; d < L.nd; ++d) {
+    int lo[3] = {0, 0, 0}, hi[3] = {L.n[0] - 1, L.n[1] - 1, L.nd == 3 ? L.n[2] - 1 : 0};
+    lo[d] = -1;
+    hi[d] = L.n[d] - 2;
+    Box bx = make_box(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+    blaunch(s->stream, bx, k_scale<R>, (R *)s->GC[d], L, bx, R(cur_dt / old_dt));
+  }
+  PerPass<R, G> pp(s);
+  R *Gb[3];
+  R **gc = pp.av_of(s->GC, Gb);
+  pp.xchng_vctr_alng(gc, 0);
+  pp.xchng_vctr_nrml(gc, L.hw - 1);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 // hook_ante_loop chain (solver_common.hpp:191-211, solver_3d.hpp:375-388,
 // mpdata_osc_3d.hpp:40-50)
 template <class R, class G>
@@ -527,6 +561,16 @@ int ante_loop(mpdx_solver *s)
   R *Gb[3];
   R **gc = pp.av_of(s->GC, Gb);
   pp.xchng_vctr_alng(gc, 0);
+  // var_dt: one rescaling for the initial advector (solver_common.hpp:199-210)
+  if (s->p.var_dt) {
+    double cfl = 0;
+    if (int rc = courant_impl<R>(s, &cfl)) return rc;
+    if (cfl > 0) {
+      const double prev_dt = s->p.dt;
+      s->p.dt *= s->p.max_courant / cfl;
+      if (int rc = scale_gc<R, G>(s, s->p.dt, prev_dt)) return rc;
+    }
+  }
   Box bi = interior_box(s->L);
   blaunch(s->stream, bi, k_div<R>, (R *)s->stat, (const R *)gc[0], (const R *)gc[1],
          (const R *)gc[s->L.nd == 3 ? 2 : 0], s->L, bi);
@@ -963,8 +1007,9 @@ void apply_rhs(mpdx_solver *s, double dt_arg)
 }
 
 template <class R, class G>
-int advance_impl(mpdx_solver *s, int64_t nt)
+int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
 {
+  const bool var_dt = s->p.var_dt;
   nt += s->timestep;
   if (s->timestep == 0) {
     if (int rc = ante_loop<R, G>(s)) return rc;
@@ -1008,7 +1053,7 @@ int advance_impl(mpdx_solver *s, int64_t nt)
   // individually (the legacy null stream cannot be captured)
   const bool graphs = fused && s->use_graphs && !s->prof_on && !s->calc_gc && s->stream != nullptr &&
                       s->p.upwind_filter_freq <= 0 && rs == MPDX_RHS_NONE;
-  while (s->timestep < nt) {
+  while (var_dt ? s->time < t_end : s->timestep < nt) {
     if (s->panic) break;
     if (graphs) {
       const int key = s->lev[0] & 1;
@@ -1032,6 +1077,23 @@ int advance_impl(mpdx_solver *s, int64_t nt)
       for (int e = 0; e < s->p.n_eqns; ++e) s->lev[e] ^= 1;
     } else {
       if (int rc = calc_gc_step<R, G>(s)) return rc;
+      // var_dt with a time-varying advector: shrink / grow dt until the
+      // Courant number is at most max_courant (solver_common.hpp:310-324)
+      if (var_dt && s->calc_gc) {
+        double cfl = 0;
+        if (int rc = courant_impl<R>(s, &cfl)) return rc;
+        int guard = 0;
+        while (cfl > 0) {
+          s->p.dt *= s->p.max_courant / cfl;
+          if (int rc = calc_gc_step<R, G>(s)) return rc;
+          if (int rc = courant_impl<R>(s, &cfl)) return rc;
+          if (!(cfl > s->p.max_courant)) break;
+          if (++guard == 64) {
+            s->err = "libmpdata++: var_dt did not bring the Courant number below max_courant";
+            return MPDX_EINVAL;
+          }
+        }
+      }
       // mpdata_rhs::hook_ante_step (solvers/mpdata_rhs.hpp:135-160)
       if (rs == MPDX_RHS_EULER_A || rs == MPDX_RHS_EULER_B) {
         if (int rc = update_rhs<R>(s, s->p.dt, 0)) return rc;
@@ -1048,7 +1110,7 @@ int advance_impl(mpdx_solver *s, int64_t nt)
     }
     HIPCHK(hipGetLastError());
     s->timestep++;
-    s->time = s->timestep * s->p.dt;
+    s->time = var_dt ? s->time + s->p.dt : s->timestep * s->p.dt;
     // mpdata_rhs::hook_post_step (solvers/mpdata_rhs.hpp:162-183)
     if (rs == MPDX_RHS_EULER_A) {
       apply_rhs<R>(s, s->p.dt);
@@ -1078,6 +1140,7 @@ void mpdx_params_default(mpdx_params *p)
   p->max_abs_div_eps = -1;
   p->guard = MPDX_GUARD_F64;
   p->path = MPDX_PATH_AUTO;
+  p->max_courant = 0.5;                     // solver_common.hpp:218
 }
 
 int mpdx_create(const mpdx_params *p, mpdx_solver **out)
@@ -1290,10 +1353,28 @@ int mpdx_get_advector(mpdx_solver *s, int d, void *buf, const int64_t strides[3]
 int mpdx_advance(mpdx_solver *s, int64_t nt)
 {
   if (nt < 0) { s->err = "libmpdata++: negative number of steps"; return MPDX_EINVAL; }
+  if (s->p.var_dt) { s->err = "libmpdata++: a var_dt solver advances by a duration (mpdx_advance_time)"; return MPDX_EINVAL; }
   if (s->esz == 8) return advance_impl<double, double>(s, nt);
   if (s->p.guard == MPDX_GUARD_NATIVE) return advance_impl<float, float>(s, nt);
   if (s->p.guard == MPDX_GUARD_F64_EXACT) return advance_impl<float, double>(s, nt);
   return advance_impl<float, GuardFast>(s, nt);
+}
+
+int mpdx_advance_time(mpdx_solver *s, double duration)
+{
+  if (!s->p.var_dt) { s->err = "libmpdata++: advancing by a duration needs var_dt"; return MPDX_EINVAL; }
+  if (!(duration >= 0)) { s->err = "libmpdata++: negative duration"; return MPDX_EINVAL; }
+  const double t_end = s->time + duration;
+  if (s->esz == 8) return advance_impl<double, double>(s, 0, t_end);
+  if (s->p.guard == MPDX_GUARD_NATIVE) return advance_impl<float, float>(s, 0, t_end);
+  if (s->p.guard == MPDX_GUARD_F64_EXACT) return advance_impl<float, double>(s, 0, t_end);
+  return advance_impl<float, GuardFast>(s, 0, t_end);
+}
+
+int mpdx_dt(const mpdx_solver *s, double *dt)
+{
+  *dt = s->p.dt;
+  return MPDX_OK;
 }
 
 int mpdx_reduce(mpdx_solver *s, int e, int op, double *out)
@@ -1387,6 +1468,10 @@ int mpdx_advector_device(mpdx_solver *s, int d, void **ptr, int64_t strides[3])
 int mpdx_set_advector_generator(mpdx_solver *s, int kind, const double *params, int n_params)
 {
   if (kind == MPDX_GC_NONE) { s->gen_kind = kind; return MPDX_OK; }
+  if (s->p.var_dt) {
+    s->err = "libmpdata++: the built-in advector generator does not follow a variable dt";
+    return MPDX_EUNSUPPORTED;
+  }
   if (kind != MPDX_GC_DEFORM || n_params < 2 || !(params[0] > 0) || !(params[1] > 0)) {
     s->err = "libmpdata++: bad advector generator parameters";
     return MPDX_EINVAL;

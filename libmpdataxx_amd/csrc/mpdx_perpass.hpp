@@ -340,6 +340,22 @@ __global__ void k_div(R *stat, const R *g0, const R *g1, const R *g2, Layout L, 
   stat[p] = v;
 }
 
+// Courant number per interior cell into stat (solver_3d.hpp:390-398,
+// solver_2d.hpp:295-303)
+template <class R>
+__global__ void k_courant(R *stat, const R *g0, const R *g1, const R *g2, Layout L, Box bx)
+{
+  int i, j, k;
+  if (!box_idx(bx, i, j, k)) return;
+  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
+  R v;
+  if (L.nd == 3)
+    v = R(0.5) * (fabs(g0[p] + g0[p - L.st[0]]) + fabs(g1[p] + g1[p - L.st[1]]) + fabs(g2[p] + g2[p - L.st[2]])) / 1;
+  else
+    v = R(0.5) * (fabs(g0[p] + g0[p - L.st[0]]) + fabs(g1[p] + g1[p - L.st[1]])) / 1;
+  stat[p] = v;
+}
+
 // reversing deformational flow from a streamfunction on cell corners
 // (the analytic advector of the reference's reversing_deform test family,
 // tests/mp3_paper_2018_JCP/reversing_deform/reversing_deform.hpp:30-105):

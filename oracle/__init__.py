@@ -38,6 +38,8 @@ class _Params(ctypes.Structure):
         ("hint_scale", ctypes.c_int * MAX_EQNS),
         ("dt", ctypes.c_double),
         ("max_abs_div_eps", ctypes.c_double),
+        ("var_dt", ctypes.c_int),
+        ("max_courant", ctypes.c_double),
     ]
 
 
@@ -79,6 +81,9 @@ def _lib(fast: bool = False):
         lib.orc_set_generator.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         lib.orc_set_rhs_scheme.argtypes = [P, ctypes.c_int, RHS_FN, ctypes.c_void_p]
         lib.orc_set_rhs.argtypes = [P, ctypes.c_int, dp]
+        lib.orc_advance_time.argtypes = [P, ctypes.c_double, ctypes.c_char_p, ctypes.c_int]
+        lib.orc_dt.argtypes = [P]
+        lib.orc_dt.restype = ctypes.c_double
         _libs[name] = lib
     return _libs[name]
 
@@ -98,7 +103,7 @@ class Oracle:
 
     def __init__(self, grid, opts=FCT, n_iters=2, n_eqns=1, bcs=None,
                  dt=0.0, upwind_filter_freq=0, hint_scale=None,
-                 max_abs_div_eps=-1.0, fast=False):
+                 max_abs_div_eps=-1.0, fast=False, var_dt=False, max_courant=0.5):
         self._lib = _lib(fast)
         nd = len(grid)
         if bcs is None:
@@ -116,6 +121,8 @@ class Oracle:
             p.hint_scale[e] = int(v)
         p.dt = float(dt)
         p.max_abs_div_eps = float(max_abs_div_eps)
+        p.var_dt = int(bool(var_dt))
+        p.max_courant = float(max_courant)
         err = ctypes.create_string_buffer(256)
         h = self._lib.orc_create(ctypes.byref(p), err, 256)
         if not h:
@@ -181,6 +188,16 @@ class Oracle:
         assert a.shape == tuple(self.grid), (a.shape, self.grid)
         if self._lib.orc_set_rhs(self._h, e, _dp(a)):
             raise ValueError("rhs not enabled or bad equation index")
+
+    def advance_time(self, duration):
+        """var_dt: advance by a duration with the adaptive time step."""
+        err = ctypes.create_string_buffer(256)
+        if self._lib.orc_advance_time(self._h, float(duration), err, 256):
+            raise RuntimeError(err.value.decode())
+
+    @property
+    def dt(self):
+        return self._lib.orc_dt(self._h)
 
     def advance(self, nt):
         err = ctypes.create_string_buffer(256)

@@ -492,6 +492,46 @@ static double max_abs_div(orc_t *s)
   return mx;
 }
 
+/* max Courant number over the interior, solver_3d.hpp:390-398 (2D:
+ * solver_2d.hpp:295-303) */
+static double courant_number(orc_t *s)
+{
+  const int nd = s->nd, n0 = s->n[0], n1 = s->n[1], n2 = nd == 3 ? s->n[2] : 1;
+  const long s0 = s->st[0], s1 = s->st[1], s2 = s->st[2];
+  double mx = -INFINITY;
+  for (int i = 0; i < n0; ++i)
+    for (int j = 0; j < n1; ++j)
+      for (int k = 0; k < n2; ++k) {
+        const long p = ix(s, i, j, k);
+        double v;
+        if (nd == 3)
+          v = 0.5 * (fabs(s->GC[0][p] + s->GC[0][p - s0]) + fabs(s->GC[1][p] + s->GC[1][p - s1])
+                     + fabs(s->GC[2][p] + s->GC[2][p - s2])) / 1;
+        else
+          v = 0.5 * (fabs(s->GC[0][p] + s->GC[0][p - s0]) + fabs(s->GC[1][p] + s->GC[1][p - s1])) / 1;
+        mx = dmax(mx, v);
+      }
+  return mx;
+}
+
+/* scale_gc, solver_3d.hpp:411-421 (solver_2d.hpp:315-324): the advector
+ * view (faces -1/2 .. n-3/2 along d, interior across) times cur/old, then
+ * its halos */
+static void scale_gc(orc_t *s, double cur_dt, double old_dt)
+{
+  const int nd = s->nd;
+  for (int d = 0; d < nd; ++d) {
+    int lo[3] = {0, 0, 0}, hi[3] = {s->n[0] - 1, s->n[1] - 1, nd == 3 ? s->n[2] - 1 : 0};
+    lo[d] = -1;
+    hi[d] = s->n[d] - 2;
+    for (int i = lo[0]; i <= hi[0]; ++i)
+      for (int j = lo[1]; j <= hi[1]; ++j)
+        for (int k = lo[2]; k <= hi[2]; ++k) s->GC[d][ix(s, i, j, k)] *= cur_dt / old_dt;
+  }
+  xchng_vctr_alng(s, s->GC, 0);
+  xchng_vctr_nrml(s, s->GC, s->hw - 1);
+}
+
 /* ---------------------------------------------------------------- driver */
 
 /* GC_unco / GC_corr bookkeeping, solvers/detail/mpdata_common.hpp:40-60 */
@@ -591,6 +631,7 @@ static void calc_gc_deform(orc_t *s)
 int orc_set_generator(orc_t *s, int kind, double T, double max_courant)
 {
   if (kind == 0) { s->gen_kind = 0; return 0; }
+  if (s->p.var_dt) return 1;   /* its amplitude does not follow dt */
   if (kind != 1 || !(T > 0) || !(max_courant > 0)) return 1;
   const int n0 = s->n[0], n1 = s->n[1];
   const double pi = 3.141592653589793;
@@ -664,28 +705,63 @@ int orc_set_rhs(orc_t *s, int e, const double *dense)
   return 0;
 }
 
+/* hook_ante_loop chain: solver_common.hpp:191-211 (advector halos along;
+ * var_dt: rescale dt and the advector to max_courant), solver_3d.hpp:375-388
+ * (divergence check), mpdata_osc_3d.hpp:40-50 (tangential advector halos,
+ * ext = halo-1), mpdata_rhs.hpp:116-133 (trapez: R^0) */
+static int ante_loop(orc_t *s, char *err, int errlen)
+{
+  xchng_vctr_alng(s, s->GC, 0);
+  if (s->p.var_dt) {
+    const double cfl = courant_number(s);
+    if (cfl > 0) {
+      const double prev_dt = s->p.dt;
+      s->p.dt *= s->p.max_courant / cfl;
+      scale_gc(s, s->p.dt, prev_dt);
+    }
+  }
+  const double eps = s->p.max_abs_div_eps < 0 ? DBL_EPSILON : s->p.max_abs_div_eps;
+  if (max_abs_div(s) > eps) {
+    seterr(err, errlen, "libmpdata++: initial advector field is divergent");
+    return 1;
+  }
+  xchng_vctr_nrml(s, s->GC, s->hw - 1);
+  if (s->rhs_scheme == 3 && update_rhs(s, s->p.dt / 2, 0)) {
+    seterr(err, errlen, "libmpdata++: update_rhs failed");
+    return 1;
+  }
+  return 0;
+}
+
+static int solve(orc_t *s, long nt, double t_end, char *err, int errlen);
+
 /* solve(): solver_common.hpp:270-358 */
 int orc_advance(orc_t *s, long nt, char *err, int errlen)
 {
-  nt += s->timestep;
-  if (s->timestep == 0) {
-    /* hook_ante_loop chain: solver_common.hpp:191-211 (advector halos along),
-     * solver_3d.hpp:375-388 (divergence check), mpdata_osc_3d.hpp:40-50
-     * (tangential advector halos, ext = halo-1) */
-    xchng_vctr_alng(s, s->GC, 0);
-    const double eps = s->p.max_abs_div_eps < 0 ? DBL_EPSILON : s->p.max_abs_div_eps;
-    if (max_abs_div(s) > eps) {
-      seterr(err, errlen, "libmpdata++: initial advector field is divergent");
-      return 1;
-    }
-    xchng_vctr_nrml(s, s->GC, s->hw - 1);
-    /* mpdata_rhs::hook_ante_loop (mpdata_rhs.hpp:116-133) */
-    if (s->rhs_scheme == 3 && update_rhs(s, s->p.dt / 2, 0)) {
-      seterr(err, errlen, "libmpdata++: update_rhs failed");
-      return 1;
-    }
+  if (s->p.var_dt) {
+    seterr(err, errlen, "libmpdata++: with var_dt advance by a duration");
+    return 1;
   }
-  while (s->timestep < nt) {
+  return solve(s, s->timestep + nt, 0, err, errlen);
+}
+
+int orc_advance_time(orc_t *s, double duration, char *err, int errlen)
+{
+  if (!s->p.var_dt) {
+    seterr(err, errlen, "libmpdata++: advance by a duration needs var_dt");
+    return 1;
+  }
+  return solve(s, 0, s->time + duration, err, errlen);
+}
+
+double orc_dt(const orc_t *s) { return s->p.dt; }
+
+static int solve(orc_t *s, long nt, double t_end, char *err, int errlen)
+{
+  const int var_dt = s->p.var_dt;
+  if (s->timestep == 0)
+    if (ante_loop(s, err, errlen)) return 1;
+  while (var_dt ? s->time < t_end : s->timestep < nt) {
     if (s->panic) break;
     /* calc_gc() at the top of each step (solver_common.hpp:306-308) */
     if (s->gen_kind == 1) calc_gc_deform(s);
@@ -704,7 +780,7 @@ int orc_advance(orc_t *s, long nt, char *err, int errlen)
       scale(s, e, -s->p.hint_scale[e]);
     }
     s->timestep++;
-    s->time = s->timestep * s->p.dt;
+    s->time = var_dt ? s->time + s->p.dt : s->timestep * s->p.dt;
     /* mpdata_rhs::hook_post_step (mpdata_rhs.hpp:162-183) */
     if (s->rhs_scheme == 1) {
       apply_rhs(s, s->p.dt);

@@ -34,6 +34,8 @@ typedef struct {
   int hint_scale[ORC_MAX_EQNS];
   double dt;
   double max_abs_div_eps;     /* < 0 selects numeric_limits<double>::epsilon() */
+  int var_dt;                 /* ct_params_t::var_dt: adaptive time step */
+  double max_courant;         /* rt_params_t::max_courant (var_dt) */
 } orc_params_t;
 
 /* returns NULL and writes a message into err on failure */
@@ -53,6 +55,12 @@ int orc_get_advector(const orc_t *s, int d, double *dense);
 
 /* runs nt more time steps; nonzero return = error (message in err) */
 int orc_advance(orc_t *s, long nt, char *err, int errlen);
+
+/* var_dt: advance by a duration (the reference's advance(real_t) with
+ * ct_params_t::var_dt, solver_common.hpp:270-274,312-336); dt is rescaled so
+ * that the Courant number is max_courant */
+int orc_advance_time(orc_t *s, double duration, char *err, int errlen);
+double orc_dt(const orc_t *s);
 
 /* time-varying advector recomputed at the start of every step (the
  * reversing_deform calc_gc pattern): kind 1 = reversing deformational flow

@@ -239,3 +239,63 @@ def test_calc_gc_callback_matches_oracle(dtype):
     s.close()
     assert calls == [k + 0.5 for k in range(nt)]
     assert rel_err(got, o.advectee()) <= (1e-12 if dtype == np.float64 else 1e-5)
+
+
+@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_var_dt_matches_oracle(dtype, path):
+    """var_dt: the initial rescaling of dt and the advector to max_courant,
+    then steps until the requested duration, against the oracle."""
+    import libmpdataxx_amd as M
+    grid, bcs = (30, 26, 20), [(1, 1), (0, 0), (1, 1)]
+    psi, gc = setup_case(grid, bcs, 55)
+    print(f"var_dt dtype={np.dtype(dtype).name} path={path}", flush=True)
+    kw = dict(opts=O.FCT, n_iters=3, bcs=bcs, dt=0.2, max_abs_div_eps=1e300, var_dt=True, max_courant=0.42)
+    o = O.Oracle(grid, **kw)
+    s = M.Solver(grid, dtype=dtype, path=path, **kw)
+    for x in (o, s):
+        x.set_advectee(psi)
+        for d in range(3):
+            x.set_advector(gc[d], d)
+    o.advance_time(1.5)
+    s.advance_time(1.5)
+    assert s.timestep == o.timestep
+    assert abs(s.dt - o.dt) <= 1e-15 * o.dt * (1 if dtype == np.float64 else 1e8)
+    assert rel_err(s.advectee().astype(np.float64), o.advectee()) <= (1e-12 if dtype == np.float64 else 1e-5)
+    with pytest.raises(Exception):
+        s.advance(1)
+    s.close()
+
+
+def test_var_dt_with_calc_gc_rescales_every_step():
+    """var_dt with a time-varying advector: the callback writes velocity x
+    dt; the library shrinks dt until the Courant number is max_courant
+    (solver_common.hpp:310-324), and the run equals a fixed-dt run at that
+    dt."""
+    import libmpdataxx_amd as M
+    grid, bcs = (24, 20, 18), [(0, 0)] * 3
+    psi, vel = setup_case(grid, bcs, 56)     # vel: Courant numbers per unit dt
+    print("var_dt with calc_gc", flush=True)
+    s = M.Solver(grid, opts=M.FCT, n_iters=2, bcs=bcs, dt=1.0, var_dt=True, max_courant=0.3,
+                 max_abs_div_eps=1e300)
+    s.set_advectee(psi)
+    for d in range(3):
+        s.set_advector(vel[d], d)
+
+    def calc_gc(t_half):
+        for d in range(3):
+            s.set_advector(vel[d] * s.dt, d)
+        return True
+    s.set_calc_gc(calc_gc)
+    s.advance_time(10.0)
+    dt = s.dt
+    steps = s.timestep
+    got = s.advectee()
+    s.close()
+    f = M.Solver(grid, opts=M.FCT, n_iters=2, bcs=bcs, dt=dt, max_abs_div_eps=1e300)
+    f.set_advectee(psi)
+    for d in range(3):
+        f.set_advector(vel[d] * dt, d)
+    f.advance(steps)
+    assert rel_err(f.advectee(), got) <= 1e-12
+    f.close()

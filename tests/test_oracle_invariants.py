@@ -223,3 +223,54 @@ def test_deform_generator_reverses():
     full = np.abs(o.advectee() - psi).max()
     assert half > 0.3
     assert full < 0.3 * half, (full, half)
+
+
+def _var_dt_case():
+    grid = (20, 18, 16)
+    psi = W.random_field(grid, 21)
+    gc = W.streamfunction_advector(grid, 22, periodic=False, max_courant=0.3)
+    return grid, psi, gc
+
+
+def _courant(o, grid):
+    """0.5 (|C(i+1/2) + C(i-1/2)| summed over d) maxed over the interior,
+    from the halo-extended advector arrays (element i <-> face i + 1/2)."""
+    n0, n1, n2 = grid
+    hw, g0 = o.full(1, 0)
+    g1, g2 = o.full(1, 1)[1], o.full(1, 2)[1]
+    I = slice(hw, hw + n0)
+    J = slice(hw, hw + n1)
+    K = slice(hw, hw + n2)
+    Im, Jm, Km = slice(hw - 1, hw + n0 - 1), slice(hw - 1, hw + n1 - 1), slice(hw - 1, hw + n2 - 1)
+    c = 0.5 * (np.abs(g0[I, J, K] + g0[Im, J, K]) + np.abs(g1[I, J, K] + g1[I, Jm, K])
+               + np.abs(g2[I, J, K] + g2[I, J, Km]))
+    return float(c.max())
+
+
+def test_var_dt_matches_prescaled_fixed_dt():
+    """var_dt (solver_common.hpp:199-210,312-336) with a constant advector
+    rescales dt and the advector once so the Courant number is max_courant;
+    the run equals a fixed-dt run whose advector is pre-scaled by the same
+    factor, and advance(duration) stops at the first step reaching it."""
+    grid, psi, gc = _var_dt_case()
+    bcs = [(O.OPEN, O.OPEN)] * 3
+    v = O.Oracle(grid, opts=O.FCT, n_iters=3, bcs=bcs, dt=0.1, var_dt=True, max_courant=0.45,
+                 max_abs_div_eps=1e300)
+    v.set_advectee(psi)
+    for d in range(3):
+        v.set_advector(gc[d], d)
+    v.advance_time(1.0)
+    ratio = v.dt / 0.1
+    assert v.time >= 1.0 and v.time - v.dt < 1.0
+    steps = v.timestep
+    f = O.Oracle(grid, opts=O.FCT, n_iters=3, bcs=bcs, dt=v.dt, max_abs_div_eps=1e300)
+    f.set_advectee(psi)
+    for d in range(3):
+        f.set_advector(gc[d] * ratio, d)
+    f.advance(steps)
+    assert np.array_equal(v.advectee(), f.advectee())
+    # the rescaled advector's Courant number is max_courant (interior
+    # cells whose faces are all inside the views)
+    assert abs(_courant(v, grid) - 0.45) < 1e-12
+    with pytest.raises(RuntimeError):
+        v.advance(1)
