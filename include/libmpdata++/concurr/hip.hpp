@@ -7,13 +7,15 @@
  * (concurr/any.hpp, concurr_common.hpp:380-453) and the same exception texts.
  * The state lives in GPU memory; advectee()/advector() return views of host
  * mirrors that are uploaded before and refreshed after every advance().
- * All work goes through the C ABI of include/mpdx.h.
+ * All work goes through the C ABI of include/mpdx.h; a solver decorated with
+ * output::hdf5 / output::hdf5_xdmf records through include/mpdx_h5.h.
  */
 #pragma once
 
 #include <array>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include <mpdx.h>
@@ -26,6 +28,16 @@ namespace libmpdataxx
 {
   namespace concurr
   {
+    namespace detail
+    {
+      // solvers decorated with an output (output/hdf5.hpp) carry a recorder type
+      template <class S, class = void> struct has_output : std::false_type {};
+      template <class S> struct has_output<S, std::void_t<typename S::output_recorder_t>> : std::true_type {};
+      struct no_output {};
+      template <class S, bool = has_output<S>::value> struct recorder_of { using type = no_output; };
+      template <class S> struct recorder_of<S, true> { using type = typename S::output_recorder_t; };
+    } // namespace detail
+
     template <class solver_t_,
               bcond::bcond_e bcxl, bcond::bcond_e bcxr,
               bcond::bcond_e bcyl = bcond::null, bcond::bcond_e bcyr = bcond::null,
@@ -53,6 +65,8 @@ namespace libmpdataxx
       std::array<std::vector<real_t>, 3> gc_host_;
       std::array<bool, 3> gc_dirty_{};
       bool panic_ = false;
+      static constexpr bool with_output = detail::has_output<solver_t_>::value;
+      typename detail::recorder_of<solver_t_>::type out_;
 
       static int bc_code(bcond::bcond_e b)
       {
@@ -115,8 +129,10 @@ namespace libmpdataxx
       public:
 
       // concurr_common ctor (concurr_common.hpp:176-187) + solver ctor checks
-      explicit hip(const typename solver_t::rt_params_t &p)
+      explicit hip(const typename solver_t::rt_params_t &p_)
       {
+        typename solver_t::rt_params_t p = p_;
+        if constexpr (with_output) out_.init(p);
         mpdx_params mp;
         mpdx_params_default(&mp);
         mp.n_dims = n_dims;
@@ -165,10 +181,12 @@ namespace libmpdataxx
         }
         else
         {
+          if constexpr (with_output) out_.start(h_, nt);
           for (advance_arg_t t = 0; t < nt; ++t)
           {
             if (panic_) break;
             check(mpdx_advance(h_, 1));
+            if constexpr (with_output) out_.post_step(h_);
           }
         }
         download();

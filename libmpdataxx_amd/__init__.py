@@ -155,6 +155,7 @@ class Solver:
             raise MpdxError(rc, L.mpdx_last_error(None).decode())
         self._L = L
         self._h = h
+        self.params = p
         self.grid = tuple(int(g) for g in grid)
         self.n_dims = nd
         self.n_eqns = n_eqns

@@ -10,6 +10,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -27,6 +28,45 @@ SOURCES = ["mpdx_capi.hip"]
 
 
 CPP_TESTS = ["revolving_sphere_3d", "rotating_cone_2d"]
+# HDF5 (output only): the image's HDF5 C library; the writer is host code
+HDF5_ROOT = os.environ.get("MPDX_HDF5_ROOT", "/opt/conda")
+H5LIB = os.path.join(HERE, "libmpdx_h5.so")
+CPP_TESTS_H5 = ["revolving_sphere_3d_h5"]
+
+
+def have_hdf5() -> bool:
+    return os.path.exists(os.path.join(HDF5_ROOT, "include", "hdf5.h"))
+
+
+def _h5_flags():
+    # HDF5 by full path, not -L: that directory also holds an older libstdc++
+    # which must not shadow the system one the ROCm runtime needs; for the
+    # same reason the writer carries its own copy of libstdc++/libgcc.
+    lib = os.path.join(HDF5_ROOT, "lib")
+    return ["-I", os.path.join(HDF5_ROOT, "include"), os.path.join(lib, "libhdf5.so"),
+            f"-Wl,-rpath,{lib}", "-static-libstdc++", "-static-libgcc", "-Wl,--exclude-libs,ALL"]
+
+
+def build_h5(verbose: bool = False) -> str | None:
+    """Compile the HDF5/XDMF writer libmpdx_h5.so (host code, g++) when the
+    HDF5 headers are installed; returns its path or None."""
+    if not have_hdf5():
+        if verbose:
+            print(f"HDF5 not found under {HDF5_ROOT}: libmpdx_h5.so not built", flush=True)
+        return None
+    tmp = H5LIB + ".tmp"
+    with tempfile.TemporaryDirectory() as td:
+        # export the C ABI only (not the template code of the bundled libstdc++)
+        vs = os.path.join(td, "exports.map")
+        with open(vs, "w") as f:
+            f.write("{ global: mpdx_h5_*; local: *; };\n")
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-fvisibility=hidden",
+               os.path.join(CSRC, "mpdx_h5.cpp"), *_h5_flags(), f"-Wl,--version-script={vs}", "-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    os.replace(tmp, H5LIB)
+    return H5LIB
 
 
 def build_cpp_tests(verbose: bool = False) -> None:
@@ -41,6 +81,14 @@ def build_cpp_tests(verbose: bool = False) -> None:
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
+    if have_hdf5():
+        for name in CPP_TESTS_H5:
+            cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                   os.path.join(src_dir, name + ".cpp"), "-L", HERE, "-lmpdx", "-lmpdx_h5",
+                   "-Wl,-rpath,$ORIGIN/../../../libmpdataxx_amd", "-o", os.path.join(out_dir, name)]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True)
 
 
 RESOURCES = os.path.join(HERE, "kernel_resources.txt")
@@ -90,5 +138,6 @@ def build(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_h5(verbose=True)
     build_cpp_tests(verbose=True)
     sys.exit(0)

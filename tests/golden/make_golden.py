@@ -9,6 +9,10 @@ Fixtures written (all data, no code):
   revolving_sphere_3d/stats_<variant>.txt    refdata/stats_<variant>.txt.gz
   rotating_cone_2d/stats_<variant>_.txt      refdata/stats_<variant>_.txt.gz
   coupled_harmosc/stats.txt                  6_coupled_harmosc/refdata/stats.txt.gz
+  revolving_sphere_3d_h5/*.xmf               refdata/fct/*.xmf (the same in every variant)
+  revolving_sphere_3d_h5/const_<variant>.txt h5dump -A of refdata/<variant>/const.h5
+  revolving_sphere_3d_h5/const_data.sha256   sha256 of the raw float32 X, Y, Z, T of
+                                              each refdata/<variant>/const.h5
   SHA256SUMS                                 checksums of the above
 
 The HDF5 datasets are float32 little-endian, C order (output/hdf5.hpp:56,
@@ -40,6 +44,34 @@ def dump_psi(h5path, out_gz):
         f.write(data)
 
 
+def h5_fixtures(rs):
+    """The output layout of the revolving-sphere runs (output::hdf5_xdmf):
+    the XDMF files (the same bytes in every variant's directory), the
+    header and attribute dump of each variant's const.h5 (h5dump -A), and
+    checksums of the raw float32 bytes of its X, Y, Z and T datasets."""
+    out = os.path.join(HERE, "revolving_sphere_3d_h5")
+    os.makedirs(out, exist_ok=True)
+    xmfs = ["timestep0000000000.xmf", "timestep0000000556.xmf", "temp.xmf"]
+    for name in xmfs:
+        data = open(os.path.join(rs, "fct", name), "rb").read()
+        for v in SPHERE_VARIANTS:
+            assert open(os.path.join(rs, v, name), "rb").read() == data, (v, name)
+        open(os.path.join(out, name), "wb").write(data)
+    sums = []
+    for v in SPHERE_VARIANTS:
+        vdir = os.path.join(rs, v)
+        txt = subprocess.run([H5DUMP, "-A", "const.h5"], cwd=vdir, capture_output=True, check=True).stdout
+        open(os.path.join(out, f"const_{v}.txt"), "wb").write(txt)
+        with tempfile.TemporaryDirectory() as tmp:
+            for ds in ("X", "Y", "Z", "T"):
+                raw = os.path.join(tmp, ds + ".bin")
+                subprocess.run([H5DUMP, "-d", "/" + ds, "-b", "LE", "-o", raw, os.path.join(vdir, "const.h5")],
+                               check=True, stdout=subprocess.DEVNULL, cwd=tmp)
+                sums.append(f"{hashlib.sha256(open(raw, 'rb').read()).hexdigest()}  {v}/const.h5:/{ds}")
+    with open(os.path.join(out, "const_data.sha256"), "w") as f:
+        f.write("\n".join(sums) + "\n")
+
+
 def main():
     sphere = os.path.join(HERE, "revolving_sphere_3d")
     cone = os.path.join(HERE, "rotating_cone_2d")
@@ -64,8 +96,9 @@ def main():
     with gzip.open(os.path.join(REF, "6_coupled_harmosc", "refdata", "stats.txt.gz"), "rb") as f, \
             open(os.path.join(osc, "stats.txt"), "wb") as g:
         shutil.copyfileobj(f, g)
+    h5_fixtures(rs)
     lines = []
-    for sub in ("revolving_sphere_3d", "rotating_cone_2d", "coupled_harmosc"):
+    for sub in ("revolving_sphere_3d", "revolving_sphere_3d_h5", "rotating_cone_2d", "coupled_harmosc"):
         for name in sorted(os.listdir(os.path.join(HERE, sub))):
             path = os.path.join(HERE, sub, name)
             digest = hashlib.sha256(open(path, "rb").read()).hexdigest()
