@@ -124,7 +124,10 @@ int mpdx_set_stream(mpdx_solver *h, void *stream);
  * in elements, for [x][y][z]; on_device != 0 means buf is device memory.
  * The advectee covers the interior (grid); the advector(d) view covers the
  * same indices as the reference's advector(d): faces -1..n_d-2 along d and,
- * for 3D components d > 0, x = -1..n0-1 (buf points at that first element). */
+ * for 3D components d > 0, x = -1..n0-1 (buf points at that first element).
+ * With on_device the call first waits for all work enqueued on the device
+ * (any stream, so a buffer still being produced elsewhere is complete) and
+ * returns when the copy is done; host copies return when done as well. */
 int mpdx_advector_extents(const mpdx_solver *h, int d, int ext[3]);
 int mpdx_set_advectee(mpdx_solver *h, int e, const void *buf, const int64_t strides[3], int on_device);
 int mpdx_get_advectee(mpdx_solver *h, int e, void *buf, const int64_t strides[3], int on_device);
@@ -188,8 +191,8 @@ int mpdx_set_advector_generator(mpdx_solver *h, int kind, const double *params, 
  * dt, at) (at = 0: R^n, at = 1: R^{n+1}, trapez only): before each call the
  * forcing arrays are zeroed over the interior, as the reference's base
  * update_rhs does (mpdata_rhs.hpp:52-70); fn fills them with mpdx_set_rhs or
- * through mpdx_rhs_device, reading the state with mpdx_get_advectee or
- * mpdx_advectee_device.  Forcing is added over the interior (apply_rhs,
+ * through mpdx_rhs_device (kernels on the solver's stream), reading the state
+ * with mpdx_get_advectee or mpdx_advectee_device.  Forcing is added over the interior (apply_rhs,
  * mpdata_rhs.hpp:72-84).  The `mixed` scheme and hint_norhs are not
  * supported. */
 #define MPDX_RHS_NONE 0

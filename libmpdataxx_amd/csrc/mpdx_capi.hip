@@ -463,6 +463,9 @@ int copy_view(mpdx_solver *s, void *arr, const int lo[3], const int hi[3], void 
   void *dbuf = buf;
   const long *bst = ust;
   std::vector<char> host;
+  // a device buffer may come from work on any stream (the library's own
+  // stream is non-blocking): wait for everything enqueued on the device
+  if (on_device) HIPCHK(hipDeviceSynchronize());
   if (!on_device) {
     if (int rc = ensure_staging(s, n * esz)) return rc;
     host.resize(n * esz);
