@@ -68,28 +68,12 @@ def profile_figure(fname, key, kernel):
     return None
 
 
-def rotation_sphere_device(torch, n, dtype, dev):
-    """rotation_sphere_3d of libmpdataxx_amd.workloads, generated on the GPU."""
-    c = (n - 1) / 2.0
-    w = 0.5 / c
-    radius, offset = 40.0 * n / 256, 48.0 * n / 256
-    i = torch.arange(n, dtype=torch.float64, device=dev)
-    r2 = ((i[:, None, None] - (c + offset)) ** 2 + (i[None, :, None] - c) ** 2
-          + (i[None, None, :] - c) ** 2)
-    psi = torch.where(r2 <= radius ** 2, 1.0, 0.0).to(dtype)
-    del r2
-    gx = (-w * (i - c)).to(dtype)[None, :, None].expand(n, n, n).contiguous()
-    ia = torch.arange(-1, n, dtype=torch.float64, device=dev)
-    gy = (w * (ia - c)).to(dtype)[:, None, None].expand(n + 1, n - 1, n).contiguous()
-    gz = torch.zeros((n + 1, n, n - 1), dtype=dtype, device=dev)
-    return psi, [gx, gy, gz]
-
-
 def run_gpu(torch, M, n, dtype, steps, warmup, path):
     dev = torch.device("cuda:0")
     s = M.Solver((n, n, n), opts=M.FCT, n_iters=3, dtype=dtype, path=path,
                  bcs=[(M.OPEN, M.OPEN)] * 3)
-    psi, gc = rotation_sphere_device(torch, n, torch.float32 if dtype == "float32" else torch.float64, dev)
+    from libmpdataxx_amd.workloads_torch import rotation_sphere_device
+    psi, gc = rotation_sphere_device(n, torch.float32 if dtype == "float32" else torch.float64, dev)
     s.set_advectee_device(psi)
     for d in range(3):
         s.set_advector_device(gc[d], d)
@@ -115,43 +99,14 @@ def run_gpu(torch, M, n, dtype, steps, warmup, path):
     return s, (t1 - t0), prof, mass0, (t3 - t2)
 
 
-def multi_tracer_fields(torch, shape, dev):
-    """The eight tracers of the multi-tracer workload on a cyclic grid of
-    period n-1 (cell n-1 duplicates cell 0): 0-3 lognormal exp(g/2) with g a
-    unit-variance sum of 8 random Fourier modes, 4-6 slotted cylinders along
-    z, 7 the constant 1."""
-    n0, n1, n2 = shape
-    out = []
-    ix = [torch.arange(n, dtype=torch.float64, device=dev) for n in shape]
-    X, Y, Z = ix[0][:, None, None], ix[1][None, :, None], ix[2][None, None, :]
-    for t in range(8):
-        g = torch.Generator().manual_seed(4000 + t)
-        if t < 4:
-            f = torch.zeros(shape, dtype=torch.float64, device=dev)
-            for _ in range(8):
-                k = torch.randint(-4, 5, (3,), generator=g).tolist()
-                ph = float(torch.rand(1, generator=g)) * 2 * math.pi
-                f += 0.5 * torch.cos(2 * math.pi * (k[0] * X / (n0 - 1) + k[1] * Y / (n1 - 1)
-                                                    + k[2] * Z / (n2 - 1)) + ph)
-            f = torch.exp(0.5 * f)
-        elif t < 7:
-            c = (torch.rand(2, generator=g) * 0.5 + 0.25).tolist()
-            cx, cy, r = c[0] * (n0 - 1), c[1] * (n1 - 1), 0.15 * (n0 - 1)
-            inside = ((X - cx) ** 2 + (Y - cy) ** 2 <= r * r) & ~((abs(X - cx) <= r / 6) & (Y - cy >= -r / 3))
-            f = torch.where(inside, 1.0, 0.1).expand(shape).clone()
-        else:
-            f = torch.ones(shape, dtype=torch.float64, device=dev)
-        out.append(f.to(torch.float32).contiguous())
-    return out
-
-
 def run_multi_tracer(torch, M, steps, warmup, shape=(512, 512, 128), n_tr=8):
     """3D MPDATA-FCT n_iters=3, cyclic, 8 tracers sharing one time-varying
     (reversing deformational) advector generated on the device every step."""
     dev = torch.device("cuda:0")
     s = M.Solver(shape, opts=M.FCT, n_iters=3, n_eqns=n_tr, dtype="float32", dt=1.0,
                  bcs=[(M.CYCLIC, M.CYCLIC)] * 3)
-    for e, f in enumerate(multi_tracer_fields(torch, shape, dev)):
+    from libmpdataxx_amd.workloads_torch import multi_tracer_fields
+    for e, f in enumerate(multi_tracer_fields(shape, dev)):
         s.set_advectee_device(f, e)
     del f
     for d in range(3):
