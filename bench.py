@@ -9,7 +9,7 @@ in FP64 for the FP32/FP64 ratio and as the accuracy reference (the FP64 GPU
 path agrees with the FP64 CPU oracle to <= 1e-12).  A second leg runs the
 multi-tracer workload (512x512x128 cyclic, 8 tracers, time-varying
 deformational advector generated on the device each step).  Inputs are
-generated on the device and resident in HBM before the timed regions.
+resident in HBM before the timed regions.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 

@@ -217,6 +217,39 @@ int mpdx_profile(mpdx_solver *h, int enable);
 int mpdx_profile_read(mpdx_solver *h, int idx, char *name, int name_len,
                       double *total_ms, int64_t *launches);
 
+/* Length, in output planes, of the x chunks the 3D interior kernel marches
+ * through (0 = automatic: as long as possible while leaving two tiles per
+ * CU).  A tuning knob of this backend with no reference counterpart; results
+ * do not depend on it. */
+int mpdx_set_x_chunk(mpdx_solver *h, int planes);
+
+/* Ensembles: independent solvers (domains of any size, dimensionality,
+ * options and precision) advanced together, as the reference advances each
+ * solver object with its own advance(nt) (concurr_common.hpp:380-385).  One
+ * hipGraph holds one time step of every member on parallel branches and is
+ * launched once per step.  Members stay owned by the caller, must outlive
+ * the ensemble, and keep working on their own between ensemble calls.  They
+ * must use the fused path and have no host callback (calc_gc, forcing), no
+ * advector generator, no upwind filter and no var_dt.  The panic flags are
+ * polled between steps: a step runs only while no member's flag is set.
+ * Stepping (advance, tune) is switched off unless the environment sets
+ * MPDX_ENSEMBLE_GRAPH=1 (MPDX_EUNSUPPORTED otherwise): the multi-member graph
+ * has crashed the host process under the HIP runtime bundled with PyTorch. */
+typedef struct mpdx_ensemble mpdx_ensemble;
+int mpdx_ensemble_create(mpdx_solver *const *members, int n, mpdx_ensemble **out);
+int mpdx_ensemble_destroy(mpdx_ensemble *e);
+int mpdx_ensemble_advance(mpdx_ensemble *e, int64_t nt);
+/* Autotuner (no reference counterpart): for each class of 3D members (x
+ * extent rounded up to a power of two) tries the x-chunk lengths 0, 16, 32,
+ * 64, 128, 256 (those that fit), timing steps_per_trial ensemble steps of
+ * each after two untimed ones, and keeps the fastest.  The trial steps are
+ * real steps of every member (use them as warm-up); *steps_taken returns
+ * their number. */
+int mpdx_ensemble_tune(mpdx_ensemble *e, int steps_per_trial, int64_t *steps_taken);
+/* x-chunk length chosen for member i (after mpdx_ensemble_tune) */
+int mpdx_ensemble_x_chunk(const mpdx_ensemble *e, int i, int *planes);
+const char *mpdx_ensemble_last_error(const mpdx_ensemble *e);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,7 +35,9 @@ EXPORTED = [
     "mpdx_panic_ptr", "mpdx_active_path", "mpdx_profile", "mpdx_profile_read",
     "mpdx_set_calc_gc", "mpdx_advector_device", "mpdx_set_advector_generator",
     "mpdx_set_rhs_scheme", "mpdx_set_rhs", "mpdx_rhs_device", "mpdx_advectee_device",
-    "mpdx_advance_time", "mpdx_dt",
+    "mpdx_advance_time", "mpdx_dt", "mpdx_set_x_chunk",
+    "mpdx_ensemble_create", "mpdx_ensemble_destroy", "mpdx_ensemble_advance", "mpdx_ensemble_tune",
+    "mpdx_ensemble_x_chunk", "mpdx_ensemble_last_error",
 ]
 GC_NONE, GC_DEFORM = 0, 1
 CALC_GC_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p)
@@ -100,6 +102,14 @@ def lib():
         L.mpdx_advectee_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), i64p]
         L.mpdx_set_rhs_scheme.argtypes = [P, ctypes.c_int, UPDATE_RHS_FN, P]
         L.mpdx_set_advector_generator.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.mpdx_set_x_chunk.argtypes = [P, ctypes.c_int]
+        L.mpdx_ensemble_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(P)]
+        L.mpdx_ensemble_destroy.argtypes = [P]
+        L.mpdx_ensemble_advance.argtypes = [P, ctypes.c_int64]
+        L.mpdx_ensemble_tune.argtypes = [P, ctypes.c_int, i64p]
+        L.mpdx_ensemble_x_chunk.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.mpdx_ensemble_last_error.restype = ctypes.c_char_p
+        L.mpdx_ensemble_last_error.argtypes = [P]
         L.mpdx_profile.argtypes = [P, ctypes.c_int]
         L.mpdx_profile_read.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_double), i64p]
@@ -319,3 +329,50 @@ class Solver:
 
     def set_panic(self, v=True):
         self._L.mpdx_panic_ptr(self._h)[0] = int(bool(v))
+
+    def set_x_chunk(self, planes):
+        """x-chunk length of the 3D interior kernel (0 = automatic)."""
+        self._check(self._L.mpdx_set_x_chunk(self._h, int(planes)))
+
+
+class Ensemble:
+    """Independent solvers advanced together by one captured step per
+    launch (mpdx_ensemble_*); the members stay usable on their own."""
+
+    def __init__(self, members):
+        L = lib()
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        h = ctypes.c_void_p()
+        rc = L.mpdx_ensemble_create(arr, len(self.members), ctypes.byref(h))
+        if rc:
+            raise MpdxError(rc, L.mpdx_ensemble_last_error(None).decode())
+        self._L = L
+        self._h = h
+
+    def _check(self, rc):
+        if rc:
+            raise MpdxError(rc, self._L.mpdx_ensemble_last_error(self._h).decode())
+
+    def advance(self, nt):
+        self._check(self._L.mpdx_ensemble_advance(self._h, int(nt)))
+
+    def tune(self, steps_per_trial=2):
+        """Autotune the x-chunk length per class of 3D members; returns the
+        number of (real) steps the trials took."""
+        n = ctypes.c_int64()
+        self._check(self._L.mpdx_ensemble_tune(self._h, int(steps_per_trial), ctypes.byref(n)))
+        return n.value
+
+    def x_chunk(self, i):
+        v = ctypes.c_int()
+        self._check(self._L.mpdx_ensemble_x_chunk(self._h, int(i), ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.mpdx_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

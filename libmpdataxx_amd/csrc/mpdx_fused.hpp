@@ -754,6 +754,10 @@ k_fz_interior(InteriorArgs<R> a)
     }
   };
 
+  // no peeling of the first iteration: a peeled copy may round differently
+  // (other contraction choices), which would make the results depend on
+  // where the x chunks start
+#pragma clang loop unroll(disable)
   for (int q = 0; q < nsteps; q += 6) {
     step(ic<0>(), q);
     if (q + 1 >= nsteps) break;

@@ -273,3 +273,119 @@ def parse_coupled_harmosc_stats(text):
         if m and cur is not None:
             cur["rms"] = float(m.group(1))
     return out
+
+
+# ---------------------------------------------------------------- ensembles
+
+class MT19937_64:
+    """The 64-bit Mersenne Twister (Matsumoto & Nishimura 2004), as
+    std::mt19937_64, so draws match a C++ host's."""
+
+    def __init__(self, seed=5489):
+        m = (1 << 64) - 1
+        self.mt = [0] * 312
+        self.mt[0] = seed & m
+        for i in range(1, 312):
+            x = self.mt[i - 1]
+            self.mt[i] = (6364136223846793005 * (x ^ (x >> 62)) + i) & m
+        self.i = 312
+
+    def __call__(self):
+        m = (1 << 64) - 1
+        if self.i >= 312:
+            mt = self.mt
+            for k in range(312):
+                y = (mt[k] & 0xFFFFFFFF80000000) | (mt[(k + 1) % 312] & 0x7FFFFFFF)
+                mt[k] = mt[(k + 156) % 312] ^ (y >> 1) ^ (0xB5026F5AA96619E9 if y & 1 else 0)
+            self.i = 0
+        x = self.mt[self.i]
+        self.i += 1
+        x ^= (x >> 29) & 0x5555555555555555
+        x ^= (x << 17) & 0x71D67FFFEDA60000
+        x ^= (x << 37) & 0xFFF7EEE000000000
+        x ^= x >> 43
+        return x & m
+
+    def uniform_int(self, lo, hi):
+        """lo..hi inclusive (modulo reduction, hand-written)."""
+        return lo + self() % (hi - lo + 1)
+
+
+def _periodic_dist(n, c):
+    per = n - 1
+    d = np.abs(np.arange(n, dtype=np.float64) - c) % per
+    return np.minimum(d, per - d)
+
+
+def level_streamfunction_advector(grid, seed, max_courant=0.5, n_modes=8, quantum=2.0 ** -20):
+    """3D cyclic advector with horizontal flow only (w = 0): a random
+    periodic 2D streamfunction of n_modes Fourier modes whose amplitude varies
+    with the level, quantised per level so that every level's discrete
+    divergence is exactly zero."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = grid
+    xs = np.arange(-2, nx) + 0.5
+    ys = np.arange(-2, ny) + 0.5
+    f = np.zeros((xs.size, ys.size))
+    for _ in range(n_modes):
+        kx, ky = rng.integers(1, 4, 2)
+        ph1, ph2 = rng.uniform(0, 2 * np.pi, 2)
+        f += rng.normal() * np.sin(2 * np.pi * kx * xs[:, None] / (nx - 1) + ph1) \
+            * np.sin(2 * np.pi * ky * ys[None, :] / (ny - 1) + ph2)
+    phz = rng.uniform(0, 2 * np.pi)
+    lev = 1.0 + 0.5 * np.sin(2 * np.pi * np.arange(nz) / (nz - 1) + phz)   # period nz-1
+    gx0 = f[:, 1:] - f[:, :-1]
+    gy0 = -(f[1:, :] - f[:-1, :])
+    scale = max_courant / (max(np.abs(gx0).max(), np.abs(gy0).max()) * lev.max())
+    fl = np.round(f[:, :, None] * (scale * lev)[None, None, :] / quantum) * quantum
+    gx_full = fl[:, 1:, :] - fl[:, :-1, :]
+    gy_full = -(fl[1:, :, :] - fl[:-1, :, :])
+    gx = np.ascontiguousarray(gx_full[1:nx + 1, 1:ny + 1, :])
+    gy = np.ascontiguousarray(gy_full[0:nx + 1, 2:ny + 1, :])
+    gz = np.zeros((nx + 1, ny, nz - 1))
+    return [gx, gy, gz]
+
+
+def ensemble_domains(n2=32, n3=32, seed=5000, max_courant=0.5):
+    """The uneven ensemble: n2 2D domains with sides drawn from 64..512 and
+    n3 3D domains with sides from 32..128 (std::mt19937_64(seed)); domain d
+    (generator seed + d) draws n_iters in {2, 3, 4} and FCT on/off, and gets a
+    divergence-free random 8-mode streamfunction advector scaled to
+    max_courant and psi = U(0.1, 1) + 3 Gaussian blobs, all cyclic (period
+    n-1 per dimension, the last cell duplicating the first).  Returns a list
+    of dicts: grid, n_iters, fct, psi, gc."""
+    g = MT19937_64(seed)
+    sizes = [tuple(g.uniform_int(64, 512) for _ in range(2)) for _ in range(n2)]
+    sizes += [tuple(g.uniform_int(32, 128) for _ in range(3)) for _ in range(n3)]
+    out = []
+    for d, grid in enumerate(sizes):
+        r = MT19937_64(seed + d)
+        n_iters = r.uniform_int(2, 4)
+        fct = bool(r() & 1)
+        field_seed = r()
+        rng = np.random.default_rng(field_seed)
+        nd = len(grid)
+        unique = tuple(n - 1 for n in grid)
+        psi = np.empty(grid)
+        core = rng.uniform(0.1, 1.0, size=unique)
+        for _ in range(3):
+            c = [rng.uniform(0, n - 1) for n in grid]
+            sig = rng.uniform(2.0, 6.0)
+            amp = rng.uniform(0.5, 2.0)
+            r2 = sum(_periodic_dist(n, cc)[tuple(slice(None) if q == k else None for q in range(nd))] ** 2
+                     for k, (n, cc) in enumerate(zip(grid, c)))
+            core = core + amp * np.exp(-r2[tuple(slice(0, u) for u in unique)] / (2 * sig * sig))
+        psi[tuple(slice(0, u) for u in unique)] = core
+        # duplicate the first cell of every dimension into the last
+        for k in range(nd):
+            src = [slice(None)] * nd
+            dst = [slice(None)] * nd
+            src[k], dst[k] = 0, grid[k] - 1
+            psi[tuple(dst)] = psi[tuple(src)]
+        adv_seed = int(field_seed % (1 << 63))
+        if nd == 2:
+            gc = streamfunction_advector(grid, adv_seed, max_courant=max_courant, n_modes=8, periodic=True)
+        else:
+            gc = level_streamfunction_advector(grid, adv_seed, max_courant=max_courant)
+        out.append({"grid": grid, "n_iters": n_iters, "fct": fct, "psi": psi, "gc": gc})
+    return out

@@ -1,0 +1,158 @@
+"""Ensembles of independent domains (mpdx_ensemble_*): the uneven ensemble
+of 64 cyclic domains (32 2D from 64^2 to 512^2, 32 3D from 32^3 to 128^3,
+n_iters 2-4, FCT on/off) stepped by one captured graph per step.
+
+The ensemble must give exactly what each member gives when advanced on its
+own (same kernels, any x-chunk length: results do not depend on it), match
+the oracle domain by domain in FP64, and conserve mass and positivity."""
+import numpy as np
+import pytest
+
+import libmpdataxx_amd as M
+import oracle as O
+from libmpdataxx_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+# Ensemble stepping captures one graph over all members' streams.  It passed
+# these tests with /opt/rocm's HIP runtime and crashed the host process
+# (SIGSEGV in the first advance) twice when PyTorch's bundled HIP runtime was
+# the one loaded; the cause is not known, so stepping is switched off in the
+# library and these cases do not run.
+STEPPING_OFF = pytest.mark.skip(reason="ensemble stepping switched off: host crash under the PyTorch-bundled "
+                                       "HIP runtime, cause not yet shown")
+
+
+@pytest.fixture(scope="module")
+def domains():
+    return W.ensemble_domains()
+
+
+def make(d, dtype):
+    nd = len(d["grid"])
+    s = M.Solver(d["grid"], opts=M.FCT if d["fct"] else 0, n_iters=d["n_iters"], dtype=dtype,
+                 bcs=[(M.CYCLIC, M.CYCLIC)] * nd)
+    s.set_advectee(d["psi"])
+    for k in range(nd):
+        s.set_advector(d["gc"][k], k)
+    return s
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("bc", [M.OPEN, M.CYCLIC])
+@pytest.mark.parametrize("opts,iters", [(M.FCT, 3), (0, 3), (0, 4), (M.IGA, 2), (M.IGA | M.FCT, 2)])
+def test_x_chunk_does_not_change_results(dtype, bc, opts, iters):
+    """Every interior cell is computed by the same code whatever the x-chunk
+    length (the march loop is not peeled, so the first planes of a chunk do
+    not get differently compiled arithmetic)."""
+    grid = (70, 36, 34)
+    print(f"x chunks dtype={np.dtype(dtype).name} bc={bc} opts={opts} n_iters={iters}", flush=True)
+    psi = W.random_field(grid, 21)
+    gc = W.streamfunction_advector(grid, 22, periodic=bc == M.CYCLIC, max_courant=0.45)
+    outs = []
+    for chunk in (0, 8, 16, 64, 256):
+        s = M.Solver(grid, opts=opts, n_iters=iters, dtype=dtype, bcs=[(bc, bc)] * 3)
+        s.set_x_chunk(chunk)
+        s.set_advectee(psi)
+        for d in range(3):
+            s.set_advector(gc[d], d)
+        s.advance(4)
+        outs.append(s.advectee())
+        s.close()
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+
+
+@STEPPING_OFF
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_ensemble_matches_members_alone(domains, dtype):
+    pick = [0, 3, 7, 31, 32, 33, 40, 63]
+    print(f"ensemble of {len(pick)} dtype={np.dtype(dtype).name}", flush=True)
+    ens_m = [make(domains[i], dtype) for i in pick]
+    alone = [make(domains[i], dtype) for i in pick]
+    e = M.Ensemble(ens_m)
+    e.advance(5)
+    for s in alone:
+        s.advance(5)
+    for a, b in zip(ens_m, alone):
+        assert a.timestep == b.timestep == 5
+        np.testing.assert_array_equal(a.advectee(), b.advectee())
+    # a member stepped on its own in between: the ensemble follows its levels
+    ens_m[2].advance(1)
+    alone[2].advance(1)
+    e.advance(3)
+    for s in alone:
+        s.advance(3)
+    for a, b in zip(ens_m, alone):
+        np.testing.assert_array_equal(a.advectee(), b.advectee())
+        assert a.timestep == b.timestep
+    e.close()
+    for s in ens_m + alone:
+        s.close()
+
+
+@STEPPING_OFF
+def test_uneven_ensemble_tuned_fp32(domains):
+    """All 64 domains in FP32: autotuned, then stepped; equal to the members
+    run alone; mass over the unique cells and positivity kept."""
+    print("uneven ensemble 64 domains FP32 (tune + 3 steps)", flush=True)
+    ens_m = [make(d, np.float32) for d in domains]
+    e = M.Ensemble(ens_m)
+    taken = e.tune(1)
+    assert taken > 0
+    e.advance(3)
+    chunks = {e.x_chunk(i) for i in range(32, 64)}
+    print(f"  tuning took {taken} steps, x chunks chosen {sorted(chunks)}", flush=True)
+    for i, (d, s) in enumerate(zip(domains, ens_m)):
+        assert s.timestep == taken + 3
+        a = s.advectee()
+        ref = make(d, np.float32)
+        ref.advance(taken + 3)
+        np.testing.assert_array_equal(a, ref.advectee(), err_msg=f"domain {i}")
+        ref.close()
+        u = tuple(slice(0, n - 1) for n in d["grid"])
+        m0 = float(d["psi"][u].astype(np.float32).astype(np.float64).sum())
+        assert abs(float(a[u].astype(np.float64).sum()) - m0) <= 1e-6 * m0, i
+        assert a.min() > 0, i
+    e.close()
+    for s in ens_m:
+        s.close()
+
+
+@STEPPING_OFF
+def test_uneven_ensemble_fp64_matches_oracle(domains):
+    print("uneven ensemble 64 domains FP64 vs oracle (2 steps)", flush=True)
+    ens_m = [make(d, np.float64) for d in domains]
+    e = M.Ensemble(ens_m)
+    e.advance(2)
+    for i, (d, s) in enumerate(zip(domains, ens_m)):
+        nd = len(d["grid"])
+        o = O.Oracle(d["grid"], opts=O.FCT if d["fct"] else 0, n_iters=d["n_iters"],
+                     bcs=[(O.CYCLIC, O.CYCLIC)] * nd)
+        o.set_advectee(d["psi"])
+        for k in range(nd):
+            o.set_advector(d["gc"][k], k)
+        o.advance(2)
+        want = o.advectee()
+        assert np.abs(s.advectee() - want).max() <= 1e-12 * np.abs(want).max(), i
+    e.close()
+    for s in ens_m:
+        s.close()
+
+
+def test_ensemble_rejects_unsupported_members(domains):
+    a = make(domains[32], np.float32)
+    b = make(domains[33], np.float32)
+    with pytest.raises(M.MpdxError, match="appears twice"):
+        M.Ensemble([a, a])
+    b.set_calc_gc(lambda t: False)
+    with pytest.raises(M.MpdxError, match="calc_gc"):
+        M.Ensemble([a, b])
+    b.set_calc_gc(None)
+    e = M.Ensemble([a, b])
+    with pytest.raises(M.MpdxError, match="negative"):
+        e.advance(-1)
+    with pytest.raises(M.MpdxError, match="switched off"):
+        e.advance(1)
+    e.close()
+    a.close()
+    b.close()
