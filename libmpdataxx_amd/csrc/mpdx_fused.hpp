@@ -399,8 +399,12 @@ k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R 
 // the tile plus a one-cell frame, so every neighbour read is the thread's
 // own LDS address plus an immediate offset; frame cells that nobody loads
 // hold zeros and reach only frame results, which are discarded.
+// tile height (y) of the FP32 interior kernel; FP64 tiles are 16 high
+#ifndef MPDX_TILE_Y_F32
+#define MPDX_TILE_Y_F32 32
+#endif
 template <class R> struct ITile {
-  static constexpr int Y = sizeof(R) == 8 ? 16 : 32, Z = 32;
+  static constexpr int Y = sizeof(R) == 8 ? 16 : MPDX_TILE_Y_F32, Z = 32;
   static constexpr int PW = Z + 2, PLANE = (Y + 2) * PW;
   static constexpr int NFRAME = 3 * Z + 3 * Y;   // FUSE0 frame loads per plane
 };
