@@ -20,11 +20,16 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MPDX_ARCH", "gfx950")
 
 FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
-SOURCES = ["mpdx_capi.hip"]
+# source -> extra flags: the FP32 interior kernels get the max-ILP
+# instruction scheduler (their occupancy is fixed by LDS, not registers)
+SOURCES = {
+    "mpdx_capi.hip": [],
+    "mpdx_interior.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+}
 
 
 CPP_TESTS = ["revolving_sphere_3d", "rotating_cone_2d"]
@@ -118,20 +123,34 @@ def _resource_report(stderr: str) -> str:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile every HIP source into libmpdx.so; returns the library path.
-    The compiler's per-kernel register / scratch / LDS usage is written to
-    kernel_resources.txt next to the library."""
+    """Compile every HIP source (each with its own flags, in parallel) and
+    link libmpdx.so; returns the library path.  The compiler's per-kernel
+    register / scratch / LDS usage is written to kernel_resources.txt next to
+    the library."""
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-Rpass-analysis=kernel-resource-usage",
-           *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stderr)
-        raise subprocess.CalledProcessError(r.returncode, cmd)
+    with tempfile.TemporaryDirectory() as td:
+        procs, objs = [], []
+        for src, extra in SOURCES.items():
+            obj = os.path.join(td, src + ".o")
+            cmd = [HIPCC, *FLAGS, *extra, "-Rpass-analysis=kernel-resource-usage", "-c",
+                   os.path.join(CSRC, src), "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+            objs.append(obj)
+        report = []
+        for cmd, p in procs:
+            _, err = p.communicate()
+            if p.returncode != 0:
+                sys.stderr.write(err)
+                raise subprocess.CalledProcessError(p.returncode, cmd)
+            report.append(_resource_report(err))
+        cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
     with open(RESOURCES, "w") as f:
-        f.write(_resource_report(r.stderr))
+        f.write("".join(report))
     os.replace(tmp, LIB)
     return LIB
 

@@ -18,6 +18,14 @@
 #include "mpdx_solver.hpp"
 #include "mpdx_perpass.hpp"
 #include "mpdx_fused.hpp"
+#include "mpdx_interior_list.hpp"
+
+// the FP32 default-guard interior kernels are instantiated in mpdx_interior.hip
+namespace mpdx {
+#define MPDX_EXTERN(R, G, F, I, S, Z) extern template __global__ void k_fz_interior<R, G, F, I, S, Z>(InteriorArgs<R>);
+MPDX_INTERIOR_F32_FAST(MPDX_EXTERN)
+#undef MPDX_EXTERN
+}  // namespace mpdx
 
 using namespace mpdx;
 

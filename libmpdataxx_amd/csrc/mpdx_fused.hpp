@@ -475,8 +475,13 @@ template <class R> __device__ __forceinline__ R iflux(R l, R r, R c)
 
 template <int N> using ic = std::integral_constant<int, N>;
 
+#ifdef MPDX_INTERIOR_WPE
+#define MPDX_INTERIOR_ATTR __attribute__((amdgpu_waves_per_eu(MPDX_INTERIOR_WPE, MPDX_INTERIOR_WPE)))
+#else
+#define MPDX_INTERIOR_ATTR
+#endif
 template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0>
-__global__ void __launch_bounds__(ITile<R>::Y * ITile<R>::Z)
+__global__ void __launch_bounds__(ITile<R>::Y * ITile<R>::Z) MPDX_INTERIOR_ATTR
 k_fz_interior(InteriorArgs<R> a)
 {
   constexpr int TY = ITile<R>::Y, TZ = ITile<R>::Z, PW = ITile<R>::PW, PL = ITile<R>::PLANE;

@@ -404,7 +404,7 @@ __global__ void k_axpy_box(R *y, const R *x, Layout L, Box bx, R alpha)
   y[p] += alpha * x[p];
 }
 
-__global__ void k_tick(int *step, int set)
+static __global__ void k_tick(int *step, int set)
 {
   if (threadIdx.x == 0 && blockIdx.x == 0) *step = set >= 0 ? set : *step + 1;
 }
@@ -471,7 +471,7 @@ __global__ void k_reduce(double2 *part, const R *a, Layout L, Box bx, int op)
   }
 }
 
-__global__ void k_reduce_final(double *out, const double2 *part, int n, int op)
+static __global__ void k_reduce_final(double *out, const double2 *part, int n, int op)
 {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double A = op == 0 ? INFINITY : (op == 1 ? -INFINITY : 0.0), C = 0.0;
