@@ -475,6 +475,31 @@ template <class R> __device__ __forceinline__ R iflux(R l, R r, R c)
 
 template <int N> using ic = std::integral_constant<int, N>;
 
+// Loads and stores through a buffer resource built from a block-uniform
+// base (scalar registers) with a 32-bit per-thread byte offset: no 64-bit
+// vector address arithmetic per access, and the running bases advance with
+// scalar adds.  Range 0xffffffff (no range check), dword 3 as for gfx9.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const char *g)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(g), (short)0, (int)0xffffffff, 0x00020000);
+}
+template <class R> __device__ __forceinline__ R buf_ld(const char *g, unsigned off)
+{
+  if constexpr (sizeof(R) == 4)
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(buf_rsrc(g), (int)off, 0, 0));
+  else
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(buf_rsrc(g), (int)off, 0, 0));
+}
+template <class R> __device__ __forceinline__ void buf_st(char *g, unsigned off, R v)
+{
+  if constexpr (sizeof(R) == 4)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), buf_rsrc(g), (int)off, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(
+                                              buf_rsrc(g), 0, 0, 0)), v),
+                                          buf_rsrc(g), (int)off, 0, 0);
+}
+
 #ifdef MPDX_INTERIOR_WPE
 #define MPDX_INTERIOR_ATTR __attribute__((amdgpu_waves_per_eu(MPDX_INTERIOR_WPE, MPDX_INTERIOR_WPE)))
 #else
@@ -519,16 +544,29 @@ k_fz_interior(InteriorArgs<R> a)
 
   // running byte addresses of the tile corner at the plane each stream reads
   // next (plane 0 = x0-2); V stores at plane q-1, psi^{k+1} at plane q-2
-  const unsigned long long sxb = (unsigned long long)sx * sizeof(R);
-  auto at = [&](const R *arr, int plane) { return (unsigned long long)(arr + base0) + plane * (long long)sxb; };
+  // typed (global) pointers with a uniform base and a 32-bit per-thread byte
+  // offset: the loads use the scalar-base addressing mode, the running
+  // pointers advance with scalar adds
+  const long sxb = sx * (long)sizeof(R);
+  auto at = [&](const R *arr, int plane) -> char * {
+    return reinterpret_cast<char *>(const_cast<R *>(arr + base0)) + plane * sxb;
+  };
   constexpr int AHEAD = FUSE0 ? 1 : 0;   // plane of the first "next" register set
-  unsigned long long g_ps = FUSE0 ? 0ull : at(a.psi, 0), g_pn = FCT || FUSE0 ? at(a.psin, AHEAD) : 0ull;
-  unsigned long long g_ux = at(a.U0, AHEAD), g_uy = at(a.U1, AHEAD), g_uz = at(a.U2, AHEAD);
-  unsigned long long g_out = at(a.out, -2);
-  unsigned long long g_v0 = STOREV ? at(a.V0, -1) : 0ull, g_v1 = STOREV ? at(a.V1, -1) : 0ull,
-                     g_v2 = STOREV ? at(a.V2, -1) : 0ull;
-  auto LD = [&](unsigned long long g) -> R { return *reinterpret_cast<const R *>(g + toff); };
-  auto ST = [&](unsigned long long g, R v) { *reinterpret_cast<R *>(g + toff) = v; };
+  char *g_ps = FUSE0 ? nullptr : at(a.psi, 0), *g_pn = FCT || FUSE0 ? at(a.psin, AHEAD) : nullptr;
+  char *g_ux = at(a.U0, AHEAD), *g_uy = at(a.U1, AHEAD), *g_uz = at(a.U2, AHEAD);
+  char *g_out = at(a.out, -2);
+  char *g_v0 = STOREV ? at(a.V0, -1) : nullptr, *g_v1 = STOREV ? at(a.V1, -1) : nullptr,
+       *g_v2 = STOREV ? at(a.V2, -1) : nullptr;
+  // buffer accesses where the scalar registers suffice; the FUSE0 kernels
+  // keep eight running streams live and would spill their descriptors
+  auto LD = [&](const char *g) -> R {
+    if constexpr (FUSE0) return *reinterpret_cast<const R *>(g + toff);
+    else return buf_ld<R>(g, toff);
+  };
+  auto ST = [&](char *g, R v) {
+    if constexpr (FUSE0) *reinterpret_cast<R *>(g + toff) = v;
+    else buf_st<R>(g, toff, v);
+  };
 
   // zero the frame of every LDS plane
   for (int i = tid; i < IL_N * 2 * (PW + TY); i += TY * TZ) {
@@ -549,7 +587,7 @@ k_fz_interior(InteriorArgs<R> a)
   // branch.
   R fr_sc = 1;
   int fr_lds = IL_N * PL + tid;
-  unsigned long long fr_g = at(a.psin, 0) + toff;
+  const char *fr_g = at(a.psin, 0) + toff;
   if (FUSE0 && tid < ITile<R>::NFRAME) {
     int fy, fz, kind;
     if (tid < 2 * TZ) { kind = 0; fy = tid < TZ ? -1 : TY; fz = tid % TZ; }
@@ -559,12 +597,12 @@ k_fz_interior(InteriorArgs<R> a)
     const int gy = min(ylo + fy, L.n[1]), gz = min(zlo + fz, L.n[2]);
     const long off = ((long)(gy - ylo) * L.st[1] + (gz - zlo)) * (long)sizeof(R);
     const R *arr = kind == 0 ? a.psin : (kind == 1 ? a.U1 : a.U2);
-    fr_g = (unsigned long long)((long long)at(arr, 0) + off);
+    fr_g = at(arr, 0) + off;
     fr_lds = (kind == 0 ? IL_PN : (kind == 1 ? IL_UY : IL_UZ)) * PL + (fy + 1) * PW + fz + 1;
     fr_sc = kind == 0 ? a.in_scale : R(1);
   }
   auto FLD = [&](int plane) -> R {
-    return *reinterpret_cast<const R *>(fr_g + plane * (long long)sxb);
+    return *reinterpret_cast<const R *>(fr_g + plane * sxb);
   };
 
   // register rings (slot = plane index mod 6)
@@ -580,7 +618,7 @@ k_fz_interior(InteriorArgs<R> a)
     n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
   } else {
     // prologue: psi^1 on plane 0 from psi^n planes -1..1 and GC planes -1, 0
-    const unsigned long long gp0 = at(a.psin, 0);
+    const char *gp0 = at(a.psin, 0);
     pn[5] = LD(gp0 - sxb) * sc;
     pn[0] = LD(gp0) * sc;
     pn[1] = LD(gp0 + sxb) * sc;
