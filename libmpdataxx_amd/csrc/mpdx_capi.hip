@@ -690,7 +690,7 @@ struct Fused {
   {
     constexpr int TX = FzBrick<ND, SHAPE>::TX, TY = FzBrick<ND, SHAPE>::TY, TZ = FzBrick<ND, SHAPE>::TZ;
     dim3 g(ND == 3 ? (bx.ext[2] + TZ - 1) / TZ : 1, (bx.ext[1] + TY - 1) / TY, (bx.ext[0] + TX - 1) / TX);
-    hipLaunchKernelGGL((k_fz_fct<R, G, ND, SHAPE, NPA, ABS, IGA>), g, dim3(256), 0, s->stream, out,
+    hipLaunchKernelGGL((k_fz_fct<R, G, ND, SHAPE, NPA, ABS, IGA>), g, dim3(TX * TY * TZ), 0, s->stream, out,
                        (const R *)psi, (const R *)psin, (const R *)V[0], (const R *)V[1],
                        (const R *)V[ND == 3 ? 2 : 0], L, in_scale, out_scale, bx);
   }

@@ -218,6 +218,11 @@ template <> struct FzBrick<2, 0> { static constexpr int TX = 8, TY = 32, TZ = 1;
 template <> struct FzBrick<2, 1> { static constexpr int TX = 128, TY = 2, TZ = 1; };
 template <> struct FzBrick<2, 2> { static constexpr int TX = 8, TY = 32, TZ = 1; };
 
+template <int ND, int SHAPE> constexpr int fz_brick_threads()
+{
+  return FzBrick<ND, SHAPE>::TX * FzBrick<ND, SHAPE>::TY * FzBrick<ND, SHAPE>::TZ;
+}
+
 // beta_up / beta_dn of one cell from psi^k (centre + 6 neighbours), the
 // psi^n bounds and the six face fluxes of the unlimited velocity
 // (formulae_mpdata_fct_3d.hpp:21-145, formulae_mpdata_fct_2d.hpp:21-120)
@@ -310,7 +315,7 @@ __device__ inline void beta_at(R &bup, R &bdn, const R *psi, const R *psin, cons
 }
 
 template <class R, class G, int ND, int SHAPE, bool NPA, bool ABS, bool IGA>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__((fz_brick_threads<ND, SHAPE>()))
 k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R *V2, Layout L,
          R in_scale, R out_scale, Box bx)
 {
