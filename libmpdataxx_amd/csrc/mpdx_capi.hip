@@ -606,8 +606,15 @@ struct Fused {
   const Layout &L;
   BcMap m;
   unsigned o;
+  // open bounds on both z sides: with an interior kernel its tiles also
+  // compute the z-boundary cells, and the boundary shell has no z slabs
+  bool zb;
 
-  explicit Fused(mpdx_solver *s_) : s(s_), pp(s_), L(s_->L), m(make_bcmap(s_)), o(s_->p.opts) {}
+  explicit Fused(mpdx_solver *s_)
+      : s(s_), pp(s_), L(s_->L), m(make_bcmap(s_)), o(s_->p.opts),
+        zb(s_->use_zbnd && s_->L.nd == 3 && s_->p.bc[2][0] == MPDX_BC_OPEN && s_->p.bc[2][1] == MPDX_BC_OPEN)
+  {
+  }
 
   R *A(void *p) const { return static_cast<R *>(p); }
 
@@ -726,8 +733,9 @@ struct Fused {
     else fct_nd<2>(out, psi, psin, V, in_scale, out_scale, bx);
   }
 
-  // the boxes of [f, l]^nd minus [f+wl, l-wh]^nd (faces x, then y, then z slabs)
-  std::vector<Box> shell(int f, int wl, int wh) const
+  // the boxes of [f, l]^nd minus [f+wl, l-wh]^nd (faces x, then y, then z
+  // slabs; the z slabs only without zb)
+  std::vector<Box> shell(int f, int wl, int wh, bool zb_tiles = false) const
   {
     std::vector<Box> out;
     int lo[3], hi[3];
@@ -748,7 +756,7 @@ struct Fused {
       return out;
     }
     int cl[3] = {lo[0], lo[1], lo[2]}, ch[3] = {hi[0], hi[1], hi[2]};
-    for (int d = 0; d < L.nd; ++d) {
+    for (int d = 0; d < (zb_tiles && zb ? 2 : L.nd); ++d) {
       int a[3] = {cl[0], cl[1], cl[2]}, b[3] = {ch[0], ch[1], ch[2]};
       b[d] = ilo[d] - 1;
       out.push_back(make_box(a[0], b[0], a[1], b[1], a[2], b[2]));
@@ -783,7 +791,16 @@ struct Fused {
     }
     a.in_scale = in_scale; a.out_scale = out_scale;
     constexpr int TY = ITile<R>::Y, TZ = ITile<R>::Z;
-    a.nb[0] = (L.n[2] - 4 + TZ - 5) / (TZ - 4);
+    // zb: the tiles cover every z cell (and store the velocity of every z
+    // face of the domain, face -1/2 included); the last tile is shifted
+    // back to end on the last halo column
+    a.zb = zb ? 1 : 0;
+    a.zs_max = std::max(L.n[2] - (TZ - 4), 0);
+    if (zb) {
+      a.lo[2] = 0; a.hi[2] = L.n[2] - 1;
+      a.vlo[2] = 0; a.vhi[2] = L.n[2] - 1;
+    }
+    a.nb[0] = (a.hi[2] - a.lo[2] + 1 + TZ - 5) / (TZ - 4);
     a.nb[1] = (L.n[1] - 4 + TY - 5) / (TY - 4);
     // x chunk: as long as possible (the 4-plane ramp of each chunk is
     // overhead) while leaving at least two tiles per CU
@@ -857,7 +874,7 @@ struct Fused {
     {
       ProfScope ps(s, "fused_antidiff");
       if (use_int)
-        for (const Box &b : shell(-1, 5, 4)) antidiff_box(V, psik, U, b);
+        for (const Box &b : shell(-1, 5, 4, true)) antidiff_box(V, psik, U, b);
       else
         antidiff_box(V, psik, U, full_box(-1));
     }
@@ -869,7 +886,7 @@ struct Fused {
       }
       ProfScope ps(s, "fused_fct");
       if (use_int)
-        for (const Box &b : shell(0, 2, 2)) fct_box(out, psik, psin, V, in_scale, os, b);
+        for (const Box &b : shell(0, 2, 2, true)) fct_box(out, psik, psin, V, in_scale, os, b);
       else
         fct_box(out, psik, psin, V, in_scale, os, full_box(0));
     } else {
@@ -879,7 +896,7 @@ struct Fused {
       }
       if (use_int) {
         ProfScope ps(s, "fused_donor_shell");
-        for (const Box &b : shell(0, 2, 2)) donor_box(out, psik, V, R(1), os, (o & O_IGA) ? 1 : 0, b);
+        for (const Box &b : shell(0, 2, 2, true)) donor_box(out, psik, V, R(1), os, (o & O_IGA) ? 1 : 0, b);
       } else {
         donor(out, psik, V, R(1), os, (o & O_IGA) ? 1 : 0);
       }
@@ -934,7 +951,7 @@ struct Fused {
           OnAux g(s, conc);
           if (it == 1) {
             ProfScope ps(s, "fused_donor_shell");
-            for (const Box &b : shell(0, 5, 5)) donor_box(out_of(0), psin, gc, in_scale, R(1), 0, b);
+            for (const Box &b : shell(0, 5, 5, true)) donor_box(out_of(0), psin, gc, in_scale, R(1), 0, b);
           }
           corrective_shell(out_of(it), out_of(it - 1), psin, it, passes, in_scale, os, true);
         }
@@ -1393,6 +1410,7 @@ int mpdx_create(const mpdx_params *p, mpdx_solver **out)
   }
   if (const char *ev = std::getenv("MPDX_NO_GRAPHS")) s->use_graphs = !(ev[0] && ev[0] != '0');
   if (const char *ev = std::getenv("MPDX_NO_CONCURRENCY")) s->use_concurrency = !(ev[0] && ev[0] != '0');
+  if (const char *ev = std::getenv("MPDX_NO_ZBND")) s->use_zbnd = !(ev[0] && ev[0] != '0');
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&

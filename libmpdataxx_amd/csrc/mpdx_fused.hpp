@@ -439,6 +439,10 @@ struct InteriorArgs {
   int nb[3];            // tiles along z, y and x chunks
   int chunk;            // output planes per x chunk
   R in_scale, out_scale;
+  // zb: open bounds on both z sides and the tiles also cover the z-boundary
+  // cells (lo[2] = 0, hi[2] = n-1); the last z tile starts at zs_max so that
+  // it ends on the last halo column
+  int zb, zs_max;
 };
 
 // max / min without the std::max argument-order semantics (they differ only
@@ -530,20 +534,40 @@ k_fz_interior(InteriorArgs<R> a)
   const Layout &L = a.L;
   const int tz = threadIdx.x, ty = threadIdx.y;
   const int tid = ty * TZ + tz;
-  const int ylo = a.lo[1] - 2 + by * (TY - 4), zlo = a.lo[2] - 2 + bz * (TZ - 4);
+  const int n2 = L.n[2];
+  const int ylo = a.lo[1] - 2 + by * (TY - 4);
+  const int zlo = (a.zb ? min(a.lo[2] + bz * (TZ - 4), a.zs_max) : a.lo[2] + bz * (TZ - 4)) - 2;
   const int y = ylo + ty, z = zlo + tz;
   const int x0 = a.lo[0] + bxc * a.chunk;
   const int x1 = min(x0 + a.chunk - 1, a.hi[0]);
   const int nsteps = x1 - x0 + 5;   // psi^k planes x0-2 .. x1+2
-  // loads: y, z >= lo-2 >= 0; on the far side rows and columns past the
-  // tile's last output cell are clamped to the first halo row (cell n),
-  // which the FUSE0 donor needs for psi^1 at cell n-1
-  const int yl = min(y, L.n[1]), zl = min(z, L.n[2]);
+  // loads: y >= lo-2 >= 0; on the far side rows and columns past the tile's
+  // last output cell are clamped to the first halo row (cell n), which the
+  // FUSE0 donor needs for psi^1 at cell n-1.  z-boundary tiles (zb) also
+  // hold the halo columns: scalars (psi^k, psi^n) are read at the open
+  // boundary's edge copy (bcond/open_3d.hpp:32-42, clamp to 0..n-1), the
+  // advector from its halo faces in memory (clamped to -hw..n)
+  const int yl = min(y, L.n[1]);
+  const int zl = a.zb ? min(max(z, -L.hw), n2) : min(z, n2);
+  const int zls = a.zb ? min(max(z, 0), n2 - 1) : min(z, n2);
   const long sx = L.st[0];
   const long base0 = L.org + (long)(x0 - 2) * sx + (long)ylo * L.st[1] + zlo;
   const unsigned toff = (unsigned)(((long)(yl - ylo) * L.st[1] + (zl - zlo)) * (long)sizeof(R));
+  const unsigned toff_s = (unsigned)(((long)(yl - ylo) * L.st[1] + (zls - zlo)) * (long)sizeof(R));
   const bool out_ok = ty >= 2 && ty < TY - 2 && tz >= 2 && tz < TZ - 2 && y <= a.hi[1] && z <= a.hi[2];
   const bool v_ok = out_ok && y >= a.vlo[1] && y <= a.vhi[1] && z >= a.vlo[2] && z <= a.vhi[2];
+  // z-boundary tiles: block flag (uniform), halo columns (z < 0 or z > n-1),
+  // the offset from a halo column to the edge column it copies, and the
+  // velocity of face -1/2 (cell -1), which belongs to the domain's velocity
+  const bool blk_zb = a.zb && (zlo < 0 || zlo + TZ - 1 > n2 - 1);
+  const bool zh = a.zb && (z < 0 || z > n2 - 1);
+  const int e_off = z < 0 ? -z : (z > n2 - 1 ? n2 - 1 - z : 0);
+  // lane offsets of the zero-divergence inputs of halo cells -1 and n: the
+  // edge column (V_x, V_y) and the V_z face (face 1/2 of cell 0, face
+  // n-3/2 of cell n-2)
+  const int zx1 = z == -1 ? 1 : (z == n2 ? -1 : 0), zx2 = z == -1 ? 1 : (z == n2 ? -2 : 0);
+  const bool vzm_ok = a.zb && z == -1 && ty >= 2 && ty < TY - 2 && y <= a.hi[1] && y >= a.vlo[1] &&
+                      y <= a.vhi[1];
   // steps whose velocity plane (p-1) is stored: planes x0..x1 within vlo..vhi
   const int vq0 = max(3, a.vlo[0] - x0 + 3), vq1 = min(nsteps - 2, a.vhi[0] - x0 + 3);
 
@@ -567,6 +591,10 @@ k_fz_interior(InteriorArgs<R> a)
   auto LD = [&](const char *g) -> R {
     if constexpr (FUSE0) return *reinterpret_cast<const R *>(g + toff);
     else return buf_ld<R>(g, toff);
+  };
+  auto LDs = [&](const char *g) -> R {   // scalar fields (psi^k, psi^n)
+    if constexpr (FUSE0) return *reinterpret_cast<const R *>(g + toff_s);
+    else return buf_ld<R>(g, toff_s);
   };
   auto ST = [&](char *g, R v) {
     if constexpr (FUSE0) *reinterpret_cast<R *>(g + toff) = v;
@@ -592,14 +620,16 @@ k_fz_interior(InteriorArgs<R> a)
   // branch.
   R fr_sc = 1;
   int fr_lds = IL_N * PL + tid;
-  const char *fr_g = at(a.psin, 0) + toff;
+  const char *fr_g = at(a.psin, 0) + toff_s;
   if (FUSE0 && tid < ITile<R>::NFRAME) {
     int fy, fz, kind;
     if (tid < 2 * TZ) { kind = 0; fy = tid < TZ ? -1 : TY; fz = tid % TZ; }
     else if (tid < 2 * TZ + 2 * TY) { kind = 0; const int r = tid - 2 * TZ; fy = r % TY; fz = r < TY ? -1 : TZ; }
     else if (tid < 3 * TZ + 2 * TY) { kind = 1; fy = -1; fz = tid - 2 * TZ - 2 * TY; }
     else { kind = 2; fy = tid - 3 * TZ - 2 * TY; fz = -1; }
-    const int gy = min(ylo + fy, L.n[1]), gz = min(zlo + fz, L.n[2]);
+    const int gy = min(ylo + fy, L.n[1]);
+    const int gz = !a.zb ? min(zlo + fz, n2) : (kind == 0 ? min(max(zlo + fz, 0), n2 - 1)
+                                                          : min(max(zlo + fz, -L.hw), n2));
     const long off = ((long)(gy - ylo) * L.st[1] + (gz - zlo)) * (long)sizeof(R);
     const R *arr = kind == 0 ? a.psin : (kind == 1 ? a.U1 : a.U2);
     fr_g = at(arr, 0) + off;
@@ -618,15 +648,15 @@ k_fz_interior(InteriorArgs<R> a)
   const R sc = a.in_scale;
 
   if (!FUSE0) {
-    n_ps = LD(g_ps);
-    if (FCT) n_pn = LD(g_pn);
+    n_ps = LDs(g_ps);
+    if (FCT) n_pn = LDs(g_pn);
     n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
   } else {
     // prologue: psi^1 on plane 0 from psi^n planes -1..1 and GC planes -1, 0
     const char *gp0 = at(a.psin, 0);
-    pn[5] = LD(gp0 - sxb) * sc;
-    pn[0] = LD(gp0) * sc;
-    pn[1] = LD(gp0 + sxb) * sc;
+    pn[5] = LDs(gp0 - sxb) * sc;
+    pn[0] = LDs(gp0) * sc;
+    pn[1] = LDs(gp0 + sxb) * sc;
     ux[5] = LD(at(a.U0, -1));
     ux[0] = LD(at(a.U0, 0));
     uy[0] = LD(at(a.U1, 0));
@@ -637,7 +667,7 @@ k_fz_interior(InteriorArgs<R> a)
     S(IL_UZ + 0, 0, 0) = uz[0];
     lds[fr_lds] = FLD(0) * fr_sc;
     // plane 1 for the first step's publish
-    n_pn = LD(g_pn); n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
+    n_pn = LDs(g_pn); n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
     fr_g += sxb;
     n_fr = *reinterpret_cast<const R *>(fr_g);
     __syncthreads();
@@ -646,6 +676,12 @@ k_fz_interior(InteriorArgs<R> a)
     const R f2 = iflux(pc, S(IL_PN, 1, 0), uy[0]), f3 = iflux(S(IL_PN, -1, 0), pc, S(IL_UY, -1, 0));
     const R f4 = iflux(pc, S(IL_PN, 0, 1), uz[0]), f5 = iflux(S(IL_PN, 0, -1), pc, S(IL_UZ, 0, -1));
     ps[0] = pc + ((-f0 + f1) + (-f2 + f3) + (-f4 + f5)) / 1;
+    if (blk_zb) {
+      // halo columns of psi^1: the open boundary's edge copy, from the
+      // edge column's lane (a tile row lies in one wave)
+      const R v = __shfl(ps[0], (int)__lane_id() + e_off);
+      if (zh) ps[0] = v;
+    }
     S(IL_PSI + 0, 0, 0) = ps[0];
   }
 
@@ -671,8 +707,8 @@ k_fz_interior(InteriorArgs<R> a)
       }
       if (q + 1 < nsteps) {
         g_ps += sxb; g_ux += sxb; g_uy += sxb; g_uz += sxb;
-        n_ps = LD(g_ps);
-        if (FCT) { g_pn += sxb; n_pn = LD(g_pn); }
+        n_ps = LDs(g_ps);
+        if (FCT) { g_pn += sxb; n_pn = LDs(g_pn); }
         n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
       }
     } else {
@@ -682,7 +718,7 @@ k_fz_interior(InteriorArgs<R> a)
       S(IL_UZ + su(1), 0, 0) = n_uz;
       S(IL_PN + su(1), 0, 0) = pn[s6(1)];
       lds[fr_lds + (fr_lds < IL_N * PL ? su(1) * PL : 0)] = n_fr * fr_sc;
-      if (q + 1 < nsteps) { g_pn += sxb; n_pn = LD(g_pn); }
+      if (q + 1 < nsteps) { g_pn += sxb; n_pn = LDs(g_pn); }
       if (q + 2 < nsteps) {
         g_ux += sxb; g_uy += sxb; g_uz += sxb;
         n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
@@ -705,6 +741,9 @@ k_fz_interior(InteriorArgs<R> a)
         const R s2 = (uz[s6(0)] + uz[s6(-1)] + S(IL_UZ + su(0), 0, -1) + S(IL_UZ + su(-1), 0, -1));
         vx = antidiff_core<R, G, 3, IGA, false, false>(ux[s6(-1)], s1, s2, P);
       }
+      // halo columns at an open z boundary: the normal-component fill sets
+      // V_x and V_y there to zero (bcond/open_3d.hpp fill_halos_vctr_nrml)
+      if (zh) vx = R(0);
       if (q >= 2) {
         {
           // V_y at face y+1/2 of plane p-1: normal y, tangential (z, x)
@@ -726,13 +765,16 @@ k_fz_interior(InteriorArgs<R> a)
           const R s2 = (S(IL_UY + su(-1), 0, 1) + uy[s6(-1)] + S(IL_UY + su(-1), -1, 1) + S(IL_UY + su(-1), -1, 0));
           vz = antidiff_core<R, G, 3, IGA, false, false>(uz[s6(-1)], s1, s2, P);
         }
+        if (zh) vy = R(0);
         S(IL_VY, 0, 0) = vy;
         S(IL_VZ, 0, 0) = vz;
       }
-      if (STOREV && v_ok && q >= vq0 && q <= vq1) {
-        ST(g_v0, vx);
-        ST(g_v1, vy);
-        ST(g_v2, vz);
+      if (STOREV && q >= vq0 && q <= vq1) {
+        if (v_ok) {
+          ST(g_v0, vx);
+          ST(g_v1, vy);
+        }
+        if (v_ok || vzm_ok) ST(g_v2, vz);
       }
     }
     if (STOREV) { g_v0 += sxb; g_v1 += sxb; g_v2 += sxb; }
@@ -752,10 +794,24 @@ k_fz_interior(InteriorArgs<R> a)
                          fmx(S(pm, -1, 0), fmx(S(pm, 0, 1), S(pm, 0, -1))));
       const R pmin = fmn(fmn(fmn(pn[s6(-1)], pn[s6(0)]), fmn(pn[s6(-2)], S(pm, 1, 0))),
                          fmn(S(pm, -1, 0), fmn(S(pm, 0, 1), S(pm, 0, -1))));
-      const R vym = S(IL_VY, -1, 0), vzm = S(IL_VZ, 0, -1);
+      const R vym = S(IL_VY, -1, 0);
+      R vzm = S(IL_VZ, 0, -1), vzp = vz;
+      if (blk_zb) {
+        // V_z on the outer halo face next to halo cell -1 (face -3/2) and
+        // halo cell n (face n+1/2) of an open z boundary: the zero-divergence
+        // fill (bcond/open_3d.hpp fill_halos_vctr_alng) from the edge cell's
+        // velocities, computed by the only thread that reads it (beta of
+        // that halo cell).  V_x of the edge column comes from its lane (a
+        // tile row lies in one wave), V_y from the published plane.
+        const int ln = (int)__lane_id();
+        const R ex = __shfl(vx, ln + zx1), exp_ = __shfl(vx_[s2(-1)], ln + zx1), ez = __shfl(vz, ln + zx2);
+        const R eyp = lds[IL_VY * PL + ti + zx1], eym = lds[IL_VY * PL + ti - PW + zx1];
+        if (z == -1) vzm = ez - (exp_ - ex) - (eym - eyp);
+        if (z == n2) vzp = ez + (exp_ - ex) + (eym - eyp);
+      }
       const R fyp = IGA ? vy : iflux(pc, yp, vy);
       const R fym = IGA ? vym : iflux(ym, pc, vym);
-      const R fzp = IGA ? vz : iflux(pc, zp, vz);
+      const R fzp = IGA ? vzp : iflux(pc, zp, vzp);
       const R fzm = IGA ? vzm : iflux(zm, pc, vzm);
       ibeta<R, G>(bu, bd, pc, ps[s6(0)], ps[s6(-2)], yp, ym, zp, zm, pmin, pmax, fpx, fpx_[s2(-1)],
                   fyp, fym, fzp, fzm);
@@ -801,6 +857,10 @@ k_fz_interior(InteriorArgs<R> a)
         const R f2 = iflux(pc, S(pp, 1, 0), uy[s6(1)]), f3 = iflux(S(pp, -1, 0), pc, S(IL_UY + su(1), -1, 0));
         const R f4 = iflux(pc, S(pp, 0, 1), uz[s6(1)]), f5 = iflux(S(pp, 0, -1), pc, S(IL_UZ + su(1), 0, -1));
         ps[s6(1)] = pc + ((-f0 + f1) + (-f2 + f3) + (-f4 + f5)) / 1;
+        if (blk_zb) {
+          const R v = __shfl(ps[s6(1)], (int)__lane_id() + e_off);
+          if (zh) ps[s6(1)] = v;
+        }
         S(IL_PSI + s3(1), 0, 0) = ps[s6(1)];
       }
     }

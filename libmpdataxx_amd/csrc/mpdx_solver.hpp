@@ -51,6 +51,9 @@ struct mpdx_solver {
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool use_concurrency = true;
+  // with open bounds on both z sides the interior tiles also compute the
+  // z-boundary cells (no z slabs in the boundary shell)
+  bool use_zbnd = true;
   int ix_chunk = 0;         // output planes per x chunk of the interior kernel (0: automatic)
   int n_cu = 256;           // compute units of the device
   // explicit forcing (mpdata_rhs): scheme, user update_rhs, forcing arrays
