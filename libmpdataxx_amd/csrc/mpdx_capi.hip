@@ -128,13 +128,26 @@ struct PerPass {
   {
     const int kind = s->p.bc[d][side];
     int mode = kind == MPDX_BC_CYCLIC ? 0 : (nrml ? 2 : 1);
-    if (L.nd == 2) rb = R_(0, 0);
-    Box bx = make_box(ra.f, ra.l, rb.f, rb.l, 1, L.hw);
+    // ra spans the rotation's b axis, rb its c axis; the box puts the
+    // memory-contiguous one last (fastest thread index)
+    Rot r = make_rot(L, d);
+    Box bx;
+    int ord = 0;
+    if (L.nd == 2) {
+      bx = make_box(1, L.hw, 0, 0, ra.f, ra.l);
+      ord = 1;
+    } else if (d == 1) {
+      // rotation (y, z, x): swap the tangential axes so that c is z
+      std::swap(r.sB, r.sC);
+      bx = make_box(rb.f, rb.l, 1, L.hw, ra.f, ra.l);
+    } else {
+      bx = make_box(ra.f, ra.l, 1, L.hw, rb.f, rb.l);
+    }
     if (bx.total <= 0) return;
     if (fb.n == 6) flush_fills();
     const int q = fb.n++;
-    fb.a[q] = a; fb.r[q] = make_rot(L, d); fb.bx[q] = bx;
-    fb.side[q] = side; fb.mode[q] = mode; fb.N[q] = L.n[d];
+    fb.a[q] = a; fb.r[q] = r; fb.bx[q] = bx;
+    fb.side[q] = side; fb.mode[q] = mode; fb.N[q] = L.n[d]; fb.ord[q] = ord;
   }
 
   void flush_fills()
@@ -162,17 +175,9 @@ struct PerPass {
     if (nd == 2) rb = R_(0, 0);
     R *avd = av[d], *av1 = av[(d + 1) % nd], *av2 = nd == 3 ? av[(d + 2) % 3] : av[0];
     int cyc[2], eopen[2], nopen = 0;
-    Box bx[2];
     for (int side = 0; side < 2; ++side) {
       cyc[side] = s->p.bc[d][side] == MPDX_BC_CYCLIC;
-      if (cyc[side]) {
-        bx[side] = make_box(1, hw, ra.f, ra.l, rb.f, rb.l);
-      } else {
-        eopen[nopen++] = side == 0 ? 0 : N - 1;
-        const int lo = side == 0 ? -hw : N - 1 + (ad ? 1 : 0);
-        const int hi = side == 0 ? -1 - (ad ? 1 : 0) : N - 2 + hw;
-        bx[side] = make_box(lo, hi, ra.f, ra.l, rb.f, rb.l);
-      }
+      if (!cyc[side]) eopen[nopen++] = side == 0 ? 0 : N - 1;
     }
     ProfScope ps(s, "pp_halo");
     if (nopen > 0) {
@@ -180,11 +185,32 @@ struct PerPass {
       hipLaunchKernelGGL(k_alng_zero<R>, dim3((unsigned)((nz + kBlock - 1) / kBlock), nopen), dim3(kBlock), 0,
                          s->stream, av1, av2, r, nd, d, eopen[0], eopen[nopen - 1], ra.f, ra.l, rb.f, rb.l);
     }
+    // the faces themselves: the thread index runs along the box's c axis,
+    // which for the y faces is x under the rotation (y, z, x); swap the
+    // tangential axes (and the components that vary along them) so that it
+    // runs along z, the contiguous one
+    int cfirst = 0;
+    if (nd == 3 && d == 1) {
+      std::swap(r.sB, r.sC);
+      std::swap(ra, rb);
+      std::swap(av1, av2);
+      cfirst = 1;
+    }
+    Box bx[2];
+    for (int side = 0; side < 2; ++side) {
+      if (cyc[side]) {
+        bx[side] = make_box(1, hw, ra.f, ra.l, rb.f, rb.l);
+      } else {
+        const int lo = side == 0 ? -hw : N - 1 + (ad ? 1 : 0);
+        const int hi = side == 0 ? -1 - (ad ? 1 : 0) : N - 2 + hw;
+        bx[side] = make_box(lo, hi, ra.f, ra.l, rb.f, rb.l);
+      }
+    }
     const unsigned gx = std::max(box_grid(bx[0]).x, box_grid(bx[1]).x);
     const unsigned gy = (unsigned)std::max(bx[0].ext[0], bx[1].ext[0]);
     if (bx[0].total > 0 || bx[1].total > 0)
       hipLaunchKernelGGL(k_alng2<R>, dim3(gx, gy, 2), dim3(kBlock), 0, s->stream, avd, (const R *)av1,
-                         (const R *)av2, r, bx[0], bx[1], nd, cyc[0], cyc[1], N);
+                         (const R *)av2, r, bx[0], bx[1], nd, cyc[0], cyc[1], N, cfirst);
   }
 
   // solver_3d.hpp:47-67, solver_2d.hpp:40-56

@@ -40,22 +40,15 @@ __device__ inline void fill_one(R *a, const Rot &r, int b, int c, int m, int sid
   a[dst] = mode == 2 ? R(0) : a[src];
 }
 
-template <class R>
-__global__ void k_fill(R *a, Rot r, Box bx, int side, int mode, int N)
-{
-  int b, c, m;
-  if (!box_idx(bx, b, c, m)) return;
-  fill_one(a, r, b, c, m, side, mode, N);
-}
-
 // up to 6 fills that touch disjoint cells and read none of each other's
 // outputs (both sides of one fill; fills of different arrays), one per
-// grid z index
+// grid z index.  The box order puts the memory-contiguous coordinate last
+// (the fastest thread index): ord 0 is (b, m, c), ord 1 (2D) is (m, c, b).
 template <class R> struct FillBatch {
   R *a[6];
   Rot r[6];
   Box bx[6];
-  int side[6], mode[6], N[6];
+  int side[6], mode[6], N[6], ord[6];
   int n;
 };
 
@@ -65,7 +58,11 @@ __global__ void k_fill_batch(FillBatch<R> fb)
   const int q = blockIdx.z;
   if ((int)blockIdx.y >= fb.bx[q].ext[0]) return;
   int b, c, m;
-  if (!box_idx(fb.bx[q], b, c, m)) return;
+  if (fb.ord[q] == 0) {
+    if (!box_idx(fb.bx[q], b, m, c)) return;
+  } else if (!box_idx(fb.bx[q], m, c, b)) {
+    return;
+  }
   fill_one(fb.a[q], fb.r[q], b, c, m, fb.side[q], fb.mode[q], fb.N[q]);
 }
 
@@ -98,10 +95,13 @@ __global__ void k_alng_zero(R *av1, R *av2, Rot r, int nd, int d, int e0, int e1
 
 // normal advector halo faces: cyclic copy (cyclic_3d.hpp:44-48,135-139) or
 // the open zero-divergence condition (open_3d.hpp:111-124,233-245); the box
-// is (ii or m, b, c)
+// is (ii or m, b, c).  av1 varies along b, av2 along c; cfirst says that the
+// reference's first difference term (component d+1) is the one along c (the
+// host swaps the tangential axes to make c the contiguous one), so the sum
+// keeps the reference's order.
 template <class R>
 __device__ inline void alng_one(R *avd, const R *av1, const R *av2, const Rot &r, int ii, int b, int c,
-                                int nd, int side, int cyclic, int N)
+                                int nd, int side, int cyclic, int N, int cfirst)
 {
   if (cyclic) {
     const int m = ii;
@@ -112,12 +112,13 @@ __device__ inline void alng_one(R *avd, const R *av1, const R *av2, const Rot &r
   const int e = side == 0 ? 0 : N - 1;
   R v;
   if (nd == 3) {
+    const R tb = av1[rr(r, e, b - 1, c)] - av1[rr(r, e, b, c)];
+    const R tc = av2[rr(r, e, b, c - 1)] - av2[rr(r, e, b, c)];
+    const R t1 = cfirst ? tc : tb, t2 = cfirst ? tb : tc;
     if (side == 0)
-      v = avd[rr(r, e, b, c)] - (av1[rr(r, e, b - 1, c)] - av1[rr(r, e, b, c)])
-                              - (av2[rr(r, e, b, c - 1)] - av2[rr(r, e, b, c)]);
+      v = avd[rr(r, e, b, c)] - t1 - t2;
     else
-      v = avd[rr(r, e - 1, b, c)] + (av1[rr(r, e, b - 1, c)] - av1[rr(r, e, b, c)])
-                                  + (av2[rr(r, e, b, c - 1)] - av2[rr(r, e, b, c)]);
+      v = avd[rr(r, e - 1, b, c)] + t1 + t2;
   } else {
     if (side == 0)
       v = avd[rr(r, e, b, 0)] - (av1[rr(r, e, b - 1, 0)] - av1[rr(r, e, b, 0)]);
@@ -127,28 +128,19 @@ __device__ inline void alng_one(R *avd, const R *av1, const R *av2, const Rot &r
   avd[rr(r, ii, b, c)] = v;
 }
 
-template <class R>
-__global__ void k_alng(R *avd, const R *av1, const R *av2, Rot r, Box bx, int nd,
-                       int side, int cyclic, int N)
-{
-  int ii, b, c;
-  if (!box_idx(bx, ii, b, c)) return;
-  alng_one(avd, av1, av2, r, ii, b, c, nd, side, cyclic, N);
-}
-
 // both sides of one dimension's normal-component fill in one launch (grid z =
 // side): each side writes its own halo faces and reads only cells the other
 // side does not write
 template <class R>
 __global__ void k_alng2(R *avd, const R *av1, const R *av2, Rot r, Box bx0, Box bx1, int nd,
-                        int cyc0, int cyc1, int N)
+                        int cyc0, int cyc1, int N, int cfirst)
 {
   const int side = blockIdx.z;
   const Box &bx = side == 0 ? bx0 : bx1;
   if ((int)blockIdx.y >= bx.ext[0]) return;
   int ii, b, c;
   if (!box_idx(bx, ii, b, c)) return;
-  alng_one(avd, av1, av2, r, ii, b, c, nd, side, side == 0 ? cyc0 : cyc1, N);
+  alng_one(avd, av1, av2, r, ii, b, c, nd, side, side == 0 ? cyc0 : cyc1, N, cfirst);
 }
 
 // ---------------------------------------------------------------- passes
