@@ -119,6 +119,11 @@ struct PerPass {
     return out;
   }
 
+  // > 0: the z faces of the normal-component fill (the zero-divergence or
+  // cyclic faces) are needed only in rows within zband of an x or y edge
+  // (the boundary shell next to z-boundary tiles)
+  int zband = 0;
+
   // Scalar-style fills along d (mode 0 cyclic / 1 edge copy from the bcond
   // kind, or 2 = zero).  Fills queue up in a batch that runs as one launch;
   // callers flush wherever a later fill reads what an earlier one wrote.
@@ -180,7 +185,8 @@ struct PerPass {
       if (!cyc[side]) eopen[nopen++] = side == 0 ? 0 : N - 1;
     }
     ProfScope ps(s, "pp_halo");
-    if (nopen > 0) {
+    // (the perpendicular-component zeroing does nothing along z)
+    if (nopen > 0 && !(nd == 3 && d == 2)) {
       const long nz = std::max(ra.l - ra.f + 1, rb.l - rb.f + 1);
       hipLaunchKernelGGL(k_alng_zero<R>, dim3((unsigned)((nz + kBlock - 1) / kBlock), nopen), dim3(kBlock), 0,
                          s->stream, av1, av2, r, nd, d, eopen[0], eopen[nopen - 1], ra.f, ra.l, rb.f, rb.l);
@@ -205,6 +211,22 @@ struct PerPass {
         const int hi = side == 0 ? -1 - (ad ? 1 : 0) : N - 2 + hw;
         bx[side] = make_box(lo, hi, ra.f, ra.l, rb.f, rb.l);
       }
+    }
+    if (zband > 0 && nd == 3 && d == 2) {
+      // bands of (x, y) rows: x bands over every y, y bands between them
+      const int Z = zband;
+      const rng bs[4][2] = {{R_(ra.f, ra.f + Z - 1), rb}, {R_(ra.l - Z + 1, ra.l), rb},
+                            {R_(ra.f + Z, ra.l - Z), R_(rb.f, rb.f + Z - 1)},
+                            {R_(ra.f + Z, ra.l - Z), R_(rb.l - Z + 1, rb.l)}};
+      BoxList bl{};
+      for (int side = 0; side < 2; ++side)
+        for (const auto &b : bs)
+          boxlist_add(bl, make_box(bx[side].lo[0], bx[side].lo[0] + bx[side].ext[0] - 1, b[0].f, b[0].l, b[1].f,
+                                   b[1].l), side);
+      if (bl.n > 0)
+        hipLaunchKernelGGL(k_alng_list<R>, dim3(boxlist_blocks(bl)), dim3(kBlock), 0, s->stream, avd,
+                           (const R *)av1, (const R *)av2, r, bl, nd, cyc[0], cyc[1], N, cfirst);
+      return;
     }
     const unsigned gx = std::max(box_grid(bx[0]).x, box_grid(bx[1]).x);
     const unsigned gy = (unsigned)std::max(bx[0].ext[0], bx[1].ext[0]);
@@ -625,6 +647,10 @@ int ante_loop(mpdx_solver *s)
 
 // ----------------------------------------------------------------- fused stepper
 
+// rows next to an x or y edge whose z halo the boundary shell reads when
+// z-boundary tiles compute the rest
+constexpr int ZBAND = 5;
+
 template <class R, class G>
 struct Fused {
   mpdx_solver *s;
@@ -653,24 +679,44 @@ struct Fused {
   }
 
   // materialize the scalar halo of psi (the reference's xchng_sclr before
-  // every pass, mpdata_osc_3d.hpp:73-202 via solver_common.hpp:127-136)
-  void psi_halo(R *a)
+  // every pass, mpdata_osc_3d.hpp:73-202 via solver_common.hpp:127-136), all
+  // slabs in one launch.  zbands: with z-boundary tiles the z halo is read
+  // only by the boundary shell, i.e. in rows within ZBAND of an x or y edge.
+  void psi_halo(R *a, bool zbands = false)
   {
     ProfScope ps(s, "fused_psi_halo");
     const int hw = L.hw;
     const int n0 = L.n[0], n1 = L.n[1], n2 = L.nd == 3 ? L.n[2] : 1;
     const int zl = L.nd == 3 ? -hw : 0, zh = L.nd == 3 ? n2 - 1 + hw : 0;
-    struct { Box b[6]; int nb; } hb;
-    hb.nb = 0;
-    hb.b[hb.nb++] = make_box(-hw, -1, -hw, n1 - 1 + hw, zl, zh);
-    hb.b[hb.nb++] = make_box(n0, n0 - 1 + hw, -hw, n1 - 1 + hw, zl, zh);
-    hb.b[hb.nb++] = make_box(0, n0 - 1, -hw, -1, zl, zh);
-    hb.b[hb.nb++] = make_box(0, n0 - 1, n1, n1 - 1 + hw, zl, zh);
+    BoxList bl{};
+    boxlist_add(bl, make_box(-hw, -1, -hw, n1 - 1 + hw, zl, zh));
+    boxlist_add(bl, make_box(n0, n0 - 1 + hw, -hw, n1 - 1 + hw, zl, zh));
+    boxlist_add(bl, make_box(0, n0 - 1, -hw, -1, zl, zh));
+    boxlist_add(bl, make_box(0, n0 - 1, n1, n1 - 1 + hw, zl, zh));
     if (L.nd == 3) {
-      hb.b[hb.nb++] = make_box(0, n0 - 1, 0, n1 - 1, -hw, -1);
-      hb.b[hb.nb++] = make_box(0, n0 - 1, 0, n1 - 1, n2, n2 - 1 + hw);
+      const int zr[2][2] = {{-hw, -1}, {n2, n2 - 1 + hw}};
+      for (const auto &r : zr) {
+        if (!zbands) {
+          boxlist_add(bl, make_box(0, n0 - 1, 0, n1 - 1, r[0], r[1]));
+        } else {
+          for (const Box &b : zband_rows(r[0], r[1])) boxlist_add(bl, b);
+        }
+      }
     }
-    for (int q = 0; q < hb.nb; ++q) blaunch(s->stream, hb.b[q], k_fz_halo<R>, a, L, m, hb.b[q]);
+    hipLaunchKernelGGL(k_fz_halo<R>, dim3(boxlist_blocks(bl)), dim3(256), 0, s->stream, a, L, m, bl);
+  }
+
+  // rows within ZBAND of an x or y edge (x bands over every y, y bands
+  // between them), over the z range [z0, z1]
+  std::vector<Box> zband_rows(int z0, int z1) const
+  {
+    const int n0 = L.n[0], n1 = L.n[1];
+    std::vector<Box> v;
+    v.push_back(make_box(0, ZBAND - 1, 0, n1 - 1, z0, z1));
+    v.push_back(make_box(n0 - ZBAND, n0 - 1, 0, n1 - 1, z0, z1));
+    v.push_back(make_box(ZBAND, n0 - 1 - ZBAND, 0, ZBAND - 1, z0, z1));
+    v.push_back(make_box(ZBAND, n0 - 1 - ZBAND, n1 - ZBAND, n1 - 1, z0, z1));
+    return v;
   }
 
   void donor_box(R *out, R *in, R **C, R in_scale, R out_scale, int flux_is_C, const Box &bx)
@@ -896,7 +942,10 @@ struct Fused {
     R **U = pp.unco(it, Ub);
     R **V = pp.corr(it, Vb);
     const bool fct = o & O_FCT;
-    psi_halo(psik);
+    // next to z-boundary tiles the shell reads the z halos only near the x
+    // and y edges
+    pp.zband = use_int && zb ? ZBAND : 0;
+    psi_halo(psik, use_int && zb);
     {
       ProfScope ps(s, "fused_antidiff");
       if (use_int)
@@ -958,7 +1007,7 @@ struct Fused {
     R *Gb[3];
     R **gc = pp.av_of(s->GC, Gb);
     const bool use_int = interior_supported(s->p, L);
-    psi_halo(psin);
+    psi_halo(psin, zb && use_int && passes > 1);
     if (passes == 1 || !use_int) {
       donor(out_of(0), psin, gc, in_scale, passes == 1 ? out_scale : R(1), 0);
       for (int it = 1; it < passes; ++it)

@@ -84,6 +84,46 @@ inline Box make_box(int a0, int a1, int b0, int b1, int c0, int c1)
   return b;
 }
 
+// Up to 16 boxes of different shapes launched as one 1D grid of 256-thread
+// blocks: block k belongs to box q when first[q] <= k < first[q + 1], and a
+// box's cells are numbered (a, b, c), c fastest.  tag is free for the
+// kernel (e.g. the side of a halo).
+struct BoxList {
+  Box bx[16];
+  unsigned first[17];
+  int tag[16];
+  int n;
+};
+
+inline void boxlist_add(BoxList &l, const Box &b, int tag = 0)
+{
+  if (b.total <= 0) return;
+  if (l.n == 0) l.first[0] = 0;
+  l.bx[l.n] = b;
+  l.tag[l.n] = tag;
+  l.first[l.n + 1] = l.first[l.n] + (unsigned)((b.total + 255) / 256);
+  ++l.n;
+}
+
+inline unsigned boxlist_blocks(const BoxList &l) { return l.n ? l.first[l.n] : 0; }
+
+__device__ inline bool boxlist_idx(const BoxList &l, int &q, int &a, int &bb, int &c)
+{
+  const unsigned blk = blockIdx.x;
+  int s = 0;
+  for (int i = 1; i < l.n; ++i) s += blk >= l.first[i];
+  q = s;
+  const Box &b = l.bx[s];
+  const unsigned t = (blk - l.first[s]) * blockDim.x + threadIdx.x;
+  if ((long)t >= b.total) return false;
+  const unsigned r = fdiv(t, b.d2);   // a * ext[1] + b
+  c = (int)(t - r * (unsigned)b.ext[2]) + b.lo[2];
+  const unsigned aa = fdiv(r, b.d1);
+  bb = (int)(r - aa * (unsigned)b.ext[1]) + b.lo[1];
+  a = (int)aa + b.lo[0];
+  return true;
+}
+
 __device__ inline bool box_idx(const Box &b, int &a, int &bb, int &c)
 {
   const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;

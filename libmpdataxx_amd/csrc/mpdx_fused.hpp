@@ -51,13 +51,15 @@ __device__ inline int map_idx(const BcMap &m, int d, int i)
 // ends up with the composition of the per-dimension rules: cyclic wrap by
 // N-1 (bcond/cyclic_3d.hpp:29-33) or open edge copy (bcond/open_3d.hpp:
 // 32-42).  Sources are interior cells, so the launch has no ordering hazard.
-// The host launches one box per slab: x slabs over the full extended (y, z)
-// range, y slabs over interior x, z slabs over interior (x, y).
+// One launch covers every slab (a box list): x slabs over the full extended
+// (y, z) range, y slabs over interior x, z slabs over interior (x, y) — or,
+// with z-boundary tiles, only the bands of (x, y) rows that the boundary
+// shell reads.
 template <class R>
-__global__ void __launch_bounds__(256) k_fz_halo(R *a, Layout L, BcMap m, Box bx)
+__global__ void __launch_bounds__(256) k_fz_halo(R *a, Layout L, BcMap m, BoxList bl)
 {
-  int i, j, k;
-  if (!box_idx(bx, i, j, k)) return;
+  int q, i, j, k;
+  if (!boxlist_idx(bl, q, i, j, k)) return;
   const int si = map_idx(m, 0, i), sj = map_idx(m, 1, j), sk = L.nd == 3 ? map_idx(m, 2, k) : 0;
   a[L.org + i * L.st[0] + j * L.st[1] + k * L.st[2]] = a[L.org + si * L.st[0] + sj * L.st[1] + sk * L.st[2]];
 }

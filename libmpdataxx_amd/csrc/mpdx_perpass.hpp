@@ -143,6 +143,18 @@ __global__ void k_alng2(R *avd, const R *av1, const R *av2, Rot r, Box bx0, Box 
   alng_one(avd, av1, av2, r, ii, b, c, nd, side, side == 0 ? cyc0 : cyc1, N, cfirst);
 }
 
+// one dimension's normal-component fill over a box list (box tag = side),
+// for fills restricted to some rows of the halo plane
+template <class R>
+__global__ void k_alng_list(R *avd, const R *av1, const R *av2, Rot r, BoxList bl, int nd, int cyc0,
+                            int cyc1, int N, int cfirst)
+{
+  int q, ii, b, c;
+  if (!boxlist_idx(bl, q, ii, b, c)) return;
+  const int side = bl.tag[q];
+  alng_one(avd, av1, av2, r, ii, b, c, nd, side, side == 0 ? cyc0 : cyc1, N, cfirst);
+}
+
 // ---------------------------------------------------------------- passes
 
 // donor-cell fluxes along d (donorcell_formulae.hpp:50-79)
