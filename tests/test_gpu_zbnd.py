@@ -6,7 +6,7 @@ zero and zero-divergence rules), and the boundary shell has no z slabs.
 In FP64 the results must match the oracle for every option set of the
 interior kernel and z extents where one tile holds both boundaries (12, 27,
 28) or the last tile overlaps its neighbour (29, 45, 61), with cyclic or
-open x bounds.  Against the boundary-shell path (MPDX_NO_ZBND=1) they may
+open x and y bounds.  Against the boundary-shell path (MPDX_NO_ZBND=1) they may
 differ only by rounding: the two kernels compute a z-boundary cell with the
 same formulae but the compiler contracts products into fused multiply-adds
 differently in each."""
@@ -43,9 +43,10 @@ NZ = [12, 27, 28, 29, 45, 61]
 def case(nz, seed):
     grid = (22, 19, nz)
     xb = M.CYCLIC if nz % 2 else M.OPEN
-    bcs = [(xb, xb), (M.OPEN, M.OPEN), (M.OPEN, M.OPEN)]
+    yb = M.CYCLIC if nz % 3 == 0 and xb == M.CYCLIC else M.OPEN
+    bcs = [(xb, xb), (yb, yb), (M.OPEN, M.OPEN)]
     psi = W.random_field(grid, seed + nz)
-    gc = W.streamfunction_advector(grid, seed + 100 + nz, periodic=(xb == M.CYCLIC, False, False),
+    gc = W.streamfunction_advector(grid, seed + 100 + nz, periodic=(xb == M.CYCLIC, yb == M.CYCLIC, False),
                                    max_courant=0.45)
     return grid, bcs, psi, gc
 
@@ -54,7 +55,7 @@ def case(nz, seed):
 @pytest.mark.parametrize("opts,iters", OPTS)
 def test_zbnd_tiles_match_oracle_fp64(nz, opts, iters, monkeypatch):
     grid, bcs, psi, gc = case(nz, 100)
-    print(f"zbnd vs oracle nz={nz} opts={opts} n_iters={iters} x bounds={bcs[0][0]}", flush=True)
+    print(f"zbnd vs oracle nz={nz} opts={opts} n_iters={iters} x/y bounds={bcs[0][0]}/{bcs[1][0]}", flush=True)
     got = run(grid, opts, iters, np.float64, bcs, psi, gc, 4, monkeypatch, True)
     oo = (O.FCT if opts & M.FCT else 0) | (O.IGA if opts & M.IGA else 0)
     ob = [(O.CYCLIC if b[0] == M.CYCLIC else O.OPEN,) * 2 for b in bcs]
