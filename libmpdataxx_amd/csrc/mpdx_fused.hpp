@@ -385,8 +385,10 @@ k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R 
 
 // One whole corrective pass (antidiffusive velocity, FCT limiter, donor-cell
 // update) for cells at distance >= 2 from every boundary, where no halo rule
-// touches any value the pass reads, so no advector halo fill is needed.
-// A block owns a (Y-4) x (Z-4) column of output cells and marches along x
+// touches any value the pass reads, so no advector halo fill is needed.  With
+// open bounds on both z sides (InteriorArgs::zb) the z tiles also cover the
+// cells next to the z boundaries and apply the open z halo rules in the tile
+// (halo columns, below).  A block owns a (Y-4) x (Z-4) column of output cells and marches along x
 // through a chunk of output planes; each thread owns one (y, z) column of the
 // (Y x Z) tile, which includes a 2-cell frame.  At step q (newest psi^k plane
 // p = x0 - 2 + q) it computes the velocity on plane p-1, beta on plane p-1,
