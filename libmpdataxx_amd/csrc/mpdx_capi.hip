@@ -218,14 +218,15 @@ struct PerPass {
       const rng bs[4][2] = {{R_(ra.f, ra.f + Z - 1), rb}, {R_(ra.l - Z + 1, ra.l), rb},
                             {R_(ra.f + Z, ra.l - Z), R_(rb.f, rb.f + Z - 1)},
                             {R_(ra.f + Z, ra.l - Z), R_(rb.l - Z + 1, rb.l)}};
-      BoxList bl{};
+      std::vector<std::pair<Box, int>> v;
       for (int side = 0; side < 2; ++side)
         for (const auto &b : bs)
-          boxlist_add(bl, make_box(bx[side].lo[0], bx[side].lo[0] + bx[side].ext[0] - 1, b[0].f, b[0].l, b[1].f,
-                                   b[1].l), side);
-      if (bl.n > 0)
+          v.emplace_back(make_box(bx[side].lo[0], bx[side].lo[0] + bx[side].ext[0] - 1, b[0].f, b[0].l, b[1].f,
+                                  b[1].l), side);
+      boxlist_launch(v, [&](const BoxList &bl) {
         hipLaunchKernelGGL(k_alng_list<R>, dim3(boxlist_blocks(bl)), dim3(kBlock), 0, s->stream, avd,
                            (const R *)av1, (const R *)av2, r, bl, nd, cyc[0], cyc[1], N, cfirst);
+      });
       return;
     }
     const unsigned gx = std::max(box_grid(bx[0]).x, box_grid(bx[1]).x);
@@ -688,22 +689,24 @@ struct Fused {
     const int hw = L.hw;
     const int n0 = L.n[0], n1 = L.n[1], n2 = L.nd == 3 ? L.n[2] : 1;
     const int zl = L.nd == 3 ? -hw : 0, zh = L.nd == 3 ? n2 - 1 + hw : 0;
-    BoxList bl{};
-    boxlist_add(bl, make_box(-hw, -1, -hw, n1 - 1 + hw, zl, zh));
-    boxlist_add(bl, make_box(n0, n0 - 1 + hw, -hw, n1 - 1 + hw, zl, zh));
-    boxlist_add(bl, make_box(0, n0 - 1, -hw, -1, zl, zh));
-    boxlist_add(bl, make_box(0, n0 - 1, n1, n1 - 1 + hw, zl, zh));
+    std::vector<std::pair<Box, int>> v;
+    v.emplace_back(make_box(-hw, -1, -hw, n1 - 1 + hw, zl, zh), 0);
+    v.emplace_back(make_box(n0, n0 - 1 + hw, -hw, n1 - 1 + hw, zl, zh), 0);
+    v.emplace_back(make_box(0, n0 - 1, -hw, -1, zl, zh), 0);
+    v.emplace_back(make_box(0, n0 - 1, n1, n1 - 1 + hw, zl, zh), 0);
     if (L.nd == 3) {
       const int zr[2][2] = {{-hw, -1}, {n2, n2 - 1 + hw}};
       for (const auto &r : zr) {
         if (!zbands) {
-          boxlist_add(bl, make_box(0, n0 - 1, 0, n1 - 1, r[0], r[1]));
+          v.emplace_back(make_box(0, n0 - 1, 0, n1 - 1, r[0], r[1]), 0);
         } else {
-          for (const Box &b : zband_rows(r[0], r[1])) boxlist_add(bl, b);
+          for (const Box &b : zband_rows(r[0], r[1])) v.emplace_back(b, 0);
         }
       }
     }
-    hipLaunchKernelGGL(k_fz_halo<R>, dim3(boxlist_blocks(bl)), dim3(256), 0, s->stream, a, L, m, bl);
+    boxlist_launch(v, [&](const BoxList &bl) {
+      hipLaunchKernelGGL(k_fz_halo<R>, dim3(boxlist_blocks(bl)), dim3(256), 0, s->stream, a, L, m, bl);
+    });
   }
 
   // rows within ZBAND of an x or y edge (x bands over every y, y bands
@@ -719,49 +722,56 @@ struct Fused {
     return v;
   }
 
-  void donor_box(R *out, R *in, R **C, R in_scale, R out_scale, int flux_is_C, const Box &bx)
+  // donor-cell update of the cells of the boxes, one launch per box (one
+  // launch over several slabs measured slower next to the interior kernel)
+  void donor_boxes(R *out, R *in, R **C, R in_scale, R out_scale, int flux_is_C, const std::vector<Box> &bxs)
   {
-    if (bx.total <= 0) return;
     R *c2 = L.nd == 3 ? C[2] : C[0];
-    const dim3 g = box_grid(bx);
-    if (o & O_NPA)
-      hipLaunchKernelGGL((k_fz_donor<R, true>), g, dim3(256), 0, s->stream, out, (const R *)in,
-                         (const R *)C[0], (const R *)C[1], (const R *)c2, L, in_scale, out_scale, flux_is_C, bx);
-    else
-      hipLaunchKernelGGL((k_fz_donor<R, false>), g, dim3(256), 0, s->stream, out, (const R *)in,
-                         (const R *)C[0], (const R *)C[1], (const R *)c2, L, in_scale, out_scale, flux_is_C, bx);
+    for (const Box &bx : bxs) {
+      if (bx.total <= 0) continue;
+      const dim3 g = box_grid(bx);
+      if (o & O_NPA)
+        hipLaunchKernelGGL((k_fz_donor<R, true>), g, dim3(256), 0, s->stream, out, (const R *)in,
+                           (const R *)C[0], (const R *)C[1], (const R *)c2, L, in_scale, out_scale, flux_is_C, bx);
+      else
+        hipLaunchKernelGGL((k_fz_donor<R, false>), g, dim3(256), 0, s->stream, out, (const R *)in,
+                           (const R *)C[0], (const R *)C[1], (const R *)c2, L, in_scale, out_scale, flux_is_C, bx);
+    }
   }
 
   void donor(R *out, R *in, R **C, R in_scale, R out_scale, int flux_is_C)
   {
     ProfScope ps(s, "fused_donor");
-    donor_box(out, in, C, in_scale, out_scale, flux_is_C, full_box(0));
+    donor_boxes(out, in, C, in_scale, out_scale, flux_is_C, {full_box(0)});
   }
 
-  // launch the antidiffusive velocity for owner cells in box bx
+  // the antidiffusive velocity for the owner cells of the boxes, one launch
+  // per box
   template <bool IGA, bool ABS, bool PFC>
-  void antidiff_t(R **V, R *psi, R **U, const Box &bx)
+  void antidiff_t(R **V, R *psi, R **U, const std::vector<Box> &bxs)
   {
-    if (bx.total <= 0) return;
-    const dim3 g = box_grid(bx);
-    if (L.nd == 3)
-      hipLaunchKernelGGL((k_fz_antidiff<R, G, 3, IGA, ABS, PFC>), g, dim3(256), 0, s->stream,
-                         V[0], V[1], V[2], (const R *)psi, (const R *)U[0], (const R *)U[1],
-                         (const R *)U[2], L, bx);
-    else
-      hipLaunchKernelGGL((k_fz_antidiff<R, G, 2, IGA, ABS, PFC>), g, dim3(256), 0, s->stream,
-                         V[0], V[1], V[0], (const R *)psi, (const R *)U[0], (const R *)U[1],
-                         (const R *)U[0], L, bx);
+    for (const Box &bx : bxs) {
+      if (bx.total <= 0) continue;
+      const dim3 g = box_grid(bx);
+      if (L.nd == 3)
+        hipLaunchKernelGGL((k_fz_antidiff<R, G, 3, IGA, ABS, PFC>), g, dim3(256), 0, s->stream,
+                           V[0], V[1], V[2], (const R *)psi, (const R *)U[0], (const R *)U[1],
+                           (const R *)U[2], L, bx);
+      else
+        hipLaunchKernelGGL((k_fz_antidiff<R, G, 2, IGA, ABS, PFC>), g, dim3(256), 0, s->stream,
+                           V[0], V[1], V[0], (const R *)psi, (const R *)U[0], (const R *)U[1],
+                           (const R *)U[0], L, bx);
+    }
   }
 
-  void antidiff_box(R **V, R *psi, R **U, const Box &bx)
+  void antidiff_boxes(R **V, R *psi, R **U, const std::vector<Box> &bxs)
   {
     const bool iga = o & O_IGA, ab = o & O_ABS, pfc = o & O_PFC;
-    if (iga) antidiff_t<true, false, false>(V, psi, U, bx);
-    else if (ab && pfc) antidiff_t<false, true, true>(V, psi, U, bx);
-    else if (ab) antidiff_t<false, true, false>(V, psi, U, bx);
-    else if (pfc) antidiff_t<false, false, true>(V, psi, U, bx);
-    else antidiff_t<false, false, false>(V, psi, U, bx);
+    if (iga) antidiff_t<true, false, false>(V, psi, U, bxs);
+    else if (ab && pfc) antidiff_t<false, true, true>(V, psi, U, bxs);
+    else if (ab) antidiff_t<false, true, false>(V, psi, U, bxs);
+    else if (pfc) antidiff_t<false, false, true>(V, psi, U, bxs);
+    else antidiff_t<false, false, false>(V, psi, U, bxs);
   }
 
   template <int ND, int SHAPE, bool NPA, bool ABS, bool IGA>
@@ -949,9 +959,9 @@ struct Fused {
     {
       ProfScope ps(s, "fused_antidiff");
       if (use_int)
-        for (const Box &b : shell(-1, 5, 4, true)) antidiff_box(V, psik, U, b);
+        antidiff_boxes(V, psik, U, shell(-1, 5, 4, true));
       else
-        antidiff_box(V, psik, U, full_box(-1));
+        antidiff_boxes(V, psik, U, {full_box(-1)});
     }
     if (fct) {
       {
@@ -971,7 +981,7 @@ struct Fused {
       }
       if (use_int) {
         ProfScope ps(s, "fused_donor_shell");
-        for (const Box &b : shell(0, 2, 2, true)) donor_box(out, psik, V, R(1), os, (o & O_IGA) ? 1 : 0, b);
+        donor_boxes(out, psik, V, R(1), os, (o & O_IGA) ? 1 : 0, shell(0, 2, 2, true));
       } else {
         donor(out, psik, V, R(1), os, (o & O_IGA) ? 1 : 0);
       }
@@ -1026,7 +1036,7 @@ struct Fused {
           OnAux g(s, conc);
           if (it == 1) {
             ProfScope ps(s, "fused_donor_shell");
-            for (const Box &b : shell(0, 5, 5, true)) donor_box(out_of(0), psin, gc, in_scale, R(1), 0, b);
+            donor_boxes(out_of(0), psin, gc, in_scale, R(1), 0, shell(0, 5, 5, true));
           }
           corrective_shell(out_of(it), out_of(it - 1), psin, it, passes, in_scale, os, true);
         }

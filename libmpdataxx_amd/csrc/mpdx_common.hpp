@@ -11,8 +11,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdint>
+#include <utility>
+#include <vector>
 
 #include "mpdx_fastdiv.hpp"
 
@@ -106,6 +109,34 @@ inline void boxlist_add(BoxList &l, const Box &b, int tag = 0)
 }
 
 inline unsigned boxlist_blocks(const BoxList &l) { return l.n ? l.first[l.n] : 0; }
+
+// Launches (box, tag) pairs as box lists: a box of 2^31 cells or more is cut
+// along its first coordinate (the flat index within a box is 32-bit), and
+// the pieces go out in lists of at most 16 boxes and 2^31 blocks.
+// launch(const BoxList &) issues one kernel.
+template <class F>
+inline void boxlist_launch(const std::vector<std::pair<Box, int>> &boxes, F launch)
+{
+  BoxList l{};
+  auto flush = [&]() {
+    if (l.n) launch(l);
+    l = BoxList{};
+  };
+  for (const auto &bt : boxes) {
+    const Box &b = bt.first;
+    if (b.total <= 0) continue;
+    const long plane = (long)b.ext[1] * b.ext[2];
+    const long rows = std::max(1L, (long)((1L << 31) - 1) / plane);
+    for (long a0 = 0; a0 < b.ext[0]; a0 += rows) {
+      const int a1 = (int)std::min<long>(b.ext[0] - 1, a0 + rows - 1);
+      const Box p = make_box(b.lo[0] + (int)a0, b.lo[0] + a1, b.lo[1], b.lo[1] + b.ext[1] - 1, b.lo[2],
+                             b.lo[2] + b.ext[2] - 1);
+      if (l.n == 16 || (long)boxlist_blocks(l) + (p.total + 255) / 256 >= (1L << 31)) flush();
+      boxlist_add(l, p, bt.second);
+    }
+  }
+  flush();
+}
 
 __device__ inline bool boxlist_idx(const BoxList &l, int &q, int &a, int &bb, int &c)
 {
