@@ -78,3 +78,19 @@ def test_zbnd_tiles_round_like_shell_path(nz, dtype, opts, iters, monkeypatch):
     b = run(grid, opts, iters, dtype, bcs, psi, gc, 4, monkeypatch, False)
     eps = np.finfo(dtype).eps
     assert np.abs(a.astype(np.float64) - b).max() <= 16 * eps * np.abs(b).max()
+
+
+@pytest.mark.parametrize("iters,scales", [(3, [0, 3]), (4, [-2, 1])])
+def test_zbnd_tiles_two_equations_hint_scale_fp64(iters, scales):
+    """Two equations with their own hint_scale (the tiles' scalar loads and
+    the halo columns see the scaled values) on a grid that takes the
+    interior kernel, against the oracle."""
+    from gpu_util import rel_err, run_gpu, run_oracle, setup_case
+    grid, bcs = (20, 18, 30), [(1, 1)] * 3
+    print(f"zbnd two equations n_iters={iters} hint_scale={scales}", flush=True)
+    psi, gc = setup_case(grid, bcs, 41)
+    psi2 = psi[::-1, :, ::-1].copy()
+    want = run_oracle(grid, bcs, O.FCT, iters, [psi, psi2], gc, 4, n_eqns=2, hint_scale=scales)
+    got = run_gpu(grid, bcs, O.FCT, iters, [psi, psi2], gc, 4, n_eqns=2, hint_scale=scales, path=2)
+    for g, w in zip(got, want):
+        assert rel_err(g, w) <= 1e-12
