@@ -164,6 +164,26 @@ def cpu_baseline(seconds=10.0):
                       f"{el:.1f} s, OpenMP {threads} threads, -Ofast"}
 
 
+def copy_bandwidth(torch, nbytes=1 << 30, reps=5):
+    """Achievable HBM rate on this GPU: a device-to-device copy of nbytes,
+    read + write bytes over the time of reps copies (HIP events on torch's
+    stream, which the copies run on)."""
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    t1.record()
+    t1.synchronize()
+    ms = t0.elapsed_time(t1)
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes * reps / (ms * 1e-3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +226,7 @@ def main():
         if default_wl else None
     valu = profile_figure("pmc_sq_interior_fct3_512cube.md", "valu_issue_util", dname) if default_wl else None
 
+    copy_gbs = copy_bandwidth(torch)
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "Gcell-updates/s", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": round(el32 * 1e3 / K, 4),
@@ -217,7 +238,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": int(traffic) if traffic else None, "avg_launch_ms": round(avg_ms, 4),
                      "launches": dlaunch, "alg_bytes_per_launch": alg_bytes,
-                     "valu_issue_util_pmc": valu},
+                     "valu_issue_util_pmc": valu, "copy_GBs": round(copy_gbs, 1),
+                     "frac_of_copy": round(achieved / copy_gbs, 4)},
         "step_roofline": {"alg_bytes_per_cell": 20, "achieved_GBs": round(value * 20, 2),
                           "frac": round(value * 20 / HBM_PEAK_GBS, 4)},
         "kernels_ms": {k: round(v[0], 3) for k, v in prof32.items()},
