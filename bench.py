@@ -197,8 +197,8 @@ def main():
     ap.add_argument("--no-multi", action="store_true", help="skip the multi-tracer leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharding design is "
-              "described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the library steps one "
+              "domain per GPU and has no multi-GPU halo exchange", file=sys.stderr)
         return 2
 
     import numpy as np

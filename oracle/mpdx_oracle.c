@@ -532,7 +532,7 @@ static void scale_gc(orc_t *s, double cur_dt, double old_dt)
   xchng_vctr_nrml(s, s->GC, s->hw - 1);
 }
 
-/* ---------------------------------------------------------------- driver */
+/* ---------------------------------------------------------------- MPDATA step */
 
 /* GC_unco / GC_corr bookkeeping, solvers/detail/mpdata_common.hpp:40-60 */
 static double **gc_unco(orc_t *s, int iter)
