@@ -10,6 +10,15 @@
  * reference never defines poisons the result instead of passing silently.
  *
  * Citations are path:line in the reference tree (libmpdata++/...).
+ *
+ * Pinning: the reference's own output files pin donor cell, the standard and
+ * iga antidiffusive velocities, FCT, open halos and the forcing schemes;
+ * exact identities pin cyclic halos, n_iters >= 3, npa, pfc, khn,
+ * hint_scale, upwind_filter_freq, var_dt and abs in its 1D reduction and
+ * FCT limiter branch.  Parity unpinned: abs in the tangential terms of a
+ * stencil whose psi changes sign, and the exact field of the deformational
+ * advector generator (only its invariants are pinned).
+ *
  * The heavy sweeps carry OpenMP pragmas so the same code doubles as the
  * multi-core CPU baseline; each element is computed independently, so the
  * result does not depend on the thread count.

@@ -1,0 +1,306 @@
+"""Exact identities that pin the oracle's option switches, where the
+reference holds no output to compare with.  Each identity is a property of
+the algorithm (or of binary floating point) that a plausible mistake in the
+switch breaks, and each comes with a control showing that the input has
+teeth (the identity fails for a variant it should not hold for).
+
+* npa (formulae/common.hpp:87-124): pospart / negpart as (x +- |x|) / 2
+  equal max(0, x) / min(0, x) exactly in binary floating point, so npa runs
+  are bit-identical to default runs.
+* pfc (formulae_mpdata_common.hpp:61-98): frac without the additive tiny
+  guard makes non-FCT MPDATA exactly equivariant under power-of-two scaling
+  of psi, also at magnitudes (2^-1000) where den + tiny != den; the default
+  guard breaks that equivariance there.
+* abs (formulae_mpdata_3d.hpp:253-295, formulae_mpdata_fct_3d.hpp:147-247):
+  the 1D reduction is the variable-sign MPDATA recursion with |psi| in the
+  ratio (Smolarkiewicz and Clark 1986); the FCT limiter's variable-sign
+  branch makes FCT|abs exactly odd in psi, which plain FCT is not.
+* khn (donorcell_formulae.hpp:154-164, kahan_sum.hpp): the compensated
+  donor-cell sum of psi and the six fluxes equals the correctly rounded sum
+  (math.fsum) where left-to-right summation loses bits.
+* hint_scale (solver_common.hpp:374-379): no effect for scale-free options;
+  with FCT (absolute epsilon in the limiter) the same as running on
+  2^-h psi and scaling back by 2^h.
+* upwind_filter_freq (mpdata_osc_3d.hpp:195-198): steps whose index is a
+  multiple of the frequency are donor-cell only; a run equals the
+  composition of n_iters = 1 and full steps.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from libmpdataxx_amd import workloads as W
+
+
+def make(grid, opts, iters, bcs, psi, gc, **kw):
+    o = O.Oracle(grid, opts=opts, n_iters=iters, bcs=bcs, max_abs_div_eps=1e300, **kw)
+    o.set_advectee(psi)
+    for d in range(len(grid)):
+        o.set_advector(gc[d], d)
+    return o
+
+
+def run(grid, opts, iters, bcs, psi, gc, nt, **kw):
+    o = make(grid, opts, iters, bcs, psi, gc, **kw)
+    o.advance(nt)
+    return o.advectee()
+
+
+def same_bits(a, b):
+    # +0 and -0 compare equal (the npa forms differ only in the sign of zero)
+    return np.array_equal(a + 0.0, b + 0.0)
+
+
+CASES = {
+    "2d": ((26, 21), [(O.OPEN, O.OPEN), (O.CYCLIC, O.CYCLIC)]),
+    "3d": ((14, 12, 11), [(O.OPEN, O.OPEN), (O.OPEN, O.OPEN), (O.CYCLIC, O.CYCLIC)]),
+}
+
+
+def case(name, seed, lo=0.5, hi=2.0):
+    grid, bcs = CASES[name]
+    rng = np.random.default_rng(seed)
+    psi = rng.uniform(lo, hi, grid)
+    gc = W.streamfunction_advector(grid, seed, max_courant=0.45,
+                                   periodic=tuple(bc[0] == O.CYCLIC for bc in bcs))
+    return grid, bcs, psi, gc
+
+
+# ------------------------------------------------------------------ npa
+
+@pytest.mark.parametrize("name", ["2d", "3d"])
+@pytest.mark.parametrize("opts,iters", [(0, 2), (O.FCT, 3), (O.IGA, 2), (O.IGA | O.FCT, 2)])
+def test_npa_is_bit_identical(name, opts, iters):
+    grid, bcs, psi, gc = case(name, 11)
+    a = run(grid, opts, iters, bcs, psi, gc, 4)
+    b = run(grid, opts | O.NPA, iters, bcs, psi, gc, 4)
+    assert same_bits(a, b)
+
+
+# ------------------------------------------------------------------ pfc
+
+@pytest.mark.parametrize("name", ["2d", "3d"])
+@pytest.mark.parametrize("iters", [2, 3])
+def test_pfc_scales_exactly_with_powers_of_two(name, iters):
+    grid, bcs, psi, gc = case(name, 12)
+    psi[tuple(slice(0, 3) for _ in grid)] = 0.0      # a zero block: frac(0, 0) = 0
+    lam = 2.0 ** -1000
+    ref = run(grid, O.PFC, iters, bcs, psi, gc, 3)
+    small = run(grid, O.PFC, iters, bcs, psi * lam, gc, 3)
+    assert np.isfinite(small).all()
+    assert same_bits(small, ref * lam)
+    # control: the default guard adds tiny to the denominator, which is no
+    # longer negligible at this magnitude
+    std = run(grid, 0, iters, bcs, psi * lam, gc, 3)
+    assert not same_bits(std, ref * lam)
+    assert np.abs(std / lam - ref).max() > 1e-12
+    # at ordinary magnitudes both guards agree bit for bit
+    assert same_bits(run(grid, 0, iters, bcs, psi, gc, 3), ref)
+
+
+# ------------------------------------------------------------------ abs
+
+def textbook_abs_1d(u, courant, iters, steps):
+    """Variable-sign MPDATA on a periodic 1D grid of unique cells: the
+    pseudo-velocity ratio uses |psi| (Smolarkiewicz and Clark 1986)."""
+    psi = u.copy()
+    for _ in range(steps):
+        U = np.full(psi.size, courant)
+        for k in range(iters):
+            right = np.roll(psi, -1)
+            F = np.maximum(U, 0) * psi + np.minimum(U, 0) * right
+            psi = psi - (F - np.roll(F, 1))
+            if k + 1 < iters:
+                right = np.roll(psi, -1)
+                a1, a0 = np.abs(right), np.abs(psi)
+                U = (np.abs(U) - U * U) * (a1 - a0) / (a1 + a0)
+    return psi
+
+
+@pytest.mark.parametrize("iters", [2, 3])
+@pytest.mark.parametrize("courant", [0.3, -0.45])
+def test_abs_reduces_to_variable_sign_textbook_1d(iters, courant):
+    n, m = 24, 5
+    x = np.arange(n - 1)
+    u = 0.4 + np.sin(2 * np.pi * x / (n - 1)) + 0.3 * np.cos(6 * np.pi * x / (n - 1))
+    assert (u > 0).any() and (u < 0).any()
+    psi = np.empty((n, m))
+    psi[:-1] = u[:, None]
+    psi[-1] = psi[0]
+    gc = [np.full((n, m), courant), np.zeros((n + 1, m - 1))]
+    bcs = [(O.CYCLIC, O.CYCLIC), (O.OPEN, O.OPEN)]
+    out = run((n, m), O.ABS, iters, bcs, psi, gc, 7)
+    want = textbook_abs_1d(u, courant, iters, 7)
+    assert np.allclose(out[:-1, 2], want, rtol=0, atol=1e-13)
+    # control: without abs the ratio of a sign-changing field differs
+    plain = run((n, m), 0, iters, bcs, psi, gc, 7)
+    assert np.abs(plain[:-1, 2] - want).max() > 1e-6
+
+
+@pytest.mark.parametrize("name", ["2d", "3d"])
+@pytest.mark.parametrize("iters", [2, 3])
+def test_fct_abs_is_odd_in_psi(name, iters):
+    grid, bcs, psi, gc = case(name, 13, lo=-1.0, hi=1.0)
+    a = run(grid, O.FCT | O.ABS, iters, bcs, psi, gc, 4)
+    b = run(grid, O.FCT | O.ABS, iters, bcs, -psi, gc, 4)
+    assert same_bits(a, -b)
+    # control: plain FCT picks the limiter factor by the sign of the velocity
+    # alone, which is not odd in psi
+    c = run(grid, O.FCT, iters, bcs, psi, gc, 4)
+    d = run(grid, O.FCT, iters, bcs, -psi, gc, 4)
+    assert not same_bits(c, -d)
+
+
+@pytest.mark.parametrize("name", ["2d", "3d"])
+def test_fct_abs_no_new_extrema(name):
+    grid, bcs, psi, gc = case(name, 14, lo=-1.0, hi=1.0)
+    out = run(grid, O.FCT | O.ABS, 3, bcs, psi, gc, 6)
+    assert out.max() <= psi.max() + 1e-14 and out.min() >= psi.min() - 1e-14
+
+
+# ------------------------------------------------------------------ khn
+
+def donor_terms(psi, faces):
+    """psi and the six signed donor-cell fluxes of every cell of a 3D grid
+    (pospart(C) psi_L + negpart(C) psi_R, donorcell_formulae.hpp:25-34) from
+    full face arrays (faces -1/2 .. n_d - 1/2 along d), open edge-copy halos."""
+    n = psi.shape
+    p = np.pad(psi, 1, mode="edge")
+    terms = [psi]
+    for d in range(3):
+        def cells(off):
+            s = [slice(1, n[k] + 1) for k in range(3)]
+            s[d] = slice(1 + off, n[d] + 1 + off)
+            return p[tuple(s)]
+        def face(off):
+            s = [slice(None)] * 3
+            s[d] = slice(off, n[d] + off)
+            return faces[d][tuple(s)]
+        Cr, Cl = face(1), face(0)          # faces i + 1/2 and i - 1/2
+        f_r = np.maximum(0, Cr) * cells(0) + np.minimum(0, Cr) * cells(1)
+        f_l = np.maximum(0, Cl) * cells(-1) + np.minimum(0, Cl) * cells(0)
+        terms += [-f_r, f_l]
+    return terms
+
+
+def advector_views(faces):
+    """The oracle's advector views (x from -1, faces along d from -1/2 for
+    d = 0 and from 1/2 otherwise) cut from full face arrays."""
+    nx = faces[1].shape[0]
+    out = [faces[0][0:nx]]        # faces -1/2 .. n_x - 3/2
+    for d in (1, 2):
+        s = [slice(None)] * 3
+        s[d] = slice(1, faces[d].shape[d] - 1)
+        v = faces[d][tuple(s)]
+        out.append(np.concatenate([v[:1], v], axis=0))   # x = -1: edge column
+    return out
+
+
+def test_khn_sum_is_correctly_rounded():
+    n = 8
+    grid = (n, n, n)
+    psi = np.ones(grid)
+    c = n // 2
+    psi[c, c, c] = 1e16
+    # converging flow around the big cell: each of its six faces brings in
+    # 0.5 from a neighbour, so psi + six halves must not lose them
+    faces = []
+    for d in range(3):
+        shape = [n, n, n]
+        shape[d] = n + 1
+        a = np.zeros(shape)
+        idx = [c, c, c]
+        idx[d] = c              # face c - 1/2
+        a[tuple(idx)] = 0.5
+        idx[d] = c + 1          # face c + 1/2
+        a[tuple(idx)] = -0.5
+        faces.append(a)
+    bcs = [(O.OPEN, O.OPEN)] * 3
+    views = advector_views(faces)
+    o = make(grid, O.KHN, 1, bcs, psi, views)
+    for d in range(3):
+        assert o.advector_shape(d) == views[d].shape
+    o.advance(1)
+    got = o.advectee()
+    terms = donor_terms(psi, faces)
+    want = np.empty(grid)
+    naive = np.empty(grid)
+    for idx in np.ndindex(*grid):
+        vals = [t[idx] for t in terms]
+        want[idx] = math.fsum(vals)
+        s = 0.0
+        for v in vals:
+            s += v
+        naive[idx] = s
+    assert got[c, c, c] == want[c, c, c] == float(10 ** 16 + 3)   # the exact sum, rounded once
+    assert naive[c, c, c] != want[c, c, c]                 # left to right loses them
+    ulp = np.spacing(np.abs(want))
+    assert np.all(np.abs(got - want) <= ulp)
+    # control: the plain (pairwise) sum of the same step
+    plain = make(grid, 0, 1, bcs, psi, views)
+    plain.advance(1)
+    assert np.abs(plain.advectee() - want).max() <= 2 * ulp.max()
+
+
+@pytest.mark.parametrize("opts,iters", [(0, 2), (O.FCT, 3)])
+def test_khn_matches_plain_sum_to_rounding(opts, iters):
+    grid, bcs, psi, gc = case("3d", 15)
+    a = run(grid, opts, iters, bcs, psi, gc, 5)
+    b = run(grid, opts | O.KHN, iters, bcs, psi, gc, 5)
+    assert np.abs(a - b).max() <= 1e-14 * np.abs(a).max()
+
+
+# ------------------------------------------------------------------ hint_scale
+
+@pytest.mark.parametrize("name", ["2d", "3d"])
+@pytest.mark.parametrize("opts,iters", [(0, 2), (O.IGA, 2), (0, 3)])
+@pytest.mark.parametrize("h", [-7, 5])
+def test_hint_scale_no_effect_when_scale_free(name, opts, iters, h):
+    grid, bcs, psi, gc = case(name, 16)
+    a = run(grid, opts, iters, bcs, psi, gc, 4)
+    b = run(grid, opts, iters, bcs, psi, gc, 4, hint_scale=[h])
+    assert same_bits(a, b)
+
+
+@pytest.mark.parametrize("name", ["2d", "3d"])
+@pytest.mark.parametrize("h", [-30, 12])
+def test_hint_scale_fct_is_prescaling(name, h):
+    # magnitudes where the limiter's absolute epsilon matters
+    grid, bcs, psi, gc = case(name, 17)
+    psi = psi * 1e-13
+    got = run(grid, O.FCT, 3, bcs, psi, gc, 3, hint_scale=[h])
+    o = make(grid, O.FCT, 3, bcs, psi * 2.0 ** -h, gc)
+    for _ in range(3):
+        o.advance(1)
+    want = o.advectee() * 2.0 ** h
+    assert same_bits(got, want)
+    # control: the scaling changes the FCT result here, in either direction
+    plain = run(grid, O.FCT, 3, bcs, psi, gc, 3)
+    assert not same_bits(got, plain)
+    flipped = make(grid, O.FCT, 3, bcs, psi * 2.0 ** h, gc)
+    flipped.advance(3)
+    assert not same_bits(got, flipped.advectee() * 2.0 ** -h)
+
+
+# ------------------------------------------------------------------ upwind_filter_freq
+
+@pytest.mark.parametrize("name", ["2d", "3d"])
+@pytest.mark.parametrize("opts,iters,freq", [(O.FCT, 3, 3), (0, 2, 2), (O.FCT, 2, 1)])
+def test_upwind_filter_is_composition(name, opts, iters, freq):
+    grid, bcs, psi, gc = case(name, 18)
+    nt = 7
+    got = run(grid, opts, iters, bcs, psi, gc, nt, upwind_filter_freq=freq)
+    cur = psi
+    for step in range(nt):
+        it = 1 if step % freq == 0 else iters
+        cur = run(grid, opts, it, bcs, cur, gc, 1)
+    assert same_bits(got, cur)
+    # control: filtering other steps (one later) gives another result
+    alt = psi
+    for step in range(nt):
+        it = 1 if (step + 1) % freq == 0 else iters
+        alt = run(grid, opts, it, bcs, alt, gc, 1)
+    if freq > 1:
+        assert not same_bits(got, alt)
