@@ -13,11 +13,10 @@
  *
  * Pinning: the reference's own output files pin donor cell, the standard and
  * iga antidiffusive velocities, FCT, open halos and the forcing schemes;
- * exact identities pin cyclic halos, n_iters >= 3, npa, pfc, khn,
- * hint_scale, upwind_filter_freq, var_dt and abs in its 1D reduction and
- * FCT limiter branch.  Parity unpinned: abs in the tangential terms of a
- * stencil whose psi changes sign, and the exact field of the deformational
- * advector generator (only its invariants are pinned).
+ * exact identities and textbook reductions pin cyclic halos, n_iters >= 3,
+ * abs, npa, pfc, khn, hint_scale, upwind_filter_freq and var_dt.  Parity
+ * unpinned: the exact field of the deformational advector generator (only
+ * its invariants are pinned).
  *
  * The heavy sweeps carry OpenMP pragmas so the same code doubles as the
  * multi-core CPU baseline; each element is computed independently, so the

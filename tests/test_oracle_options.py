@@ -13,8 +13,11 @@ teeth (the identity fails for a variant it should not hold for).
   guard breaks that equivariance there.
 * abs (formulae_mpdata_3d.hpp:253-295, formulae_mpdata_fct_3d.hpp:147-247):
   the 1D reduction is the variable-sign MPDATA recursion with |psi| in the
-  ratio (Smolarkiewicz and Clark 1986); the FCT limiter's variable-sign
-  branch makes FCT|abs exactly odd in psi, which plain FCT is not.
+  ratio (Smolarkiewicz and Clark 1986); on cyclic 2D and 3D grids one
+  corrective pass equals the published pseudo-velocity with |psi| in the
+  normal and both tangential ratios, applied to the oracle's own donor-cell
+  result; the FCT limiter's variable-sign branch makes FCT|abs exactly odd
+  in psi, which plain FCT is not.
 * khn (donorcell_formulae.hpp:154-164, kahan_sum.hpp): the compensated
   donor-cell sum of psi and the six fluxes equals the correctly rounded sum
   (math.fsum) where left-to-right summation loses bits.
@@ -151,6 +154,79 @@ def test_fct_abs_is_odd_in_psi(name, iters):
     c = run(grid, O.FCT, iters, bcs, psi, gc, 4)
     d = run(grid, O.FCT, iters, bcs, -psi, gc, 4)
     assert not same_bits(c, -d)
+
+
+def textbook_abs_pass(p1, gc, lo, hi):
+    """One corrective pass of variable-sign MPDATA (Smolarkiewicz 1984
+    eq. 13-14 and its 3D form, with |psi| in every ratio as in Smolarkiewicz
+    and Clark 1986) on the cells lo..hi of every axis:
+    V_d = |u|(1-|u|) A - u / 2 sum_t ubar_t B_t, A the |psi| ratio across the
+    face, B_t the tangential |psi| ratio along t and ubar_t the 4-point mean
+    of the t component around the face; then the donor-cell update of p1
+    with V.  gc[d][x+1, y(, z)] is the d component at the + face of cell
+    (x, y(, z)) along d (the oracle's advector views)."""
+    nd = p1.ndim
+    tiny = np.finfo(np.float64).tiny
+    P = np.abs(p1)
+
+    def e(d, k=1):
+        v = [0] * nd
+        v[d] = k
+        return tuple(v)
+
+    def add(*cs):
+        return tuple(sum(x) for x in zip(*cs))
+
+    def face(d, c):
+        return gc[d][(c[0] + 1,) + tuple(c[1:])]
+
+    def vel(d, c):
+        u = face(d, c)
+        r = add(c, e(d))
+        A = (P[r] - P[c]) / (P[r] + P[c] + tiny)
+        tang = 0.0
+        for t in range(nd):
+            if t == d:
+                continue
+            q = [P[add(r, e(t))], P[add(c, e(t))], P[add(r, e(t, -1))], P[add(c, e(t, -1))]]
+            B = (q[0] + q[1] - q[2] - q[3]) / (q[0] + q[1] + q[2] + q[3] + tiny)
+            ub = (face(t, r) + face(t, c) + face(t, add(r, e(t, -1))) + face(t, add(c, e(t, -1)))) / 4
+            tang += ub * B
+        return abs(u) * (1 - abs(u)) * A - u * tang / 2
+
+    def flux(v, l, r):
+        return max(0.0, v) * l + min(0.0, v) * r
+
+    out = {}
+    for c in np.ndindex(*([hi - lo + 1] * nd)):
+        c = tuple(x + lo for x in c)
+        total = p1[c]
+        for d in range(nd):
+            m = add(c, e(d, -1))
+            total -= flux(vel(d, c), p1[c], p1[add(c, e(d))]) - flux(vel(d, m), p1[m], p1[c])
+        out[c] = total
+    return out
+
+
+@pytest.mark.parametrize("grid", [(20, 18), (13, 12, 11)])
+def test_abs_tangential_terms(grid):
+    # sign-changing field: most tangential stencils mix signs
+    bcs = [(O.CYCLIC, O.CYCLIC)] * len(grid)
+    rng = np.random.default_rng(19)
+    psi = rng.uniform(-1.0, 1.0, grid)
+    gc = W.streamfunction_advector(grid, 19, max_courant=0.45, periodic=True)
+    p1 = run(grid, O.ABS, 1, bcs, psi, gc, 1)             # the donor-cell pass
+    got = run(grid, O.ABS, 2, bcs, psi, gc, 1)
+    lo, hi = 3, min(grid) - 4                               # no halo within reach
+    want = textbook_abs_pass(p1, gc, lo, hi)
+    err = max(abs(got[k] - v) for k, v in want.items())
+    assert err <= 1e-14
+    box = lambda c: p1[tuple(slice(x - 1, x + 2) for x in c)]
+    mixed = sum(1 for c in want if (box(c) > 0).any() and (box(c) < 0).any())
+    assert mixed > len(want) // 2
+    # control: the plain ratios differ on these stencils
+    plain = run(grid, 0, 2, bcs, psi, gc, 1)
+    assert max(abs(plain[k] - v) for k, v in want.items()) > 1e-6
 
 
 @pytest.mark.parametrize("name", ["2d", "3d"])
