@@ -922,7 +922,7 @@ struct Fused {
   // writes, so with an interior kernel the shell runs on the auxiliary
   // stream next to it (fork / join through events; a captured step keeps
   // the same dependencies).  Kernel timing runs everything on one stream.
-  bool concurrent() const { return s->aux && s->use_concurrency && !s->prof_on; }
+  bool concurrent() const { return s->aux && s->ev_fork && s->ev_join && s->use_concurrency && !s->prof_on; }
 
   void fork()
   {
@@ -1493,9 +1493,14 @@ int mpdx_create(const mpdx_params *p, mpdx_solver **out)
     (void)hipGetLastError();
     s->use_concurrency = false;
   }
-  if (const char *ev = std::getenv("MPDX_NO_GRAPHS")) s->use_graphs = !(ev[0] && ev[0] != '0');
-  if (const char *ev = std::getenv("MPDX_NO_CONCURRENCY")) s->use_concurrency = !(ev[0] && ev[0] != '0');
-  if (const char *ev = std::getenv("MPDX_NO_ZBND")) s->use_zbnd = !(ev[0] && ev[0] != '0');
+  // run-time switches only turn features off ("0" or empty leaves them on)
+  auto off = [](const char *name) {
+    const char *ev = std::getenv(name);
+    return ev && ev[0] && ev[0] != '0';
+  };
+  if (off("MPDX_NO_GRAPHS")) s->use_graphs = false;
+  if (off("MPDX_NO_CONCURRENCY")) s->use_concurrency = false;
+  if (off("MPDX_NO_ZBND")) s->use_zbnd = false;
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -1568,13 +1573,18 @@ const char *mpdx_last_error(const mpdx_solver *s)
 int mpdx_set_stream(mpdx_solver *s, void *stream)
 {
   HIPCHK(hipStreamSynchronize(s->stream));
-  if (s->own_stream) HIPCHK(hipStreamDestroy(s->stream));
-  s->own_stream = false;
-  s->stream = static_cast<hipStream_t>(stream);
-  if (!s->stream) {
-    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-    s->own_stream = true;
+  // the new stream first, so that a failure leaves the solver on its old one
+  hipStream_t next = static_cast<hipStream_t>(stream);
+  bool own_next = false;
+  if (!next) {
+    HIPCHK(hipStreamCreateWithFlags(&next, hipStreamNonBlocking));
+    own_next = true;
   }
+  hipStream_t old = s->stream;
+  const bool own_old = s->own_stream;
+  s->stream = next;
+  s->own_stream = own_next;
+  if (own_old && old) (void)hipStreamDestroy(old);
   return MPDX_OK;
 }
 
