@@ -7,7 +7,9 @@
  * where the reference's concurr::serial/openmp would run the solver objects.
  * Plain pointers and sizes only; all calls return 0 on success and a nonzero
  * MPDX_E* code on failure, with the message (libmpdata++'s own wording where
- * the reference throws) available from mpdx_last_error().
+ * the reference throws) available from mpdx_last_error().  A null handle or
+ * output pointer gives MPDX_EINVAL, its message from mpdx_last_error(NULL)
+ * (mpdx_ensemble_last_error(NULL) for ensembles).
  *
  * Reference interfaces replaced (paths relative to the reference tree):
  *   mpdx_create            concurr::serial ctor -> concurr_common ctor
@@ -153,7 +155,8 @@ int mpdx_time(const mpdx_solver *h, double *t);
 int mpdx_timestep(const mpdx_solver *h, int64_t *ts);
 /* host flag polled once per step; set nonzero to make advance() return early */
 int *mpdx_panic_ptr(mpdx_solver *h);
-/* which path advance() uses for this configuration (MPDX_PATH_PERPASS/FUSED) */
+/* which path advance() uses for this configuration (MPDX_PATH_PERPASS/FUSED;
+ * -1 for a null handle) */
 int mpdx_active_path(const mpdx_solver *h);
 
 /* Time-varying advector (the reference's calc_gc() hook, solver_common.hpp:

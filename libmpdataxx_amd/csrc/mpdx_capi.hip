@@ -1393,6 +1393,17 @@ bool ensemble_panic(const mpdx_ensemble *e)
 
 // ----------------------------------------------------------------- C ABI
 
+// A null handle or output pointer is an error (MPDX_EINVAL) rather than a
+// crash; the message is then available from mpdx_last_error(NULL) /
+// mpdx_ensemble_last_error(NULL).
+#define MPDX_NEED(cond, what)                                     \
+  do {                                                            \
+    if (!(cond)) {                                                \
+      g_create_error = "libmpdata++: null " what;                 \
+      return MPDX_EINVAL;                                         \
+    }                                                             \
+  } while (0)
+
 extern "C" {
 
 void mpdx_params_default(mpdx_params *p)
@@ -1411,6 +1422,7 @@ void mpdx_params_default(mpdx_params *p)
 
 int mpdx_create(const mpdx_params *p, mpdx_solver **out)
 {
+  MPDX_NEED(out, "output pointer");
   *out = nullptr;
   const uint32_t unsupported = MPDX_OPT_TOT | MPDX_OPT_NUG | MPDX_OPT_DFL | ~0x1FFu;
   auto fail = [&](int code, const char *msg) {
@@ -1572,6 +1584,7 @@ const char *mpdx_last_error(const mpdx_solver *s)
 
 int mpdx_set_stream(mpdx_solver *s, void *stream)
 {
+  MPDX_NEED(s, "solver handle");
   HIPCHK(hipStreamSynchronize(s->stream));
   // the new stream first, so that a failure leaves the solver on its old one
   hipStream_t next = static_cast<hipStream_t>(stream);
@@ -1590,6 +1603,8 @@ int mpdx_set_stream(mpdx_solver *s, void *stream)
 
 int mpdx_advector_extents(const mpdx_solver *s, int d, int ext[3])
 {
+  MPDX_NEED(s, "solver handle");
+  MPDX_NEED(ext, "output pointer");
   if (d < 0 || d >= s->L.nd) return MPDX_EINVAL;
   int lo[3], hi[3];
   adv_bounds(s, d, lo, hi);
@@ -1599,6 +1614,7 @@ int mpdx_advector_extents(const mpdx_solver *s, int d, int ext[3])
 
 int mpdx_set_advectee(mpdx_solver *s, int e, const void *buf, const int64_t strides[3], int on_device)
 {
+  MPDX_NEED(s, "solver handle");
   if (e < 0 || e >= s->p.n_eqns) { s->err = "libmpdata++: bad equation index"; return MPDX_EINVAL; }
   int lo[3] = {0, 0, 0}, hi[3] = {s->L.n[0] - 1, s->L.n[1] - 1, s->L.nd == 3 ? s->L.n[2] - 1 : 0};
   return copy_view(s, s->psi[e][s->lev[e]], lo, hi, const_cast<void *>(buf), strides, on_device, true);
@@ -1606,6 +1622,7 @@ int mpdx_set_advectee(mpdx_solver *s, int e, const void *buf, const int64_t stri
 
 int mpdx_get_advectee(mpdx_solver *s, int e, void *buf, const int64_t strides[3], int on_device)
 {
+  MPDX_NEED(s, "solver handle");
   if (e < 0 || e >= s->p.n_eqns) { s->err = "libmpdata++: bad equation index"; return MPDX_EINVAL; }
   int lo[3] = {0, 0, 0}, hi[3] = {s->L.n[0] - 1, s->L.n[1] - 1, s->L.nd == 3 ? s->L.n[2] - 1 : 0};
   return copy_view(s, s->psi[e][s->lev[e]], lo, hi, buf, strides, on_device, false);
@@ -1613,6 +1630,7 @@ int mpdx_get_advectee(mpdx_solver *s, int e, void *buf, const int64_t strides[3]
 
 int mpdx_set_advector(mpdx_solver *s, int d, const void *buf, const int64_t strides[3], int on_device)
 {
+  MPDX_NEED(s, "solver handle");
   if (d < 0 || d >= s->L.nd) { s->err = "libmpdata++: bad advector component"; return MPDX_EINVAL; }
   int lo[3], hi[3];
   adv_bounds(s, d, lo, hi);
@@ -1621,6 +1639,7 @@ int mpdx_set_advector(mpdx_solver *s, int d, const void *buf, const int64_t stri
 
 int mpdx_get_advector(mpdx_solver *s, int d, void *buf, const int64_t strides[3], int on_device)
 {
+  MPDX_NEED(s, "solver handle");
   if (d < 0 || d >= s->L.nd) { s->err = "libmpdata++: bad advector component"; return MPDX_EINVAL; }
   int lo[3], hi[3];
   adv_bounds(s, d, lo, hi);
@@ -1629,6 +1648,7 @@ int mpdx_get_advector(mpdx_solver *s, int d, void *buf, const int64_t strides[3]
 
 int mpdx_advance(mpdx_solver *s, int64_t nt)
 {
+  MPDX_NEED(s, "solver handle");
   if (nt < 0) { s->err = "libmpdata++: negative number of steps"; return MPDX_EINVAL; }
   if (s->p.var_dt) { s->err = "libmpdata++: a var_dt solver advances by a duration (mpdx_advance_time)"; return MPDX_EINVAL; }
   if (s->esz == 8) return advance_impl<double, double>(s, nt);
@@ -1639,6 +1659,7 @@ int mpdx_advance(mpdx_solver *s, int64_t nt)
 
 int mpdx_advance_time(mpdx_solver *s, double duration)
 {
+  MPDX_NEED(s, "solver handle");
   if (!s->p.var_dt) { s->err = "libmpdata++: advancing by a duration needs var_dt"; return MPDX_EINVAL; }
   if (!(duration >= 0)) { s->err = "libmpdata++: negative duration"; return MPDX_EINVAL; }
   const double t_end = s->time + duration;
@@ -1650,12 +1671,16 @@ int mpdx_advance_time(mpdx_solver *s, double duration)
 
 int mpdx_dt(const mpdx_solver *s, double *dt)
 {
+  MPDX_NEED(s, "solver handle");
+  MPDX_NEED(dt, "output pointer");
   *dt = s->p.dt;
   return MPDX_OK;
 }
 
 int mpdx_reduce(mpdx_solver *s, int e, int op, double *out)
 {
+  MPDX_NEED(s, "solver handle");
+  MPDX_NEED(out, "output pointer");
   if (e < 0 || e >= s->p.n_eqns || op < 0 || op > 2) { s->err = "libmpdata++: bad reduce arguments"; return MPDX_EINVAL; }
   if (s->esz == 8) return reduce_impl<double>(s, s->psi[e][s->lev[e]], op, out);
   return reduce_impl<float>(s, s->psi[e][s->lev[e]], op, out);
@@ -1663,22 +1688,27 @@ int mpdx_reduce(mpdx_solver *s, int e, int op, double *out)
 
 int mpdx_time(const mpdx_solver *s, double *t)
 {
+  MPDX_NEED(s, "solver handle");
+  MPDX_NEED(t, "output pointer");
   *t = s->time;
   return MPDX_OK;
 }
 
 int mpdx_timestep(const mpdx_solver *s, int64_t *ts)
 {
+  MPDX_NEED(s, "solver handle");
+  MPDX_NEED(ts, "output pointer");
   *ts = s->timestep;
   return MPDX_OK;
 }
 
 int *mpdx_panic_ptr(mpdx_solver *s) { return &s->panic; }
 
-int mpdx_active_path(const mpdx_solver *s) { return s->active_path; }
+int mpdx_active_path(const mpdx_solver *s) { return s ? s->active_path : -1; }
 
 int mpdx_set_calc_gc(mpdx_solver *s, mpdx_calc_gc_fn fn, void *user)
 {
+  MPDX_NEED(s, "solver handle");
   s->calc_gc = fn;
   s->calc_gc_user = user;
   return MPDX_OK;
@@ -1686,6 +1716,7 @@ int mpdx_set_calc_gc(mpdx_solver *s, mpdx_calc_gc_fn fn, void *user)
 
 int mpdx_set_rhs_scheme(mpdx_solver *s, int scheme, mpdx_update_rhs_fn fn, void *user)
 {
+  MPDX_NEED(s, "solver handle");
   if (scheme < MPDX_RHS_NONE || scheme > MPDX_RHS_TRAPEZ) {
     s->err = "libmpdata++: unsupported rhs scheme";
     return MPDX_EUNSUPPORTED;
@@ -1701,6 +1732,7 @@ int mpdx_set_rhs_scheme(mpdx_solver *s, int scheme, mpdx_update_rhs_fn fn, void 
 
 int mpdx_set_rhs(mpdx_solver *s, int e, const void *buf, const int64_t strides[3], int on_device)
 {
+  MPDX_NEED(s, "solver handle");
   if (e < 0 || e >= s->p.n_eqns || !s->rhs[e]) {
     s->err = "libmpdata++: no forcing array for this equation (set an rhs scheme first)";
     return MPDX_EINVAL;
@@ -1718,6 +1750,7 @@ static int interior_device(mpdx_solver *s, void *arr, void **ptr, int64_t stride
 
 int mpdx_rhs_device(mpdx_solver *s, int e, void **ptr, int64_t strides[3])
 {
+  MPDX_NEED(s, "solver handle");
   if (e < 0 || e >= s->p.n_eqns || !s->rhs[e]) {
     s->err = "libmpdata++: no forcing array for this equation (set an rhs scheme first)";
     return MPDX_EINVAL;
@@ -1727,12 +1760,14 @@ int mpdx_rhs_device(mpdx_solver *s, int e, void **ptr, int64_t strides[3])
 
 int mpdx_advectee_device(mpdx_solver *s, int e, void **ptr, int64_t strides[3])
 {
+  MPDX_NEED(s, "solver handle");
   if (e < 0 || e >= s->p.n_eqns) { s->err = "libmpdata++: bad equation index"; return MPDX_EINVAL; }
   return interior_device(s, s->psi[e][s->lev[e]], ptr, strides);
 }
 
 int mpdx_advector_device(mpdx_solver *s, int d, void **ptr, int64_t strides[3])
 {
+  MPDX_NEED(s, "solver handle");
   if (d < 0 || d >= s->L.nd) { s->err = "libmpdata++: bad advector component"; return MPDX_EINVAL; }
   int lo[3], hi[3];
   adv_bounds(s, d, lo, hi);
@@ -1744,6 +1779,7 @@ int mpdx_advector_device(mpdx_solver *s, int d, void **ptr, int64_t strides[3])
 
 int mpdx_set_advector_generator(mpdx_solver *s, int kind, const double *params, int n_params)
 {
+  MPDX_NEED(s, "solver handle");
   if (kind == MPDX_GC_NONE) { s->gen_kind = kind; return MPDX_OK; }
   if (s->p.var_dt) {
     s->err = "libmpdata++: the built-in advector generator does not follow a variable dt";
@@ -1786,6 +1822,7 @@ int mpdx_set_advector_generator(mpdx_solver *s, int kind, const double *params, 
 
 int mpdx_profile(mpdx_solver *s, int enable)
 {
+  MPDX_NEED(s, "solver handle");
   HIPCHK(hipStreamSynchronize(s->stream));
   prof_collect(s);
   s->prof.clear();
@@ -1797,6 +1834,7 @@ int mpdx_profile(mpdx_solver *s, int enable)
 int mpdx_profile_read(mpdx_solver *s, int idx, char *name, int name_len, double *total_ms,
                       int64_t *launches)
 {
+  MPDX_NEED(s, "solver handle");
   HIPCHK(hipStreamSynchronize(s->stream));
   prof_collect(s);
   if (idx < 0 || idx >= (int)s->prof_order.size()) return MPDX_EINVAL;
@@ -1813,6 +1851,7 @@ int mpdx_profile_read(mpdx_solver *s, int idx, char *name, int name_len, double 
 
 int mpdx_set_x_chunk(mpdx_solver *s, int planes)
 {
+  MPDX_NEED(s, "solver handle");
   if (planes != 0 && (planes < 8 || planes > 4096)) {
     s->err = "libmpdata++: x chunk must be 0 (automatic) or 8..4096 planes";
     return MPDX_EINVAL;
@@ -1898,6 +1937,7 @@ static bool ensemble_enabled(mpdx_ensemble *e)
 
 int mpdx_ensemble_advance(mpdx_ensemble *e, int64_t nt)
 {
+  MPDX_NEED(e, "ensemble handle");
   if (nt < 0) { e->err = "libmpdata++: negative number of steps"; return MPDX_EINVAL; }
   if (!ensemble_enabled(e)) return MPDX_EUNSUPPORTED;
   if (nt == 0) return MPDX_OK;
@@ -1912,6 +1952,7 @@ int mpdx_ensemble_advance(mpdx_ensemble *e, int64_t nt)
 
 int mpdx_ensemble_tune(mpdx_ensemble *e, int steps_per_trial, int64_t *steps_taken)
 {
+  MPDX_NEED(e, "ensemble handle");
   if (steps_taken) *steps_taken = 0;
   if (steps_per_trial < 1) { e->err = "libmpdata++: steps_per_trial must be >= 1"; return MPDX_EINVAL; }
   if (!ensemble_enabled(e)) return MPDX_EUNSUPPORTED;

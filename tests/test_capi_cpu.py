@@ -49,6 +49,43 @@ def test_create_rejects_like_reference(kw, msg):
         M.Solver(**kw)
 
 
+def test_null_handles_are_errors():
+    """Every entry point that takes a solver or ensemble handle returns
+    MPDX_EINVAL for a null one (the check comes before any HIP call), with
+    the message from mpdx_last_error(NULL)."""
+    L = M.lib()
+    d, i64 = ctypes.c_double(), ctypes.c_int64()
+    ext = (ctypes.c_int * 3)()
+    st = (ctypes.c_int64 * 3)()
+    buf = (ctypes.c_double * 8)()
+    calls = {
+        "mpdx_set_stream": (None, None),
+        "mpdx_advector_extents": (None, 0, ext),
+        "mpdx_set_advectee": (None, 0, buf, st, 0),
+        "mpdx_get_advectee": (None, 0, buf, st, 0),
+        "mpdx_set_advector": (None, 0, buf, st, 0),
+        "mpdx_get_advector": (None, 0, buf, st, 0),
+        "mpdx_advance": (None, 1),
+        "mpdx_advance_time": (None, 1.0),
+        "mpdx_dt": (None, ctypes.byref(d)),
+        "mpdx_reduce": (None, 0, 0, ctypes.byref(d)),
+        "mpdx_time": (None, ctypes.byref(d)),
+        "mpdx_timestep": (None, ctypes.byref(i64)),
+        "mpdx_set_x_chunk": (None, 0),
+        "mpdx_profile": (None, 1),
+        "mpdx_ensemble_advance": (None, 1),
+        "mpdx_ensemble_tune": (None, 1, None),
+    }
+    for name, args in calls.items():
+        fn = getattr(L, name)
+        assert fn(*args) == 1, name                       # MPDX_EINVAL
+        assert b"null" in L.mpdx_last_error(None), name
+    assert L.mpdx_active_path(None) == -1
+    p = M.Params()
+    L.mpdx_params_default(ctypes.byref(p))
+    assert L.mpdx_create(ctypes.byref(p), None) == 1
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(M, "LIB_PATH", str(tmp_path / "absent.so"))
     monkeypatch.setattr(M, "_lib", None)
