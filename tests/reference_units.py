@@ -1,0 +1,55 @@
+"""Inputs of two of the reference's unit tests that bound the FCT path,
+built as the reference programs build them (real_t = float), in the views
+of this backend (advectee [x][y]; advector(0) over faces -1/2 .. n_x-3/2,
+advector(1) over x = -1 .. n_x-1 and faces 1/2 .. n_y-3/2).
+
+* cone_bugs (tests/unit/cone_bugs/test_issue.cpp): 15 x 14 cyclic, FCT,
+  n_iters = 2, a cone of height 4 on a background of 1 under a uniform
+  Courant number 0.3 along both axes; after 2 steps the minimum is exactly 1.
+* hint_scale (tests/unit/hint_scale/hint_scale.cpp): 4 x 4, cyclic in x and
+  open in y, FCT | iga, n_iters = 2, psi ~ 8e7 next to zeros; with
+  hint_scale = 24 one step keeps max <= the initial max and min >= 0.  The
+  program's comma lists are written over the arrays' raw memory (interior
+  values, then halo NaNs); the values below are the interior part of each
+  row.  Its other half (values below zero without scaling) needs the
+  program's fesetround(FE_DOWNWARD) float arithmetic and is not reproduced.
+"""
+import numpy as np
+
+F32_EPS = float(np.finfo(np.float32).eps)
+
+
+def cone_bugs_case():
+    nx, ny = 15, 14
+    f = np.float32
+    dx = dy = f(1)
+    h, h0, r = f(4), f(1), f(12) * dx
+    x0, y0 = f(1), f(0)
+    i = np.arange(nx, dtype=f)[:, None]
+    j = np.arange(ny, dtype=f)[None, :]
+    d2 = (i * dx - x0) ** 2 + (j * dy - y0) ** 2
+    psi = h0 + np.where(d2 - r * r <= 0, h * (f(1) - d2 / (r * r)) ** 2, f(0))
+    gc = [np.full((nx, ny), f(0.3)), np.full((nx + 1, ny - 1), f(0.3))]
+    return dict(grid=(nx, ny), bcs=[(0, 0), (0, 0)], n_iters=2, nt=2,
+                psi=psi.astype(np.float64), gc=[g.astype(np.float64) for g in gc])
+
+
+def hint_scale_case():
+    f = np.float32
+    psi = np.array([[0, 0, 8.16638e+07, 8.16339e+07],
+                    [0, 0, 7.8215e+07, 8.15976e+07],
+                    [0, 0, 8.16724e+07, 8.16339e+07],
+                    [0, 0, 8.16638e+07, 8.16339e+07]], dtype=f)
+    nan = np.nan
+    gx = np.array([[nan, nan, nan, nan],
+                   [0.000572957855183631, 0.000286478927591815, -0.000286478927591815, -0.000572957855183631],
+                   [-0.00114591582678258, -0.000572957913391292, 0.000572957855183631, 0.00114591582678258],
+                   [0.00057295779697597, 0.000286478898487985, -0.000286478869384154, -0.00057295779697597]],
+                  dtype=f)
+    gy = np.array([[nan, nan, nan],
+                   [0, 0, 0],
+                   [0.000859436870086938, 0.00171887374017387, 0.000859436928294599],
+                   [-0.000859436753671616, -0.00171887350734323, -0.000859436870086938],
+                   [-1.70754757555791e-10, -3.41509515111582e-10, -1.70754757555791e-10]], dtype=f)
+    return dict(grid=(4, 4), bcs=[(0, 0), (1, 1)], n_iters=2, nt=1,
+                psi=psi.astype(np.float64), gc=[gx.astype(np.float64), gy.astype(np.float64)])

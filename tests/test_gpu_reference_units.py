@@ -1,0 +1,65 @@
+"""The reference's own FCT unit tests on the GPU paths (inputs in
+tests/reference_units.py): cone_bugs (tests/unit/cone_bugs/test_issue.cpp,
+the minimum stays exactly 1 after 2 steps) and the bounds of hint_scale
+(tests/unit/hint_scale/hint_scale.cpp, hint_scale = 24: max <= the initial
+max, min >= 0), in FP32 with the reference's float semantics and with the
+default guards, and in FP64 against the oracle."""
+import numpy as np
+import pytest
+
+import libmpdataxx_amd as M
+import oracle as O
+from reference_units import F32_EPS, cone_bugs_case, hint_scale_case
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [(np.float32, M.GUARD_NATIVE), (np.float32, M.GUARD_F64), (np.float64, M.GUARD_F64)]
+PATHS = [M.PATH_FUSED, M.PATH_PERPASS]
+
+
+def run_gpu(case, opts, dtype, guard, path, hint=0):
+    s = M.Solver(case["grid"], opts=opts, n_iters=case["n_iters"], bcs=case["bcs"], dtype=dtype,
+                 guard=guard, path=path, hint_scale=[hint], max_abs_div_eps=F32_EPS)
+    s.set_advectee(case["psi"])
+    for d in range(len(case["grid"])):
+        s.set_advector(case["gc"][d], d)
+    s.advance(case["nt"])
+    out = s.advectee().astype(np.float64)
+    s.close()
+    return out
+
+
+def run_oracle(case, opts, hint=0):
+    o = O.Oracle(case["grid"], opts=opts, n_iters=case["n_iters"], bcs=case["bcs"],
+                 hint_scale=[hint], max_abs_div_eps=F32_EPS)
+    o.set_advectee(case["psi"])
+    for d in range(len(case["grid"])):
+        o.set_advector(case["gc"][d], d)
+    o.advance(case["nt"])
+    return o.advectee()
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("dtype,guard", VARIANTS)
+def test_cone_bugs_minimum_stays_one(dtype, guard, path):
+    c = cone_bugs_case()
+    print(f"cone_bugs dtype={np.dtype(dtype).name} guard={guard} path={path}", flush=True)
+    out = run_gpu(c, M.FCT, dtype, guard, path)
+    assert out.min() == 1.0
+    if dtype == np.float64:
+        want = run_oracle(c, M.FCT)
+        assert np.abs(out - want).max() <= 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("dtype,guard", VARIANTS)
+def test_hint_scale_keeps_bounds(dtype, guard, path):
+    c = hint_scale_case()
+    print(f"hint_scale dtype={np.dtype(dtype).name} guard={guard} path={path}", flush=True)
+    out = run_gpu(c, M.FCT | M.IGA, dtype, guard, path, hint=24)
+    assert np.isfinite(out).all()
+    assert out.max() <= c["psi"].max()
+    assert out.min() >= 0.0
+    if dtype == np.float64:
+        want = run_oracle(c, M.FCT | M.IGA, hint=24)
+        assert np.abs(out - want).max() <= 1e-12 * np.abs(want).max()

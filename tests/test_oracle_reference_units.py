@@ -1,0 +1,38 @@
+"""The oracle on the known answers of the reference's own FCT unit tests
+(tests/unit/cone_bugs/test_issue.cpp, tests/unit/hint_scale/hint_scale.cpp;
+inputs in tests/reference_units.py)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from reference_units import F32_EPS, cone_bugs_case, hint_scale_case
+
+
+def run(case, opts, hint=0):
+    o = O.Oracle(case["grid"], opts=opts, n_iters=case["n_iters"], bcs=case["bcs"],
+                 hint_scale=[hint], max_abs_div_eps=F32_EPS)
+    o.set_advectee(case["psi"])
+    for d in range(len(case["grid"])):
+        o.set_advector(case["gc"][d], d)
+    o.advance(case["nt"])
+    return o.advectee()
+
+
+def test_cone_bugs_minimum_stays_one():
+    # test_issue.cpp:100-101: slv.min() - 1 == 0 after 2 steps
+    c = cone_bugs_case()
+    assert c["psi"].min() == 1.0 and c["psi"].max() > 4.0
+    out = run(c, O.FCT)
+    assert out.min() == 1.0
+    # control: the same cone without the limiter dips below the background
+    assert run(c, 0).min() < 1.0
+
+
+@pytest.mark.parametrize("hint", [0, 24])
+def test_hint_scale_keeps_bounds(hint):
+    # hint_scale.cpp:56-59: max <= initial max and min >= 0 after one step
+    c = hint_scale_case()
+    out = run(c, O.FCT | O.IGA, hint)
+    assert np.isfinite(out).all()
+    assert out.max() <= c["psi"].max()
+    assert out.min() >= 0.0
