@@ -12,8 +12,9 @@ mistake (dropped term, wrong sign, wrong index or iteration chaining) fails.
   1984): V_k = (|U_k| - U_k^2) (psi_R - psi_L) / (psi_R + psi_L) with
   U_k = V_{k-1} -- pins the GC_unco / GC_corr chaining for n_iters = 2..4.
 * FCT creates no new extrema; standard MPDATA keeps positive fields positive.
-* basic MPDATA is second-order accurate (error ratio ~4 on grid doubling,
-  as in the reference's sandbox/convergence_2d_3d test).
+* basic MPDATA is second-order accurate (error ratio ~4 on grid doubling);
+  the reference's own convergence_2d case gives order 2 for basic and
+  iga|fct (tests/sandbox/convergence_2d_3d/convergence_2d.cpp).
 """
 import numpy as np
 import pytest
@@ -161,6 +162,21 @@ def test_basic_mpdata_second_order():
     assert 1.7 < order < 2.5, order
     u1, u2 = _smooth_error(33, 1), _smooth_error(65, 1)
     assert np.log2(u1 / u2) < 1.3       # upwind is first order
+
+
+def _convergence_2d_error(np_, opts):
+    from reference_units import convergence_2d_case, l2_error
+    c = convergence_2d_case(np_)
+    out = run(c["grid"], c["bcs"], opts, c["n_iters"], c["psi"], c["gc"], c["nt"])
+    return l2_error(out, c)
+
+
+@pytest.mark.parametrize("opts", [0, O.IGA | O.FCT])
+def test_reference_convergence_2d_orders(opts):
+    # convergence_2d.cpp:68-92: round(log2(err(33) / err(65))) == 2 for
+    # basic and for iga|fct, both with n_iters = 2
+    e1, e2 = _convergence_2d_error(33, opts), _convergence_2d_error(65, opts)
+    assert round(np.log2(e1 / e2)) == 2, np.log2(e1 / e2)
 
 
 def _deform_oracle(grid, bcs, opts, iters, psi, T, courant, dt=1.0):
