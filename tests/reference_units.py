@@ -13,10 +13,12 @@ advector(1) over x = -1 .. n_x-1 and faces 1/2 .. n_y-3/2).
   values, then halo NaNs); the values below are the interior part of each
   row.  Its other half (values below zero without scaling) needs the
   program's fesetround(FE_DOWNWARD) float arithmetic and is not reproduced.
-* convergence_2d (tests/sandbox/convergence_2d_3d/convergence_2d.cpp:18-92):
-  psi = 2 + sin(2 pi i dx) sin(2 pi j dy) on an np x np cyclic grid, advector
-  (1, 2) dt / dx with dt = 0.2 dx, one period; round(log2(L2 error at 33 /
-  L2 error at 65)) is 2 for basic and for iga|fct with n_iters = 2.
+* convergence_2d / _3d (tests/sandbox/convergence_2d_3d/convergence_2d.cpp:
+  18-92, convergence_3d.cpp:19-97): psi = 2 + product of sin(2 pi i dx) over
+  the axes on an np^d cyclic grid, advector (1, 2(, 3)) dt / dx with
+  dt = 0.2 dx (2D) or 0.1 dx (3D), one period; round(log2(L2 error at the
+  coarse / fine np)) is 2 for basic and for iga|fct with n_iters = 2 (2D:
+  33 / 65, 3D: 17 / 33).
 """
 import numpy as np
 
@@ -59,15 +61,24 @@ def hint_scale_case():
                 psi=psi.astype(np.float64), gc=[gx.astype(np.float64), gy.astype(np.float64)])
 
 
-def convergence_2d_case(np_):
+def convergence_case(np_, nd=2):
     time, dx = 1.0, 1.0 / (np_ - 1)
-    dy, dt = dx, 0.2 * dx
+    dy = dz = dx
+    dt = (0.2 if nd == 2 else 0.1) * dx
     nt = int(time / dt)
-    i = np.arange(np_)[:, None]
-    j = np.arange(np_)[None, :]
-    psi = 2 + np.sin(2 * np.pi * i * dx) * np.sin(2 * np.pi * j * dy)
-    gc = [np.full((np_, np_), 1.0 * dt / dx), np.full((np_ + 1, np_ - 1), 2.0 * dt / dy)]
-    return dict(grid=(np_, np_), bcs=[(0, 0), (0, 0)], n_iters=2, nt=nt, psi=psi, gc=gc, time=time)
+    ax = np.arange(np_)
+    if nd == 2:
+        psi = 2 + np.sin(2 * np.pi * ax[:, None] * dx) * np.sin(2 * np.pi * ax[None, :] * dy)
+        gc = [np.full((np_, np_), 1.0 * dt / dx), np.full((np_ + 1, np_ - 1), 2.0 * dt / dy)]
+    else:
+        psi = 2 + (np.sin(2 * np.pi * ax[:, None, None] * dx) * np.sin(2 * np.pi * ax[None, :, None] * dy)
+                   * np.sin(2 * np.pi * ax[None, None, :] * dz))
+        gc = [np.full((np_, np_, np_), 1.0 * dt / dx), np.full((np_ + 1, np_ - 1, np_), 2.0 * dt / dy),
+              np.full((np_ + 1, np_, np_ - 1), 3.0 * dt / dy)]
+    return dict(grid=(np_,) * nd, bcs=[(0, 0)] * nd, n_iters=2, nt=nt, psi=psi, gc=gc, time=time)
+
+
+CONVERGENCE_SIZES = {2: (33, 65), 3: (17, 33)}
 
 
 def l2_error(out, case):

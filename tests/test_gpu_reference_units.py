@@ -4,13 +4,14 @@ the minimum stays exactly 1 after 2 steps) and the bounds of hint_scale
 (tests/unit/hint_scale/hint_scale.cpp, hint_scale = 24: max <= the initial
 max, min >= 0), in FP32 with the reference's float semantics and with the
 default guards, and in FP64 against the oracle; and the convergence orders
-of tests/sandbox/convergence_2d_3d/convergence_2d.cpp."""
+of tests/sandbox/convergence_2d_3d (2D and 3D)."""
 import numpy as np
 import pytest
 
 import libmpdataxx_amd as M
 import oracle as O
-from reference_units import F32_EPS, cone_bugs_case, convergence_2d_case, hint_scale_case, l2_error
+from reference_units import (CONVERGENCE_SIZES, F32_EPS, cone_bugs_case, convergence_case,
+                             hint_scale_case, l2_error)
 
 pytestmark = pytest.mark.gpu
 
@@ -69,14 +70,15 @@ def test_hint_scale_keeps_bounds(dtype, guard, path):
 @pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("opts", [0, M.IGA | M.FCT])
-def test_convergence_2d_orders(opts, dtype, path):
-    print(f"convergence_2d opts={opts} dtype={np.dtype(dtype).name} path={path}", flush=True)
+@pytest.mark.parametrize("nd", [2, 3])
+def test_convergence_orders(nd, opts, dtype, path):
+    print(f"convergence nd={nd} opts={opts} dtype={np.dtype(dtype).name} path={path}", flush=True)
     errs = []
-    for np_ in (33, 65):
-        c = convergence_2d_case(np_)
+    for np_ in CONVERGENCE_SIZES[nd]:
+        c = convergence_case(np_, nd)
         s = M.Solver(c["grid"], opts=opts, n_iters=c["n_iters"], bcs=c["bcs"], dtype=dtype, path=path)
         s.set_advectee(c["psi"])
-        for d in range(2):
+        for d in range(nd):
             s.set_advector(c["gc"][d], d)
         s.advance(c["nt"])
         errs.append(l2_error(s.advectee().astype(np.float64), c))

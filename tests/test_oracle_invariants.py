@@ -13,8 +13,8 @@ mistake (dropped term, wrong sign, wrong index or iteration chaining) fails.
   U_k = V_{k-1} -- pins the GC_unco / GC_corr chaining for n_iters = 2..4.
 * FCT creates no new extrema; standard MPDATA keeps positive fields positive.
 * basic MPDATA is second-order accurate (error ratio ~4 on grid doubling);
-  the reference's own convergence_2d case gives order 2 for basic and
-  iga|fct (tests/sandbox/convergence_2d_3d/convergence_2d.cpp).
+  the reference's own convergence_2d / _3d cases give order 2 for basic and
+  iga|fct (tests/sandbox/convergence_2d_3d).
 """
 import numpy as np
 import pytest
@@ -164,18 +164,22 @@ def test_basic_mpdata_second_order():
     assert np.log2(u1 / u2) < 1.3       # upwind is first order
 
 
-def _convergence_2d_error(np_, opts):
-    from reference_units import convergence_2d_case, l2_error
-    c = convergence_2d_case(np_)
+def _convergence_error(np_, nd, opts):
+    from reference_units import convergence_case, l2_error
+    c = convergence_case(np_, nd)
     out = run(c["grid"], c["bcs"], opts, c["n_iters"], c["psi"], c["gc"], c["nt"])
     return l2_error(out, c)
 
 
+@pytest.mark.parametrize("nd", [2, 3])
 @pytest.mark.parametrize("opts", [0, O.IGA | O.FCT])
-def test_reference_convergence_2d_orders(opts):
-    # convergence_2d.cpp:68-92: round(log2(err(33) / err(65))) == 2 for
-    # basic and for iga|fct, both with n_iters = 2
-    e1, e2 = _convergence_2d_error(33, opts), _convergence_2d_error(65, opts)
+def test_reference_convergence_orders(opts, nd):
+    # convergence_2d.cpp:68-92, convergence_3d.cpp:72-97: the order
+    # round(log2(err(coarse) / err(fine))) is 2 for basic and for iga|fct,
+    # both with n_iters = 2
+    from reference_units import CONVERGENCE_SIZES
+    n1, n2 = CONVERGENCE_SIZES[nd]
+    e1, e2 = _convergence_error(n1, nd, opts), _convergence_error(n2, nd, opts)
     assert round(np.log2(e1 / e2)) == 2, np.log2(e1 / e2)
 
 
