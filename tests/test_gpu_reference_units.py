@@ -4,7 +4,8 @@ the minimum stays exactly 1 after 2 steps) and the bounds of hint_scale
 (tests/unit/hint_scale/hint_scale.cpp, hint_scale = 24: max <= the initial
 max, min >= 0), in FP32 with the reference's float semantics and with the
 default guards, and in FP64 against the oracle; and the convergence orders
-of tests/sandbox/convergence_2d_3d (2D and 3D)."""
+of tests/sandbox/convergence_2d_3d (2D and 3D); and the order independence
+of the compensated mass sum (tests/unit/kahan_sum)."""
 import numpy as np
 import pytest
 
@@ -84,3 +85,24 @@ def test_convergence_orders(nd, opts, dtype, path):
         errs.append(l2_error(s.advectee().astype(np.float64), c))
         s.close()
     assert round(np.log2(errs[0] / errs[1])) == 2, errs
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_kahan_mass_sum_order_independent(dtype):
+    # tests/unit/kahan_sum/test_kahan_sum.cpp: one 1e8 and many 1e-8; the
+    # compensated sums of the field and of its reverse agree within 1.4e-8
+    # (an uncompensated sum loses the small terms in one order only)
+    print(f"kahan mass sum dtype={np.dtype(dtype).name}", flush=True)
+    grid = (10, 6, 5)
+    a = np.full(grid, 1e-8, dtype=dtype)
+    a[0] = 1e8
+    sums = []
+    for field in (a, a[::-1].copy()):
+        s = M.Solver(grid, opts=M.FCT, n_iters=2, dtype=dtype)
+        s.set_advectee(field.astype(np.float64))
+        sums.append(s.reduce(M.RED_SUM))
+        s.close()
+    import math
+    exact = math.fsum(a.astype(np.float64).ravel())
+    assert abs(sums[0] - sums[1]) <= 1.4e-8
+    assert abs(sums[0] - exact) <= np.spacing(exact)
