@@ -4,8 +4,9 @@ the minimum stays exactly 1 after 2 steps) and the bounds of hint_scale
 (tests/unit/hint_scale/hint_scale.cpp, hint_scale = 24: max <= the initial
 max, min >= 0), in FP32 with the reference's float semantics and with the
 default guards, and in FP64 against the oracle; and the convergence orders
-of tests/sandbox/convergence_2d_3d (2D and 3D); and the order independence
-of the compensated mass sum (tests/unit/kahan_sum)."""
+of tests/sandbox/convergence_2d_3d (2D and 3D); the order independence of
+the compensated mass sum (tests/unit/kahan_sum); and the orders and the
+variable-time-step gain of tests/unit/var_dt."""
 import numpy as np
 import pytest
 
@@ -106,3 +107,54 @@ def test_kahan_mass_sum_order_independent(dtype):
     exact = math.fsum(a.astype(np.float64).ravel())
     assert abs(sums[0] - sums[1]) <= 1.4e-8
     assert abs(sums[0] - exact) <= np.spacing(exact)
+
+
+def _var_dt_error(np_, cfl, var_dt):
+    """tests/unit/var_dt/var_dt_test.cpp:18-68 in double, the 1D grid as
+    np x 3 cyclic with no flow along y: psi = 2 + sin(i dx), dx = 2 pi /
+    (np - 1), calc_gc sets GC_x = dt / dx u_0 cos(pi t_half / tau) (tau = 256,
+    u_0 = 1) each step with the current dt; time = tau sqrt(2) / 2; relative
+    L2 error against 2 + sin(i dx - sin(pi u_0 t / tau) tau / pi)."""
+    import ctypes
+    pi, tau, u0 = np.pi, 256.0, 1.0
+    time = tau * np.sqrt(2.0) / 2
+    dx = 2.0 * pi / (np_ - 1)
+    dt = cfl * dx
+    grid = (np_, 3)
+    s = M.Solver(grid, opts=0, n_iters=2, bcs=[(M.CYCLIC, M.CYCLIC)] * 2, dtype=np.float64,
+                 dt=dt, var_dt=var_dt, max_courant=cfl)
+    x = np.arange(np_) * dx
+    s.set_advectee(np.repeat((2.0 + np.sin(x))[:, None], 3, axis=1))
+    s.set_advector(np.zeros(s.advector_shape(0)), 0)
+    s.set_advector(np.zeros(s.advector_shape(1)), 1)
+    cur = ctypes.c_double()
+    L = M.lib()
+    shape0 = s.advector_shape(0)
+
+    def calc_gc(t_half):
+        L.mpdx_dt(s._h, ctypes.byref(cur))
+        s.set_advector(np.full(shape0, cur.value / dx * u0 * np.cos(pi * t_half / tau)), 0)
+        return True
+
+    s.set_calc_gc(calc_gc)
+    if var_dt:
+        s.advance_time(time)
+    else:
+        s.advance(int(time / dt))
+    t = s.time
+    got = s.advectee()[:, 1]
+    s.close()
+    true = 2 + np.sin(x - np.sin(pi * u0 * t / tau) * tau / pi)
+    return np.sqrt(np.sum((true - got) ** 2) / np.sum(true ** 2))
+
+
+@pytest.mark.parametrize("cfl", [0.1, 0.5, 0.9])
+def test_var_dt_orders_and_gain(cfl):
+    # var_dt_test.cpp:70-95: order 2 with a constant and with a variable
+    # time step, and the variable one more accurate on the fine grid
+    print(f"var_dt cfl={cfl}", flush=True)
+    e = {(n, v): _var_dt_error(n, cfl, v) for n in (33, 65) for v in (False, True)}
+    ord_cdt = round(np.log2(e[33, False] / e[65, False]))
+    ord_vdt = round(np.log2(e[33, True] / e[65, True]))
+    assert ord_cdt == 2 and ord_vdt == 2, e
+    assert e[65, True] < e[65, False], e
