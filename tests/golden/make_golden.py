@@ -72,6 +72,28 @@ def h5_fixtures(rs):
         f.write("\n".join(sums) + "\n")
 
 
+def basic_example_records():
+    """0_basic_example/refdata/log-orig.gz is the gnuplot script of the
+    example's output: six records (t/dt = 0, 20, ..., 100) of the 101 cell
+    values, each printed twice (histeps), blocks ended by 'e'.  Kept: one
+    line per record with the 101 values as printed."""
+    out = os.path.join(HERE, "basic_example")
+    os.makedirs(out, exist_ok=True)
+    text = gzip.open(os.path.join(REF, "0_basic_example", "refdata", "log-orig.gz"), "rt").read()
+    body = text.split("\n")
+    start = next(k for k, l in enumerate(body) if l.startswith("splot")) + 1
+    records, cur = [], []
+    for l in body[start:]:
+        if l == "e":
+            records.append(cur[::2])
+            cur = []
+        elif l.strip():
+            cur.append(l.strip())
+    with open(os.path.join(out, "records.txt"), "w") as f:
+        for r in records:
+            f.write(" ".join(r) + "\n")
+
+
 def main():
     sphere = os.path.join(HERE, "revolving_sphere_3d")
     cone = os.path.join(HERE, "rotating_cone_2d")
@@ -97,8 +119,10 @@ def main():
             open(os.path.join(osc, "stats.txt"), "wb") as g:
         shutil.copyfileobj(f, g)
     h5_fixtures(rs)
+    basic_example_records()
     lines = []
-    for sub in ("revolving_sphere_3d", "revolving_sphere_3d_h5", "rotating_cone_2d", "coupled_harmosc"):
+    for sub in ("revolving_sphere_3d", "revolving_sphere_3d_h5", "rotating_cone_2d", "coupled_harmosc",
+                "basic_example"):
         for name in sorted(os.listdir(os.path.join(HERE, sub))):
             path = os.path.join(HERE, sub, name)
             digest = hashlib.sha256(open(path, "rb").read()).hexdigest()

@@ -84,3 +84,21 @@ CONVERGENCE_SIZES = {2: (33, 65), 3: (17, 33)}
 def l2_error(out, case):
     n = case["psi"].size
     return np.sqrt(np.sum((out - case["psi"]) ** 2) / n) / case["time"]
+
+
+def basic_example_case():
+    """tests/paper_2015_GMD/0_basic_example/basic_example.cpp: 1D, 101 open
+    cells, dx = 0.1, the default options (iga | fct, n_iters = 2), a Witch
+    of Agnesi -0.5 + 1 / ((dx (i - 50))^2 + 1) under Courant 0.5, output
+    every 20 steps up to 100; here as 101 x 4 with no flow along y."""
+    nx, ny, dx = 101, 4, 0.1
+    i = np.arange(nx)
+    psi = np.repeat((-.5 + 1 / ((dx * (i - (nx - 1) / 2.)) ** 2 + 1))[:, None], ny, axis=1)
+    gc = [np.full((nx, ny), .5), np.zeros((nx + 1, ny - 1))]
+    return dict(grid=(nx, ny), bcs=[(1, 1), (1, 1)], n_iters=2, psi=psi, gc=gc, outfreq=20, nt=100)
+
+
+def basic_example_records(golden_dir):
+    import os
+    path = os.path.join(golden_dir, "basic_example", "records.txt")
+    return [np.array([float(v) for v in line.split()]) for line in open(path)]

@@ -158,3 +158,23 @@ def test_var_dt_orders_and_gain(cfl):
     ord_vdt = round(np.log2(e[33, True] / e[65, True]))
     assert ord_cdt == 2 and ord_vdt == 2, e
     assert e[65, True] < e[65, False], e
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_basic_example_log(dtype, path, golden_dir):
+    # paper_2015_GMD/0_basic_example: the six printed records (5 decimals)
+    from reference_units import basic_example_case, basic_example_records
+    print(f"basic_example dtype={np.dtype(dtype).name} path={path}", flush=True)
+    c = basic_example_case()
+    want = basic_example_records(golden_dir)
+    s = M.Solver(c["grid"], opts=M.IGA | M.FCT, n_iters=c["n_iters"], bcs=c["bcs"], dtype=dtype, path=path)
+    s.set_advectee(c["psi"])
+    for d in range(2):
+        s.set_advector(c["gc"][d], d)
+    for k, rec in enumerate(want):
+        if k:
+            s.advance(c["outfreq"])
+        got = s.advectee()[:, 1].astype(np.float64)
+        assert np.abs(got - rec).max() <= 5e-6 + (1e-6 if dtype == np.float32 else 1e-12), k
+    s.close()

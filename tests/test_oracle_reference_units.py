@@ -36,3 +36,26 @@ def test_hint_scale_keeps_bounds(hint):
     assert np.isfinite(out).all()
     assert out.max() <= c["psi"].max()
     assert out.min() >= 0.0
+
+
+def test_basic_example_log(golden_dir):
+    # paper_2015_GMD/0_basic_example: the six printed records (5 decimals)
+    from reference_units import basic_example_case, basic_example_records
+    c = basic_example_case()
+    want = basic_example_records(golden_dir)
+    o = O.Oracle(c["grid"], opts=O.IGA | O.FCT, n_iters=c["n_iters"], bcs=c["bcs"])
+    o.set_advectee(c["psi"])
+    for d in range(2):
+        o.set_advector(c["gc"][d], d)
+    for k, rec in enumerate(want):
+        if k:
+            o.advance(c["outfreq"])
+        got = o.advectee()[:, 1]
+        assert np.abs(got - rec).max() <= 5e-6 + 1e-12, k
+    # control: the basic scheme (no iga / fct) prints other numbers
+    b = O.Oracle(c["grid"], opts=0, n_iters=2, bcs=c["bcs"])
+    b.set_advectee(c["psi"])
+    for d in range(2):
+        b.set_advector(c["gc"][d], d)
+    b.advance(c["nt"])
+    assert np.abs(b.advectee()[:, 1] - want[-1]).max() > 1e-4
