@@ -72,6 +72,38 @@ def h5_fixtures(rs):
         f.write("\n".join(sums) + "\n")
 
 
+def advscheme_opts_records():
+    """1_advscheme_opts/refdata/log-orig.gz: the same gnuplot layout, 20
+    blocks (options abs, iga, iga|tot, iga|fct, iga|tot|fct; per option the
+    two equations at t/dt = 0 and 1200) of the 601 cell values.  Kept: one
+    line per block, in the log's order."""
+    out = os.path.join(HERE, "advscheme_opts")
+    os.makedirs(out, exist_ok=True)
+    text = gzip.open(os.path.join(REF, "1_advscheme_opts", "refdata", "log-orig.gz"), "rt").read()
+    blocks, cur, data = [], [], False
+    for l in text.split("\n"):
+        if l.startswith("splot") or l.startswith("plot"):
+            data = True
+            continue
+        if not data:
+            continue
+        if l == "e":
+            # histeps plots print every value twice
+            doubled = len(cur) % 2 == 0 and cur[0::2] == cur[1::2]
+            blocks.append(cur[::2] if doubled else cur)
+            cur = []
+        elif l.strip() and not l.startswith("set") and not l.startswith("reset"):
+            try:
+                float(l)
+            except ValueError:
+                data = False
+                continue
+            cur.append(l.strip())
+    with open(os.path.join(out, "records.txt"), "w") as f:
+        for r in blocks:
+            f.write(" ".join(r) + "\n")
+
+
 def basic_example_records():
     """0_basic_example/refdata/log-orig.gz is the gnuplot script of the
     example's output: six records (t/dt = 0, 20, ..., 100) of the 101 cell
@@ -120,9 +152,10 @@ def main():
         shutil.copyfileobj(f, g)
     h5_fixtures(rs)
     basic_example_records()
+    advscheme_opts_records()
     lines = []
     for sub in ("revolving_sphere_3d", "revolving_sphere_3d_h5", "rotating_cone_2d", "coupled_harmosc",
-                "basic_example"):
+                "basic_example", "advscheme_opts"):
         for name in sorted(os.listdir(os.path.join(HERE, sub))):
             path = os.path.join(HERE, sub, name)
             digest = hashlib.sha256(open(path, "rb").read()).hexdigest()

@@ -102,3 +102,37 @@ def basic_example_records(golden_dir):
     import os
     path = os.path.join(golden_dir, "basic_example", "records.txt")
     return [np.array([float(v) for v in line.split()]) for line in open(path)]
+
+
+# option groups of 1_advscheme_opts' log, in order; the tot ones are not
+# built by this backend
+ADVSCHEME_GROUPS = ["abs", "iga", "iga|tot", "iga|fct", "iga|tot|fct"]
+
+
+def advscheme_opts_case():
+    """tests/paper_2015_GMD/1_advscheme_opts/advscheme_opts.cpp: 1D, 601
+    cyclic cells, two equations (2 / 4 and -1 / +1 steps over cells
+    76..124), Courant -0.75, n_iters = 2, 1200 steps; here as 601 x 4 with
+    no flow along y."""
+    nx, ny = 601, 4
+    i = np.arange(nx)
+    out = (i <= 75) | (i >= 125)
+    psi = [np.repeat(np.where(out, 2.0, 4.0)[:, None], ny, axis=1),
+           np.repeat(np.where(out, -1.0, 1.0)[:, None], ny, axis=1)]
+    gc = [np.full((nx, ny), -.75), np.zeros((nx + 1, ny - 1))]
+    return dict(grid=(nx, ny), bcs=[(0, 0), (1, 1)], n_iters=2, psi=psi, gc=gc, nt=1200)
+
+
+def advscheme_opts_finals(golden_dir, group):
+    """final fields (equation 0, equation 1) of option group `group` from the
+    log: of its four blocks, the two that are not the initial fields"""
+    import os
+    path = os.path.join(golden_dir, "advscheme_opts", "records.txt")
+    blocks = [np.array([float(v) for v in line.split()]) for line in open(path)]
+    g = blocks[4 * group:4 * group + 4]
+    c = advscheme_opts_case()
+    init = [p[:, 0] for p in c["psi"]]
+    finals = [b for b in g if not any(np.array_equal(b, x) for x in init)]
+    assert len(finals) == 2
+    finals.sort(key=lambda b: -b.mean())        # equation 0 (2 / 4) first
+    return finals

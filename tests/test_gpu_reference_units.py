@@ -178,3 +178,26 @@ def test_basic_example_log(dtype, path, golden_dir):
         got = s.advectee()[:, 1].astype(np.float64)
         assert np.abs(got - rec).max() <= 5e-6 + (1e-6 if dtype == np.float32 else 1e-12), k
     s.close()
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("group,opts", [(0, M.ABS), (1, M.IGA), (3, M.IGA | M.FCT)])
+def test_advscheme_opts_log(group, opts, dtype, path, golden_dir):
+    # paper_2015_GMD/1_advscheme_opts: both equations after 1200 steps
+    from reference_units import advscheme_opts_case, advscheme_opts_finals
+    print(f"advscheme_opts group={group} dtype={np.dtype(dtype).name} path={path}", flush=True)
+    c = advscheme_opts_case()
+    want = advscheme_opts_finals(golden_dir, group)
+    s = M.Solver(c["grid"], opts=opts, n_iters=c["n_iters"], n_eqns=2, bcs=c["bcs"], dtype=dtype, path=path)
+    for e in range(2):
+        s.set_advectee(c["psi"][e], e)
+    for d in range(2):
+        s.set_advector(c["gc"][d], d)
+    s.advance(c["nt"])
+    # FP64: the printed 5 decimals; FP32: its rounding accumulated over 1200
+    # steps of the non-monotone iga scheme at a step (~2e-5 relative measured)
+    tol = 5e-6 + (2e-4 if dtype == np.float32 else 1e-12)
+    for e in range(2):
+        assert np.abs(s.advectee(e)[:, 1].astype(np.float64) - want[e]).max() <= tol, e
+    s.close()

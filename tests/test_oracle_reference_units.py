@@ -59,3 +59,20 @@ def test_basic_example_log(golden_dir):
         b.set_advector(c["gc"][d], d)
     b.advance(c["nt"])
     assert np.abs(b.advectee()[:, 1] - want[-1]).max() > 1e-4
+
+
+@pytest.mark.parametrize("group,opts", [(0, O.ABS), (1, O.IGA), (3, O.IGA | O.FCT)])
+def test_advscheme_opts_log(golden_dir, group, opts):
+    # paper_2015_GMD/1_advscheme_opts: both equations after 1200 steps, as
+    # printed (5 decimals), for abs, iga and iga|fct
+    from reference_units import advscheme_opts_case, advscheme_opts_finals
+    c = advscheme_opts_case()
+    want = advscheme_opts_finals(golden_dir, group)
+    o = O.Oracle(c["grid"], opts=opts, n_iters=c["n_iters"], n_eqns=2, bcs=c["bcs"])
+    for e in range(2):
+        o.set_advectee(c["psi"][e], e)
+    for d in range(2):
+        o.set_advector(c["gc"][d], d)
+    o.advance(c["nt"])
+    for e in range(2):
+        assert np.abs(o.advectee(e)[:, 1] - want[e]).max() <= 5e-6 + 1e-12, e
