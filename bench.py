@@ -6,10 +6,12 @@ grid, FP32 state/advector/fluxes with FP64 guards and FP64 Kahan mass sums;
 solid-body rotation about z (max Courant 0.5) moving a discontinuous sphere.
 A "step" is one time step of the whole grid.  The same problem is also run
 in FP64 for the FP32/FP64 ratio and as the accuracy reference (the FP64 GPU
-path agrees with the FP64 CPU oracle to <= 1e-12).  A second leg runs the
-multi-tracer workload (512x512x128 cyclic, 8 tracers, time-varying
-deformational advector generated on the device each step).  Inputs are
-resident in HBM before the timed regions.
+path agrees with the FP64 CPU oracle to <= 1e-12), and in FP32 with the
+guarded ratios evaluated in FP64.  Further legs: C2 (256^3 FP64), C1 (2D
+256^2 FP64 cyclic), the multi-tracer workload (512x512x128 cyclic, 8
+tracers, time-varying deformational advector generated on the device each
+step) and the CPU baseline.  Inputs are resident in HBM before the timed
+regions.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -19,7 +21,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -27,28 +28,22 @@ import time
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 METRIC = "Gcell-updates/s, 3D MPDATA-FCT iters=3, FP32 vs FP64, % of HBM peak"
 
-# Compulsory HBM words per cell for one launch of each kernel class (reads
-# of the arrays it consumes + writes of the arrays it produces; halo re-reads,
-# values kept on chip and reductions excluded), and which cells one launch
-# covers: "all" = n^3, "interior" = (n-4)^3 (cells >= 2 from every boundary).
-ALG_WORDS = {
-    "fused_interior_fct_v": (9, "interior"),  # psi^k, U x3, psi^n in; psi^k+1, V x3 out
-    "fused_interior_fct": (6, "interior"),    # psi^k, U x3, psi^n in; psi^k+1 out
-    "fused_interior_v": (8, "interior"),      # psi^k, U x3 in; psi^k+1, V x3 out
-    "fused_interior": (5, "interior"),        # psi^k, U x3 in; psi^k+1 out
-    "fused_first_fct_v": (8, "interior"),     # psi^n, GC x3 in; psi^2, V x3 out (psi^1 on chip)
-    "fused_first_fct": (5, "interior"),       # psi^n, GC x3 in; psi^2 out
-    "fused_first_v": (8, "interior"),
-    "fused_first": (5, "interior"),
-    "fused_donor": (5, "all"),                # psi^n, GC x3 in; psi^1 out
-    "fused_psi_halo": (0, "all"),             # halo copies only
-    "pp_antidiff": (5, "all"),                # psi, U_d, U_d+1, U_d+2 in; V_d out
-    "pp_flux": (3, "all"),
-    "pp_donorcell": (5, "all"),
-    "pp_betas": (10, "all"),
-    "pp_gc_mono": (5, "all"),
-    "pp_fct_init": (3, "all"),
-}
+# Algorithmic bytes per cell-update of one time step: every tracer read once
+# and written once, every advector component read once; halo re-reads,
+# intermediates (psi of the intermediate passes, antidiffusive velocities)
+# and reductions excluded: prod(grid) x (2 T + D) x sizeof(real).
+def alg_bytes_per_cell(tracers=1, dims=3, real_bytes=4):
+    return (2 * tracers + dims) * real_bytes
+
+
+def launch_cells(name, n):
+    """Cells whose new value one launch of a kernel class computes, for the
+    open-boundary n^3 workload: the interior kernels output the cells >= 2
+    from the x and y boundaries and every z cell (their z-boundary tiles
+    cover both open z boundaries); the other classes cover the whole grid."""
+    if name.startswith("fused_first") or name.startswith("fused_interior"):
+        return (n - 4) * (n - 4) * n
+    return n ** 3
 
 
 PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
@@ -68,9 +63,9 @@ def profile_figure(fname, key, kernel):
     return None
 
 
-def run_gpu(torch, M, n, dtype, steps, warmup, path):
+def run_gpu(torch, M, n, dtype, steps, warmup, path, guard=0, profile=True):
     dev = torch.device("cuda:0")
-    s = M.Solver((n, n, n), opts=M.FCT, n_iters=3, dtype=dtype, path=path,
+    s = M.Solver((n, n, n), opts=M.FCT, n_iters=3, dtype=dtype, path=path, guard=guard,
                  bcs=[(M.OPEN, M.OPEN)] * 3)
     from libmpdataxx_amd.workloads_torch import rotation_sphere_device
     psi, gc = rotation_sphere_device(n, torch.float32 if dtype == "float32" else torch.float64, dev)
@@ -87,6 +82,8 @@ def run_gpu(torch, M, n, dtype, steps, warmup, path):
     s.advance(steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    if not profile:
+        return s, (t1 - t0), {}, mass0, 0.0
     # the same number of steps again, launched eagerly with a HIP event pair
     # around every kernel on the solver's stream (per-kernel durations)
     s.profile(True)
@@ -135,33 +132,75 @@ def run_multi_tracer(torch, M, steps, warmup, shape=(512, 512, 128), n_tr=8):
     return res
 
 
-def cpu_baseline(seconds=10.0):
-    """FP64 oracle (reference release-style flags, OpenMP) on a 128^3 sample
-    of the same workload; FP64 like the reference's double-precision runs."""
-    threads = int(os.environ.get("MPDX_CPU_THREADS", min(16, os.cpu_count() or 1)))
-    os.environ["OMP_NUM_THREADS"] = str(threads)
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-    import oracle as O
-    from libmpdataxx_amd import workloads as W
-    n = 128
-    psi, gc = W.rotation_sphere_3d(n)
-    o = O.Oracle((n, n, n), opts=O.FCT, n_iters=3, fast=True)
-    o.set_advectee(psi)
-    for d in range(3):
-        o.set_advector(gc[d], d)
-    o.advance(1)
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
-        o.advance(1)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or steps >= 5000:
-            break
-    return {"value": n ** 3 * steps / el / 1e9, "unit": "Gcell-updates/s", "cores": threads,
+def _cpu_leg(threads, seconds, runs, grid):
+    """One child process timing the oracle's release-flag build with
+    `threads` OpenMP threads (oracle/timing.py); returns its JSON."""
+    import subprocess
+    root = os.path.dirname(os.path.abspath(__file__))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    r = subprocess.run([sys.executable, "-m", "oracle.timing", "--threads", str(threads), "--grid", str(grid),
+                        "--seconds", str(seconds), "--runs", str(runs)],
+                       cwd=root, env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline run failed: {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(seconds=3.0, runs=3, grid=128):
+    """The reference's CPU path as far as it can be run here: the FP64 oracle
+    (same unfused per-pass structure and OpenMP y-slab threading as
+    concurr::openmp, -Ofast, x86-64-v4 or -v3 to match the host) on a 128^3
+    sample of the headline workload, timed with this process's CPU share of
+    threads (OMP_NUM_THREADS, else every online CPU) and with one thread;
+    median of `runs` timings of about `seconds` each, after a warm-up step."""
+    nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS") or nproc)
+    full = _cpu_leg(threads, seconds, runs, grid)
+    one = _cpu_leg(1, seconds, runs, grid) if threads != 1 else full
+    return {"value": round(full["gcell_per_s"], 6), "unit": "Gcell-updates/s", "cores": threads,
             "kind": "port",
-            "sample": f"{n}^3 open, FCT n_iters=3, FP64, {steps} steps after 1 warm-up step, "
-                      f"{el:.1f} s, OpenMP {threads} threads, -Ofast"}
+            "sample": f"{grid}^3 open, FCT n_iters=3, FP64, median of {runs} runs of {full['steps_per_run']} steps "
+                      f"after 1 warm-up step, OpenMP {threads} threads, -Ofast -march={full['isa']}",
+            "one_thread": {"value": round(one["gcell_per_s"], 6), "steps_per_run": one["steps_per_run"],
+                           "run_seconds": one["run_seconds"]},
+            "run_seconds": full["run_seconds"], "nproc": nproc,
+            "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "cpu_model": _cpu_model(), "isa": full["isa"],
+            "note": "the reference's concurr::openmp itself needs Blitz++ and Boost, absent here; threads = the "
+                    "process's CPU share (OMP_NUM_THREADS) rather than every host core"}
+
+
+def run_2d_c1(torch, M, steps, warmup, n=256):
+    """C1: 2D MPDATA n_iters=2, FP64, 256^2 cyclic, uniform Courant
+    (0.25, 0.25), Gaussian cone on 0.5."""
+    import numpy as np
+    from libmpdataxx_amd import workloads as W
+    psi, gc = W.gaussian_2d_cyclic(n)
+    s = M.Solver((n, n), opts=0, n_iters=2, bcs=[(M.CYCLIC, M.CYCLIC)] * 2, dtype=np.float64)
+    s.set_advectee(psi)
+    for d in range(2):
+        s.set_advector(gc[d], d)
+    s.advance(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.advance(steps)
+    el = time.perf_counter() - t0
+    s.close()
+    return {"value": round(n * n * steps / el / 1e9, 4), "unit": "Gcell-updates/s",
+            "ms_per_step": round(el * 1e3 / steps, 4), "steps": steps, "dtype": "f64",
+            "workload": f"2D MPDATA n_iters=2, {n}^2 cyclic, uniform Courant (0.25, 0.25), Gaussian cone",
+            "step_frac_of_hbm": round(n * n * steps / el * alg_bytes_per_cell(1, 2, 8) / 1e9 / HBM_PEAK_GBS, 5)}
 
 
 def copy_bandwidth(torch, nbytes=1 << 30, reps=5):
@@ -193,34 +232,35 @@ def main():
     ap.add_argument("--path", type=int, default=0, help="0 auto, 1 per-pass, 2 fused")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=3.0)
     ap.add_argument("--no-multi", action="store_true", help="skip the multi-tracer leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the FP64-exact-guard, C1 (2D 256^2) and C2 (3D 256^3 FP64) legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the library steps one "
               "domain per GPU and has no multi-GPU halo exchange", file=sys.stderr)
         return 2
 
-    import numpy as np
     import torch
     import libmpdataxx_amd as M
 
     n, K, W = args.grid, args.steps, args.warmup
     cells = n ** 3
+    print(f"bench: {n}^3 FP32 FCT n_iters=3, {W} warm-up + {K} timed steps", file=sys.stderr, flush=True)
     s32, el32, prof32, m0_32, elp32 = run_gpu(torch, M, n, "float32", K, W, args.path)
     value = cells * K / el32 / 1e9
     path_name = {1: "per-pass", 2: "fused"}.get(s32.active_path, "?")
+    bpc = alg_bytes_per_cell(1, 3, 4)          # 20 B per FP32 cell-update
 
     # dominant kernel: largest summed device time in the profiled steps
     dom = max(prof32.items(), key=lambda kv: kv[1][0])
     dname, (dms, dlaunch) = dom
     avg_ms = dms / max(dlaunch, 1)
-    words, span = ALG_WORDS.get(dname, (0, "all"))
-    launch_cells = (n - 4) ** 3 if span == "interior" else cells
-    alg_bytes = words * 4 * launch_cells
+    alg_bytes = bpc * launch_cells(dname, n)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    # HBM bytes per launch (FETCH_SIZE / WRITE_SIZE, copy-calibrated) and VALU
-    # issue utilisation, recorded for the default 512^3 workload only
+    # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) and VALU
+    # issue utilisation, recorded by rocprofv3 for the default 512^3 workload
     default_wl = n == 512 and s32.active_path == 2
     traffic = profile_figure("pmc_traffic_fct3_512cube.md", "traffic_bytes_per_launch", dname) \
         if default_wl else None
@@ -232,45 +272,80 @@ def main():
         "steps": K, "warmup": W, "ms_per_step": round(el32 * 1e3 / K, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (solid-body rotation of a discontinuous sphere, generated on device)",
-        "config": {"workload": f"3D MPDATA-FCT n_iters=3, {n}^3, open bconds, FP32 state, FP64 guards",
+        "config": {"workload": f"3D MPDATA-FCT n_iters=3, {n}^3, open bconds, FP32 state and arithmetic, "
+                               "FP64 guard constants, FP64 Kahan mass sums",
                    "grid": [n, n, n], "n_iters": 3, "opts": "fct", "bcond": "open", "path": path_name},
         "roofline": {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": int(traffic) if traffic else None, "avg_launch_ms": round(avg_ms, 4),
-                     "launches": dlaunch, "alg_bytes_per_launch": alg_bytes,
+                     "traffic": int(traffic) if traffic else None,
+                     "traffic_over_alg": round(traffic / alg_bytes, 3) if traffic else None,
+                     "avg_launch_ms": round(avg_ms, 4), "launches": dlaunch,
+                     "alg_bytes_per_cell": bpc, "cells_per_launch": launch_cells(dname, n),
+                     "alg_bytes_per_launch": alg_bytes,
                      "valu_issue_util_pmc": valu, "copy_GBs": round(copy_gbs, 1),
                      "frac_of_copy": round(achieved / copy_gbs, 4)},
-        "step_roofline": {"alg_bytes_per_cell": 20, "achieved_GBs": round(value * 20, 2),
-                          "frac": round(value * 20 / HBM_PEAK_GBS, 4)},
+        "step_roofline": {"alg_bytes_per_cell": bpc, "achieved_GBs": round(value * bpc, 2),
+                          "frac": round(value * bpc / HBM_PEAK_GBS, 4)},
         "kernels_ms": {k: round(v[0], 3) for k, v in prof32.items()},
         "profiled_ms_per_step": round(elp32 * 1e3 / K, 4),
     }
 
     if not args.no_fp64:
+        print("bench: FP64 leg", file=sys.stderr, flush=True)
         s64, el64, prof64, m0_64, _ = run_gpu(torch, M, n, "float64", K, W, args.path)
         v64 = cells * K / el64 / 1e9
-        a32 = torch.empty((n, n, n), dtype=torch.float32, device="cuda:0")
         a64 = torch.empty((n, n, n), dtype=torch.float64, device="cuda:0")
-        s32.get_advectee_device(a32)
         s64.get_advectee_device(a64)
-        diff = a32.double() - a64
-        l2 = float(torch.sqrt((diff ** 2).sum() / (a64 ** 2).sum()))
-        linf = float(diff.abs().max())
-        m32, m64 = s32.mass(), s64.mass()
+
+        def accuracy(sx, m0):
+            a = torch.empty((n, n, n), dtype=torch.float32, device="cuda:0")
+            sx.get_advectee_device(a)
+            diff = a.double() - a64
+            res = {"l2_rel": float(torch.sqrt((diff ** 2).sum() / (a64 ** 2).sum())),
+                   "linf": float(diff.abs().max()),
+                   "mass_rel_err": abs(sx.mass() - m0) / abs(m0), "min": sx.min(), "max": sx.max()}
+            del a, diff
+            return res
         out["fp64"] = {"value": round(v64, 4), "ms_per_step": round(el64 * 1e3 / K, 4),
+                       "mass_rel_err": abs(s64.mass() - m0_64) / abs(m0_64),
                        "kernels_ms": {k: round(v[0], 3) for k, v in prof64.items()}}
         out["fp32_over_fp64"] = round(value / v64, 4)
-        out["accuracy_vs_fp64"] = {"l2_rel": l2, "linf": linf,
-                                   "mass_rel_err_fp32": abs(m32 - m0_32) / abs(m0_32),
-                                   "mass_rel_err_fp64": abs(m64 - m0_64) / abs(m0_64),
-                                   "min_fp32": s32.min(), "steps": 2 * K + W}
-        del a32, a64, diff
+        out["accuracy_vs_fp64"] = dict(accuracy(s32, m0_32), steps=2 * K + W)
+        s32.close()
+        if not args.no_extra:
+            # FP32 state with the guarded ratios evaluated in FP64 (IEEE FP64
+            # division, FP64 constants), same steps as the FP64 run
+            print("bench: FP32 FP64-exact-guard leg", file=sys.stderr, flush=True)
+            sx, elx, _, m0_x, _ = run_gpu(torch, M, n, "float32", K, W, args.path, guard=M.GUARD_F64_EXACT,
+                                          profile=False)
+            vx = cells * K / elx / 1e9
+            sx.advance(K)
+            out["fp32_f64_exact_guards"] = {"value": round(vx, 4), "ms_per_step": round(elx * 1e3 / K, 4),
+                                            "over_fp64": round(vx / v64, 4),
+                                            "accuracy_vs_fp64": dict(accuracy(sx, m0_x), steps=2 * K + W)}
+            sx.close()
+        del a64
         s64.close()
-    s32.close()
+    else:
+        s32.close()
+    torch.cuda.empty_cache()
 
+    if not args.no_extra:
+        print("bench: C2 256^3 FP64 leg", file=sys.stderr, flush=True)
+        s2, el2, _, _, _ = run_gpu(torch, M, 256, "float64", K, W, args.path, profile=False)
+        s2.close()
+        out["c2_fp64_256"] = {"value": round(256 ** 3 * K / el2 / 1e9, 4), "unit": "Gcell-updates/s",
+                              "ms_per_step": round(el2 * 1e3 / K, 4), "dtype": "f64",
+                              "workload": "3D MPDATA-FCT n_iters=3, 256^3 open, solid-body rotation",
+                              "step_frac_of_hbm": round(256 ** 3 * K / el2 * alg_bytes_per_cell(1, 3, 8) / 1e9
+                                                        / HBM_PEAK_GBS, 4)}
+        print("bench: C1 256^2 FP64 leg", file=sys.stderr, flush=True)
+        out["c1_2d_fp64_256"] = run_2d_c1(torch, M, 10 * K, W)
     if not args.no_multi:
+        print("bench: multi-tracer leg", file=sys.stderr, flush=True)
         out["multi_tracer"] = run_multi_tracer(torch, M, max(K // 2, 2), min(W, 2))
     if not args.no_cpu:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)
     return 0

@@ -36,18 +36,36 @@ namespace libmpdataxx
     std::array<int, n_dims> base() const { return lbound_; }
     std::array<int, n_dims> shape() const { return extent_; }
     std::size_t size() const { std::size_t n = 1; for (int d = 0; d < n_dims; ++d) n *= extent_[d]; return n; }
-    real_t *dataFirst() const { return data_; }
+    // like blitz::Array, a const view gives read-only elements
+    real_t *dataFirst() { return data_; }
+    const real_t *dataFirst() const { return data_; }
     std::ptrdiff_t stride(int d) const { return stride_[d]; }
 
     template <class... I>
-    real_t &operator()(I... idx) const
+    real_t &operator()(I... idx)
+    {
+      return data_[offset(idx...)];
+    }
+
+    template <class... I>
+    const real_t &operator()(I... idx) const
+    {
+      return data_[offset(idx...)];
+    }
+
+    private:
+
+    template <class... I>
+    std::ptrdiff_t offset(I... idx) const
     {
       static_assert(sizeof...(I) == n_dims, "wrong number of indices");
       const int ix[n_dims] = {idx...};
       std::ptrdiff_t off = 0;
       for (int d = 0; d < n_dims; ++d) off += (ix[d] - lbound_[d]) * stride_[d];
-      return data_[off];
+      return off;
     }
+
+    public:
 
     // blitz-like scalar assignment
     arr_t &operator=(real_t v)

@@ -6,12 +6,19 @@
  * <solver_t, bcxl, bcxr, bcyl, bcyr, bczl, bczr>, same public interface
  * (concurr/any.hpp, concurr_common.hpp:380-453) and the same exception texts.
  * The state lives in GPU memory; advectee()/advector() return views of host
- * mirrors that are uploaded before and refreshed after every advance().
+ * mirrors.  A mirror is downloaded when it is first accessed after an
+ * advance() (not after every step), and uploaded before the next advance()
+ * or reduction only if a writeable view of it was handed out in between;
+ * advectee_global() gives a read-only view that triggers no upload.
+ * advance(nt) is one library call for all nt steps (the library polls the
+ * panic flag between steps); with an output decorator it is one call per
+ * stretch of steps between records.
  * All work goes through the C ABI of include/mpdx.h; a solver decorated with
  * output::hdf5 / output::hdf5_xdmf records through include/mpdx_h5.h.
  */
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <stdexcept>
 #include <string>
@@ -60,10 +67,14 @@ namespace libmpdataxx
       private:
 
       mpdx_solver *h_ = nullptr;
-      std::vector<std::vector<real_t>> psi_host_;
-      std::vector<bool> psi_dirty_;
-      std::array<std::vector<real_t>, 3> gc_host_;
-      std::array<bool, 3> gc_dirty_{};
+      // host mirror states: stale (the device is newer), clean (equal to
+      // the device), lent (a writeable view was handed out: upload before
+      // the device uses the array again)
+      enum mirror_state { stale, clean, lent };
+      mutable std::vector<std::vector<real_t>> psi_host_;
+      mutable std::vector<mirror_state> psi_state_;
+      mutable std::array<std::vector<real_t>, 3> gc_host_;
+      mutable std::array<mirror_state, 3> gc_state_{};
       bool panic_ = false;
       static constexpr bool with_output = detail::has_output<solver_t_>::value;
       typename detail::recorder_of<solver_t_>::type out_;
@@ -96,34 +107,60 @@ namespace libmpdataxx
 
       std::array<int, n_dims> grid_{};
 
-      void upload()
+      void advector_strides(int d, int64_t st[3]) const
+      {
+        int ext3[3];
+        check(mpdx_advector_extents(h_, d, ext3));
+        std::array<int, n_dims> ext{};
+        for (int q = 0; q < n_dims; ++q) ext[q] = ext3[q];
+        dense_strides(ext, st);
+      }
+
+      // mirrors handed out writeable go back to the device
+      void upload() const
       {
         int64_t st[3];
         for (int e = 0; e < n_eqns; ++e)
-          if (psi_dirty_[e])
+          if (psi_state_[e] == lent)
           {
             dense_strides(interior_extent(), st);
             check(mpdx_set_advectee(h_, e, psi_host_[e].data(), st, 0));
-            psi_dirty_[e] = false;
+            psi_state_[e] = clean;
           }
         for (int d = 0; d < n_dims; ++d)
-          if (gc_dirty_[d])
+          if (gc_state_[d] == lent)
           {
-            int ext3[3];
-            check(mpdx_advector_extents(h_, d, ext3));
-            std::array<int, n_dims> ext{};
-            for (int q = 0; q < n_dims; ++q) ext[q] = ext3[q];
-            dense_strides(ext, st);
+            advector_strides(d, st);
             check(mpdx_set_advector(h_, d, gc_host_[d].data(), st, 0));
-            gc_dirty_[d] = false;
+            gc_state_[d] = clean;
           }
       }
 
-      void download()
+      // the mirror of equation e / component d, current
+      void fetch_advectee(int e) const
       {
+        if (psi_state_.at(e) != stale) return;
         int64_t st[3];
         dense_strides(interior_extent(), st);
-        for (int e = 0; e < n_eqns; ++e) check(mpdx_get_advectee(h_, e, psi_host_[e].data(), st, 0));
+        check(mpdx_get_advectee(h_, e, psi_host_[e].data(), st, 0));
+        psi_state_[e] = clean;
+      }
+
+      void fetch_advector(int d) const
+      {
+        if (gc_state_.at(d) != stale) return;
+        int64_t st[3];
+        advector_strides(d, st);
+        check(mpdx_get_advector(h_, d, gc_host_[d].data(), st, 0));
+        gc_state_[d] = clean;
+      }
+
+      // after an advance the device holds newer state (the advector too: a
+      // var_dt rescale or a calc_gc hook may have changed it)
+      void device_advanced()
+      {
+        for (auto &st : psi_state_) st = stale;
+        for (int d = 0; d < n_dims; ++d) gc_state_[d] = stale;
       }
 
       public:
@@ -154,7 +191,9 @@ namespace libmpdataxx
         std::size_t n = 1;
         for (int d = 0; d < n_dims; ++d) n *= grid_[d];
         psi_host_.assign(n_eqns, std::vector<real_t>(n, real_t(0)));
-        psi_dirty_.assign(n_eqns, false);
+        psi_state_.assign(n_eqns, stale);
+        // the library polls this object's panic flag between steps
+        check(mpdx_watch_panic(h_, reinterpret_cast<const volatile unsigned char *>(&panic_)));
         for (int d = 0; d < n_dims; ++d)
         {
           int ext3[3];
@@ -162,6 +201,7 @@ namespace libmpdataxx
           std::size_t m = 1;
           for (int q = 0; q < n_dims; ++q) m *= ext3[q];
           gc_host_[d].assign(m, real_t(0));
+          gc_state_[d] = stale;
         }
       }
 
@@ -179,30 +219,54 @@ namespace libmpdataxx
         {
           if (!panic_) check(mpdx_advance_time(h_, nt));
         }
-        else
+        else if constexpr (with_output)
         {
-          if constexpr (with_output) out_.start(h_, nt);
-          for (advance_arg_t t = 0; t < nt; ++t)
+          // records between steps (hook_post_step, output_common.hpp:145-156):
+          // one library call per stretch of steps up to the next record
+          out_.start(h_, nt);
+          int64_t left = int64_t(nt);
+          while (left > 0 && !panic_)
           {
-            if (panic_) break;
-            check(mpdx_advance(h_, 1));
-            if constexpr (with_output) out_.post_step(h_);
+            int64_t ts0, ts1;
+            mpdx_timestep(h_, &ts0);
+            const int64_t k = std::min<int64_t>(left, out_.steps_to_next_record(h_));
+            check(mpdx_advance(h_, k));
+            mpdx_timestep(h_, &ts1);
+            left -= ts1 - ts0;
+            if (ts1 - ts0 < k) break;      // stopped by the panic flag
+            out_.post_step(h_);
           }
         }
-        download();
+        else
+        {
+          check(mpdx_advance(h_, int64_t(nt)));
+        }
+        device_advanced();
       }
 
       // sharedmem.hpp:522-531: interior of the current time level, base 0
       arr_t advectee(int e = 0) override
       {
-        psi_dirty_.at(e) = true;
+        fetch_advectee(e);
+        psi_state_[e] = lent;
         std::array<int, n_dims> lb{};
         return arr_t(psi_host_[e].data(), lb, interior_extent());
       }
 
-      const arr_t advectee_global(int e = 0) override { return advectee(e); }
+      // read-only: no upload follows
+      const arr_t advectee_global(int e = 0) override
+      {
+        fetch_advectee(e);
+        std::array<int, n_dims> lb{};
+        return arr_t(psi_host_[e].data(), lb, interior_extent());
+      }
 
-      void advectee_global_set(const arr_t arr, int e = 0) override { advectee(e).assign(arr); }
+      void advectee_global_set(const arr_t arr, int e = 0) override
+      {
+        std::array<int, n_dims> lb{};
+        arr_t(psi_host_.at(e).data(), lb, interior_extent()).assign(arr);
+        psi_state_[e] = lent;
+      }
 
       // sharedmem.hpp:543-577: faces -1..n_d-2 along d (index i <-> face i+1/2),
       // x from -1 for the components d > 0 in 3D
@@ -213,7 +277,8 @@ namespace libmpdataxx
         std::array<int, n_dims> lb{}, ext{};
         for (int q = 0; q < n_dims; ++q) ext[q] = ext3[q];
         lb[0] = -1;
-        gc_dirty_.at(d) = true;
+        fetch_advector(d);
+        gc_state_[d] = lent;
         return arr_t(gc_host_[d].data(), lb, ext);
       }
 
@@ -235,6 +300,7 @@ namespace libmpdataxx
       const real_t min(int e = 0) const override
       {
         double v;
+        upload();
         check(mpdx_reduce(h_, e, MPDX_RED_MIN, &v));
         return real_t(v);
       }
@@ -242,6 +308,7 @@ namespace libmpdataxx
       const real_t max(int e = 0) const override
       {
         double v;
+        upload();
         check(mpdx_reduce(h_, e, MPDX_RED_MAX, &v));
         return real_t(v);
       }
@@ -250,6 +317,7 @@ namespace libmpdataxx
       double mass(int e = 0) const
       {
         double v;
+        upload();
         check(mpdx_reduce(h_, e, MPDX_RED_SUM, &v));
         return v;
       }

@@ -114,6 +114,20 @@ namespace libmpdataxx
             if ((ts - t) % int64_t(outfreq_) == 0 && (ts - t) >= int64_t(outstart_)) record(h);
         }
 
+        // steps from the current one to the next step whose post_step
+        // records (at least 1), so that the steps in between can go to the
+        // library in one call
+        int64_t steps_to_next_record(mpdx_solver *h) const
+        {
+          int64_t ts;
+          mpdx_timestep(h, &ts);
+          const int64_t f = int64_t(outfreq_), horizon = f + int64_t(outstart_) + outwindow_ + 1;
+          for (int64_t k = 1; k < horizon; ++k)
+            for (int t = 0; t < outwindow_; ++t)
+              if ((ts + k - t) % f == 0 && (ts + k - t) >= int64_t(outstart_)) return k;
+          return horizon;
+        }
+
         void record(mpdx_solver *h)
         {
           int64_t ts;

@@ -153,8 +153,18 @@ int mpdx_dt(const mpdx_solver *h, double *dt);
 int mpdx_reduce(mpdx_solver *h, int e, int op, double *out);
 int mpdx_time(const mpdx_solver *h, double *t);
 int mpdx_timestep(const mpdx_solver *h, int64_t *ts);
-/* host flag polled once per step; set nonzero to make advance() return early */
+/* host flag polled once per step; set nonzero to make advance() return early.
+ * advance() keeps at most a few steps queued on the GPU ahead of the host, so
+ * a flag set while it runs (another thread, a signal handler) stops it within
+ * those steps; the steps already queued complete. */
 int *mpdx_panic_ptr(mpdx_solver *h);
+/* a second flag, owned by the caller (a byte, nonzero = stop), polled like
+ * the library's own: the C++ backend registers its bool panic_ so that the
+ * reference's panic_ptr() (concurr_common.hpp:435-438, polled by
+ * solver_common::solve, solver_common.hpp:301-302) works across one
+ * multi-step advance() call.  NULL removes it.  The flag must outlive the
+ * solver or be removed first. */
+int mpdx_watch_panic(mpdx_solver *h, const volatile unsigned char *flag);
 /* which path advance() uses for this configuration (MPDX_PATH_PERPASS/FUSED;
  * -1 for a null handle) */
 int mpdx_active_path(const mpdx_solver *h);

@@ -32,7 +32,7 @@ EXPORTED = [
     "mpdx_set_stream", "mpdx_advector_extents", "mpdx_set_advectee",
     "mpdx_get_advectee", "mpdx_set_advector", "mpdx_get_advector",
     "mpdx_advance", "mpdx_reduce", "mpdx_time", "mpdx_timestep",
-    "mpdx_panic_ptr", "mpdx_active_path", "mpdx_profile", "mpdx_profile_read",
+    "mpdx_panic_ptr", "mpdx_watch_panic", "mpdx_active_path", "mpdx_profile", "mpdx_profile_read",
     "mpdx_set_calc_gc", "mpdx_advector_device", "mpdx_set_advector_generator",
     "mpdx_set_rhs_scheme", "mpdx_set_rhs", "mpdx_rhs_device", "mpdx_advectee_device",
     "mpdx_advance_time", "mpdx_dt", "mpdx_set_x_chunk",
@@ -95,6 +95,7 @@ def lib():
         L.mpdx_timestep.argtypes = [P, i64p]
         L.mpdx_panic_ptr.restype = ctypes.POINTER(ctypes.c_int)
         L.mpdx_panic_ptr.argtypes = [P]
+        L.mpdx_watch_panic.argtypes = [P, P]
         L.mpdx_active_path.argtypes = [P]
         L.mpdx_set_calc_gc.argtypes = [P, CALC_GC_FN, P]
         L.mpdx_advector_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), i64p]
@@ -213,16 +214,32 @@ class Solver:
         return out
 
     # device (torch) copies ----------------------------------------------
+    def _device_view(self, t, shape, what):
+        """Element strides of a device tensor, after checking that the kernel
+        that copies it stays inside it: the shape of the view, the element
+        type of the solver, memory on a GPU and non-negative strides.  Raises
+        ValueError before anything is launched."""
+        if not getattr(t, "is_cuda", False):
+            raise ValueError(f"{what}: expected a GPU tensor")
+        want = {4: "torch.float32", 8: "torch.float64"}[self.dtype.itemsize]
+        if str(t.dtype) != want:
+            raise ValueError(f"{what}: dtype {t.dtype}, the solver's is {want}")
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f"{what}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+        if any(s < 0 for s in t.stride()):
+            raise ValueError(f"{what}: negative strides")
+        return (ctypes.c_int64 * 3)(*(list(t.stride()) + [0] * (3 - t.dim())))
+
     def set_advectee_device(self, t, e=0):
-        st = (ctypes.c_int64 * 3)(*(list(t.stride()) + [0] * (3 - t.dim())))
+        st = self._device_view(t, self.grid, "advectee")
         self._check(self._L.mpdx_set_advectee(self._h, e, t.data_ptr(), st, 1))
 
     def get_advectee_device(self, t, e=0):
-        st = (ctypes.c_int64 * 3)(*(list(t.stride()) + [0] * (3 - t.dim())))
+        st = self._device_view(t, self.grid, "advectee")
         self._check(self._L.mpdx_get_advectee(self._h, e, t.data_ptr(), st, 1))
 
     def set_advector_device(self, t, d):
-        st = (ctypes.c_int64 * 3)(*(list(t.stride()) + [0] * (3 - t.dim())))
+        st = self._device_view(t, self.advector_shape(d), f"advector({d})")
         self._check(self._L.mpdx_set_advector(self._h, d, t.data_ptr(), st, 1))
 
     # stepping and diagnostics ---------------------------------------------

@@ -403,6 +403,21 @@ struct PerPass {
 
 // ----------------------------------------------------------------- helpers
 
+// captured time steps hold kernel arguments (buffer addresses, launch
+// shapes, the generator's kernel); whatever changes one of them drops both
+void drop_step_graphs(mpdx_solver *s)
+{
+  for (int q = 0; q < 2; ++q) {
+    if (s->graph[q]) (void)hipGraphExecDestroy(s->graph[q]);
+    s->graph[q] = nullptr;
+  }
+}
+
+bool panicked(const mpdx_solver *s)
+{
+  return s->panic || (s->panic_ext && *s->panic_ext);
+}
+
 int fill_nan(mpdx_solver *s, void *p)
 {
   if (s->esz == 8)
@@ -1057,10 +1072,12 @@ int fused_step(mpdx_solver *s)
 }
 
 // time-varying advector for this step (solver_common.hpp:306-308), then the
-// halo refill that the reference's calc_gc implementations perform
+// halo refill that the reference's calc_gc implementations perform; *changed
+// says whether the advector changed (calc_gc's return value)
 template <class R, class G>
-int calc_gc_step(mpdx_solver *s)
+int calc_gc_step(mpdx_solver *s, bool *changed_out = nullptr)
 {
+  if (changed_out) *changed_out = false;
   if (!s->calc_gc && s->gen_kind == MPDX_GC_NONE) return 0;
   const double t_half = s->time + 0.5 * s->p.dt;
   bool changed = false;
@@ -1091,6 +1108,7 @@ int calc_gc_step(mpdx_solver *s)
     pp.xchng_vctr_nrml(gc, s->L.hw - 1);
   }
   HIPCHK(hipGetLastError());
+  if (changed_out) *changed_out = changed;
   return 0;
 }
 
@@ -1143,10 +1161,20 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
   if (gen) {
     const long n = nt - s->timestep;
     if (n > s->gen_amp_cap) {
+      // a captured step reads the table by address: drop the graphs before
+      // the table moves, and free the old one once nothing uses it
+      HIPCHK(hipStreamSynchronize(s->stream));
+      drop_step_graphs(s);
+      const long cap = std::max(n, std::max(2 * s->gen_amp_cap, 64L));
       void *pa = nullptr;
-      if (int rc = dev_alloc(s, &pa, sizeof(double) * (size_t)n)) return rc;
+      if (int rc = dev_alloc(s, &pa, sizeof(double) * (size_t)cap)) return rc;
+      if (s->gen_amp) {
+        for (auto &p : s->allocs)
+          if (p == s->gen_amp) p = nullptr;
+        (void)hipFree(s->gen_amp);
+      }
       s->gen_amp = (double *)pa;
-      s->gen_amp_cap = n;
+      s->gen_amp_cap = cap;
     }
     if (!s->gen_step) {
       void *pc = nullptr;
@@ -1167,8 +1195,21 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
   // individually (the legacy null stream cannot be captured)
   const bool graphs = fused && s->use_graphs && !s->prof_on && !s->calc_gc && s->stream != nullptr &&
                       s->p.upwind_filter_freq <= 0 && rs == MPDX_RHS_NONE;
+  // bounded queue: before step q the host waits for step q - kInflight, so
+  // the panic flag is looked at while at most kInflight steps are queued
+  if (!s->inflight[0])
+    for (int q = 0; q < mpdx_solver::kInflight; ++q)
+      if (hipEventCreateWithFlags(&s->inflight[q], hipEventDisableTiming) != hipSuccess) {
+        s->inflight[q] = nullptr;
+        (void)hipGetLastError();
+      }
+  int64_t queued = 0;
   while (var_dt ? s->time < t_end : s->timestep < nt) {
-    if (s->panic) break;
+    if (queued >= mpdx_solver::kInflight) {
+      hipEvent_t ev = s->inflight[queued % mpdx_solver::kInflight];
+      if (ev) HIPCHK(hipEventSynchronize(ev));
+    }
+    if (panicked(s)) break;
     if (graphs) {
       const int key = s->lev[0] & 1;
       if (!s->graph[key]) {
@@ -1190,18 +1231,21 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
       HIPCHK(hipGraphLaunch(s->graph[key], s->stream));
       for (int e = 0; e < s->p.n_eqns; ++e) s->lev[e] ^= 1;
     } else {
-      if (int rc = calc_gc_step<R, G>(s)) return rc;
+      bool var_gc = false;
+      if (int rc = calc_gc_step<R, G>(s, &var_gc)) return rc;
       // var_dt with a time-varying advector: shrink / grow dt until the
-      // Courant number is at most max_courant (solver_common.hpp:310-324)
-      if (var_dt && s->calc_gc) {
+      // Courant number is at most max_courant, only in steps whose calc_gc
+      // reported a changed advector (solver_common.hpp:306-324); the
+      // comparisons are in real_t like the reference's
+      if (var_dt && var_gc) {
         double cfl = 0;
         if (int rc = courant_impl<R>(s, &cfl)) return rc;
         int guard = 0;
-        while (cfl > 0) {
+        while ((R)cfl > R(0)) {
           s->p.dt *= s->p.max_courant / cfl;
           if (int rc = calc_gc_step<R, G>(s)) return rc;
           if (int rc = courant_impl<R>(s, &cfl)) return rc;
-          if (!(cfl > s->p.max_courant)) break;
+          if (!((R)cfl > (R)s->p.max_courant)) break;
           if (++guard == 64) {
             s->err = "libmpdata++: var_dt did not bring the Courant number below max_courant";
             return MPDX_EINVAL;
@@ -1223,6 +1267,8 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
       if (gen) hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, s->stream, s->gen_step, -1);
     }
     HIPCHK(hipGetLastError());
+    if (hipEvent_t ev = s->inflight[queued % mpdx_solver::kInflight]) HIPCHK(hipEventRecord(ev, s->stream));
+    ++queued;
     s->timestep++;
     s->time = var_dt ? s->time + s->p.dt : s->timestep * s->p.dt;
     // mpdata_rhs::hook_post_step (solvers/mpdata_rhs.hpp:162-183)
@@ -1561,8 +1607,7 @@ int mpdx_destroy(mpdx_solver *s)
 {
   if (!s) return MPDX_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  for (int q = 0; q < 2; ++q)
-    if (s->graph[q]) (void)hipGraphExecDestroy(s->graph[q]);
+  drop_step_graphs(s);
   for (void *p : s->allocs)
     if (p) (void)hipFree(p);
   fused_free(s);
@@ -1572,6 +1617,8 @@ int mpdx_destroy(mpdx_solver *s)
   }
   if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
   if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+  for (hipEvent_t ev : s->inflight)
+    if (ev) (void)hipEventDestroy(ev);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return MPDX_OK;
@@ -1615,6 +1662,8 @@ int mpdx_advector_extents(const mpdx_solver *s, int d, int ext[3])
 int mpdx_set_advectee(mpdx_solver *s, int e, const void *buf, const int64_t strides[3], int on_device)
 {
   MPDX_NEED(s, "solver handle");
+  MPDX_NEED(buf, "buffer");
+  MPDX_NEED(strides, "strides");
   if (e < 0 || e >= s->p.n_eqns) { s->err = "libmpdata++: bad equation index"; return MPDX_EINVAL; }
   int lo[3] = {0, 0, 0}, hi[3] = {s->L.n[0] - 1, s->L.n[1] - 1, s->L.nd == 3 ? s->L.n[2] - 1 : 0};
   return copy_view(s, s->psi[e][s->lev[e]], lo, hi, const_cast<void *>(buf), strides, on_device, true);
@@ -1623,6 +1672,8 @@ int mpdx_set_advectee(mpdx_solver *s, int e, const void *buf, const int64_t stri
 int mpdx_get_advectee(mpdx_solver *s, int e, void *buf, const int64_t strides[3], int on_device)
 {
   MPDX_NEED(s, "solver handle");
+  MPDX_NEED(buf, "buffer");
+  MPDX_NEED(strides, "strides");
   if (e < 0 || e >= s->p.n_eqns) { s->err = "libmpdata++: bad equation index"; return MPDX_EINVAL; }
   int lo[3] = {0, 0, 0}, hi[3] = {s->L.n[0] - 1, s->L.n[1] - 1, s->L.nd == 3 ? s->L.n[2] - 1 : 0};
   return copy_view(s, s->psi[e][s->lev[e]], lo, hi, buf, strides, on_device, false);
@@ -1631,6 +1682,8 @@ int mpdx_get_advectee(mpdx_solver *s, int e, void *buf, const int64_t strides[3]
 int mpdx_set_advector(mpdx_solver *s, int d, const void *buf, const int64_t strides[3], int on_device)
 {
   MPDX_NEED(s, "solver handle");
+  MPDX_NEED(buf, "buffer");
+  MPDX_NEED(strides, "strides");
   if (d < 0 || d >= s->L.nd) { s->err = "libmpdata++: bad advector component"; return MPDX_EINVAL; }
   int lo[3], hi[3];
   adv_bounds(s, d, lo, hi);
@@ -1640,6 +1693,8 @@ int mpdx_set_advector(mpdx_solver *s, int d, const void *buf, const int64_t stri
 int mpdx_get_advector(mpdx_solver *s, int d, void *buf, const int64_t strides[3], int on_device)
 {
   MPDX_NEED(s, "solver handle");
+  MPDX_NEED(buf, "buffer");
+  MPDX_NEED(strides, "strides");
   if (d < 0 || d >= s->L.nd) { s->err = "libmpdata++: bad advector component"; return MPDX_EINVAL; }
   int lo[3], hi[3];
   adv_bounds(s, d, lo, hi);
@@ -1702,7 +1757,21 @@ int mpdx_timestep(const mpdx_solver *s, int64_t *ts)
   return MPDX_OK;
 }
 
-int *mpdx_panic_ptr(mpdx_solver *s) { return &s->panic; }
+int *mpdx_panic_ptr(mpdx_solver *s)
+{
+  if (!s) {
+    g_create_error = "libmpdata++: null solver handle";
+    return nullptr;
+  }
+  return &s->panic;
+}
+
+int mpdx_watch_panic(mpdx_solver *s, const volatile unsigned char *flag)
+{
+  MPDX_NEED(s, "solver handle");
+  s->panic_ext = flag;
+  return MPDX_OK;
+}
 
 int mpdx_active_path(const mpdx_solver *s) { return s ? s->active_path : -1; }
 
@@ -1733,6 +1802,8 @@ int mpdx_set_rhs_scheme(mpdx_solver *s, int scheme, mpdx_update_rhs_fn fn, void 
 int mpdx_set_rhs(mpdx_solver *s, int e, const void *buf, const int64_t strides[3], int on_device)
 {
   MPDX_NEED(s, "solver handle");
+  MPDX_NEED(buf, "buffer");
+  MPDX_NEED(strides, "strides");
   if (e < 0 || e >= s->p.n_eqns || !s->rhs[e]) {
     s->err = "libmpdata++: no forcing array for this equation (set an rhs scheme first)";
     return MPDX_EINVAL;
@@ -1751,6 +1822,8 @@ static int interior_device(mpdx_solver *s, void *arr, void **ptr, int64_t stride
 int mpdx_rhs_device(mpdx_solver *s, int e, void **ptr, int64_t strides[3])
 {
   MPDX_NEED(s, "solver handle");
+  MPDX_NEED(ptr, "output pointer");
+  MPDX_NEED(strides, "strides");
   if (e < 0 || e >= s->p.n_eqns || !s->rhs[e]) {
     s->err = "libmpdata++: no forcing array for this equation (set an rhs scheme first)";
     return MPDX_EINVAL;
@@ -1761,6 +1834,8 @@ int mpdx_rhs_device(mpdx_solver *s, int e, void **ptr, int64_t strides[3])
 int mpdx_advectee_device(mpdx_solver *s, int e, void **ptr, int64_t strides[3])
 {
   MPDX_NEED(s, "solver handle");
+  MPDX_NEED(ptr, "output pointer");
+  MPDX_NEED(strides, "strides");
   if (e < 0 || e >= s->p.n_eqns) { s->err = "libmpdata++: bad equation index"; return MPDX_EINVAL; }
   return interior_device(s, s->psi[e][s->lev[e]], ptr, strides);
 }
@@ -1768,6 +1843,8 @@ int mpdx_advectee_device(mpdx_solver *s, int e, void **ptr, int64_t strides[3])
 int mpdx_advector_device(mpdx_solver *s, int d, void **ptr, int64_t strides[3])
 {
   MPDX_NEED(s, "solver handle");
+  MPDX_NEED(ptr, "output pointer");
+  MPDX_NEED(strides, "strides");
   if (d < 0 || d >= s->L.nd) { s->err = "libmpdata++: bad advector component"; return MPDX_EINVAL; }
   int lo[3], hi[3];
   adv_bounds(s, d, lo, hi);
@@ -1780,12 +1857,15 @@ int mpdx_advector_device(mpdx_solver *s, int d, void **ptr, int64_t strides[3])
 int mpdx_set_advector_generator(mpdx_solver *s, int kind, const double *params, int n_params)
 {
   MPDX_NEED(s, "solver handle");
+  // a captured step holds the generator's kernel (or its absence)
+  HIPCHK(hipStreamSynchronize(s->stream));
+  drop_step_graphs(s);
   if (kind == MPDX_GC_NONE) { s->gen_kind = kind; return MPDX_OK; }
   if (s->p.var_dt) {
     s->err = "libmpdata++: the built-in advector generator does not follow a variable dt";
     return MPDX_EUNSUPPORTED;
   }
-  if (kind != MPDX_GC_DEFORM || n_params < 2 || !(params[0] > 0) || !(params[1] > 0)) {
+  if (kind != MPDX_GC_DEFORM || n_params < 2 || !params || !(params[0] > 0) || !(params[1] > 0)) {
     s->err = "libmpdata++: bad advector generator parameters";
     return MPDX_EINVAL;
   }
@@ -1858,10 +1938,7 @@ int mpdx_set_x_chunk(mpdx_solver *s, int planes)
   }
   HIPCHK(hipStreamSynchronize(s->stream));
   s->ix_chunk = planes;
-  for (int q = 0; q < 2; ++q) {     // captured steps hold the old launch shape
-    if (s->graph[q]) (void)hipGraphExecDestroy(s->graph[q]);
-    s->graph[q] = nullptr;
-  }
+  drop_step_graphs(s);              // captured steps hold the old launch shape
   return MPDX_OK;
 }
 
@@ -2010,7 +2087,9 @@ int mpdx_ensemble_tune(mpdx_ensemble *e, int steps_per_trial, int64_t *steps_tak
 
 int mpdx_ensemble_x_chunk(const mpdx_ensemble *e, int i, int *planes)
 {
-  if (!e || i < 0 || i >= (int)e->m.size() || !planes) return MPDX_EINVAL;
+  MPDX_NEED(e, "ensemble handle");
+  MPDX_NEED(planes, "output pointer");
+  if (i < 0 || i >= (int)e->m.size()) return MPDX_EINVAL;
   *planes = e->m[i]->ix_chunk;
   return MPDX_OK;
 }

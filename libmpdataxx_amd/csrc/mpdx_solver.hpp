@@ -40,6 +40,12 @@ struct mpdx_solver {
   int64_t timestep = 0;
   double time = 0;
   int panic = 0;
+  const volatile unsigned char *panic_ext = nullptr;   // caller's flag (mpdx_watch_panic)
+  // one event per queued step of advance(): the host waits for the step
+  // kInflight back before queueing another, so a panic flag takes effect
+  // within kInflight steps
+  static constexpr int kInflight = 6;
+  hipEvent_t inflight[kInflight] = {};
   int active_path = MPDX_PATH_PERPASS;
   std::string err;
   mpdx_fused_state *fz = nullptr;

@@ -52,8 +52,33 @@ def build(force: bool = False) -> None:
         subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
 
 
+# x86-64-v4 = v3 + AVX-512 F/BW/CD/DQ/VL
+_V4_FLAGS = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+
+
+def host_isa() -> str:
+    """'x86-64-v4' when this host's CPU reports the AVX-512 subsets of that
+    level, else 'x86-64-v3'."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return "x86-64-v4" if _V4_FLAGS <= set(line.split(":", 1)[1].split()) else "x86-64-v3"
+    except OSError:
+        pass
+    return "x86-64-v3"
+
+
+def fast_lib_name() -> str:
+    """The release-flag build that best matches this host's CPU."""
+    v4 = os.path.join(_HERE, "libmpdx_oracle_fast_v4.so")
+    if host_isa() == "x86-64-v4" and os.path.exists(v4):
+        return "libmpdx_oracle_fast_v4.so"
+    return "libmpdx_oracle_fast.so"
+
+
 def _lib(fast: bool = False):
-    name = "libmpdx_oracle_fast.so" if fast else "libmpdx_oracle.so"
+    name = fast_lib_name() if fast else "libmpdx_oracle.so"
     if name not in _libs:
         path = os.path.join(_HERE, name)
         if not os.path.exists(path):

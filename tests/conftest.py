@@ -3,6 +3,12 @@ import sys
 
 import pytest
 
+# The oracle is an OpenMP library with a barrier between passes; on the small
+# grids of most tests several threads per process (more so with pytest-xdist
+# workers) spend their time spinning at those barriers.  One thread unless
+# the environment asks for more (the GPU host sets its own CPU share).
+os.environ.setdefault("OMP_NUM_THREADS", "1")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

@@ -73,17 +73,60 @@ def test_null_handles_are_errors():
         "mpdx_timestep": (None, ctypes.byref(i64)),
         "mpdx_set_x_chunk": (None, 0),
         "mpdx_profile": (None, 1),
+        "mpdx_profile_read": (None, 0, None, 0, None, None),
+        "mpdx_set_calc_gc": (None, M.CALC_GC_FN(), None),
+        "mpdx_set_rhs_scheme": (None, 0, M.UPDATE_RHS_FN(), None),
+        "mpdx_set_rhs": (None, 0, buf, st, 0),
+        "mpdx_set_advector_generator": (None, 1, None, 0),
+        "mpdx_advector_device": (None, 0, ctypes.byref(ctypes.c_void_p()), st),
+        "mpdx_advectee_device": (None, 0, ctypes.byref(ctypes.c_void_p()), st),
+        "mpdx_rhs_device": (None, 0, ctypes.byref(ctypes.c_void_p()), st),
         "mpdx_ensemble_advance": (None, 1),
         "mpdx_ensemble_tune": (None, 1, None),
+        "mpdx_ensemble_x_chunk": (None, 0, ctypes.byref(ctypes.c_int())),
     }
     for name, args in calls.items():
         fn = getattr(L, name)
         assert fn(*args) == 1, name                       # MPDX_EINVAL
         assert b"null" in L.mpdx_last_error(None), name
     assert L.mpdx_active_path(None) == -1
+    assert not L.mpdx_panic_ptr(None)                     # a null pointer, not a crash
+    assert b"null" in L.mpdx_last_error(None)
     p = M.Params()
     L.mpdx_params_default(ctypes.byref(p))
     assert L.mpdx_create(ctypes.byref(p), None) == 1
+
+
+def test_null_buffers_are_errors():
+    """Null buffers, strides and output pointers are rejected before the
+    handle is used: a stand-in handle (host memory that no call may
+    dereference) with a null pointer argument gives MPDX_EINVAL."""
+    L = M.lib()
+    fake = ctypes.create_string_buffer(256)
+    h = ctypes.cast(fake, ctypes.c_void_p)
+    st = (ctypes.c_int64 * 3)(1, 1, 1)
+    buf = (ctypes.c_double * 8)()
+    pp = ctypes.byref(ctypes.c_void_p())
+    calls = {
+        "mpdx_set_advectee": [(h, 0, None, st, 0), (h, 0, buf, None, 0)],
+        "mpdx_get_advectee": [(h, 0, None, st, 0), (h, 0, buf, None, 0)],
+        "mpdx_set_advector": [(h, 0, None, st, 0), (h, 0, buf, None, 0)],
+        "mpdx_get_advector": [(h, 0, None, st, 0), (h, 0, buf, None, 0)],
+        "mpdx_set_rhs": [(h, 0, None, st, 0), (h, 0, buf, None, 0)],
+        "mpdx_advector_device": [(h, 0, None, st), (h, 0, pp, None)],
+        "mpdx_advectee_device": [(h, 0, None, st), (h, 0, pp, None)],
+        "mpdx_rhs_device": [(h, 0, None, st), (h, 0, pp, None)],
+        "mpdx_dt": [(h, None)],
+        "mpdx_time": [(h, None)],
+        "mpdx_timestep": [(h, None)],
+        "mpdx_reduce": [(h, 0, 0, None)],
+        "mpdx_advector_extents": [(h, 0, None)],
+    }
+    for name, variants in calls.items():
+        fn = getattr(L, name)
+        for args in variants:
+            assert fn(*args) == 1, (name, args)
+            assert b"null" in L.mpdx_last_error(None), name
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
@@ -139,3 +182,52 @@ def test_interior_kernels_do_not_spill():
     for r in rows:
         f = dict(kv.split("=") for kv in r[1:])
         assert f["vspill"] == "0" and f["scratch"] == "0", r
+
+
+class _NoCalls:
+    """Stands in for the C library: any call is a test failure."""
+
+    def __getattr__(self, name):
+        raise AssertionError(f"{name} called although the arguments are invalid")
+
+
+class _FakeTensor:
+    def __init__(self, shape, dtype="torch.float32", cuda=True, stride=None):
+        self.shape, self.dtype, self.is_cuda = tuple(shape), dtype, cuda
+        st, s = [], 1
+        for n in reversed(shape):
+            st.insert(0, s)
+            s *= n
+        self._stride = tuple(stride) if stride else tuple(st)
+
+    def stride(self):
+        return self._stride
+
+    def dim(self):
+        return len(self.shape)
+
+    def data_ptr(self):
+        return 4096
+
+
+@pytest.mark.parametrize("t,msg", [
+    (_FakeTensor((6, 5, 4), cuda=False), "GPU tensor"),
+    (_FakeTensor((6, 5, 4), dtype="torch.float64"), "dtype"),
+    (_FakeTensor((6, 5, 3)), "shape"),
+    (_FakeTensor((5, 5, 4)), "shape"),
+    (_FakeTensor((6, 5, 4), stride=(20, 4, -1)), "negative"),
+])
+def test_device_views_are_checked_before_launch(t, msg):
+    """set/get_advectee_device and set_advector_device pass a tensor's raw
+    address and strides to a copy kernel; a tensor of the wrong shape,
+    element type or memory would make that kernel read or write outside it,
+    so the binding rejects it before calling the library."""
+    import numpy as np
+    s = M.Solver.__new__(M.Solver)
+    s._L, s._h = _NoCalls(), None
+    s.grid, s.dtype, s.n_dims = (6, 5, 4), np.dtype(np.float32), 3
+    s.advector_shape = lambda d: (6, 5, 4)
+    for call in (lambda: s.set_advectee_device(t), lambda: s.get_advectee_device(t),
+                 lambda: s.set_advector_device(t, 0)):
+        with pytest.raises(ValueError, match=msg):
+            call()

@@ -47,19 +47,28 @@ def test_c1_gaussian_2d_cyclic_matches_oracle():
     s.close()
 
 
-def test_c2_rotation_sphere_64_matches_oracle():
-    print("C2 at 64^3 FP64 20 steps vs oracle", flush=True)
-    psi, gc = W.rotation_sphere_3d(64)
-    s = M.Solver((64,) * 3, opts=M.FCT, n_iters=3, dtype=np.float64)
-    o = O.Oracle((64,) * 3, opts=O.FCT, n_iters=3)
-    for x in (s, o):
-        x.set_advectee(psi)
-        for d in range(3):
-            x.set_advector(gc[d], d)
-    s.advance(20)
-    o.advance(20)
-    assert np.abs(s.advectee() - o.advectee()).max() <= 1e-12
+@pytest.mark.parametrize("n,steps", [(64, 20), (256, 10)])
+def test_c2_rotation_sphere_matches_oracle(n, steps):
+    """C2 against the strict FP64 oracle, at 64^3 and at its full 256^3
+    (fields in [0, 1]: absolute agreement 1e-12)."""
+    print(f"C2 at {n}^3 FP64 {steps} steps vs oracle", flush=True)
+    psi, gc = W.rotation_sphere_3d(n)
+    s = M.Solver((n,) * 3, opts=M.FCT, n_iters=3, dtype=np.float64)
+    s.set_advectee(psi)
+    for d in range(3):
+        s.set_advector(gc[d], d)
+    s.advance(steps)
+    got = s.advectee()
     s.close()
+    o = O.Oracle((n,) * 3, opts=O.FCT, n_iters=3)
+    o.set_advectee(psi)
+    for d in range(3):
+        o.set_advector(gc[d], d)
+    del psi, gc
+    o.advance(steps)
+    want = o.advectee()
+    del o
+    assert np.abs(got - want).max() <= 1e-12
 
 
 def _sphere(n, dtype, steps):

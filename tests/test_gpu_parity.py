@@ -96,19 +96,26 @@ def test_fused_larger_grid_fp64(nd):
 INTERIOR_OPTS = ["basic", "iters3", "fct", "fct3", "iga", "iga_fct"]
 
 
+# (element type, guard mode): FP64, FP32 with the default FP64-constant
+# guards, FP32 with the guarded ratios evaluated in FP64
+PRECISIONS = {"f64": (np.float64, 0), "f32": (np.float32, 0), "f32_exact": (np.float32, 2)}
+
+
 @pytest.mark.parametrize("name", INTERIOR_OPTS)
 @pytest.mark.parametrize("bi", range(4))
-@pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_fused_interior_matches_oracle(dtype, bi, name):
+@pytest.mark.parametrize("prec", list(PRECISIONS))
+def test_fused_interior_matches_oracle(prec, bi, name):
     """3D grids large enough for the x-marching interior kernel, with several
-    x chunks and y/z tiles, every boundary combination."""
+    x chunks and y/z tiles, every boundary combination; FP64 within 1e-12 of
+    the oracle, FP32 (either guard mode) within 1e-5."""
+    dtype, guard = PRECISIONS[prec]
     opts, iters = OPTS[name]
     bcs = BCS3[bi]
     grid = [(140, 14, 36), (14, 70, 36), (30, 26, 75), (40, 33, 20)][bi]
     psi, gc = setup_case(grid, bcs, 100 + bi)
-    print(f"interior case grid={grid} bcs={bcs} opts={name} dtype={np.dtype(dtype).name}", flush=True)
+    print(f"interior case grid={grid} bcs={bcs} opts={name} prec={prec}", flush=True)
     want = run_oracle(grid, bcs, opts, iters, psi, gc, 3)[0]
-    got = run_gpu(grid, bcs, opts, iters, psi, gc, 3, dtype=dtype, path=2)[0]
+    got = run_gpu(grid, bcs, opts, iters, psi, gc, 3, dtype=dtype, path=2, guard=guard)[0]
     assert not np.isnan(got).any()
     assert rel_err(got, want) <= (1e-12 if dtype == np.float64 else 1e-5)
 
@@ -299,3 +306,87 @@ def test_var_dt_with_calc_gc_rescales_every_step():
     f.advance(steps)
     assert rel_err(f.advectee(), got) <= 1e-12
     f.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_generator_across_growing_advance_calls(dtype):
+    """advance(2) then advance(10) with the on-device generator: the second
+    call needs a longer amplitude table than the first captured its graph
+    with, so the graphs must be recaptured; the advector and trajectory equal
+    the oracle's 12 steps (advector bit for bit)."""
+    import libmpdataxx_amd as M
+    grid, bcs = (40, 34, 14), [(0, 0)] * 3
+    psi = W_random(grid, 310)
+    T, c = 20.0, 0.4
+    print(f"generator advance(2)+advance(10) dtype={np.dtype(dtype).name}", flush=True)
+    o = O.Oracle(grid, opts=O.FCT, n_iters=3, bcs=bcs, dt=1.0, max_abs_div_eps=1e300)
+    s = M.Solver(grid, opts=O.FCT, n_iters=3, bcs=bcs, dt=1.0, dtype=dtype, max_abs_div_eps=1e300)
+    for x in (o, s):
+        x.set_advectee(psi)
+        for d in range(3):
+            x.set_advector(np.zeros(x.advector_shape(d)), d)
+    o.set_generator(1, T, c)
+    s.set_advector_generator(M.GC_DEFORM, T, c)
+    o.advance(12)
+    s.advance(2)
+    s.advance(10)
+    for d in range(3):
+        assert np.array_equal(s.advector(d).astype(np.float64), o.advector(d))
+    got = s.advectee().astype(np.float64)
+    s.close()
+    assert rel_err(got, o.advectee()) <= (1e-12 if dtype == np.float64 else 1e-5)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_generator_switched_on_after_first_advance(dtype):
+    """Steps with a fixed advector (captured graphs without the generator),
+    then the generator switched on: the next steps must run it."""
+    import libmpdataxx_amd as M
+    grid, bcs = (40, 34, 14), [(0, 0)] * 3
+    psi, gc = setup_case(grid, bcs, 311)
+    T, c = 20.0, 0.4
+    print(f"generator switched on dtype={np.dtype(dtype).name}", flush=True)
+    o = O.Oracle(grid, opts=O.FCT, n_iters=3, bcs=bcs, dt=1.0, max_abs_div_eps=1e300)
+    s = M.Solver(grid, opts=O.FCT, n_iters=3, bcs=bcs, dt=1.0, dtype=dtype, max_abs_div_eps=1e300)
+    for x in (o, s):
+        x.set_advectee(psi)
+        for d in range(3):
+            x.set_advector(gc[d], d)
+    o.advance(3)
+    s.advance(3)
+    o.set_generator(1, T, c)
+    s.set_advector_generator(M.GC_DEFORM, T, c)
+    o.advance(4)
+    s.advance(4)
+    for d in range(3):
+        assert np.array_equal(s.advector(d).astype(np.float64), o.advector(d))
+    got = s.advectee().astype(np.float64)
+    s.close()
+    assert rel_err(got, o.advectee()) <= (1e-12 if dtype == np.float64 else 1e-5)
+
+
+def test_var_dt_calc_gc_unchanged_keeps_dt():
+    """A calc_gc hook that reports an unchanged advector: the reference
+    rescales dt only when calc_gc returns true (solver_common.hpp:306-324),
+    so dt, time and the field equal a var_dt run without the hook."""
+    import libmpdataxx_amd as M
+    grid, bcs = (24, 20, 18), [(0, 0)] * 3
+    psi, gc = setup_case(grid, bcs, 57)
+    print("var_dt with an unchanged calc_gc", flush=True)
+    runs = []
+    for hook in (False, True):
+        s = M.Solver(grid, opts=M.FCT, n_iters=2, bcs=bcs, dt=0.5, var_dt=True, max_courant=0.3,
+                     max_abs_div_eps=1e300)
+        s.set_advectee(psi)
+        for d in range(3):
+            s.set_advector(gc[d], d)
+        calls = []
+        if hook:
+            s.set_calc_gc(lambda t_half: calls.append(t_half) and False)
+        s.advance_time(4.0)
+        runs.append((s.dt, s.time, s.timestep, s.advectee(), len(calls)))
+        s.close()
+    (dt0, t0, n0, a0, _), (dt1, t1, n1, a1, ncalls) = runs
+    assert ncalls == n1 > 0
+    assert dt1 == dt0 and t1 == t0 and n1 == n0
+    assert np.array_equal(a1, a0)

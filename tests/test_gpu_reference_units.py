@@ -17,7 +17,8 @@ from reference_units import (CONVERGENCE_SIZES, F32_EPS, cone_bugs_case, converg
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [(np.float32, M.GUARD_NATIVE), (np.float32, M.GUARD_F64), (np.float64, M.GUARD_F64)]
+VARIANTS = [(np.float32, M.GUARD_NATIVE), (np.float32, M.GUARD_F64), (np.float32, M.GUARD_F64_EXACT),
+            (np.float64, M.GUARD_F64)]
 PATHS = [M.PATH_FUSED, M.PATH_PERPASS]
 
 

@@ -294,3 +294,64 @@ def test_var_dt_matches_prescaled_fixed_dt():
     assert abs(_courant(v, grid) - 0.45) < 1e-12
     with pytest.raises(RuntimeError):
         v.advance(1)
+
+
+def _deform_closed_form(grid, T, courant, t_half, half=0.5, swap=False):
+    """The advector of the reversing deformational flow evaluated directly
+    from its definition, independently of the oracle's code: streamfunction
+    Psi(x, y, t) = sin^2(pi x) sin^2(pi y) cos(pi t / T) at the cell corners
+    x = (a + 1/2)/(n0 - 1), y = (b + 1/2)/(n1 - 1) (period n - 1, like the
+    cyclic grid), scaled so that the largest face value at cos = 1 is the
+    Courant number and rounded to a multiple of 2^-17; GC_x(i, j) =
+    Psi(i, j) - Psi(i, j - 1), GC_y(i, j) = Psi(i - 1, j) - Psi(i, j), GC_z = 0,
+    returned over the reference's advector views.  `half` and `swap` plant
+    the mistakes the comparison must catch (corner offset, axes)."""
+    import math
+    n0, n1 = grid[0], grid[1]
+    # periods along x and y (swap: each axis normalised by the other's extent)
+    l0, l1 = (n1 - 1, n0 - 1) if swap else (n0 - 1, n1 - 1)
+    # corners on one full period, indexed modulo n - 1
+    sx = np.array([math.sin(math.pi * (a + half) / l0) ** 2 for a in range(n0 - 1)])
+    sy = np.array([math.sin(math.pi * (b + half) / l1) ** 2 for b in range(n1 - 1)])
+    p0 = np.outer(sx, sy)                                   # unscaled corners
+    mx = max(np.abs(p0 - np.roll(p0, 1, axis=1)).max(), np.abs(np.roll(p0, 1, axis=0) - p0).max())
+    amp = courant / mx * math.cos(math.pi * t_half / T)
+    q = np.rint(amp * sx[:, None] * sy[None, :] * 131072.0) / 131072.0
+    m0, m1 = len(sx), len(sy)
+
+    def c(a, b):
+        return q[np.asarray(a) % m0][:, None] if np.ndim(b) == 0 else q[np.ix_(np.asarray(a) % m0,
+                                                                                np.asarray(b) % m1)]
+    nz = grid[2] if len(grid) == 3 else None
+    i0 = np.arange(-1, grid[0] - 1)                        # x faces -1 .. n0-2
+    j0 = np.arange(0, grid[1])
+    gx = c(i0, j0) - c(i0, j0 - 1)
+    i1 = np.arange(-1, grid[0])                            # y component: x = -1 .. n0-1
+    j1 = np.arange(0, grid[1] - 1)                         # y faces 0 .. n1-2
+    gy = c(i1 - 1, j1) - c(i1, j1)
+    if nz is None:
+        return [gx, gy]
+    return [np.repeat(gx[:, :, None], nz, axis=2), np.repeat(gy[:, :, None], nz, axis=2),
+            np.zeros((grid[0] + 1, grid[1], nz - 1))]
+
+
+@pytest.mark.parametrize("grid", [(40, 34, 12), (60, 50), (33, 21, 9)])
+def test_deform_generator_equals_closed_form(grid):
+    """The oracle's generated advector equals, bit for bit, the closed form
+    evaluated from the flow's definition (cyclic grid, after 3 steps: the
+    advector of t + dt/2 = 2.5).  Evaluated with a corner offset of a
+    instead of a + 1/2, or with x and y exchanged, the closed form differs
+    from the oracle, so either mistake in the generator would fail here."""
+    nd = len(grid)
+    bcs = [(O.CYCLIC, O.CYCLIC)] * nd
+    T, c, nt = 16.0, 0.45, 3
+    o = _deform_oracle(grid, bcs, O.FCT, 3, np.ones(grid), T, c)
+    o.advance(nt)
+    got = [o.advector(d) for d in range(nd)]
+    want = _deform_closed_form(grid, T, c, nt - 0.5)
+    for d in range(nd):
+        assert got[d].shape == want[d].shape, d
+        assert np.array_equal(got[d], want[d]), (d, np.abs(got[d] - want[d]).max())
+    for plant in (dict(half=0.0), dict(swap=True)):
+        bad = _deform_closed_form(grid, T, c, nt - 0.5, **plant)
+        assert any(not np.array_equal(got[d], bad[d]) for d in range(2)), plant
