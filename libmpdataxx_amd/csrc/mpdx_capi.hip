@@ -18,6 +18,7 @@
 #include "mpdx_solver.hpp"
 #include "mpdx_perpass.hpp"
 #include "mpdx_fused.hpp"
+#include "mpdx_reduce.hpp"
 #include "mpdx_interior_list.hpp"
 
 // the FP32 default-guard interior kernels are instantiated in mpdx_interior.hip
@@ -41,6 +42,7 @@ thread_local std::string g_create_error;
 }  // namespace
 
 
+
 #define HIPCHK(expr)                                                        \
   do {                                                                      \
     hipError_t e_ = (expr);                                                 \
@@ -54,7 +56,6 @@ thread_local std::string g_create_error;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kRedBlocks = 1024;
 
 template <class K, class... A>
 void launch(hipStream_t st, long total, K kern, A... args)
@@ -577,30 +578,56 @@ int copy_view(mpdx_solver *s, void *arr, const int lo[3], const int hi[3], void 
   return 0;
 }
 
-template <class R>
-int reduce_impl(mpdx_solver *s, const void *arr, int op, double *out)
+// reduction of kind K over the interior into out_dev (device memory),
+// block partials in part (red_parts(n0) entries)
+template <int K, class R>
+void red_launch(hipStream_t st, double2 *part, double *out_dev, const R *a, const R *g1, const R *g2,
+                const Layout &L)
 {
-  Box bi = interior_box(s->L);
-  hipLaunchKernelGGL(k_reduce<R>, dim3(kRedBlocks), dim3(kBlock), 0, s->stream, s->red_part,
-                     (const R *)arr, s->L, bi, op);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1), 0, s->stream, s->red_out, s->red_part,
-                     kRedBlocks, op);
+  const Box bi = interior_box(L);
+  const int S = red_split(bi.ext[0]);
+  const int nparts = bi.ext[0] * S;
+  hipLaunchKernelGGL((k_red_part<K, R>), dim3((unsigned)nparts), dim3(kRedThreads), 0, st, part, a, g1, g2, L, bi, S);
+  hipLaunchKernelGGL((k_red_final<K>), dim3(1), dim3(kRedFinalThreads), 0, st, out_dev, (const double2 *)part,
+                     nparts);
+}
+
+// min / max / FP64 compensated sum (op = MPDX_RED_*) of arr into out_dev
+template <class R>
+void red_field(hipStream_t st, double2 *part, double *out_dev, const R *arr, int op, const Layout &L)
+{
+  if (op == MPDX_RED_MIN) red_launch<RK_MIN, R>(st, part, out_dev, arr, arr, arr, L);
+  else if (op == MPDX_RED_MAX) red_launch<RK_MAX, R>(st, part, out_dev, arr, arr, arr, L);
+  else red_launch<RK_SUM, R>(st, part, out_dev, arr, arr, arr, L);
+}
+
+// the result of a reduction enqueued into s->red_out, on the host
+int red_result(mpdx_solver *s, double *out)
+{
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, s->red_out, sizeof(double), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
 }
 
+template <class R>
+int reduce_impl(mpdx_solver *s, const void *arr, int op, double *out)
+{
+  {
+    ProfScope ps(s, op == MPDX_RED_MIN ? "reduce_min" : (op == MPDX_RED_MAX ? "reduce_max" : "reduce_sum"));
+    red_field<R>(s->stream, s->red_part, s->red_out, (const R *)arr, op, s->L);
+  }
+  return red_result(s, out);
+}
+
 // max Courant number of the advector (solver_3d.hpp:390-398)
 template <class R>
 int courant_impl(mpdx_solver *s, double *cfl)
 {
-  if (!s->stat)
-    if (int rc = alloc_array(s, &s->stat)) return rc;
-  Box bi = interior_box(s->L);
-  blaunch(s->stream, bi, k_courant<R>, (R *)s->stat, (const R *)s->GC[0], (const R *)s->GC[1],
-          (const R *)s->GC[s->L.nd == 3 ? 2 : 0], s->L, bi);
-  return reduce_impl<R>(s, s->stat, MPDX_RED_MAX, cfl);
+  const R *g = (const R *)s->GC[0];
+  red_launch<RK_CFL, R>(s->stream, s->red_part, s->red_out, g, (const R *)s->GC[1],
+                        (const R *)s->GC[s->L.nd == 3 ? 2 : 0], s->L);
+  return red_result(s, cfl);
 }
 
 // scale_gc (solver_3d.hpp:411-421, solver_2d.hpp:315-324): faces -1/2 ..
@@ -630,9 +657,6 @@ int scale_gc(mpdx_solver *s, double cur_dt, double old_dt)
 template <class R, class G>
 int ante_loop(mpdx_solver *s)
 {
-  if (!s->stat) {
-    if (int rc = alloc_array(s, &s->stat)) return rc;
-  }
   PerPass<R, G> pp(s);
   R *Gb[3];
   R **gc = pp.av_of(s->GC, Gb);
@@ -647,11 +671,10 @@ int ante_loop(mpdx_solver *s)
       if (int rc = scale_gc<R, G>(s, s->p.dt, prev_dt)) return rc;
     }
   }
-  Box bi = interior_box(s->L);
-  blaunch(s->stream, bi, k_div<R>, (R *)s->stat, (const R *)gc[0], (const R *)gc[1],
-         (const R *)gc[s->L.nd == 3 ? 2 : 0], s->L, bi);
+  red_launch<RK_DIV, R>(s->stream, s->red_part, s->red_out, (const R *)gc[0], (const R *)gc[1],
+                        (const R *)gc[s->L.nd == 3 ? 2 : 0], s->L);
   double mx = 0;
-  if (int rc = reduce_impl<R>(s, s->stat, MPDX_RED_MAX, &mx)) return rc;
+  if (int rc = red_result(s, &mx)) return rc;
   const double eps = s->p.max_abs_div_eps < 0 ? (double)lim<R>::eps() : s->p.max_abs_div_eps;
   if ((R)mx > (R)eps) {
     s->err = "libmpdata++: initial advector field is divergent";
@@ -1576,7 +1599,7 @@ int mpdx_create(const mpdx_params *p, mpdx_solver **out)
     rc = rc || alloc_array(s, &s->psi[e][1]);
   }
   for (int d = 0; d < L.nd && !rc; ++d) rc = alloc_array(s, &s->GC[d]);
-  if (!rc) rc = dev_alloc(s, (void **)&s->red_part, sizeof(double2) * kRedBlocks);
+  if (!rc) rc = dev_alloc(s, (void **)&s->red_part, sizeof(double2) * (size_t)red_parts(L.n[0]));
   if (!rc) rc = dev_alloc(s, (void **)&s->red_out, sizeof(double));
   if (!rc) {
     const bool fused_ok = fused_supported(s->p, L);

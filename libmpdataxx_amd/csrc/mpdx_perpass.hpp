@@ -329,37 +329,6 @@ __global__ void k_scale(R *a, Layout L, Box bx, R factor)
   a[L.org + i * L.st[0] + j * L.st[1] + k * L.st[2]] *= factor;
 }
 
-// |div GC| per interior cell into stat (solver_3d.hpp:400-409, solver_2d.hpp:305-313)
-template <class R>
-__global__ void k_div(R *stat, const R *g0, const R *g1, const R *g2, Layout L, Box bx)
-{
-  int i, j, k;
-  if (!box_idx(bx, i, j, k)) return;
-  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
-  R v;
-  if (L.nd == 3)
-    v = fabs((g0[p] - g0[p - L.st[0]]) + (g1[p] - g1[p - L.st[1]]) + (g2[p] - g2[p - L.st[2]])) / 1;
-  else
-    v = fabs((g0[p] - g0[p - L.st[0]]) + (g1[p] - g1[p - L.st[1]])) / 1;
-  stat[p] = v;
-}
-
-// Courant number per interior cell into stat (solver_3d.hpp:390-398,
-// solver_2d.hpp:295-303)
-template <class R>
-__global__ void k_courant(R *stat, const R *g0, const R *g1, const R *g2, Layout L, Box bx)
-{
-  int i, j, k;
-  if (!box_idx(bx, i, j, k)) return;
-  const long p = L.org + i * L.st[0] + j * L.st[1] + k * L.st[2];
-  R v;
-  if (L.nd == 3)
-    v = R(0.5) * (fabs(g0[p] + g0[p - L.st[0]]) + fabs(g1[p] + g1[p - L.st[1]]) + fabs(g2[p] + g2[p - L.st[2]])) / 1;
-  else
-    v = R(0.5) * (fabs(g0[p] + g0[p - L.st[0]]) + fabs(g1[p] + g1[p - L.st[1]])) / 1;
-  stat[p] = v;
-}
-
 // reversing deformational flow from a streamfunction on cell corners
 // (the analytic advector of the reference's reversing_deform test family,
 // tests/mp3_paper_2018_JCP/reversing_deform/reversing_deform.hpp:30-105):
@@ -429,69 +398,6 @@ __global__ void k_copy_box(R *dst, long dorg, long ds0, long ds1, long ds2,
   int i, j, k;
   if (!box_idx(bx, i, j, k)) return;
   dst[dorg + i * ds0 + j * ds1 + k * ds2] = src[sorg + i * ss0 + j * ss1 + k * ss2];
-}
-
-// block-partial min / max / Kahan sum over an interior box; partials are
-// (value, compensation) pairs in FP64, combined in a fixed order afterwards
-template <class R>
-__global__ void k_reduce(double2 *part, const R *a, Layout L, Box bx, int op)
-{
-  __shared__ double2 sh[256];
-  double acc = op == 0 ? INFINITY : (op == 1 ? -INFINITY : 0.0), cc = 0.0;
-  // rows (a, b) block-cyclic over the grid, cells c strided over the block
-  const unsigned rows = (unsigned)bx.ext[0] * (unsigned)bx.ext[1];
-  for (unsigned row = blockIdx.x; row < rows; row += gridDim.x) {
-    const unsigned qa = fdiv(row, bx.d1);
-    const int i = (int)qa + bx.lo[0], j = (int)(row - qa * (unsigned)bx.ext[1]) + bx.lo[1];
-    const long base = L.org + i * L.st[0] + j * L.st[1];
-    for (int kk = threadIdx.x; kk < bx.ext[2]; kk += blockDim.x) {
-      const double v = double(a[base + (kk + bx.lo[2]) * L.st[2]]);
-      if (op == 0) acc = smin(acc, v);
-      else if (op == 1) acc = smax(acc, v);
-      else {
-        const double y = v - cc, s = acc + y;
-        cc = (s - acc) - y;
-        acc = s;
-      }
-    }
-  }
-  sh[threadIdx.x] = make_double2(acc, cc);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double A = op == 0 ? INFINITY : (op == 1 ? -INFINITY : 0.0), C = 0.0;
-    for (int q = 0; q < (int)blockDim.x; ++q) {
-      if (op == 0) A = smin(A, sh[q].x);
-      else if (op == 1) A = smax(A, sh[q].x);
-      else {
-        const double in2[2] = {sh[q].x, -sh[q].y};
-        for (int w = 0; w < 2; ++w) {
-          const double y = in2[w] - C, s = A + y;
-          C = (s - A) - y;
-          A = s;
-        }
-      }
-    }
-    part[blockIdx.x] = make_double2(A, C);
-  }
-}
-
-static __global__ void k_reduce_final(double *out, const double2 *part, int n, int op)
-{
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double A = op == 0 ? INFINITY : (op == 1 ? -INFINITY : 0.0), C = 0.0;
-  for (int q = 0; q < n; ++q) {
-    if (op == 0) A = smin(A, part[q].x);
-    else if (op == 1) A = smax(A, part[q].x);
-    else {
-      const double in2[2] = {part[q].x, -part[q].y};
-      for (int w = 0; w < 2; ++w) {
-        const double y = in2[w] - C, s = A + y;
-        C = (s - A) - y;
-        A = s;
-      }
-    }
-  }
-  *out = A;
 }
 
 }  // namespace mpdx

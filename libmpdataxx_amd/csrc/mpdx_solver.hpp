@@ -29,9 +29,9 @@ struct mpdx_solver {
   void *tmp[2][3] = {};
   void *flux[3] = {};
   void *mono[3] = {};
-  void *pmin = nullptr, *pmax = nullptr, *bup = nullptr, *bdn = nullptr, *stat = nullptr;
+  void *pmin = nullptr, *pmax = nullptr, *bup = nullptr, *bdn = nullptr;
   bool perpass_ready = false;
-  double2 *red_part = nullptr;
+  double2 *red_part = nullptr;   // block partials of a reduction (mpdx::red_parts(n0))
   double *red_out = nullptr;
   void *staging = nullptr;
   size_t staging_bytes = 0;
