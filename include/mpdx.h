@@ -151,6 +151,17 @@ int mpdx_advance_time(mpdx_solver *h, double duration);
 int mpdx_dt(const mpdx_solver *h, double *dt);
 
 int mpdx_reduce(mpdx_solver *h, int e, int op, double *out);
+/* No counterpart in the reference: mpdx_reduce for callers that cannot
+ * allocate or block (code under HIP graph capture, tight serving loops).
+ * Same kernels and result bit for bit, but the block partials live in a
+ * fixed 128 KiB device array per GPU (nothing is allocated), and the result
+ * is written by the GPU to out (device memory, or host memory the device can
+ * write) in the solver's stream order: the call returns without waiting.
+ * Grids with more than 8192 x planes are refused (MPDX_EINVAL).  Every call
+ * on one GPU uses the same array: calls that overlap in time on one GPU
+ * (different streams, threads or solvers) race and must be ordered by the
+ * caller. */
+int mpdx_reduce_noalloc(mpdx_solver *h, int e, int op, double *out);
 int mpdx_time(const mpdx_solver *h, double *t);
 int mpdx_timestep(const mpdx_solver *h, int64_t *ts);
 /* host flag polled once per step; set nonzero to make advance() return early.

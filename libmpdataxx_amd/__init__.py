@@ -31,7 +31,7 @@ EXPORTED = [
     "mpdx_params_default", "mpdx_create", "mpdx_destroy", "mpdx_last_error",
     "mpdx_set_stream", "mpdx_advector_extents", "mpdx_set_advectee",
     "mpdx_get_advectee", "mpdx_set_advector", "mpdx_get_advector",
-    "mpdx_advance", "mpdx_reduce", "mpdx_time", "mpdx_timestep",
+    "mpdx_advance", "mpdx_reduce", "mpdx_reduce_noalloc", "mpdx_time", "mpdx_timestep",
     "mpdx_panic_ptr", "mpdx_watch_panic", "mpdx_active_path", "mpdx_profile", "mpdx_profile_read",
     "mpdx_set_calc_gc", "mpdx_advector_device", "mpdx_set_advector_generator",
     "mpdx_set_rhs_scheme", "mpdx_set_rhs", "mpdx_rhs_device", "mpdx_advectee_device",
@@ -91,6 +91,7 @@ def lib():
         L.mpdx_advance_time.argtypes = [P, ctypes.c_double]
         L.mpdx_dt.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.mpdx_reduce.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.mpdx_reduce_noalloc.argtypes = [P, ctypes.c_int, ctypes.c_int, P]
         L.mpdx_time.argtypes = [P, ctypes.POINTER(ctypes.c_double)]
         L.mpdx_timestep.argtypes = [P, i64p]
         L.mpdx_panic_ptr.restype = ctypes.POINTER(ctypes.c_int)

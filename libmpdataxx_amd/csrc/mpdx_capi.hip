@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
@@ -41,6 +42,49 @@ thread_local std::string g_create_error;
 
 }  // namespace
 
+// Fixed scratch of mpdx_reduce_noalloc: the block partials of one reduction
+// (a (hi, lo) pair each), 8192 x 16 bytes = 128 KiB per GPU for the life of
+// the process.  Host code never uses this name as an address: each device's
+// copy is looked up once (hipGetSymbolAddress, in mpdx_create with that
+// device current) and kept in g_scratch.
+static constexpr int kScratchParts = 8192;
+static __device__ __align__(256) double2 scratch[kScratchParts];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, double2 *> g_scratch;   // device ordinal -> its scratch
+
+// the scratch of the current device, looked up on its first use
+int scratch_for_current_device(int *dev_out, std::string &err)
+{
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    err = std::string("libmpdata++: HIP error: ") + hipGetErrorString(e);
+    return MPDX_EHIP;
+  }
+  *dev_out = dev;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch.count(dev)) return MPDX_OK;
+  void *p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    err = std::string("libmpdata++: cannot locate the reduction scratch: ") + hipGetErrorString(e);
+    return MPDX_EHIP;
+  }
+  g_scratch[dev] = static_cast<double2 *>(p);
+  return MPDX_OK;
+}
+
+double2 *scratch_of(int dev)
+{
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(dev);
+  return it == g_scratch.end() ? nullptr : it->second;
+}
+
+}  // namespace
 
 
 #define HIPCHK(expr)                                                        \
@@ -1525,8 +1569,14 @@ int mpdx_create(const mpdx_params *p, mpdx_solver **out)
       return fail(MPDX_EUNSUPPORTED, "libmpdata++: grid too large for the HIP backend's index space");
   }
 
+  int device = 0;
+  {
+    std::string err;
+    if (int rc = scratch_for_current_device(&device, err)) return fail(rc, err.c_str());
+  }
   mpdx_solver *s = new mpdx_solver();
   s->p = *p;
+  s->device = device;
   s->esz = p->real_bytes;
   Layout &L = s->L;
   L.nd = p->n_dims;
@@ -1762,6 +1812,24 @@ int mpdx_reduce(mpdx_solver *s, int e, int op, double *out)
   if (e < 0 || e >= s->p.n_eqns || op < 0 || op > 2) { s->err = "libmpdata++: bad reduce arguments"; return MPDX_EINVAL; }
   if (s->esz == 8) return reduce_impl<double>(s, s->psi[e][s->lev[e]], op, out);
   return reduce_impl<float>(s, s->psi[e][s->lev[e]], op, out);
+}
+
+int mpdx_reduce_noalloc(mpdx_solver *s, int e, int op, double *out)
+{
+  MPDX_NEED(s, "solver handle");
+  MPDX_NEED(out, "output pointer");
+  if (e < 0 || e >= s->p.n_eqns || op < 0 || op > 2) { s->err = "libmpdata++: bad reduce arguments"; return MPDX_EINVAL; }
+  if (red_parts(s->L.n[0]) > kScratchParts) {
+    s->err = "libmpdata++: mpdx_reduce_noalloc: more than " + std::to_string(kScratchParts) +
+             " x planes (its fixed scratch holds " + std::to_string(kScratchParts) + " block partials)";
+    return MPDX_EINVAL;
+  }
+  double2 *part = scratch_of(s->device);
+  if (!part) { s->err = "libmpdata++: no reduction scratch for this device"; return MPDX_EHIP; }
+  if (s->esz == 8) red_field<double>(s->stream, part, out, (const double *)s->psi[e][s->lev[e]], op, s->L);
+  else red_field<float>(s->stream, part, out, (const float *)s->psi[e][s->lev[e]], op, s->L);
+  HIPCHK(hipGetLastError());
+  return MPDX_OK;
 }
 
 int mpdx_time(const mpdx_solver *s, double *t)

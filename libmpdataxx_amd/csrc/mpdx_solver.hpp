@@ -22,6 +22,7 @@ struct mpdx_solver {
   mpdx_params p;
   mpdx::Layout L;
   int esz = 8;
+  int device = 0;            // HIP device current at creation
   std::vector<void *> allocs;
   void *psi[MPDX_MAX_EQNS][2] = {};
   int lev[MPDX_MAX_EQNS] = {};
