@@ -10,7 +10,7 @@ path agrees with the FP64 CPU oracle to <= 1e-12), and in FP32 with the
 guarded ratios evaluated in FP64.  Further legs: C2 (256^3 FP64), C1 (2D
 256^2 FP64 cyclic), the multi-tracer workload (512x512x128 cyclic, 8
 tracers, time-varying deformational advector generated on the device each
-step) and the CPU baseline.  Inputs are resident in HBM before the timed
+step), the uneven ensemble of 64 domains and the CPU baseline.  Inputs are resident in HBM before the timed
 regions.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -188,6 +188,51 @@ def cpu_baseline(seconds=3.0, runs=3, grid=128):
                     "process's CPU share (OMP_NUM_THREADS) rather than every host core"}
 
 
+def run_ensemble(torch, M, steps, warmup, tune_steps=1):
+    """C5: the uneven ensemble of 64 cyclic FP32 domains (32 2D from 64^2 to
+    512^2, 32 3D from 32^3 to 128^3, n_iters 2-4, FCT on/off), autotuned,
+    then stepped as one ensemble; beside it the same members advanced one
+    after another with their own advance()."""
+    import numpy as np
+    from libmpdataxx_amd import workloads as W
+    doms = W.ensemble_domains()
+    mem = []
+    for d in doms:
+        nd = len(d["grid"])
+        s = M.Solver(d["grid"], opts=M.FCT if d["fct"] else 0, n_iters=d["n_iters"], dtype=np.float32,
+                     bcs=[(M.CYCLIC, M.CYCLIC)] * nd)
+        s.set_advectee(d["psi"])
+        for k in range(nd):
+            s.set_advector(d["gc"][k], k)
+        mem.append(s)
+    cells = sum(int(np.prod(d["grid"])) for d in doms)
+    e = M.Ensemble(mem)
+    taken = e.tune(tune_steps)
+    e.advance(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.advance(steps)
+    el = time.perf_counter() - t0
+    launches, tasks = e.stats()
+    tiles = sorted({(e.tile_y(i), e.x_chunk(i)) for i in range(len(mem)) if len(doms[i]["grid"]) == 3})
+    e.close()
+    for s in mem:
+        s.advance(warmup)
+    t0 = time.perf_counter()
+    for s in mem:
+        s.advance(steps)
+    el_seq = time.perf_counter() - t0
+    for s in mem:
+        s.close()
+    return {"value": round(cells * steps / el / 1e9, 4), "unit": "Gcell-updates/s",
+            "ms_per_step": round(el * 1e3 / steps, 4), "steps": steps, "cells": cells, "dtype": "f32",
+            "workload": "64 uneven cyclic domains (32 2D 64^2-512^2, 32 3D 32^3-128^3), n_iters 2-4, FCT on/off",
+            "batched_launches_per_step": launches, "tasks_per_step": tasks, "tuning_steps": taken,
+            "tuned_3d_tile_y_x_chunk": tiles,
+            "members_one_by_one": {"value": round(cells * steps / el_seq / 1e9, 4),
+                                   "ms_per_step": round(el_seq * 1e3 / steps, 4)}}
+
+
 def run_2d_c1(torch, M, steps, warmup, n=256):
     """C1: 2D MPDATA n_iters=2, FP64, 256^2 cyclic, uniform Courant
     (0.25, 0.25), Gaussian cone on 0.5."""
@@ -241,6 +286,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=3.0)
     ap.add_argument("--no-multi", action="store_true", help="skip the multi-tracer leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the uneven-ensemble leg")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the FP64-exact-guard, C1 (2D 256^2) and C2 (3D 256^3 FP64) legs")
     args = ap.parse_args()
@@ -355,6 +401,9 @@ def main():
     if not args.no_multi:
         print("bench: multi-tracer leg", file=sys.stderr, flush=True)
         out["multi_tracer"] = run_multi_tracer(torch, M, max(K // 2, 2), min(W, 2))
+    if not args.no_ensemble:
+        print("bench: ensemble leg", file=sys.stderr, flush=True)
+        out["ensemble"] = run_ensemble(torch, M, K, W)
     if not args.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -247,31 +247,43 @@ int mpdx_profile_read(mpdx_solver *h, int idx, char *name, int name_len,
  * do not depend on it. */
 int mpdx_set_x_chunk(mpdx_solver *h, int planes);
 
+/* Interior tile height of the 3D interior kernel: 32 (one 1024-thread block
+ * per CU) or 16 rows (two 512-thread blocks per CU) for FP32 solvers with the
+ * default guards; 0 = the default (32 in FP32; FP64 and the other guard
+ * modes always use their own default).  A tuning knob with no reference
+ * counterpart; results do not depend on it. */
+int mpdx_set_tile_y(mpdx_solver *h, int rows);
+
 /* Ensembles: independent solvers (domains of any size, dimensionality,
  * options and precision) advanced together, as the reference advances each
  * solver object with its own advance(nt) (concurr_common.hpp:380-385).  One
- * hipGraph holds one time step of every member on parallel branches and is
- * launched once per step.  Members stay owned by the caller, must outlive
- * the ensemble, and keep working on their own between ensemble calls.  They
- * must use the fused path and have no host callback (calc_gc, forcing), no
- * advector generator, no upwind filter and no var_dt.  The panic flags are
- * polled between steps: a step runs only while no member's flag is set.
- * Stepping (advance, tune) is switched off unless the environment sets
- * MPDX_ENSEMBLE_GRAPH=1 (MPDX_EUNSUPPORTED otherwise): the multi-member graph
- * has crashed the host process under the HIP runtime bundled with PyTorch. */
+ * time step of every member is recorded as the launches its own step makes
+ * (same kernels, same arguments), launches of the same kernel are merged
+ * across members into single batched launches (a device table of per-member
+ * tasks), and the batched launches are captured on the ensemble's own stream
+ * as one linear hipGraph that is replayed once per step.  Members stay owned
+ * by the caller, must outlive the ensemble, live on one device, and keep
+ * working on their own between ensemble calls (their pending work is
+ * finished before an ensemble step starts).  They must use the fused path
+ * and have no host callback (calc_gc, forcing), no advector generator, no
+ * upwind filter and no var_dt.  The panic flags are polled between steps. */
 typedef struct mpdx_ensemble mpdx_ensemble;
 int mpdx_ensemble_create(mpdx_solver *const *members, int n, mpdx_ensemble **out);
 int mpdx_ensemble_destroy(mpdx_ensemble *e);
 int mpdx_ensemble_advance(mpdx_ensemble *e, int64_t nt);
+/* Batched launches and tasks of one ensemble step (of the last step's plan) */
+int mpdx_ensemble_stats(const mpdx_ensemble *e, int *launches, int64_t *tasks);
 /* Autotuner (no reference counterpart): for each class of 3D members (x
- * extent rounded up to a power of two) tries the x-chunk lengths 0, 16, 32,
- * 64, 128, 256 (those that fit), timing steps_per_trial ensemble steps of
- * each after two untimed ones, and keeps the fastest.  The trial steps are
- * real steps of every member (use them as warm-up); *steps_taken returns
- * their number. */
+ * extent rounded up to a power of two) tries the interior tile heights
+ * (FP32 default-guard classes: 32 and 16 rows) with the x-chunk lengths 0,
+ * 16, 32, 64, 128, 256 (those that fit), timing steps_per_trial ensemble
+ * steps of each after two untimed ones, and keeps the fastest pair.  The
+ * trial steps are real steps of every member (use them as warm-up);
+ * *steps_taken returns their number. */
 int mpdx_ensemble_tune(mpdx_ensemble *e, int steps_per_trial, int64_t *steps_taken);
-/* x-chunk length chosen for member i (after mpdx_ensemble_tune) */
+/* x-chunk length and interior tile height chosen for member i */
 int mpdx_ensemble_x_chunk(const mpdx_ensemble *e, int i, int *planes);
+int mpdx_ensemble_tile_y(const mpdx_ensemble *e, int i, int *rows);
 const char *mpdx_ensemble_last_error(const mpdx_ensemble *e);
 
 #ifdef __cplusplus

@@ -35,9 +35,9 @@ EXPORTED = [
     "mpdx_panic_ptr", "mpdx_watch_panic", "mpdx_active_path", "mpdx_profile", "mpdx_profile_read",
     "mpdx_set_calc_gc", "mpdx_advector_device", "mpdx_set_advector_generator",
     "mpdx_set_rhs_scheme", "mpdx_set_rhs", "mpdx_rhs_device", "mpdx_advectee_device",
-    "mpdx_advance_time", "mpdx_dt", "mpdx_set_x_chunk",
+    "mpdx_advance_time", "mpdx_dt", "mpdx_set_x_chunk", "mpdx_set_tile_y",
     "mpdx_ensemble_create", "mpdx_ensemble_destroy", "mpdx_ensemble_advance", "mpdx_ensemble_tune",
-    "mpdx_ensemble_x_chunk", "mpdx_ensemble_last_error",
+    "mpdx_ensemble_x_chunk", "mpdx_ensemble_tile_y", "mpdx_ensemble_stats", "mpdx_ensemble_last_error",
 ]
 GC_NONE, GC_DEFORM = 0, 1
 CALC_GC_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p)
@@ -105,6 +105,9 @@ def lib():
         L.mpdx_set_rhs_scheme.argtypes = [P, ctypes.c_int, UPDATE_RHS_FN, P]
         L.mpdx_set_advector_generator.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.mpdx_set_x_chunk.argtypes = [P, ctypes.c_int]
+        L.mpdx_set_tile_y.argtypes = [P, ctypes.c_int]
+        L.mpdx_ensemble_tile_y.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.mpdx_ensemble_stats.argtypes = [P, ctypes.POINTER(ctypes.c_int), i64p]
         L.mpdx_ensemble_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(P)]
         L.mpdx_ensemble_destroy.argtypes = [P]
         L.mpdx_ensemble_advance.argtypes = [P, ctypes.c_int64]
@@ -352,10 +355,15 @@ class Solver:
         """x-chunk length of the 3D interior kernel (0 = automatic)."""
         self._check(self._L.mpdx_set_x_chunk(self._h, int(planes)))
 
+    def set_tile_y(self, rows):
+        """Interior tile height (0 = default, 16 or 32; FP32 default guards)."""
+        self._check(self._L.mpdx_set_tile_y(self._h, int(rows)))
+
 
 class Ensemble:
-    """Independent solvers advanced together by one captured step per
-    launch (mpdx_ensemble_*); the members stay usable on their own."""
+    """Independent solvers advanced together: each step is one graph of
+    batched launches over all members (mpdx_ensemble_*); the members stay
+    usable on their own."""
 
     def __init__(self, members):
         L = lib()
@@ -386,6 +394,17 @@ class Ensemble:
         v = ctypes.c_int()
         self._check(self._L.mpdx_ensemble_x_chunk(self._h, int(i), ctypes.byref(v)))
         return v.value
+
+    def tile_y(self, i):
+        v = ctypes.c_int()
+        self._check(self._L.mpdx_ensemble_tile_y(self._h, int(i), ctypes.byref(v)))
+        return v.value
+
+    def stats(self):
+        """(batched launches, tasks) of one ensemble step."""
+        n, t = ctypes.c_int(), ctypes.c_int64()
+        self._check(self._L.mpdx_ensemble_stats(self._h, ctypes.byref(n), ctypes.byref(t)))
+        return n.value, t.value
 
     def close(self):
         if getattr(self, "_h", None):

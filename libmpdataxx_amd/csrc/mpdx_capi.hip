@@ -14,17 +14,22 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mpdx_solver.hpp"
 #include "mpdx_perpass.hpp"
 #include "mpdx_fused.hpp"
 #include "mpdx_reduce.hpp"
+#include "mpdx_tasks.hpp"
 #include "mpdx_interior_list.hpp"
 
 // the FP32 default-guard interior kernels are instantiated in mpdx_interior.hip
 namespace mpdx {
-#define MPDX_EXTERN(R, G, F, I, S, Z) extern template __global__ void k_fz_interior<R, G, F, I, S, Z>(InteriorArgs<R>);
+#define MPDX_EXTERN(R, G, F, I, S, Z, T)                                                       \
+  extern template __global__ void k_fz_interior<R, G, F, I, S, Z, T>(InteriorArgs<R>);         \
+  extern template __global__ void k_task_list<InteriorTask<R, G, F, I, S, Z, T>>(              \
+      const InteriorTask<R, G, F, I, S, Z, T> *, const unsigned *, int);
 MPDX_INTERIOR_F32_FAST(MPDX_EXTERN)
 #undef MPDX_EXTERN
 }  // namespace mpdx
@@ -139,6 +144,36 @@ void pblaunch(mpdx_solver *s, const char *name, const Box &bx, K kern, A... args
   blaunch(s->stream, bx, kern, args...);
 }
 
+template <class T>
+void launch_task_list(hipStream_t st, const void *tasks, const unsigned *first, int n, unsigned blocks)
+{
+  hipLaunchKernelGGL(k_task_list<T>, dim3(blocks), dim3(T::kThreads / T::kBlockY, T::kBlockY), 0, st,
+                     static_cast<const T *>(tasks), first, n);
+}
+
+// records task t, whose own launch has grid g, into the solver's recorder
+template <class T>
+void record(mpdx_solver *s, T t, dim3 g)
+{
+  t.nx = g.x;
+  t.ny = g.y;
+  t.nz = g.z;
+  if (t.blocks() == 0) return;
+  RecOp &op = s->rec->next();
+  op.fn = &launch_task_list<T>;
+  op.tsize = sizeof(T);
+  op.blocks = t.blocks();
+  op.bytes.resize(sizeof(T));
+  std::memcpy(op.bytes.data(), &t, sizeof(T));
+}
+
+// a group of mutually independent recorded launches (no-op when launching)
+struct RecGroup {
+  mpdx_solver *s;
+  explicit RecGroup(mpdx_solver *s_) : s(s_) { if (s->rec) s->rec->begin(); }
+  ~RecGroup() { if (s->rec) s->rec->end(); }
+};
+
 rng interior(const mpdx_solver *s, int d) { return R_(0, s->L.n[d] - 1); }
 
 Box interior_box(const Layout &L)
@@ -203,6 +238,17 @@ struct PerPass {
   void flush_fills()
   {
     if (fb.n == 0) return;
+    if (s->rec) {
+      RecGroup grp(s);
+      for (int q = 0; q < fb.n; ++q) {
+        FillTask<R> t{};
+        t.a = fb.a[q]; t.r = fb.r[q]; t.bx = fb.bx[q];
+        t.side = fb.side[q]; t.mode = fb.mode[q]; t.N = fb.N[q]; t.ord = fb.ord[q];
+        record(s, t, box_grid(fb.bx[q]));
+      }
+      fb.n = 0;
+      return;
+    }
     unsigned gx = 0, gy = 0;
     for (int q = 0; q < fb.n; ++q) {
       gx = std::max(gx, box_grid(fb.bx[q]).x);
@@ -233,8 +279,18 @@ struct PerPass {
     // (the perpendicular-component zeroing does nothing along z)
     if (nopen > 0 && !(nd == 3 && d == 2)) {
       const long nz = std::max(ra.l - ra.f + 1, rb.l - rb.f + 1);
-      hipLaunchKernelGGL(k_alng_zero<R>, dim3((unsigned)((nz + kBlock - 1) / kBlock), nopen), dim3(kBlock), 0,
-                         s->stream, av1, av2, r, nd, d, eopen[0], eopen[nopen - 1], ra.f, ra.l, rb.f, rb.l);
+      if (s->rec) {
+        RecGroup grp(s);
+        for (int q = 0; q < nopen; ++q) {
+          AlngZeroTask<R> t{};
+          t.av1 = av1; t.av2 = av2; t.r = r; t.nd = nd; t.d = d; t.e = eopen[q];
+          t.raf = ra.f; t.ral = ra.l; t.rbf = rb.f; t.rbl = rb.l;
+          record(s, t, dim3((unsigned)((nz + kBlock - 1) / kBlock)));
+        }
+      } else {
+        hipLaunchKernelGGL(k_alng_zero<R>, dim3((unsigned)((nz + kBlock - 1) / kBlock), nopen), dim3(kBlock), 0,
+                           s->stream, av1, av2, r, nd, d, eopen[0], eopen[nopen - 1], ra.f, ra.l, rb.f, rb.l);
+      }
     }
     // the faces themselves: the thread index runs along the box's c axis,
     // which for the y faces is x under the rotation (y, z, x); swap the
@@ -268,10 +324,21 @@ struct PerPass {
         for (const auto &b : bs)
           v.emplace_back(make_box(bx[side].lo[0], bx[side].lo[0] + bx[side].ext[0] - 1, b[0].f, b[0].l, b[1].f,
                                   b[1].l), side);
+      if (s->rec) {
+        RecGroup grp(s);
+        for (const auto &bt : v) record_alng(avd, av1, av2, r, bt.first, nd, bt.second, cyc[bt.second], N, cfirst);
+        return;
+      }
       boxlist_launch(v, [&](const BoxList &bl) {
         hipLaunchKernelGGL(k_alng_list<R>, dim3(boxlist_blocks(bl)), dim3(kBlock), 0, s->stream, avd,
                            (const R *)av1, (const R *)av2, r, bl, nd, cyc[0], cyc[1], N, cfirst);
       });
+      return;
+    }
+    if (s->rec) {
+      RecGroup grp(s);
+      for (int side = 0; side < 2; ++side)
+        if (bx[side].total > 0) record_alng(avd, av1, av2, r, bx[side], nd, side, cyc[side], N, cfirst);
       return;
     }
     const unsigned gx = std::max(box_grid(bx[0]).x, box_grid(bx[1]).x);
@@ -279,6 +346,15 @@ struct PerPass {
     if (bx[0].total > 0 || bx[1].total > 0)
       hipLaunchKernelGGL(k_alng2<R>, dim3(gx, gy, 2), dim3(kBlock), 0, s->stream, avd, (const R *)av1,
                          (const R *)av2, r, bx[0], bx[1], nd, cyc[0], cyc[1], N, cfirst);
+  }
+
+  void record_alng(R *avd, R *av1, R *av2, const Rot &r, const Box &bx, int nd, int side, int cyc, int N,
+                   int cfirst)
+  {
+    AlngTask<R> t{};
+    t.avd = avd; t.av1 = av1; t.av2 = av2; t.r = r; t.bx = bx;
+    t.nd = nd; t.side = side; t.cyc = cyc; t.N = N; t.cfirst = cfirst;
+    record(s, t, box_grid(bx));
   }
 
   // solver_3d.hpp:47-67, solver_2d.hpp:40-56
@@ -786,6 +862,15 @@ struct Fused {
         }
       }
     }
+    if (s->rec) {
+      RecGroup grp(s);
+      for (const auto &bt : v) {
+        HaloTask<R> t{};
+        t.a = a; t.L = L; t.m = m; t.bx = bt.first;
+        record(s, t, box_grid(bt.first));
+      }
+      return;
+    }
     boxlist_launch(v, [&](const BoxList &bl) {
       hipLaunchKernelGGL(k_fz_halo<R>, dim3(boxlist_blocks(bl)), dim3(256), 0, s->stream, a, L, m, bl);
     });
@@ -809,9 +894,20 @@ struct Fused {
   void donor_boxes(R *out, R *in, R **C, R in_scale, R out_scale, int flux_is_C, const std::vector<Box> &bxs)
   {
     R *c2 = L.nd == 3 ? C[2] : C[0];
+    RecGroup grp(s);
     for (const Box &bx : bxs) {
       if (bx.total <= 0) continue;
       const dim3 g = box_grid(bx);
+      if (s->rec) {
+        auto fill = [&](auto t) {
+          t.out = out; t.in = in; t.C0 = C[0]; t.C1 = C[1]; t.C2 = c2; t.L = L;
+          t.in_scale = in_scale; t.out_scale = out_scale; t.flux_is_C = flux_is_C; t.bx = bx;
+          record(s, t, g);
+        };
+        if (o & O_NPA) fill(DonorTask<R, true>{});
+        else fill(DonorTask<R, false>{});
+        continue;
+      }
       if (o & O_NPA)
         hipLaunchKernelGGL((k_fz_donor<R, true>), g, dim3(256), 0, s->stream, out, (const R *)in,
                            (const R *)C[0], (const R *)C[1], (const R *)c2, L, in_scale, out_scale, flux_is_C, bx);
@@ -832,9 +928,20 @@ struct Fused {
   template <bool IGA, bool ABS, bool PFC>
   void antidiff_t(R **V, R *psi, R **U, const std::vector<Box> &bxs)
   {
+    RecGroup grp(s);
     for (const Box &bx : bxs) {
       if (bx.total <= 0) continue;
       const dim3 g = box_grid(bx);
+      if (s->rec) {
+        auto fill = [&](auto t, int nd) {
+          t.V0 = V[0]; t.V1 = V[1]; t.V2 = nd == 3 ? V[2] : V[0]; t.psi = psi;
+          t.U0 = U[0]; t.U1 = U[1]; t.U2 = nd == 3 ? U[2] : U[0]; t.L = L; t.bx = bx;
+          record(s, t, g);
+        };
+        if (L.nd == 3) fill(AntidiffTask<R, G, 3, IGA, ABS, PFC>{}, 3);
+        else fill(AntidiffTask<R, G, 2, IGA, ABS, PFC>{}, 2);
+        continue;
+      }
       if (L.nd == 3)
         hipLaunchKernelGGL((k_fz_antidiff<R, G, 3, IGA, ABS, PFC>), g, dim3(256), 0, s->stream,
                            V[0], V[1], V[2], (const R *)psi, (const R *)U[0], (const R *)U[1],
@@ -861,6 +968,13 @@ struct Fused {
   {
     constexpr int TX = FzBrick<ND, SHAPE>::TX, TY = FzBrick<ND, SHAPE>::TY, TZ = FzBrick<ND, SHAPE>::TZ;
     dim3 g(ND == 3 ? (bx.ext[2] + TZ - 1) / TZ : 1, (bx.ext[1] + TY - 1) / TY, (bx.ext[0] + TX - 1) / TX);
+    if (s->rec) {
+      FctTask<R, G, ND, SHAPE, NPA, ABS, IGA> t{};
+      t.out = out; t.psi = psi; t.psin = psin; t.V0 = V[0]; t.V1 = V[1]; t.V2 = V[ND == 3 ? 2 : 0];
+      t.L = L; t.in_scale = in_scale; t.out_scale = out_scale; t.bx = bx;
+      record(s, t, g);
+      return;
+    }
     hipLaunchKernelGGL((k_fz_fct<R, G, ND, SHAPE, NPA, ABS, IGA>), g, dim3(TX * TY * TZ), 0, s->stream, out,
                        (const R *)psi, (const R *)psin, (const R *)V[0], (const R *)V[1],
                        (const R *)V[ND == 3 ? 2 : 0], L, in_scale, out_scale, bx);
@@ -938,8 +1052,28 @@ struct Fused {
     return make_box(f, L.n[0] - 1, f, L.n[1] - 1, L.nd == 3 ? f : 0, L.nd == 3 ? L.n[2] - 1 : 0);
   }
 
+  // the interior tile height: the solver's choice for the FP32 default-guard
+  // kernels (16 or 32), the element type's default otherwise
+  static constexpr bool kTileChoice = std::is_same<R, float>::value && std::is_same<G, GuardFast>::value;
+  int tile_y() const
+  {
+    if (kTileChoice && s->tile_y == 16) return 16;
+    return itile_y_default<R>();
+  }
+
   template <bool FCT, bool IGA, bool STOREV, bool FUSE0>
   void interior_t(R *out, R *psi, R *psin, R **U, R **V, R in_scale, R out_scale)
+  {
+    if constexpr (kTileChoice) {
+      if (tile_y() == 16) return interior_ty<FCT, IGA, STOREV, FUSE0, 16>(out, psi, psin, U, V, in_scale, out_scale);
+      return interior_ty<FCT, IGA, STOREV, FUSE0, 32>(out, psi, psin, U, V, in_scale, out_scale);
+    } else {
+      return interior_ty<FCT, IGA, STOREV, FUSE0, itile_y_default<R>()>(out, psi, psin, U, V, in_scale, out_scale);
+    }
+  }
+
+  template <bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV>
+  void interior_ty(R *out, R *psi, R *psin, R **U, R **V, R in_scale, R out_scale)
   {
     InteriorArgs<R> a;
     a.out = out; a.psi = psi; a.psin = psin;
@@ -954,7 +1088,7 @@ struct Fused {
       a.vlo[d] = 4; a.vhi[d] = L.n[d] - 5;
     }
     a.in_scale = in_scale; a.out_scale = out_scale;
-    constexpr int TY = ITile<R>::Y, TZ = ITile<R>::Z;
+    constexpr int TY = ITile<TYV>::Y, TZ = ITile<TYV>::Z;
     // zb: the tiles cover every z cell (and store the velocity of every z
     // face of the domain, face -1/2 included); the last tile is shifted
     // back to end on the last halo column
@@ -978,7 +1112,13 @@ struct Fused {
     a.nb[2] = (L.n[0] - 4 + a.chunk - 1) / a.chunk;
     const int ntile = a.nb[0] * a.nb[1] * a.nb[2];
     const dim3 g(8 * ((ntile + 7) / 8));   // tiles are spread over the 8 XCDs in the kernel
-    hipLaunchKernelGGL((k_fz_interior<R, G, FCT, IGA, STOREV, FUSE0>), g, dim3(TZ, TY), 0, s->stream, a);
+    if (s->rec) {
+      InteriorTask<R, G, FCT, IGA, STOREV, FUSE0, TYV> t{};
+      t.a = a;
+      record(s, t, g);
+      return;
+    }
+    hipLaunchKernelGGL((k_fz_interior<R, G, FCT, IGA, STOREV, FUSE0, TYV>), g, dim3(TZ, TY), 0, s->stream, a);
   }
 
   // psi: psi^k, or unused with fuse0 (psi^1 computed from psin and U = GC)
@@ -1004,7 +1144,10 @@ struct Fused {
   // writes, so with an interior kernel the shell runs on the auxiliary
   // stream next to it (fork / join through events; a captured step keeps
   // the same dependencies).  Kernel timing runs everything on one stream.
-  bool concurrent() const { return s->aux && s->ev_fork && s->ev_join && s->use_concurrency && !s->prof_on; }
+  bool concurrent() const
+  {
+    return s->aux && s->ev_fork && s->ev_join && s->use_concurrency && !s->prof_on && !s->rec;
+  }
 
   void fork()
   {
@@ -1052,9 +1195,10 @@ struct Fused {
         pp.xchng_vctr_nrml(V, 0);
       }
       ProfScope ps(s, "fused_fct");
-      if (use_int)
+      if (use_int) {
+        RecGroup grp(s);
         for (const Box &b : shell(0, 2, 2, true)) fct_box(out, psik, psin, V, in_scale, os, b);
-      else
+      } else
         fct_box(out, psik, psin, V, in_scale, os, full_box(0));
     } else {
       if (it != passes - 1) {
@@ -1371,17 +1515,39 @@ int step_dispatch(mpdx_solver *s)
 
 // ----------------------------------------------------------------- ensembles
 
-// Independent solvers stepped together: one captured hipGraph per state of
-// the members' time levels (two alternate), each holding one fused step of
-// every member on its own branch (fork / join events off the ensemble's
-// stream).
+// Independent solvers stepped together.  One time step of every member is
+// recorded as tasks (the launches its own step would make, with its own
+// arrays, boxes and scales), the tasks are merged across members into
+// batched launches of k_task_list (one launch per kernel and round of the
+// schedule below), and the batched launches are captured on the ensemble's
+// stream as one linear hipGraph, replayed once per step.  A plan is keyed by
+// the members' time levels; two alternate.
+namespace {
+
+struct EnsBatch {
+  TaskLaunchFn fn;
+  size_t toff, foff;   // byte offsets of the tasks and of their block prefix table
+  int n;
+  unsigned blocks;
+};
+
+struct EnsPlan {
+  std::vector<int> key;              // members' time levels the plan was recorded at
+  void *dev = nullptr;               // tasks and prefix tables of all batches
+  std::vector<EnsBatch> batches;
+  hipGraphExec_t graph = nullptr;
+  size_t tasks = 0;
+};
+
+}  // namespace
+
 struct mpdx_ensemble {
   std::vector<mpdx_solver *> m;
   hipStream_t stream = nullptr;
-  std::vector<hipEvent_t> fork, join;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
-  std::vector<int> key[2];       // members' time levels each graph was captured for
+  EnsPlan plan[2];
   int next_slot = 0;
+  static constexpr int kInflight = 6;
+  hipEvent_t inflight[kInflight] = {};
   std::string err;
 };
 
@@ -1405,13 +1571,16 @@ std::vector<int> level_key(const mpdx_ensemble *e)
   return k;
 }
 
-void drop_graphs(mpdx_ensemble *e)
+void free_plan(EnsPlan &p)
 {
-  for (int q = 0; q < 2; ++q) {
-    if (e->graph[q]) (void)hipGraphExecDestroy(e->graph[q]);
-    e->graph[q] = nullptr;
-    e->key[q].clear();
-  }
+  if (p.graph) (void)hipGraphExecDestroy(p.graph);
+  if (p.dev) (void)hipFree(p.dev);
+  p = EnsPlan();
+}
+
+void drop_plans(mpdx_ensemble *e)
+{
+  for (EnsPlan &p : e->plan) free_plan(p);
   e->next_slot = 0;
 }
 
@@ -1421,41 +1590,129 @@ int member_error(mpdx_ensemble *e, size_t i, const mpdx_solver *s, int rc)
   return rc;
 }
 
-// one step of every member, captured on the ensemble's stream
-int capture_step(mpdx_ensemble *e, hipGraphExec_t *out)
+// The merge: every member's recorded step is a sequence of groups of
+// mutually independent launches.  Each round takes the kernel that heads
+// the most members' current groups and launches all of its pending tasks in
+// those groups as one batch; a member moves to its next group once its
+// current one is done.  Each member's groups therefore run in order, and a
+// batch holds tasks of different members (or independent tasks of one).
+void schedule(const std::vector<TaskRecorder> &recs, std::vector<std::vector<const RecOp *>> &out)
 {
-  std::vector<int> saved = level_key(e);
-  hipGraph_t g = nullptr;
-  EHIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  const size_t n = recs.size();
+  std::vector<size_t> g(n, 0);
+  std::vector<std::vector<char>> taken(n);
+  auto settle = [&](size_t i) {
+    while (g[i] < recs[i].groups.size()) {
+      const auto &grp = recs[i].groups[g[i]];
+      taken[i].resize(grp.size(), 0);
+      bool done = true;
+      for (size_t q = 0; q < grp.size(); ++q) done = done && taken[i][q];
+      if (!done) return;
+      ++g[i];
+      taken[i].clear();
+    }
+  };
+  for (size_t i = 0; i < n; ++i) settle(i);
+  for (;;) {
+    std::vector<std::pair<TaskLaunchFn, int>> count;   // kernel, members whose head holds it
+    for (size_t i = 0; i < n; ++i) {
+      if (g[i] >= recs[i].groups.size()) continue;
+      const auto &grp = recs[i].groups[g[i]];
+      std::vector<TaskLaunchFn> seen;
+      for (size_t q = 0; q < grp.size(); ++q) {
+        if (taken[i][q] || std::find(seen.begin(), seen.end(), grp[q].fn) != seen.end()) continue;
+        seen.push_back(grp[q].fn);
+        auto it = std::find_if(count.begin(), count.end(), [&](const auto &c) { return c.first == grp[q].fn; });
+        if (it == count.end()) count.emplace_back(grp[q].fn, 1);
+        else ++it->second;
+      }
+    }
+    if (count.empty()) break;
+    TaskLaunchFn best = count[0].first;
+    int most = count[0].second;
+    for (const auto &c : count)
+      if (c.second > most) { best = c.first; most = c.second; }
+    std::vector<const RecOp *> batch;
+    for (size_t i = 0; i < n; ++i) {
+      if (g[i] >= recs[i].groups.size()) continue;
+      const auto &grp = recs[i].groups[g[i]];
+      for (size_t q = 0; q < grp.size(); ++q)
+        if (!taken[i][q] && grp[q].fn == best) {
+          batch.push_back(&grp[q]);
+          taken[i][q] = 1;
+        }
+      settle(i);
+    }
+    out.push_back(std::move(batch));
+  }
+}
+
+// records one step of every member, merges it into batches, uploads the
+// tasks and captures the batched launches as one graph
+int build_plan(mpdx_ensemble *e, EnsPlan &p)
+{
+  const std::vector<int> saved = level_key(e);
+  std::vector<TaskRecorder> recs(e->m.size());
   int rc = 0;
+  size_t fail_at = 0;
   for (size_t i = 0; i < e->m.size() && !rc; ++i) {
     mpdx_solver *s = e->m[i];
-    (void)hipEventRecord(e->fork[i], e->stream);
-    (void)hipStreamWaitEvent(s->stream, e->fork[i], 0);
+    s->rec = &recs[i];
     rc = step_dispatch(s);
-    if (rc) {
-      rc = member_error(e, i, s, rc);
-      break;
-    }
-    (void)hipEventRecord(e->join[i], s->stream);
-    (void)hipStreamWaitEvent(e->stream, e->join[i], 0);
+    s->rec = nullptr;
+    fail_at = i;
   }
-  hipError_t ec = hipStreamEndCapture(e->stream, &g);
   size_t q = 0;
   for (mpdx_solver *s : e->m)
     for (int r = 0; r < s->p.n_eqns; ++r) s->lev[r] = saved[q++];
-  if (rc) {
-    if (g) (void)hipGraphDestroy(g);
-    return rc;
+  if (rc) return member_error(e, fail_at, e->m[fail_at], rc);
+
+  std::vector<std::vector<const RecOp *>> batches;
+  schedule(recs, batches);
+  // pack: per batch its tasks, then its prefix table, each 256-byte aligned
+  std::vector<unsigned char> blob;
+  auto align = [&]() { blob.resize((blob.size() + 255) / 256 * 256); };
+  for (const auto &b : batches) {
+    EnsBatch eb;
+    eb.fn = b[0]->fn;
+    eb.n = (int)b.size();
+    align();
+    eb.toff = blob.size();
+    for (const RecOp *op : b) blob.insert(blob.end(), op->bytes.begin(), op->bytes.end());
+    align();
+    eb.foff = blob.size();
+    unsigned first = 0;
+    for (const RecOp *op : b) {
+      const unsigned char *f = reinterpret_cast<const unsigned char *>(&first);
+      blob.insert(blob.end(), f, f + sizeof(unsigned));
+      first += op->blocks;
+    }
+    eb.blocks = first;
+    p.batches.push_back(eb);
+    p.tasks += b.size();
   }
+  if (blob.empty()) blob.resize(256);
+  EHIPCHK(hipMalloc(&p.dev, blob.size()));
+  EHIPCHK(hipMemcpy(p.dev, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  hipGraph_t graph = nullptr;
+  EHIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  char *dev = static_cast<char *>(p.dev);
+  for (const EnsBatch &b : p.batches)
+    b.fn(e->stream, dev + b.toff, reinterpret_cast<const unsigned *>(dev + b.foff), b.n, b.blocks);
+  const hipError_t el = hipGetLastError();
+  const hipError_t ec = hipStreamEndCapture(e->stream, &graph);
+  EHIPCHK(el);
   EHIPCHK(ec);
-  hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
+  const hipError_t ei = hipGraphInstantiate(&p.graph, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
   EHIPCHK(ei);
+  p.key = saved;
   return 0;
 }
 
-// first-advance work of members that have not stepped yet (hook_ante_loop)
+// first-advance work of members that have not stepped yet (hook_ante_loop),
+// and every member's own pending work finished before the ensemble's stream
+// touches its arrays
 int ensemble_ante(mpdx_ensemble *e)
 {
   for (size_t i = 0; i < e->m.size(); ++i) {
@@ -1470,23 +1727,25 @@ int ensemble_ante(mpdx_ensemble *e)
   return 0;
 }
 
-// launches one ensemble step (capturing the graph for the current levels
-// when needed) and advances every member's clock
+// launches one ensemble step (building the plan for the current levels when
+// needed) and advances every member's clock
 int ensemble_step(mpdx_ensemble *e)
 {
-  std::vector<int> k = level_key(e);
+  const std::vector<int> k = level_key(e);
   int slot = -1;
   for (int q = 0; q < 2; ++q)
-    if (e->graph[q] && e->key[q] == k) slot = q;
+    if (e->plan[q].graph && e->plan[q].key == k) slot = q;
   if (slot < 0) {
     slot = e->next_slot;
     e->next_slot ^= 1;
-    if (e->graph[slot]) (void)hipGraphExecDestroy(e->graph[slot]);
-    e->graph[slot] = nullptr;
-    if (int rc = capture_step(e, &e->graph[slot])) return rc;
-    e->key[slot] = k;
+    EHIPCHK(hipStreamSynchronize(e->stream));   // the slot's old graph may still run
+    free_plan(e->plan[slot]);
+    if (int rc = build_plan(e, e->plan[slot])) {
+      free_plan(e->plan[slot]);
+      return rc;
+    }
   }
-  EHIPCHK(hipGraphLaunch(e->graph[slot], e->stream));
+  EHIPCHK(hipGraphLaunch(e->plan[slot].graph, e->stream));
   for (mpdx_solver *s : e->m) {
     for (int r = 0; r < s->p.n_eqns; ++r) s->lev[r] ^= 1;
     s->timestep++;
@@ -1498,8 +1757,28 @@ int ensemble_step(mpdx_ensemble *e)
 bool ensemble_panic(const mpdx_ensemble *e)
 {
   for (const mpdx_solver *s : e->m)
-    if (s->panic) return true;
+    if (panicked(s)) return true;
   return false;
+}
+
+// nt steps with at most kInflight of them queued ahead of the host (panic
+// flags are polled between steps)
+int ensemble_run(mpdx_ensemble *e, int64_t nt)
+{
+  if (!e->inflight[0])
+    for (hipEvent_t &ev : e->inflight)
+      if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        ev = nullptr;
+        (void)hipGetLastError();
+      }
+  for (int64_t t = 0; t < nt; ++t) {
+    if (t >= mpdx_ensemble::kInflight)
+      if (hipEvent_t ev = e->inflight[t % mpdx_ensemble::kInflight]) EHIPCHK(hipEventSynchronize(ev));
+    if (ensemble_panic(e)) break;
+    if (int rc = ensemble_step(e)) return rc;
+    if (hipEvent_t ev = e->inflight[t % mpdx_ensemble::kInflight]) EHIPCHK(hipEventRecord(ev, e->stream));
+  }
+  return 0;
 }
 
 }  // namespace
@@ -2033,9 +2312,22 @@ int mpdx_set_x_chunk(mpdx_solver *s, int planes)
   return MPDX_OK;
 }
 
+int mpdx_set_tile_y(mpdx_solver *s, int rows)
+{
+  MPDX_NEED(s, "solver handle");
+  if (rows != 0 && rows != 16 && rows != 32) {
+    s->err = "libmpdata++: interior tile height must be 0 (default), 16 or 32";
+    return MPDX_EINVAL;
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->tile_y = rows;
+  drop_step_graphs(s);              // captured steps hold the old kernels
+  return MPDX_OK;
+}
+
 int mpdx_ensemble_create(mpdx_solver *const *members, int n, mpdx_ensemble **out)
 {
-  if (!out) return MPDX_EINVAL;
+  MPDX_NEED(out, "output pointer");
   *out = nullptr;
   auto reject = [&](const std::string &msg) {
     g_create_error = "libmpdata++: " + msg;
@@ -2053,18 +2345,15 @@ int mpdx_ensemble_create(mpdx_solver *const *members, int n, mpdx_ensemble **out
       return reject(who + " has a calc_gc hook, forcing or an advector generator");
     if (s->p.upwind_filter_freq > 0 || s->p.var_dt) return reject(who + " uses upwind_filter_freq or var_dt");
     if (!s->stream) return reject(who + " runs on the legacy null stream");
+    if (s->device != members[0]->device) return reject(who + " lives on another device");
   }
   mpdx_ensemble *e = new mpdx_ensemble;
   e->m.assign(members, members + n);
-  bool ok = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess;
-  e->fork.assign(n, nullptr);
-  e->join.assign(n, nullptr);
-  for (int i = 0; i < n && ok; ++i)
-    ok = hipEventCreateWithFlags(&e->fork[i], hipEventDisableTiming) == hipSuccess &&
-         hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) == hipSuccess;
-  if (!ok) {
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    e->stream = nullptr;
     mpdx_ensemble_destroy(e);
-    g_create_error = "libmpdata++: cannot create the ensemble's stream / events";
+    g_create_error = "libmpdata++: cannot create the ensemble's stream";
     return MPDX_EHIP;
   }
   *out = e;
@@ -2075,11 +2364,9 @@ int mpdx_ensemble_destroy(mpdx_ensemble *e)
 {
   if (!e) return MPDX_OK;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
-  drop_graphs(e);
-  for (hipEvent_t v : e->fork)
-    if (v) (void)hipEventDestroy(v);
-  for (hipEvent_t v : e->join)
-    if (v) (void)hipEventDestroy(v);
+  drop_plans(e);
+  for (hipEvent_t ev : e->inflight)
+    if (ev) (void)hipEventDestroy(ev);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return MPDX_OK;
@@ -2090,31 +2377,23 @@ const char *mpdx_ensemble_last_error(const mpdx_ensemble *e)
   return e ? e->err.c_str() : g_create_error.c_str();
 }
 
-// The multi-member graph crashed the host process (SIGSEGV inside the first
-// advance) in the runs where the process used the HIP runtime bundled with
-// PyTorch, and ran correctly with /opt/rocm's; until the cause is known the
-// stepping entry points refuse unless MPDX_ENSEMBLE_GRAPH=1 is set.
-static bool ensemble_enabled(mpdx_ensemble *e)
-{
-  const char *v = std::getenv("MPDX_ENSEMBLE_GRAPH");
-  if (v && std::strcmp(v, "1") == 0) return true;
-  e->err = "libmpdata++: ensemble stepping is switched off (set MPDX_ENSEMBLE_GRAPH=1 to enable it; "
-           "it has crashed the host process under some HIP runtimes)";
-  return false;
-}
-
 int mpdx_ensemble_advance(mpdx_ensemble *e, int64_t nt)
 {
   MPDX_NEED(e, "ensemble handle");
   if (nt < 0) { e->err = "libmpdata++: negative number of steps"; return MPDX_EINVAL; }
-  if (!ensemble_enabled(e)) return MPDX_EUNSUPPORTED;
   if (nt == 0) return MPDX_OK;
   if (int rc = ensemble_ante(e)) return rc;
-  for (int64_t t = 0; t < nt; ++t) {
-    if (ensemble_panic(e)) break;
-    if (int rc = ensemble_step(e)) return rc;
-  }
+  if (int rc = ensemble_run(e, nt)) return rc;
   EHIPCHK(hipStreamSynchronize(e->stream));
+  return MPDX_OK;
+}
+
+int mpdx_ensemble_stats(const mpdx_ensemble *e, int *launches, int64_t *tasks)
+{
+  MPDX_NEED(e, "ensemble handle");
+  const EnsPlan &p = e->plan[0].graph ? e->plan[0] : e->plan[1];
+  if (launches) *launches = (int)p.batches.size();
+  if (tasks) *tasks = (int64_t)p.tasks;
   return MPDX_OK;
 }
 
@@ -2123,12 +2402,12 @@ int mpdx_ensemble_tune(mpdx_ensemble *e, int steps_per_trial, int64_t *steps_tak
   MPDX_NEED(e, "ensemble handle");
   if (steps_taken) *steps_taken = 0;
   if (steps_per_trial < 1) { e->err = "libmpdata++: steps_per_trial must be >= 1"; return MPDX_EINVAL; }
-  if (!ensemble_enabled(e)) return MPDX_EUNSUPPORTED;
   if (int rc = ensemble_ante(e)) return rc;
-  // classes: 3D members by x extent rounded up to a power of two
+  // classes: 3D members by x extent rounded up to a power of two; the FP32
+  // default-guard ones may take either interior tile height
   std::map<int, std::vector<mpdx_solver *>> cls;
   for (mpdx_solver *s : e->m)
-    if (s->L.nd == 3) {
+    if (s->L.nd == 3 && interior_supported(s->p, s->L)) {
       int b = 16;
       while (b < s->L.n[0] - 4) b *= 2;
       cls[b].push_back(s);
@@ -2143,15 +2422,24 @@ int mpdx_ensemble_tune(mpdx_ensemble *e, int steps_per_trial, int64_t *steps_tak
   int rc = 0;
   int64_t taken = 0;
   for (auto &kv : cls) {
-    int best = kv.second.front()->ix_chunk;
+    bool tiles = true;
+    for (mpdx_solver *s : kv.second) tiles = tiles && s->esz == 4 && s->p.guard == MPDX_GUARD_F64;
+    std::vector<std::pair<int, int>> trials;   // (tile height, x chunk)
+    for (int ty : tiles ? std::vector<int>{0, 16} : std::vector<int>{0})
+      for (int c : {0, 16, 32, 64, 128, 256})
+        if (c <= kv.first) trials.emplace_back(ty, c);
+    std::pair<int, int> best(kv.second.front()->tile_y, kv.second.front()->ix_chunk);
     float best_ms = -1;
-    for (int c : {0, 16, 32, 64, 128, 256}) {
-      if (c > kv.first) break;
-      for (mpdx_solver *s : kv.second) s->ix_chunk = c;
-      drop_graphs(e);
-      for (int q = 0; q < 2 && !rc; ++q) rc = ensemble_step(e);   // both level states captured
+    for (const auto &tr : trials) {
+      for (mpdx_solver *s : kv.second) {
+        s->tile_y = tr.first;
+        s->ix_chunk = tr.second;
+      }
+      if (hipStreamSynchronize(e->stream) != hipSuccess) rc = MPDX_EHIP;
+      drop_plans(e);
+      if (!rc) rc = ensemble_run(e, 2);   // builds both level states' plans
       if (!rc && hipEventRecord(t0, e->stream) != hipSuccess) rc = MPDX_EHIP;
-      for (int q = 0; q < steps_per_trial && !rc; ++q) rc = ensemble_step(e);
+      if (!rc) rc = ensemble_run(e, steps_per_trial);
       if (!rc && hipEventRecord(t1, e->stream) != hipSuccess) rc = MPDX_EHIP;
       float ms = 0;
       if (!rc && (hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
@@ -2160,20 +2448,23 @@ int mpdx_ensemble_tune(mpdx_ensemble *e, int steps_per_trial, int64_t *steps_tak
       taken += 2 + steps_per_trial;
       if (best_ms < 0 || ms < best_ms) {
         best_ms = ms;
-        best = c;
+        best = tr;
       }
     }
-    for (mpdx_solver *s : kv.second) s->ix_chunk = best;
+    for (mpdx_solver *s : kv.second) {
+      s->tile_y = best.first;
+      s->ix_chunk = best.second;
+      drop_step_graphs(s);
+    }
     if (rc) break;
   }
-  drop_graphs(e);
+  (void)hipStreamSynchronize(e->stream);
+  drop_plans(e);
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   if (steps_taken) *steps_taken = taken;
   if (rc == MPDX_EHIP && e->err.empty()) e->err = "libmpdata++: HIP error while tuning";
-  if (rc) return rc;
-  EHIPCHK(hipStreamSynchronize(e->stream));
-  return MPDX_OK;
+  return rc;
 }
 
 int mpdx_ensemble_x_chunk(const mpdx_ensemble *e, int i, int *planes)
@@ -2182,6 +2473,16 @@ int mpdx_ensemble_x_chunk(const mpdx_ensemble *e, int i, int *planes)
   MPDX_NEED(planes, "output pointer");
   if (i < 0 || i >= (int)e->m.size()) return MPDX_EINVAL;
   *planes = e->m[i]->ix_chunk;
+  return MPDX_OK;
+}
+
+int mpdx_ensemble_tile_y(const mpdx_ensemble *e, int i, int *rows)
+{
+  MPDX_NEED(e, "ensemble handle");
+  MPDX_NEED(rows, "output pointer");
+  if (i < 0 || i >= (int)e->m.size()) return MPDX_EINVAL;
+  const mpdx_solver *s = e->m[i];
+  *rows = s->tile_y ? s->tile_y : (s->esz == 8 ? 16 : 32);
   return MPDX_OK;
 }
 

@@ -61,6 +61,18 @@ __device__ inline bool boxlist_idx(const BoxList &l, int &q, int &a, int &bb, in
   return boxlist_cell(l, blockIdx.x, threadIdx.x, q, a, bb, c);
 }
 
+// cell of this thread in block (gx, gy) of a Box launch of 256-thread blocks
+__device__ inline bool box_idx_at(const Box &b, unsigned gx, unsigned gy, int &a, int &bb, int &c)
+{
+  const unsigned t = gx * 256u + threadIdx.x;
+  if (t >= b.plane) return false;
+  const unsigned q = fdiv(t, b.d2);
+  a = (int)gy + b.lo[0];
+  bb = (int)q + b.lo[1];
+  c = (int)(t - q * (unsigned)b.ext[2]) + b.lo[2];
+  return true;
+}
+
 __device__ inline bool box_idx(const Box &b, int &a, int &bb, int &c)
 {
   const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;

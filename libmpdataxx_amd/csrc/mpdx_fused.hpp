@@ -56,12 +56,18 @@ __device__ inline int map_idx(const BcMap &m, int d, int i)
 // with z-boundary tiles, only the bands of (x, y) rows that the boundary
 // shell reads.
 template <class R>
+__device__ __forceinline__ void fz_halo_cell(R *a, const Layout &L, const BcMap &m, int i, int j, int k)
+{
+  const int si = map_idx(m, 0, i), sj = map_idx(m, 1, j), sk = L.nd == 3 ? map_idx(m, 2, k) : 0;
+  a[L.org + i * L.st[0] + j * L.st[1] + k * L.st[2]] = a[L.org + si * L.st[0] + sj * L.st[1] + sk * L.st[2]];
+}
+
+template <class R>
 __global__ void __launch_bounds__(256) k_fz_halo(R *a, Layout L, BcMap m, BoxList bl)
 {
   int q, i, j, k;
   if (!boxlist_idx(bl, q, i, j, k)) return;
-  const int si = map_idx(m, 0, i), sj = map_idx(m, 1, j), sk = L.nd == 3 ? map_idx(m, 2, k) : 0;
-  a[L.org + i * L.st[0] + j * L.st[1] + k * L.st[2]] = a[L.org + si * L.st[0] + sj * L.st[1] + sk * L.st[2]];
+  fz_halo_cell(a, L, m, i, j, k);
 }
 
 // element index of cell (i, j, k); the (j, k) part in 32-bit arithmetic
@@ -83,13 +89,10 @@ __device__ __forceinline__ R rel(const R *A, long p, const Layout &L, int dx, in
 // ---------------------------------------------------------------- donor-cell pass
 
 template <class R, bool NPA>
-__global__ void __launch_bounds__(256)
-k_fz_donor(R *out, const R *in, const R *C0, const R *C1, const R *C2, Layout L,
-           R in_scale, R out_scale, int flux_is_C, Box bx)
+__device__ __forceinline__ void fz_donor_cell(R *out, const R *in, const R *C0, const R *C1, const R *C2,
+                                              const Layout &L, R in_scale, R out_scale, int flux_is_C, int i,
+                                              int j, int k)
 {
-  // cells of the box bx (last coordinate fastest)
-  int i, j, k;
-  if (!box_idx(bx, i, j, k)) return;
   const long p = cell_index(L, i, j, k);
   const R pc = in[p] * in_scale;
   R f[6];
@@ -114,6 +117,17 @@ k_fz_donor(R *out, const R *in, const R *C0, const R *C1, const R *C2, Layout L,
   if (L.nd == 3) r = pc + ((-f[0] + f[1]) + (-f[2] + f[3]) + (-f[4] + f[5])) / 1;
   else r = pc + ((-f[0] + f[1]) + (-f[2] + f[3])) / 1;
   out[p] = r * out_scale;
+}
+
+template <class R, bool NPA>
+__global__ void __launch_bounds__(256)
+k_fz_donor(R *out, const R *in, const R *C0, const R *C1, const R *C2, Layout L,
+           R in_scale, R out_scale, int flux_is_C, Box bx)
+{
+  // cells of the box bx (last coordinate fastest)
+  int i, j, k;
+  if (!box_idx(bx, i, j, k)) return;
+  fz_donor_cell<R, NPA>(out, in, C0, C1, C2, L, in_scale, out_scale, flux_is_C, i, j, k);
 }
 
 // ---------------------------------------------------------------- antidiffusive velocity
@@ -187,13 +201,9 @@ __device__ inline R antidiff_at(const R *psi, const R *Ud, const R *U1, const R 
 // all components over the box [-1, n-1]^nd; component d is written where the
 // other coordinates are interior (the reference's im x j x k ranges)
 template <class R, class G, int ND, bool IGA, bool ABS, bool PFC>
-__global__ void __launch_bounds__(256)
-k_fz_antidiff(R *V0, R *V1, R *V2, const R *psi, const R *U0, const R *U1, const R *U2,
-              Layout L, Box bx)
+__device__ __forceinline__ void fz_antidiff_cell(R *V0, R *V1, R *V2, const R *psi, const R *U0, const R *U1,
+                                                 const R *U2, const Layout &L, int i, int j, int k)
 {
-  // bx: sub-box of the owner cells [-1, n-1]^nd (last coordinate fastest)
-  int i, j, k;
-  if (!box_idx(bx, i, j, k)) return;
   const bool in0 = i >= 0, in1 = j >= 0, in2 = ND == 2 || k >= 0;
   const long p = cell_index(L, i, j, k);
   if (ND == 3) {
@@ -204,6 +214,17 @@ k_fz_antidiff(R *V0, R *V1, R *V2, const R *psi, const R *U0, const R *U1, const
     if (in1) V0[p] = antidiff_at<R, G, 2, 0, IGA, ABS, PFC>(psi, U0, U1, U1, L, p);
     if (in0) V1[p] = antidiff_at<R, G, 2, 1, IGA, ABS, PFC>(psi, U1, U0, U0, L, p);
   }
+}
+
+template <class R, class G, int ND, bool IGA, bool ABS, bool PFC>
+__global__ void __launch_bounds__(256)
+k_fz_antidiff(R *V0, R *V1, R *V2, const R *psi, const R *U0, const R *U1, const R *U2,
+              Layout L, Box bx)
+{
+  // bx: sub-box of the owner cells [-1, n-1]^nd (last coordinate fastest)
+  int i, j, k;
+  if (!box_idx(bx, i, j, k)) return;
+  fz_antidiff_cell<R, G, ND, IGA, ABS, PFC>(V0, V1, V2, psi, U0, U1, U2, L, i, j, k);
 }
 
 // ---------------------------------------------------------------- FCT + donor-cell
@@ -316,16 +337,17 @@ __device__ inline void beta_at(R &bup, R &bdn, const R *psi, const R *psin, cons
   else beta_core<R, G, 2, NPA>(bup, bdn, pc, xp, xm, yp, ym, zp, zm, pmin, pmax, fxp, fxm, fyp, fym, fzp, fzm);
 }
 
+// brick (gz, gy, gx) of the box: gz along z (3D), gy along y, gx along x
 template <class R, class G, int ND, int SHAPE, bool NPA, bool ABS, bool IGA>
-__global__ void __launch_bounds__((fz_brick_threads<ND, SHAPE>()))
-k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R *V2, Layout L,
-         R in_scale, R out_scale, Box bx)
+__device__ __forceinline__ void fz_fct_block(R *out, const R *psi, const R *psin, const R *V0, const R *V1,
+                                             const R *V2, const Layout &L, R in_scale, R out_scale,
+                                             const Box &bx, unsigned gz, unsigned gy, unsigned gx)
 {
   constexpr int TX = FzBrick<ND, SHAPE>::TX, TY = FzBrick<ND, SHAPE>::TY, TZ = FzBrick<ND, SHAPE>::TZ;
   constexpr int BX = TX + 2, BY = TY + 2, BZ = ND == 3 ? TZ + 2 : 1;
   __shared__ R sbu[BX * BY * BZ], sbd[BX * BY * BZ];
-  const int i0 = bx.lo[0] + blockIdx.z * TX, j0 = bx.lo[1] + blockIdx.y * TY;
-  const int k0 = ND == 3 ? bx.lo[2] + blockIdx.x * TZ : 0;
+  const int i0 = bx.lo[0] + gx * TX, j0 = bx.lo[1] + gy * TY;
+  const int k0 = ND == 3 ? bx.lo[2] + gz * TZ : 0;
   const int tid = threadIdx.x;
   // beta over the brick plus a one-cell frame (cells -1..n of the domain)
   for (int q = tid; q < BX * BY * BZ; q += blockDim.x) {
@@ -381,6 +403,15 @@ k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R 
   out[p] = r * out_scale;
 }
 
+template <class R, class G, int ND, int SHAPE, bool NPA, bool ABS, bool IGA>
+__global__ void __launch_bounds__((fz_brick_threads<ND, SHAPE>()))
+k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R *V2, Layout L,
+         R in_scale, R out_scale, Box bx)
+{
+  fz_fct_block<R, G, ND, SHAPE, NPA, ABS, IGA>(out, psi, psin, V0, V1, V2, L, in_scale, out_scale, bx,
+                                               blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 // ---------------------------------------------------------------- interior corrective pass (3D)
 
 // One whole corrective pass (antidiffusive velocity, FCT limiter, donor-cell
@@ -408,15 +439,16 @@ k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R 
 // the tile plus a one-cell frame, so every neighbour read is the thread's
 // own LDS address plus an immediate offset; frame cells that nobody loads
 // hold zeros and reach only frame results, which are discarded.
-// tile height (y) of the FP32 interior kernel; FP64 tiles are 16 high
-#ifndef MPDX_TILE_Y_F32
-#define MPDX_TILE_Y_F32 32
-#endif
-template <class R> struct ITile {
-  static constexpr int Y = sizeof(R) == 8 ? 16 : MPDX_TILE_Y_F32, Z = 32;
+// Tile of TYV (y) x 32 (z) threads.  FP32 tiles are 32 high (one
+// 1024-thread block per CU, 106-116 KB of LDS) or 16 high (512 threads, two
+// blocks per CU); FP64 tiles are 16 high (the 32-high rings would need 212 KB
+// of LDS).
+template <int TYV> struct ITile {
+  static constexpr int Y = TYV, Z = 32;
   static constexpr int PW = Z + 2, PLANE = (Y + 2) * PW;
   static constexpr int NFRAME = 3 * Z + 3 * Y;   // FUSE0 frame loads per plane
 };
+template <class R> constexpr int itile_y_default() { return sizeof(R) == 8 ? 16 : 32; }
 
 // LDS planes of the interior kernel
 enum {
@@ -518,11 +550,13 @@ template <class R> __device__ __forceinline__ void buf_st(char *g, unsigned off,
 #else
 #define MPDX_INTERIOR_ATTR
 #endif
-template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0>
-__global__ void __launch_bounds__(ITile<R>::Y * ITile<R>::Z) MPDX_INTERIOR_ATTR
-k_fz_interior(InteriorArgs<R> a)
+// block blk of the launch over a's tiles (a launch of 8 * ceil(tiles / 8)
+// blocks; blk % 8 is the XCD the block runs on when the launch starts on a
+// multiple of 8)
+template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV>
+__device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned blk)
 {
-  constexpr int TY = ITile<R>::Y, TZ = ITile<R>::Z, PW = ITile<R>::PW, PL = ITile<R>::PLANE;
+  constexpr int TY = ITile<TYV>::Y, TZ = ITile<TYV>::Z, PW = ITile<TYV>::PW, PL = ITile<TYV>::PLANE;
   __shared__ R lds[IL_N * PL + (FUSE0 ? TY * TZ : 0)];
 
   // XCD-aware tile order: workgroup w runs on XCD w % 8, and XCD x takes the
@@ -531,7 +565,7 @@ k_fz_interior(InteriorArgs<R> a)
   const int nbz = a.nb[0], nby = a.nb[1];
   const int ntile = nbz * nby * a.nb[2];
   const int per = (ntile + 7) / 8;
-  const int t = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  const int t = (int)(blk & 7) * per + (int)(blk >> 3);
   if (t >= ntile) return;
   const int bz = t % nbz, by = (t / nbz) % nby, bxc = t / (nbz * nby);
 
@@ -625,7 +659,7 @@ k_fz_interior(InteriorArgs<R> a)
   R fr_sc = 1;
   int fr_lds = IL_N * PL + tid;
   const char *fr_g = at(a.psin, 0) + toff_s;
-  if (FUSE0 && tid < ITile<R>::NFRAME) {
+  if (FUSE0 && tid < ITile<TYV>::NFRAME) {
     int fy, fz, kind;
     if (tid < 2 * TZ) { kind = 0; fy = tid < TZ ? -1 : TY; fz = tid % TZ; }
     else if (tid < 2 * TZ + 2 * TY) { kind = 0; const int r = tid - 2 * TZ; fy = r % TY; fz = r < TY ? -1 : TZ; }
@@ -887,6 +921,13 @@ k_fz_interior(InteriorArgs<R> a)
     if (q + 5 >= nsteps) break;
     step(ic<5>(), q + 5);
   }
+}
+
+template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV>
+__global__ void __launch_bounds__(ITile<TYV>::Y * ITile<TYV>::Z) MPDX_INTERIOR_ATTR
+k_fz_interior(InteriorArgs<R> a)
+{
+  fz_interior_block<R, G, FCT, IGA, STOREV, FUSE0, TYV>(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- host side

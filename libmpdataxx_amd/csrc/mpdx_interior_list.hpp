@@ -1,12 +1,14 @@
 // The FP32 interior kernels with the default (fast FP64-epsilon) guards:
-// k_fz_interior<float, GuardFast, FCT, IGA, STOREV, FUSE0> for every option
-// combination the library launches.  X(R, G, FCT, IGA, STOREV, FUSE0) is
-// applied to each.
+// k_fz_interior<float, GuardFast, FCT, IGA, STOREV, FUSE0, TY> for every
+// option combination the library launches and both tile heights (32, 16).
+// X(R, G, FCT, IGA, STOREV, FUSE0, TY) is applied to each.
 #pragma once
 
-#define MPDX_IV_S(X, F, I, Z)                                         \
-  X(float, ::mpdx::GuardFast, F, I, true, Z)                          \
-  X(float, ::mpdx::GuardFast, F, I, false, Z)
+#define MPDX_IV_T(X, F, I, S, Z)                                      \
+  X(float, ::mpdx::GuardFast, F, I, S, Z, 32)                         \
+  X(float, ::mpdx::GuardFast, F, I, S, Z, 16)
+
+#define MPDX_IV_S(X, F, I, Z) MPDX_IV_T(X, F, I, true, Z) MPDX_IV_T(X, F, I, false, Z)
 
 #define MPDX_IV_Z(X, F, I) MPDX_IV_S(X, F, I, true) MPDX_IV_S(X, F, I, false)
 

@@ -69,13 +69,9 @@ __global__ void k_fill_batch(FillBatch<R> fb)
 // open-boundary zeroing of perpendicular advector components at the edge
 // (open_3d.hpp:79-103,201-225; open_2d.hpp fill_halos_vctr_alng)
 template <class R>
-__global__ void k_alng_zero(R *av1, R *av2, Rot r, int nd, int d, int e0, int e1,
-                            int raf, int ral, int rbf, int rbl)
+__device__ __forceinline__ void alng_zero_one(R *av1, R *av2, const Rot &r, int nd, int d, int e, int raf,
+                                              int ral, int rbf, int rbl, int t)
 {
-  // grid y: the edge e0 (first side) or e1 (second side); the two touch
-  // different cells
-  const int e = blockIdx.y == 0 ? e0 : e1;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (nd == 3) {
     if (d == 0 && t <= ral - raf) {
       int b = raf + t;
@@ -91,6 +87,16 @@ __global__ void k_alng_zero(R *av1, R *av2, Rot r, int nd, int d, int e0, int e1
     av1[rr(r, e, raf - 1, 0)] = R(0);
     av1[rr(r, e, ral, 0)] = R(0);
   }
+}
+
+template <class R>
+__global__ void k_alng_zero(R *av1, R *av2, Rot r, int nd, int d, int e0, int e1,
+                            int raf, int ral, int rbf, int rbl)
+{
+  // grid y: the edge e0 (first side) or e1 (second side); the two touch
+  // different cells
+  const int e = blockIdx.y == 0 ? e0 : e1;
+  alng_zero_one(av1, av2, r, nd, d, e, raf, ral, rbf, rbl, (int)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // normal advector halo faces: cyclic copy (cyclic_3d.hpp:44-48,135-139) or

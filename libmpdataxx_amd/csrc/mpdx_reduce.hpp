@@ -5,9 +5,9 @@
 // solver_2d.hpp:305-313) and the Courant number (solver_3d.hpp:390-398,
 // solver_2d.hpp:295-303), computed on the fly rather than stored.
 //
-// Two launches.  k_red_part: block b covers a contiguous run of cells of one
+// Two launches.  k_red_part: block b covers a contiguous run of rows of one
 // x plane (plane b / S, part b % S of it, S parts per plane); each thread
-// keeps an FP64 accumulator over its strided cells, the 64 lanes of a wave
+// keeps an FP64 accumulator over its cells of each row, the 64 lanes of a wave
 // combine theirs with cross-lane shuffles (a butterfly), the waves of the
 // block through LDS, and the block writes one partial.  k_red_final: one
 // block combines the partials the same way.  Sums are compensated at every
@@ -109,38 +109,40 @@ __device__ __forceinline__ double red_value(const R *a, const R *g1, const R *g2
 }
 
 // partial b of kind K over the box bx (bx.ext[0] planes, S parts each);
-// RK_DIV / RK_CFL read the advector components a, g1, g2
+// RK_DIV / RK_CFL read the advector components a, g1, g2.  A plane is walked
+// as rows of contiguous cells (3D: ext[1] rows of ext[2] cells along z; 2D:
+// one row of ext[1] cells along y); the S parts of a plane split its rows,
+// or, with a single row, the row itself.
 template <int K, class R>
 __global__ void __launch_bounds__(kRedThreads)
 k_red_part(double2 *part, const R *a, const R *g1, const R *g2, Layout L, Box bx, int S)
 {
   const int b = blockIdx.x;
   const int plane_i = b / S, s = b - plane_i * S;
-  const unsigned plane = bx.plane;
-  const unsigned q0 = (unsigned)(((unsigned long long)plane * s) / S);
-  const unsigned q1 = (unsigned)(((unsigned long long)plane * (s + 1)) / S);
-  const long base = L.org + (long)(plane_i + bx.lo[0]) * L.st[0];
+  const bool d3 = L.nd == 3;
+  const int rows = d3 ? bx.ext[1] : 1, len = d3 ? bx.ext[2] : bx.ext[1];
+  int r0 = 0, r1 = rows, k0 = 0, k1 = len;
+  if (rows > 1) {
+    r0 = (int)((long)rows * s / S);
+    r1 = (int)((long)rows * (s + 1) / S);
+  } else {
+    k0 = (int)((long)len * s / S);
+    k1 = (int)((long)len * (s + 1) / S);
+  }
+  // element of (row r, cell k): first + r * rst + k
+  const long first = L.org + (long)(plane_i + bx.lo[0]) * L.st[0] + (long)bx.lo[1] * L.st[1] +
+                     (d3 ? (long)bx.lo[2] * L.st[2] : 0);
+  const long rst = d3 ? L.st[1] : 0;
   double hi = red_identity<K>(), lo = 0.0;
-  // loads in flight per thread: 32 bytes of the field (RK_DIV / RK_CFL read
-  // three arrays per cell)
-  constexpr int U = (K == RK_DIV || K == RK_CFL || sizeof(R) == 8) ? 4 : 8;
-  for (unsigned q = q0 + threadIdx.x; q < q1; q += U * kRedThreads) {
-    double v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const unsigned qq = q + u * kRedThreads;
-      v[u] = red_identity<K>();
-      if (qq < q1) {
-        const unsigned jj = fdiv(qq, bx.d2);
-        const int j = (int)jj + bx.lo[1], k = (int)(qq - jj * (unsigned)bx.ext[2]) + bx.lo[2];
-        v[u] = red_value<K>(a, g1, g2, L, base + (long)j * L.st[1] + (long)k * L.st[2]);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if constexpr (K == RK_MIN) hi = smin(hi, v[u]);
-      else if constexpr (K == RK_MAX || K == RK_DIV || K == RK_CFL) hi = smax(hi, v[u]);
-      else acc_add(hi, lo, v[u]);
+  for (int r = r0; r < r1; ++r) {
+    const long row = first + (long)r * rst;
+    for (int k = k0 + (int)threadIdx.x; k < k1; k += 2 * kRedThreads) {
+      const bool two = k + kRedThreads < k1;
+      const double v0 = red_value<K>(a, g1, g2, L, row + k);
+      const double v1 = two ? red_value<K>(a, g1, g2, L, row + k + kRedThreads) : red_identity<K>();
+      if constexpr (K == RK_MIN) hi = smin(smin(hi, v0), v1);
+      else if constexpr (K == RK_MAX || K == RK_DIV || K == RK_CFL) hi = smax(smax(hi, v0), v1);
+      else { acc_add(hi, lo, v0); acc_add(hi, lo, v1); }
     }
   }
   constexpr int KC = K == RK_SUM ? RK_SUM : (K == RK_MIN ? RK_MIN : RK_MAX);

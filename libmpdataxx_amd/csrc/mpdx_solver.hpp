@@ -11,6 +11,42 @@
 
 struct mpdx_fused_state;
 
+namespace mpdx {
+
+// launches k_task_list<T> over n tasks at `tasks` (device) with the block
+// prefix table `first` (device) and `blocks` blocks in all
+using TaskLaunchFn = void (*)(hipStream_t, const void *tasks, const unsigned *first, int n, unsigned blocks);
+
+// one launch of a solver's time step, recorded as a task of a batched kernel
+struct RecOp {
+  TaskLaunchFn fn = nullptr;
+  unsigned tsize = 0, blocks = 0;
+  std::vector<unsigned char> bytes;
+};
+
+// The launches of one time step of a solver, recorded instead of launched
+// (ensembles batch them across members).  Launches made between begin() and
+// end() form one group: they touch disjoint cells and read nothing the
+// others write, so they may run in any order or together; a group runs
+// after the previous one has finished, as launches on one stream would.
+struct TaskRecorder {
+  std::vector<std::vector<RecOp>> groups;
+  int depth = 0;
+  void begin()
+  {
+    if (depth++ == 0) groups.emplace_back();
+  }
+  void end() { --depth; }
+  RecOp &next()
+  {
+    if (depth == 0) groups.emplace_back();
+    groups.back().emplace_back();
+    return groups.back().back();
+  }
+};
+
+}  // namespace mpdx
+
 // per-kernel-class timing with HIP events on the solver's stream
 struct mpdx_prof_entry {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -50,6 +86,8 @@ struct mpdx_solver {
   int active_path = MPDX_PATH_PERPASS;
   std::string err;
   mpdx_fused_state *fz = nullptr;
+  mpdx::TaskRecorder *rec = nullptr;   // set: launches are recorded, not made
+  int tile_y = 0;            // interior tile height (0: the element type's default)
   // one captured time step per time-level parity (fused path)
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool use_graphs = true;

@@ -1,0 +1,161 @@
+// Batched launches for ensembles of independent solvers.
+//
+// A task is the argument set of one launch of a fused-path kernel (its
+// pointers, layout, box and scales) together with the shape of the grid that
+// launch would have (nx x ny x nz blocks).  k_task_list runs many tasks of
+// one kernel as a single 1D grid: block b belongs to task q when
+// first[q] <= b < first[q + 1] (a binary search over first[], which lives in
+// device memory next to the tasks), and runs block b - first[q] of that task
+// with the same device code the task's own launch would run.  So a batched
+// step computes exactly what each member's own step computes.
+#pragma once
+
+#include "mpdx_fused.hpp"
+
+namespace mpdx {
+
+struct TaskBlk {
+  unsigned x, y, z;
+};
+
+// blocks of a task's own grid; first = prefix sums of blocks, n tasks
+template <class T>
+__global__ void __launch_bounds__(T::kThreads) k_task_list(const T *tasks, const unsigned *first, int n)
+{
+  const unsigned b = blockIdx.x;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  // a private copy: the task's fields stay in registers instead of being
+  // reloaded from memory after every store the body makes
+  const T t = tasks[lo];
+  const unsigned l = b - first[lo];
+  t.run(TaskBlk{l % t.nx, (l / t.nx) % t.ny, l / (t.nx * t.ny)});
+}
+
+// fields every task carries: its grid shape; blocks are kThreads threads,
+// kBlockY rows of kThreads / kBlockY (all 1D but the interior kernel's)
+struct TaskGrid {
+  static constexpr int kBlockY = 1;
+  unsigned nx = 1, ny = 1, nz = 1;
+  unsigned blocks() const { return nx * ny * nz; }
+};
+
+// one box of the scalar halo (k_fz_halo)
+template <class R> struct HaloTask : TaskGrid {
+  static constexpr int kThreads = 256;
+  R *a;
+  Layout L;
+  BcMap m;
+  Box bx;
+  __device__ void run(TaskBlk b) const
+  {
+    int i, j, k;
+    if (box_idx_at(bx, b.x, b.y, i, j, k)) fz_halo_cell(a, L, m, i, j, k);
+  }
+};
+
+// one box of the donor-cell pass (k_fz_donor)
+template <class R, bool NPA> struct DonorTask : TaskGrid {
+  static constexpr int kThreads = 256;
+  R *out;
+  const R *in, *C0, *C1, *C2;
+  Layout L;
+  R in_scale, out_scale;
+  int flux_is_C;
+  Box bx;
+  __device__ void run(TaskBlk b) const
+  {
+    int i, j, k;
+    if (box_idx_at(bx, b.x, b.y, i, j, k))
+      fz_donor_cell<R, NPA>(out, in, C0, C1, C2, L, in_scale, out_scale, flux_is_C, i, j, k);
+  }
+};
+
+// one box of the antidiffusive velocity (k_fz_antidiff)
+template <class R, class G, int ND, bool IGA, bool ABS, bool PFC> struct AntidiffTask : TaskGrid {
+  static constexpr int kThreads = 256;
+  R *V0, *V1, *V2;
+  const R *psi, *U0, *U1, *U2;
+  Layout L;
+  Box bx;
+  __device__ void run(TaskBlk b) const
+  {
+    int i, j, k;
+    if (box_idx_at(bx, b.x, b.y, i, j, k)) fz_antidiff_cell<R, G, ND, IGA, ABS, PFC>(V0, V1, V2, psi, U0, U1, U2, L, i, j, k);
+  }
+};
+
+// one box of the FCT limiter with the donor-cell update (k_fz_fct)
+template <class R, class G, int ND, int SHAPE, bool NPA, bool ABS, bool IGA> struct FctTask : TaskGrid {
+  static constexpr int kThreads = fz_brick_threads<ND, SHAPE>();
+  R *out;
+  const R *psi, *psin, *V0, *V1, *V2;
+  Layout L;
+  R in_scale, out_scale;
+  Box bx;
+  __device__ void run(TaskBlk b) const
+  {
+    fz_fct_block<R, G, ND, SHAPE, NPA, ABS, IGA>(out, psi, psin, V0, V1, V2, L, in_scale, out_scale, bx, b.x, b.y, b.z);
+  }
+};
+
+// one halo fill of a scalar or tangential advector component (k_fill_batch)
+template <class R> struct FillTask : TaskGrid {
+  static constexpr int kThreads = 256;
+  R *a;
+  Rot r;
+  Box bx;
+  int side, mode, N, ord;
+  __device__ void run(TaskBlk b) const
+  {
+    int bb, c, m;
+    if (ord == 0) {
+      if (!box_idx_at(bx, b.x, b.y, bb, m, c)) return;
+    } else if (!box_idx_at(bx, b.x, b.y, m, c, bb)) {
+      return;
+    }
+    fill_one(a, r, bb, c, m, side, mode, N);
+  }
+};
+
+// one side of an open boundary's perpendicular-component zeroing (k_alng_zero)
+template <class R> struct AlngZeroTask : TaskGrid {
+  static constexpr int kThreads = 256;
+  R *av1, *av2;
+  Rot r;
+  int nd, d, e, raf, ral, rbf, rbl;
+  __device__ void run(TaskBlk b) const
+  {
+    alng_zero_one(av1, av2, r, nd, d, e, raf, ral, rbf, rbl, (int)(b.x * 256u + threadIdx.x));
+  }
+};
+
+// one box of a normal-component halo fill (k_alng2, k_alng_list)
+template <class R> struct AlngTask : TaskGrid {
+  static constexpr int kThreads = 256;
+  R *avd;
+  const R *av1, *av2;
+  Rot r;
+  Box bx;
+  int nd, side, cyc, N, cfirst;
+  __device__ void run(TaskBlk b) const
+  {
+    int ii, bb, c;
+    if (box_idx_at(bx, b.x, b.y, ii, bb, c)) alng_one(avd, av1, av2, r, ii, bb, c, nd, side, cyc, N, cfirst);
+  }
+};
+
+// one launch of the interior corrective pass (k_fz_interior); its block
+// count is a multiple of 8, so a batch of these keeps the XCD tile order
+template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV>
+struct InteriorTask : TaskGrid {
+  static constexpr int kThreads = ITile<TYV>::Y * ITile<TYV>::Z, kBlockY = ITile<TYV>::Y;
+  InteriorArgs<R> a;
+  __device__ void run(TaskBlk b) const { fz_interior_block<R, G, FCT, IGA, STOREV, FUSE0, TYV>(a, b.x); }
+};
+
+}  // namespace mpdx

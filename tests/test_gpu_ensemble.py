@@ -1,10 +1,12 @@
 """Ensembles of independent domains (mpdx_ensemble_*): the uneven ensemble
 of 64 cyclic domains (32 2D from 64^2 to 512^2, 32 3D from 32^3 to 128^3,
-n_iters 2-4, FCT on/off) stepped by one captured graph per step.
+n_iters 2-4, FCT on/off) stepped by one graph of batched launches per step
+(each launch runs one kernel's tasks of many members).
 
 The ensemble must give exactly what each member gives when advanced on its
-own (same kernels, any x-chunk length: results do not depend on it), match
-the oracle domain by domain in FP64, and conserve mass and positivity."""
+own (same device code, any x-chunk length and tile height: results do not
+depend on them), match the oracle domain by domain in FP64, and conserve
+mass and positivity."""
 import numpy as np
 import pytest
 
@@ -13,13 +15,6 @@ import oracle as O
 from libmpdataxx_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
-# Ensemble stepping captures one graph over all members' streams.  It passed
-# these tests with /opt/rocm's HIP runtime and crashed the host process
-# (SIGSEGV in the first advance) twice when PyTorch's bundled HIP runtime was
-# the one loaded; the cause is not known, so stepping is switched off in the
-# library and these cases do not run.
-STEPPING_OFF = pytest.mark.skip(reason="ensemble stepping switched off: host crash under the PyTorch-bundled "
-                                       "HIP runtime, cause not yet shown")
 
 
 @pytest.fixture(scope="module")
@@ -42,16 +37,17 @@ def make(d, dtype):
 @pytest.mark.parametrize("opts,iters", [(M.FCT, 3), (0, 3), (0, 4), (M.IGA, 2), (M.IGA | M.FCT, 2)])
 def test_x_chunk_does_not_change_results(dtype, bc, opts, iters):
     """Every interior cell is computed by the same code whatever the x-chunk
-    length (the march loop is not peeled, so the first planes of a chunk do
-    not get differently compiled arithmetic)."""
+    length and the tile height (the march loop is not peeled, so the first
+    planes of a chunk do not get differently compiled arithmetic)."""
     grid = (70, 36, 34)
     print(f"x chunks dtype={np.dtype(dtype).name} bc={bc} opts={opts} n_iters={iters}", flush=True)
     psi = W.random_field(grid, 21)
     gc = W.streamfunction_advector(grid, 22, periodic=bc == M.CYCLIC, max_courant=0.45)
     outs = []
-    for chunk in (0, 8, 16, 64, 256):
+    for chunk, tile in ((0, 0), (8, 0), (16, 16), (64, 0), (256, 16)):
         s = M.Solver(grid, opts=opts, n_iters=iters, dtype=dtype, bcs=[(bc, bc)] * 3)
         s.set_x_chunk(chunk)
+        s.set_tile_y(tile)
         s.set_advectee(psi)
         for d in range(3):
             s.set_advector(gc[d], d)
@@ -62,7 +58,6 @@ def test_x_chunk_does_not_change_results(dtype, bc, opts, iters):
         np.testing.assert_array_equal(o, outs[0])
 
 
-@STEPPING_OFF
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_ensemble_matches_members_alone(domains, dtype):
     pick = [0, 3, 7, 31, 32, 33, 40, 63]
@@ -71,6 +66,9 @@ def test_ensemble_matches_members_alone(domains, dtype):
     alone = [make(domains[i], dtype) for i in pick]
     e = M.Ensemble(ens_m)
     e.advance(5)
+    launches, tasks = e.stats()
+    print(f"  one step: {launches} batched launches for {tasks} tasks", flush=True)
+    assert launches < tasks
     for s in alone:
         s.advance(5)
     for a, b in zip(ens_m, alone):
@@ -90,7 +88,6 @@ def test_ensemble_matches_members_alone(domains, dtype):
         s.close()
 
 
-@STEPPING_OFF
 def test_uneven_ensemble_tuned_fp32(domains):
     """All 64 domains in FP32: autotuned, then stepped; equal to the members
     run alone; mass over the unique cells and positivity kept."""
@@ -101,7 +98,11 @@ def test_uneven_ensemble_tuned_fp32(domains):
     assert taken > 0
     e.advance(3)
     chunks = {e.x_chunk(i) for i in range(32, 64)}
-    print(f"  tuning took {taken} steps, x chunks chosen {sorted(chunks)}", flush=True)
+    tiles = {e.tile_y(i) for i in range(32, 64)}
+    launches, tasks = e.stats()
+    print(f"  tuning took {taken} steps, x chunks chosen {sorted(chunks)}, tile heights {sorted(tiles)}; "
+          f"a step is {launches} batched launches for {tasks} tasks", flush=True)
+    assert tiles <= {16, 32}
     for i, (d, s) in enumerate(zip(domains, ens_m)):
         assert s.timestep == taken + 3
         a = s.advectee()
@@ -118,7 +119,6 @@ def test_uneven_ensemble_tuned_fp32(domains):
         s.close()
 
 
-@STEPPING_OFF
 def test_uneven_ensemble_fp64_matches_oracle(domains):
     print("uneven ensemble 64 domains FP64 vs oracle (2 steps)", flush=True)
     ens_m = [make(d, np.float64) for d in domains]
@@ -151,8 +151,6 @@ def test_ensemble_rejects_unsupported_members(domains):
     e = M.Ensemble([a, b])
     with pytest.raises(M.MpdxError, match="negative"):
         e.advance(-1)
-    with pytest.raises(M.MpdxError, match="switched off"):
-        e.advance(1)
     e.close()
     a.close()
     b.close()
