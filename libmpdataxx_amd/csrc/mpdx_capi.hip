@@ -446,12 +446,15 @@ struct PerPass {
     return make_box(-1, L.n[0], -1, L.n[1], L.nd == 3 ? -1 : 0, L.nd == 3 ? L.n[2] : 0);
   }
 
+  // the GC_unco / GC_corr ping-pong arrays (of equation tmp_sel when the
+  // equations run batched)
+  void *(*vel())[3] { return s->tmp_sel > 0 ? s->tmp_eq[s->tmp_sel] : s->tmp; }
   R **unco(int iter, R *buf[3])
   {
     if (iter == 1) return av_of(s->GC, buf);
-    return av_of(iter % 2 ? s->tmp[1] : s->tmp[0], buf);
+    return av_of(iter % 2 ? vel()[1] : vel()[0], buf);
   }
-  R **corr(int iter, R *buf[3]) { return av_of(iter % 2 ? s->tmp[0] : s->tmp[1], buf); }
+  R **corr(int iter, R *buf[3]) { return av_of(iter % 2 ? vel()[0] : vel()[1], buf); }
 
   // mpdata_fct_3d.hpp:57-110
   void fct_adjust(R *psi, int iter)
@@ -531,6 +534,8 @@ void drop_step_graphs(mpdx_solver *s)
   for (int q = 0; q < 2; ++q) {
     if (s->graph[q]) (void)hipGraphExecDestroy(s->graph[q]);
     s->graph[q] = nullptr;
+    if (s->eqplan[q].dev) (void)hipFree(s->eqplan[q].dev);
+    s->eqplan[q] = BatchPlan();
   }
 }
 
@@ -1349,6 +1354,120 @@ void apply_rhs(mpdx_solver *s, double dt_arg)
     blaunch(s->stream, bi, k_axpy_box<R>, (R *)s->psi[e][s->lev[e]], (const R *)s->rhs[e], s->L, bi, R(dt_arg));
 }
 
+// ----------------------------------------------------------------- batched launches
+
+// Recorded launches of several independent task sequences (the equations of
+// one solver, or the members of an ensemble) merged into batched launches
+// of k_task_list: the tasks of each batch and their block prefix table live
+// in one device buffer.
+void free_batch_plan(BatchPlan &p)
+{
+  if (p.dev) (void)hipFree(p.dev);
+  p = BatchPlan();
+}
+
+// The merge: every sequence's recorded step (a member's, an equation's) is
+// a sequence of groups of mutually independent launches.  Each round takes the kernel that heads
+// the most members' current groups and launches all of its pending tasks in
+// those groups as one batch; a member moves to its next group once its
+// current one is done.  Each member's groups therefore run in order, and a
+// batch holds tasks of different members (or independent tasks of one).
+void schedule(const std::vector<TaskRecorder> &recs, std::vector<std::vector<const RecOp *>> &out)
+{
+  const size_t n = recs.size();
+  std::vector<size_t> g(n, 0);
+  std::vector<std::vector<char>> taken(n);
+  auto settle = [&](size_t i) {
+    while (g[i] < recs[i].groups.size()) {
+      const auto &grp = recs[i].groups[g[i]];
+      taken[i].resize(grp.size(), 0);
+      bool done = true;
+      for (size_t q = 0; q < grp.size(); ++q) done = done && taken[i][q];
+      if (!done) return;
+      ++g[i];
+      taken[i].clear();
+    }
+  };
+  for (size_t i = 0; i < n; ++i) settle(i);
+  for (;;) {
+    std::vector<std::pair<TaskLaunchFn, int>> count;   // kernel, members whose head holds it
+    for (size_t i = 0; i < n; ++i) {
+      if (g[i] >= recs[i].groups.size()) continue;
+      const auto &grp = recs[i].groups[g[i]];
+      std::vector<TaskLaunchFn> seen;
+      for (size_t q = 0; q < grp.size(); ++q) {
+        if (taken[i][q] || std::find(seen.begin(), seen.end(), grp[q].fn) != seen.end()) continue;
+        seen.push_back(grp[q].fn);
+        auto it = std::find_if(count.begin(), count.end(), [&](const auto &c) { return c.first == grp[q].fn; });
+        if (it == count.end()) count.emplace_back(grp[q].fn, 1);
+        else ++it->second;
+      }
+    }
+    if (count.empty()) break;
+    TaskLaunchFn best = count[0].first;
+    int most = count[0].second;
+    for (const auto &c : count)
+      if (c.second > most) { best = c.first; most = c.second; }
+    std::vector<const RecOp *> batch;
+    for (size_t i = 0; i < n; ++i) {
+      if (g[i] >= recs[i].groups.size()) continue;
+      const auto &grp = recs[i].groups[g[i]];
+      for (size_t q = 0; q < grp.size(); ++q)
+        if (!taken[i][q] && grp[q].fn == best) {
+          batch.push_back(&grp[q]);
+          taken[i][q] = 1;
+        }
+      settle(i);
+    }
+    out.push_back(std::move(batch));
+  }
+}
+
+// merges the recorded sequences into batches and uploads their tasks and
+// prefix tables (synchronously; not while a stream is being captured)
+int pack_plan(const std::vector<TaskRecorder> &recs, BatchPlan &p, std::string &err)
+{
+  std::vector<std::vector<const RecOp *>> batches;
+  schedule(recs, batches);
+  // per batch its tasks, then its prefix table, each 256-byte aligned
+  std::vector<unsigned char> blob;
+  auto align = [&]() { blob.resize((blob.size() + 255) / 256 * 256); };
+  for (const auto &b : batches) {
+    EnsBatch eb;
+    eb.fn = b[0]->fn;
+    eb.n = (int)b.size();
+    align();
+    eb.toff = blob.size();
+    for (const RecOp *op : b) blob.insert(blob.end(), op->bytes.begin(), op->bytes.end());
+    align();
+    eb.foff = blob.size();
+    unsigned first = 0;
+    for (const RecOp *op : b) {
+      const unsigned char *f = reinterpret_cast<const unsigned char *>(&first);
+      blob.insert(blob.end(), f, f + sizeof(unsigned));
+      first += op->blocks;
+    }
+    eb.blocks = first;
+    p.batches.push_back(eb);
+    p.tasks += b.size();
+  }
+  if (blob.empty()) blob.resize(256);
+  hipError_t e = hipMalloc(&p.dev, blob.size());
+  if (e == hipSuccess) e = hipMemcpy(p.dev, blob.data(), blob.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    err = std::string("libmpdata++: HIP error: ") + hipGetErrorString(e) + " (uploading batched tasks)";
+    return MPDX_EHIP;
+  }
+  return 0;
+}
+
+void launch_plan(const BatchPlan &p, hipStream_t st)
+{
+  char *dev = static_cast<char *>(p.dev);
+  for (const EnsBatch &b : p.batches)
+    b.fn(st, dev + b.toff, reinterpret_cast<const unsigned *>(dev + b.foff), b.n, b.blocks);
+}
+
 template <class R, class G>
 int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
 {
@@ -1426,10 +1545,40 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
       if (!s->graph[key]) {
         int lev_save[MPDX_MAX_EQNS];
         for (int e = 0; e < s->p.n_eqns; ++e) lev_save[e] = s->lev[e];
+        // several equations: the launches of each equation's step are
+        // recorded and merged into batched launches (one per kernel and
+        // round, covering every equation), uploaded before the capture
+        const bool batched = s->batch_eqns && s->p.n_eqns > 1;
+        if (batched) {
+          // equations 1.. get velocity scratch of their own
+          for (int e = 1; e < s->p.n_eqns; ++e)
+            for (int d = 0; d < s->L.nd; ++d) {
+              if (s->p.n_iters > 1)
+                if (int rc = alloc_missing(s, &s->tmp_eq[e][0][d])) return rc;
+              if (s->p.n_iters > 2)
+                if (int rc = alloc_missing(s, &s->tmp_eq[e][1][d])) return rc;
+            }
+          std::vector<TaskRecorder> recs(s->p.n_eqns);
+          for (int e = 0; e < s->p.n_eqns; ++e) {
+            s->rec = &recs[e];
+            s->tmp_sel = e;
+            Fused<R, G> fz(s);
+            fz.step_eqn(e);
+            s->rec = nullptr;
+            s->tmp_sel = -1;
+          }
+          for (int e = 0; e < s->p.n_eqns; ++e) s->lev[e] = lev_save[e];
+          if (s->eqplan[key].dev) (void)hipFree(s->eqplan[key].dev);
+          s->eqplan[key] = BatchPlan();
+          if (int rc = pack_plan(recs, s->eqplan[key], s->err)) return rc;
+        }
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
         int rc = calc_gc_step<R, G>(s);
-        if (!rc) rc = fused_step<R, G>(s);
+        if (!rc) {
+          if (batched) launch_plan(s->eqplan[key], s->stream);
+          else rc = fused_step<R, G>(s);
+        }
         if (gen) hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, s->stream, s->gen_step, -1);
         hipError_t ec = hipStreamEndCapture(s->stream, &g);
         for (int e = 0; e < s->p.n_eqns; ++e) s->lev[e] = lev_save[e];
@@ -1524,19 +1673,10 @@ int step_dispatch(mpdx_solver *s)
 // the members' time levels; two alternate.
 namespace {
 
-struct EnsBatch {
-  TaskLaunchFn fn;
-  size_t toff, foff;   // byte offsets of the tasks and of their block prefix table
-  int n;
-  unsigned blocks;
-};
-
 struct EnsPlan {
   std::vector<int> key;              // members' time levels the plan was recorded at
-  void *dev = nullptr;               // tasks and prefix tables of all batches
-  std::vector<EnsBatch> batches;
+  BatchPlan bp;
   hipGraphExec_t graph = nullptr;
-  size_t tasks = 0;
 };
 
 }  // namespace
@@ -1574,7 +1714,7 @@ std::vector<int> level_key(const mpdx_ensemble *e)
 void free_plan(EnsPlan &p)
 {
   if (p.graph) (void)hipGraphExecDestroy(p.graph);
-  if (p.dev) (void)hipFree(p.dev);
+  free_batch_plan(p.bp);
   p = EnsPlan();
 }
 
@@ -1588,63 +1728,6 @@ int member_error(mpdx_ensemble *e, size_t i, const mpdx_solver *s, int rc)
 {
   e->err = "libmpdata++: ensemble member " + std::to_string(i) + ": " + s->err;
   return rc;
-}
-
-// The merge: every member's recorded step is a sequence of groups of
-// mutually independent launches.  Each round takes the kernel that heads
-// the most members' current groups and launches all of its pending tasks in
-// those groups as one batch; a member moves to its next group once its
-// current one is done.  Each member's groups therefore run in order, and a
-// batch holds tasks of different members (or independent tasks of one).
-void schedule(const std::vector<TaskRecorder> &recs, std::vector<std::vector<const RecOp *>> &out)
-{
-  const size_t n = recs.size();
-  std::vector<size_t> g(n, 0);
-  std::vector<std::vector<char>> taken(n);
-  auto settle = [&](size_t i) {
-    while (g[i] < recs[i].groups.size()) {
-      const auto &grp = recs[i].groups[g[i]];
-      taken[i].resize(grp.size(), 0);
-      bool done = true;
-      for (size_t q = 0; q < grp.size(); ++q) done = done && taken[i][q];
-      if (!done) return;
-      ++g[i];
-      taken[i].clear();
-    }
-  };
-  for (size_t i = 0; i < n; ++i) settle(i);
-  for (;;) {
-    std::vector<std::pair<TaskLaunchFn, int>> count;   // kernel, members whose head holds it
-    for (size_t i = 0; i < n; ++i) {
-      if (g[i] >= recs[i].groups.size()) continue;
-      const auto &grp = recs[i].groups[g[i]];
-      std::vector<TaskLaunchFn> seen;
-      for (size_t q = 0; q < grp.size(); ++q) {
-        if (taken[i][q] || std::find(seen.begin(), seen.end(), grp[q].fn) != seen.end()) continue;
-        seen.push_back(grp[q].fn);
-        auto it = std::find_if(count.begin(), count.end(), [&](const auto &c) { return c.first == grp[q].fn; });
-        if (it == count.end()) count.emplace_back(grp[q].fn, 1);
-        else ++it->second;
-      }
-    }
-    if (count.empty()) break;
-    TaskLaunchFn best = count[0].first;
-    int most = count[0].second;
-    for (const auto &c : count)
-      if (c.second > most) { best = c.first; most = c.second; }
-    std::vector<const RecOp *> batch;
-    for (size_t i = 0; i < n; ++i) {
-      if (g[i] >= recs[i].groups.size()) continue;
-      const auto &grp = recs[i].groups[g[i]];
-      for (size_t q = 0; q < grp.size(); ++q)
-        if (!taken[i][q] && grp[q].fn == best) {
-          batch.push_back(&grp[q]);
-          taken[i][q] = 1;
-        }
-      settle(i);
-    }
-    out.push_back(std::move(batch));
-  }
 }
 
 // records one step of every member, merges it into batches, uploads the
@@ -1667,38 +1750,10 @@ int build_plan(mpdx_ensemble *e, EnsPlan &p)
     for (int r = 0; r < s->p.n_eqns; ++r) s->lev[r] = saved[q++];
   if (rc) return member_error(e, fail_at, e->m[fail_at], rc);
 
-  std::vector<std::vector<const RecOp *>> batches;
-  schedule(recs, batches);
-  // pack: per batch its tasks, then its prefix table, each 256-byte aligned
-  std::vector<unsigned char> blob;
-  auto align = [&]() { blob.resize((blob.size() + 255) / 256 * 256); };
-  for (const auto &b : batches) {
-    EnsBatch eb;
-    eb.fn = b[0]->fn;
-    eb.n = (int)b.size();
-    align();
-    eb.toff = blob.size();
-    for (const RecOp *op : b) blob.insert(blob.end(), op->bytes.begin(), op->bytes.end());
-    align();
-    eb.foff = blob.size();
-    unsigned first = 0;
-    for (const RecOp *op : b) {
-      const unsigned char *f = reinterpret_cast<const unsigned char *>(&first);
-      blob.insert(blob.end(), f, f + sizeof(unsigned));
-      first += op->blocks;
-    }
-    eb.blocks = first;
-    p.batches.push_back(eb);
-    p.tasks += b.size();
-  }
-  if (blob.empty()) blob.resize(256);
-  EHIPCHK(hipMalloc(&p.dev, blob.size()));
-  EHIPCHK(hipMemcpy(p.dev, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  if (int rc2 = pack_plan(recs, p.bp, e->err)) return rc2;
   hipGraph_t graph = nullptr;
   EHIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-  char *dev = static_cast<char *>(p.dev);
-  for (const EnsBatch &b : p.batches)
-    b.fn(e->stream, dev + b.toff, reinterpret_cast<const unsigned *>(dev + b.foff), b.n, b.blocks);
+  launch_plan(p.bp, e->stream);
   const hipError_t el = hipGetLastError();
   const hipError_t ec = hipStreamEndCapture(e->stream, &graph);
   EHIPCHK(el);
@@ -1911,6 +1966,7 @@ int mpdx_create(const mpdx_params *p, mpdx_solver **out)
   if (off("MPDX_NO_GRAPHS")) s->use_graphs = false;
   if (off("MPDX_NO_CONCURRENCY")) s->use_concurrency = false;
   if (off("MPDX_NO_ZBND")) s->use_zbnd = false;
+  if (off("MPDX_BATCH_EQNS")) s->batch_eqns = true;   // (a switch that turns the batched path on)
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -2392,8 +2448,8 @@ int mpdx_ensemble_stats(const mpdx_ensemble *e, int *launches, int64_t *tasks)
 {
   MPDX_NEED(e, "ensemble handle");
   const EnsPlan &p = e->plan[0].graph ? e->plan[0] : e->plan[1];
-  if (launches) *launches = (int)p.batches.size();
-  if (tasks) *tasks = (int64_t)p.tasks;
+  if (launches) *launches = (int)p.bp.batches.size();
+  if (tasks) *tasks = (int64_t)p.bp.tasks;
   return MPDX_OK;
 }
 

@@ -45,6 +45,22 @@ struct TaskRecorder {
   }
 };
 
+// a batched launch of k_task_list: its kernel, the byte offsets of its
+// tasks and of their block prefix table in BatchPlan::dev, tasks, blocks
+struct EnsBatch {
+  TaskLaunchFn fn;
+  size_t toff, foff;
+  int n;
+  unsigned blocks;
+};
+
+// recorded launches of several independent sequences merged into batches
+struct BatchPlan {
+  void *dev = nullptr;
+  std::vector<EnsBatch> batches;
+  size_t tasks = 0;
+};
+
 }  // namespace mpdx
 
 // per-kernel-class timing with HIP events on the solver's stream
@@ -64,6 +80,10 @@ struct mpdx_solver {
   int lev[MPDX_MAX_EQNS] = {};
   void *GC[3] = {};
   void *tmp[2][3] = {};
+  // antidiffusive-velocity scratch of equations 1.. when the equations'
+  // steps run batched (equation 0 uses tmp); tmp_sel >= 0 selects a set
+  void *tmp_eq[MPDX_MAX_EQNS][2][3] = {};
+  int tmp_sel = -1;
   void *flux[3] = {};
   void *mono[3] = {};
   void *pmin = nullptr, *pmax = nullptr, *bup = nullptr, *bdn = nullptr;
@@ -88,9 +108,13 @@ struct mpdx_solver {
   mpdx_fused_state *fz = nullptr;
   mpdx::TaskRecorder *rec = nullptr;   // set: launches are recorded, not made
   int tile_y = 0;            // interior tile height (0: the element type's default)
-  // one captured time step per time-level parity (fused path)
+  // one captured time step per time-level parity (fused path); with several
+  // equations the step's launches are batched across them (eqplan: the
+  // device tables the captured step reads)
   hipGraphExec_t graph[2] = {nullptr, nullptr};
+  mpdx::BatchPlan eqplan[2];
   bool use_graphs = true;
+  bool batch_eqns = false;   // MPDX_BATCH_EQNS=1: measured slower for 8 tracers (shell loses its overlap)
   // auxiliary stream (boundary shell next to the interior kernel) and its
   // fork / join events
   hipStream_t aux = nullptr;
