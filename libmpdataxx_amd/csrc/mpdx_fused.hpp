@@ -142,7 +142,11 @@ k_fz_donor(R *out, const R *in, const R *C0, const R *C1, const R *C2, Layout L,
 // are powers of two, so they are applied once at the end: |u|(1-|u|) frac
 // - u (s1 ndy + s2 ndz) / 8 (2D: - u s1 ndy / 8), with |u|(1-|u|) as one
 // fused multiply-add.
-template <class R, class G, int ND, bool IGA, bool ABS, bool PFC, class PF>
+// PAIRS (the interior kernel): the four-point sums of the tangential ratios
+// as sums of their two pairs, (P+ + P-) +- (M+ + M-), four additions where
+// the reference's left-to-right order takes five (rounding differs in the
+// last bit)
+template <class R, class G, int ND, bool IGA, bool ABS, bool PFC, class PF, bool PAIRS = false>
 __device__ inline R antidiff_core(R u, R s1, R s2, PF P)
 {
   auto Q = [&](int da, int db, int dc) -> R {
@@ -164,11 +168,19 @@ __device__ inline R antidiff_core(R u, R s1, R s2, PF P)
   const R a1 = Q(1, 0, 0), a0 = Q(0, 0, 0);
   const R fx = frac<PFC, G>(a1 - a0, a1 + a0);
   const R b11 = Q(1, 1, 0), b01 = Q(0, 1, 0), b1m = Q(1, -1, 0), b0m = Q(0, -1, 0);
-  const R ndy = frac<PFC, G>(b11 + b01 - b1m - b0m, b11 + b01 + b1m + b0m);
+  auto ratio = [](R pp, R p0, R mp, R m0) -> R {
+    if constexpr (PAIRS) {
+      const R P = pp + p0, M = mp + m0;
+      return frac<PFC, G>(P - M, P + M);
+    } else {
+      return frac<PFC, G>(pp + p0 - mp - m0, pp + p0 + mp + m0);
+    }
+  };
+  const R ndy = ratio(b11, b01, b1m, b0m);
   const R w = fma(-au, au, au) * fx;
   if (ND == 3) {
     const R c11 = Q(1, 0, 1), c01 = Q(0, 0, 1), c1m = Q(1, 0, -1), c0m = Q(0, 0, -1);
-    const R ndz = frac<PFC, G>(c11 + c01 - c1m - c0m, c11 + c01 + c1m + c0m);
+    const R ndz = ratio(c11, c01, c1m, c0m);
     return fma(-(u * R(0.125)), fma(s2, ndz, s1 * ndy), w);
   }
   return fma(-(u * R(0.125)), s1 * ndy, w);
@@ -496,6 +508,9 @@ __device__ __forceinline__ void ibeta(R &bup, R &bdn, R pc, R xp, R xm, R yp, R 
 {
   const R mx = fmx(fmx(fmx(pmax, pc), fmx(xp, xm)), fmx(fmx(yp, ym), fmx(zp, zm)));
   const R mn = fmn(fmn(fmn(pmin, pc), fmn(xp, xm)), fmn(fmn(yp, ym), fmn(zp, zm)));
+  // sums of non-negative terms: no cancellation (rewriting them as (A +- B)/2
+  // with A = sum |f| and B the net inflow cancels when every face flows the
+  // same way, and a denominator that should be 0 comes out as +-1 ulp of A)
   const R z = 0;
   const R up_den = (fmx(z, fxm) - fmn(z, fxp)) + (fmx(z, fym) - fmn(z, fyp)) + (fmx(z, fzm) - fmn(z, fzp));
   const R dn_den = (fmx(z, fxp) - fmn(z, fxm)) + (fmx(z, fyp) - fmn(z, fym)) + (fmx(z, fzp) - fmn(z, fzm));
@@ -509,6 +524,14 @@ __device__ __forceinline__ R imono(R v, R bul, R bdl, R bur, R bdr)
   const R m_dn_up = fmn(R(1), fmn(bdl, bur));
   const R m_up_dn = fmn(R(1), fmn(bul, bdr));
   return v * (v > 0 ? m_dn_up : m_up_dn);
+}
+
+// donor-cell flux of the limited velocity m = v * limiter (limiter >= 0):
+// upwind by the sign of v, which is the sign of m wherever m != 0 (the
+// flux is then 0 either way, up to the sign of a zero)
+template <class R> __device__ __forceinline__ R iflux_v(R l, R r, R m, R v)
+{
+  return m * (v > 0 ? l : r);
 }
 
 // donor-cell flux pospart(c) l + negpart(c) r: the same value (up to the
@@ -777,7 +800,7 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
         };
         const R s1 = (uy[s6(0)] + uy[s6(-1)] + S(IL_UY + su(0), -1, 0) + S(IL_UY + su(-1), -1, 0));
         const R s2 = (uz[s6(0)] + uz[s6(-1)] + S(IL_UZ + su(0), 0, -1) + S(IL_UZ + su(-1), 0, -1));
-        vx = antidiff_core<R, G, 3, IGA, false, false>(ux[s6(-1)], s1, s2, P);
+        vx = antidiff_core<R, G, 3, IGA, false, false, decltype(P), true>(ux[s6(-1)], s1, s2, P);
       }
       // halo columns at an open z boundary: the normal-component fill sets
       // V_x and V_y there to zero (bcond/open_3d.hpp fill_halos_vctr_nrml)
@@ -791,7 +814,7 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
           };
           const R s1 = (S(IL_UZ + su(-1), 1, 0) + uz[s6(-1)] + S(IL_UZ + su(-1), 1, -1) + S(IL_UZ + su(-1), 0, -1));
           const R s2 = (S(IL_UX + s3(-1), 1, 0) + ux[s6(-1)] + S(IL_UX + s3(-2), 1, 0) + ux[s6(-2)]);
-          vy = antidiff_core<R, G, 3, IGA, false, false>(uy[s6(-1)], s1, s2, P);
+          vy = antidiff_core<R, G, 3, IGA, false, false, decltype(P), true>(uy[s6(-1)], s1, s2, P);
         }
         {
           // V_z at face z+1/2 of plane p-1: normal z, tangential (x, y)
@@ -801,7 +824,7 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
           };
           const R s1 = (S(IL_UX + s3(-1), 0, 1) + ux[s6(-1)] + S(IL_UX + s3(-2), 0, 1) + ux[s6(-2)]);
           const R s2 = (S(IL_UY + su(-1), 0, 1) + uy[s6(-1)] + S(IL_UY + su(-1), -1, 1) + S(IL_UY + su(-1), -1, 0));
-          vz = antidiff_core<R, G, 3, IGA, false, false>(uz[s6(-1)], s1, s2, P);
+          vz = antidiff_core<R, G, 3, IGA, false, false, decltype(P), true>(uz[s6(-1)], s1, s2, P);
         }
         if (zh) vy = R(0);
         S(IL_VY, 0, 0) = vy;
@@ -862,7 +885,7 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
     if (q >= 3) {
       // x face owned by p-2 (between planes p-2 and p-1)
       const R mx = FCT ? imono(vx_[s2(-1)], bu_[s2(-1)], bd_[s2(-1)], bu, bd) : vx_[s2(-1)];
-      const R fx = IGA ? mx : iflux(ps[s6(-2)], ps[s6(-1)], mx);
+      const R fx = IGA ? mx : iflux_v(ps[s6(-2)], ps[s6(-1)], mx, vx_[s2(-1)]);
       // y and z faces of plane p-1 (flux ring slot s2(-1)); plane p-2 is s2(-2)
       if (!FCT) {
         yp = S(IL_PSI + s3(-1), 1, 0);
@@ -870,8 +893,8 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
       }
       const R my = FCT ? imono(vy, bu, bd, S(IL_BU, 1, 0), S(IL_BD, 1, 0)) : vy;
       const R mz = FCT ? imono(vz, bu, bd, S(IL_BU, 0, 1), S(IL_BD, 0, 1)) : vz;
-      S(IL_FY + s2(-1), 0, 0) = IGA ? my : iflux(ps[s6(-1)], yp, my);
-      S(IL_FZ + s2(-1), 0, 0) = IGA ? mz : iflux(ps[s6(-1)], zp, mz);
+      S(IL_FY + s2(-1), 0, 0) = IGA ? my : iflux_v(ps[s6(-1)], yp, my, vy);
+      S(IL_FZ + s2(-1), 0, 0) = IGA ? mz : iflux_v(ps[s6(-1)], zp, mz, vz);
       if (q >= 4 && out_ok) {
         const R r = ps[s6(-2)] + ((-fx + fx_[s2(-1)]) + (-S(IL_FY + s2(-2), 0, 0) + S(IL_FY + s2(-2), -1, 0))
                                   + (-S(IL_FZ + s2(-2), 0, 0) + S(IL_FZ + s2(-2), 0, -1))) / 1;
