@@ -217,14 +217,30 @@ int mpdx_set_advector_generator(mpdx_solver *h, int kind, const double *params, 
  * update_rhs does (mpdata_rhs.hpp:52-70); fn fills them with mpdx_set_rhs or
  * through mpdx_rhs_device (kernels on the solver's stream), reading the state
  * with mpdx_get_advectee or mpdx_advectee_device.  Forcing is added over the interior (apply_rhs,
- * mpdata_rhs.hpp:72-84).  The `mixed` scheme and hint_norhs are not
- * supported. */
+ * mpdata_rhs.hpp:72-84).  hint_norhs is not supported. */
 #define MPDX_RHS_NONE 0
 #define MPDX_RHS_EULER_A 1
 #define MPDX_RHS_EULER_B 2
 #define MPDX_RHS_TRAPEZ 3
+#define MPDX_RHS_MIXED 4
 typedef int (*mpdx_update_rhs_fn)(mpdx_solver *h, double dt, int at, void *user);
 int mpdx_set_rhs_scheme(mpdx_solver *h, int scheme, mpdx_update_rhs_fn fn, void *user);
+/* The mixed scheme (rhs_scheme_t::mixed, mpdata_rhs.hpp:24,112-124): the
+ * library applies no forcing itself but calls hook(h, MPDX_HOOK_*, user) where
+ * the reference calls hook_mixed_rhs_ante_loop / _ante_step / _post_step
+ * (the first advance's hook_ante_loop, before and after every step), and the
+ * hook implements the scheme with the two calls below: mpdx_update_rhs (the
+ * reference's update_rhs: zero the forcing over the interior, then the
+ * scheme's update function fills it) and mpdx_apply_rhs (apply_rhs: state
+ * += dt_arg * forcing over the interior).  Both also work for the other
+ * schemes. */
+#define MPDX_HOOK_ANTE_LOOP 0
+#define MPDX_HOOK_ANTE_STEP 1
+#define MPDX_HOOK_POST_STEP 2
+typedef int (*mpdx_rhs_hook_fn)(mpdx_solver *h, int hook, void *user);
+int mpdx_set_rhs_hooks(mpdx_solver *h, mpdx_rhs_hook_fn hook, void *user);
+int mpdx_update_rhs(mpdx_solver *h, double dt, int at);
+int mpdx_apply_rhs(mpdx_solver *h, double dt_arg);
 /* copy the forcing of equation e (interior, like mpdx_set_advectee) */
 int mpdx_set_rhs(mpdx_solver *h, int e, const void *buf, const int64_t strides[3], int on_device);
 /* device address of element (0, 0, 0) of the forcing / the current state of

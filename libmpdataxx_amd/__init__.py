@@ -35,14 +35,17 @@ EXPORTED = [
     "mpdx_panic_ptr", "mpdx_watch_panic", "mpdx_active_path", "mpdx_profile", "mpdx_profile_read",
     "mpdx_set_calc_gc", "mpdx_advector_device", "mpdx_set_advector_generator",
     "mpdx_set_rhs_scheme", "mpdx_set_rhs", "mpdx_rhs_device", "mpdx_advectee_device",
+    "mpdx_set_rhs_hooks", "mpdx_update_rhs", "mpdx_apply_rhs",
     "mpdx_advance_time", "mpdx_dt", "mpdx_set_x_chunk", "mpdx_set_tile_y",
     "mpdx_ensemble_create", "mpdx_ensemble_destroy", "mpdx_ensemble_advance", "mpdx_ensemble_tune",
     "mpdx_ensemble_x_chunk", "mpdx_ensemble_tile_y", "mpdx_ensemble_stats", "mpdx_ensemble_last_error",
 ]
 GC_NONE, GC_DEFORM = 0, 1
 CALC_GC_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p)
-RHS_NONE, RHS_EULER_A, RHS_EULER_B, RHS_TRAPEZ = 0, 1, 2, 3
+RHS_NONE, RHS_EULER_A, RHS_EULER_B, RHS_TRAPEZ, RHS_MIXED = 0, 1, 2, 3, 4
+HOOK_ANTE_LOOP, HOOK_ANTE_STEP, HOOK_POST_STEP = 0, 1, 2
 UPDATE_RHS_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_void_p)
+RHS_HOOK_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p)
 
 
 class Params(ctypes.Structure):
@@ -103,6 +106,9 @@ def lib():
         L.mpdx_rhs_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), i64p]
         L.mpdx_advectee_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), i64p]
         L.mpdx_set_rhs_scheme.argtypes = [P, ctypes.c_int, UPDATE_RHS_FN, P]
+        L.mpdx_set_rhs_hooks.argtypes = [P, RHS_HOOK_FN, P]
+        L.mpdx_update_rhs.argtypes = [P, ctypes.c_double, ctypes.c_int]
+        L.mpdx_apply_rhs.argtypes = [P, ctypes.c_double]
         L.mpdx_set_advector_generator.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.mpdx_set_x_chunk.argtypes = [P, ctypes.c_int]
         L.mpdx_set_tile_y.argtypes = [P, ctypes.c_int]
@@ -338,6 +344,24 @@ class Solver:
                 return -1
         self._rhs_cb = UPDATE_RHS_FN(tramp)
         self._check(self._L.mpdx_set_rhs_scheme(self._h, int(scheme), self._rhs_cb, None))
+
+    def set_rhs_hooks(self, hook):
+        """The mixed scheme's hooks: hook(which) with which = HOOK_ANTE_LOOP,
+        HOOK_ANTE_STEP or HOOK_POST_STEP; it calls update_rhs / apply_rhs."""
+        def tramp(h, which, user):
+            try:
+                hook(which)
+                return 0
+            except Exception:          # noqa: BLE001 - reported as an error code
+                return -1
+        self._rhs_hook = RHS_HOOK_FN(tramp)
+        self._check(self._L.mpdx_set_rhs_hooks(self._h, self._rhs_hook, None))
+
+    def update_rhs(self, dt, at):
+        self._check(self._L.mpdx_update_rhs(self._h, float(dt), int(at)))
+
+    def apply_rhs(self, dt):
+        self._check(self._L.mpdx_apply_rhs(self._h, float(dt)))
 
     def set_rhs(self, e, arr):
         a = np.ascontiguousarray(arr, dtype=self.dtype)

@@ -1468,6 +1468,21 @@ void launch_plan(const BatchPlan &p, hipStream_t st)
     b.fn(st, dev + b.toff, reinterpret_cast<const unsigned *>(dev + b.foff), b.n, b.blocks);
 }
 
+// the user's hooks of the mixed scheme (mpdata_rhs.hpp:112-124): they
+// implement it with mpdx_update_rhs / mpdx_apply_rhs
+int rhs_hook(mpdx_solver *s, int hook)
+{
+  if (!s->rhs_hook) {
+    s->err = "libmpdata++: the mixed rhs scheme needs hooks (mpdx_set_rhs_hooks)";
+    return MPDX_EINVAL;
+  }
+  if (s->rhs_hook(s, hook, s->rhs_hook_user) < 0) {
+    if (s->err.empty()) s->err = "libmpdata++: rhs hook failed";
+    return MPDX_EINVAL;
+  }
+  return 0;
+}
+
 template <class R, class G>
 int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
 {
@@ -1475,9 +1490,11 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
   nt += s->timestep;
   if (s->timestep == 0) {
     if (int rc = ante_loop<R, G>(s)) return rc;
-    // mpdata_rhs::hook_ante_loop (solvers/mpdata_rhs.hpp:116-133)
+    // mpdata_rhs::hook_ante_loop (solvers/mpdata_rhs.hpp:126-145)
     if (s->rhs_scheme == MPDX_RHS_TRAPEZ)
       if (int rc = update_rhs<R>(s, s->p.dt / 2, 0)) return rc;
+    if (s->rhs_scheme == MPDX_RHS_MIXED)
+      if (int rc = rhs_hook(s, MPDX_HOOK_ANTE_LOOP)) return rc;
   }
   const int rs = s->rhs_scheme;
   const bool fused = s->active_path == MPDX_PATH_FUSED;
@@ -1618,6 +1635,8 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
         if (rs == MPDX_RHS_EULER_B) apply_rhs<R>(s, s->p.dt);
       } else if (rs == MPDX_RHS_TRAPEZ) {
         apply_rhs<R>(s, s->p.dt / 2);
+      } else if (rs == MPDX_RHS_MIXED) {
+        if (int rc = rhs_hook(s, MPDX_HOOK_ANTE_STEP)) return rc;
       }
       if (fused) {
         if (int rc = fused_step<R, G>(s)) return rc;
@@ -1637,6 +1656,8 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
     } else if (rs == MPDX_RHS_TRAPEZ) {
       if (int rc = update_rhs<R>(s, s->p.dt / 2, 1)) return rc;
       apply_rhs<R>(s, s->p.dt / 2);
+    } else if (rs == MPDX_RHS_MIXED) {
+      if (int rc = rhs_hook(s, MPDX_HOOK_POST_STEP)) return rc;
     }
   }
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -2212,7 +2233,7 @@ int mpdx_set_calc_gc(mpdx_solver *s, mpdx_calc_gc_fn fn, void *user)
 int mpdx_set_rhs_scheme(mpdx_solver *s, int scheme, mpdx_update_rhs_fn fn, void *user)
 {
   MPDX_NEED(s, "solver handle");
-  if (scheme < MPDX_RHS_NONE || scheme > MPDX_RHS_TRAPEZ) {
+  if (scheme < MPDX_RHS_NONE || scheme > MPDX_RHS_MIXED) {
     s->err = "libmpdata++: unsupported rhs scheme";
     return MPDX_EUNSUPPORTED;
   }
@@ -2222,6 +2243,32 @@ int mpdx_set_rhs_scheme(mpdx_solver *s, int scheme, mpdx_update_rhs_fn fn, void 
   s->rhs_scheme = scheme;
   s->rhs_fn = fn;
   s->rhs_user = user;
+  return MPDX_OK;
+}
+
+int mpdx_set_rhs_hooks(mpdx_solver *s, mpdx_rhs_hook_fn hook, void *user)
+{
+  MPDX_NEED(s, "solver handle");
+  s->rhs_hook = hook;
+  s->rhs_hook_user = user;
+  return MPDX_OK;
+}
+
+int mpdx_update_rhs(mpdx_solver *s, double dt, int at)
+{
+  MPDX_NEED(s, "solver handle");
+  if (s->rhs_scheme == MPDX_RHS_NONE) { s->err = "libmpdata++: no rhs scheme set"; return MPDX_EINVAL; }
+  if (s->esz == 8) return update_rhs<double>(s, dt, at);
+  return update_rhs<float>(s, dt, at);
+}
+
+int mpdx_apply_rhs(mpdx_solver *s, double dt_arg)
+{
+  MPDX_NEED(s, "solver handle");
+  if (s->rhs_scheme == MPDX_RHS_NONE) { s->err = "libmpdata++: no rhs scheme set"; return MPDX_EINVAL; }
+  if (s->esz == 8) apply_rhs<double>(s, dt_arg);
+  else apply_rhs<float>(s, dt_arg);
+  HIPCHK(hipGetLastError());
   return MPDX_OK;
 }
 

@@ -129,6 +129,8 @@ struct mpdx_solver {
   int rhs_scheme = MPDX_RHS_NONE;
   mpdx_update_rhs_fn rhs_fn = nullptr;
   void *rhs_user = nullptr;
+  mpdx_rhs_hook_fn rhs_hook = nullptr;   // the mixed scheme's hooks
+  void *rhs_hook_user = nullptr;
   void *rhs[MPDX_MAX_EQNS] = {};
   mpdx_calc_gc_fn calc_gc = nullptr;
   void *calc_gc_user = nullptr;

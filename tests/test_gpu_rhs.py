@@ -73,3 +73,62 @@ def test_rhs_schemes_match_oracle(dtype, scheme):
     for e in range(2):
         assert rel_err(s.advectee(e).astype(np.float64), o.advectee(e)) <= (1e-12 if dtype == np.float64 else 1e-5)
     s.close()
+
+
+@pytest.mark.parametrize("scheme", [O.RHS_EULER_A, O.RHS_EULER_B, O.RHS_TRAPEZ])
+def test_mixed_scheme_hooks_reproduce_each_scheme(scheme):
+    """The mixed scheme (rhs_scheme_t::mixed): its three hooks, written to do
+    what mpdata_rhs.hpp does for euler_a / euler_b / trapez with the same
+    update_rhs / apply_rhs calls, give bit for bit the fields of that scheme."""
+    import libmpdataxx_amd as M
+    grid, bcs = (24, 20, 30), [(0, 0), (1, 1), (0, 0)]
+    psi, gc = setup_case(grid, bcs, 17)
+    print(f"mixed scheme as scheme={scheme}", flush=True)
+
+    def make():
+        s = M.Solver(grid, opts=M.FCT, n_iters=2, n_eqns=1, bcs=bcs, dt=0.5, max_abs_div_eps=1e300)
+        s.set_advectee(psi)
+        for d in range(3):
+            s.set_advector(gc[d], d)
+        return s
+
+    def forcing(s):
+        def upd(dt, at):
+            a = s.advectee(0)
+            s.set_rhs(0, 0.05 - 0.2 * a * a + 0.01 * at)
+        return upd
+
+    ref = make()
+    ref.set_rhs_scheme(scheme, forcing(ref))
+    ref.advance(4)
+    want = ref.advectee()
+    ref.close()
+
+    s = make()
+    s.set_rhs_scheme(M.RHS_MIXED, forcing(s))
+    dt = 0.5
+    calls = []
+
+    def hook(which):
+        calls.append(which)
+        if which == M.HOOK_ANTE_LOOP and scheme == O.RHS_TRAPEZ:
+            s.update_rhs(dt / 2, 0)
+        elif which == M.HOOK_ANTE_STEP:
+            if scheme in (O.RHS_EULER_A, O.RHS_EULER_B):
+                s.update_rhs(dt, 0)
+            if scheme == O.RHS_EULER_B:
+                s.apply_rhs(dt)
+            if scheme == O.RHS_TRAPEZ:
+                s.apply_rhs(dt / 2)
+        elif which == M.HOOK_POST_STEP:
+            if scheme == O.RHS_EULER_A:
+                s.apply_rhs(dt)
+            if scheme == O.RHS_TRAPEZ:
+                s.update_rhs(dt / 2, 1)
+                s.apply_rhs(dt / 2)
+    s.set_rhs_hooks(hook)
+    s.advance(4)
+    got = s.advectee()
+    s.close()
+    assert calls == [M.HOOK_ANTE_LOOP] + [M.HOOK_ANTE_STEP, M.HOOK_POST_STEP] * 4
+    assert np.array_equal(got, want)
