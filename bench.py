@@ -134,7 +134,7 @@ def run_multi_tracer(torch, M, steps, warmup, shape=(512, 512, 128), n_tr=8):
                                   f"{n_tr} tracers, reversing deformational advector generated each step",
                       "grid": list(shape), "tracers": n_tr},
            "alg_bytes_per_cell_step": 2 * n_tr * 4 + 3 * 4,
-           "constant_tracer_max_dev": dev_const}
+           "constant_tracer_max_dev": dev_const, "z_tiles": s.z_tiles}
     s.close()
     return res
 

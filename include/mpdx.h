@@ -179,6 +179,14 @@ int mpdx_watch_panic(mpdx_solver *h, const volatile unsigned char *flag);
 /* which path advance() uses for this configuration (MPDX_PATH_PERPASS/FUSED;
  * -1 for a null handle) */
 int mpdx_active_path(const mpdx_solver *h);
+/* how the last advance() call's interior kernel treated the z bounds (3D
+ * fused path; no counterpart in the reference, which has no such kernel):
+ * 0 = the boundary shell computed the cells next to them, 1 = open bounds on
+ * both z sides, computed in the interior kernel's tiles, 2 = cyclic bounds on
+ * both z sides whose fields repeat plane z = 0 at z = n-1, read by the tiles
+ * through the periodic images of the halo columns (decided at the start of
+ * each call by comparing those planes on the device); -1 for a null handle */
+int mpdx_z_tiles(const mpdx_solver *h);
 
 /* Time-varying advector (the reference's calc_gc() hook, solver_common.hpp:
  * 119-120,308, e.g. tests/mp3_paper_2018_JCP/reversing_deform/

@@ -32,7 +32,7 @@ EXPORTED = [
     "mpdx_set_stream", "mpdx_advector_extents", "mpdx_set_advectee",
     "mpdx_get_advectee", "mpdx_set_advector", "mpdx_get_advector",
     "mpdx_advance", "mpdx_reduce", "mpdx_reduce_noalloc", "mpdx_time", "mpdx_timestep",
-    "mpdx_panic_ptr", "mpdx_watch_panic", "mpdx_active_path", "mpdx_profile", "mpdx_profile_read",
+    "mpdx_panic_ptr", "mpdx_watch_panic", "mpdx_active_path", "mpdx_z_tiles", "mpdx_profile", "mpdx_profile_read",
     "mpdx_set_calc_gc", "mpdx_advector_device", "mpdx_set_advector_generator",
     "mpdx_set_rhs_scheme", "mpdx_set_rhs", "mpdx_rhs_device", "mpdx_advectee_device",
     "mpdx_set_rhs_hooks", "mpdx_update_rhs", "mpdx_apply_rhs",
@@ -101,6 +101,7 @@ def lib():
         L.mpdx_panic_ptr.argtypes = [P]
         L.mpdx_watch_panic.argtypes = [P, P]
         L.mpdx_active_path.argtypes = [P]
+        L.mpdx_z_tiles.argtypes = [P]
         L.mpdx_set_calc_gc.argtypes = [P, CALC_GC_FN, P]
         L.mpdx_advector_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), i64p]
         L.mpdx_rhs_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), i64p]
@@ -196,6 +197,13 @@ class Solver:
     @property
     def active_path(self):
         return self._L.mpdx_active_path(self._h)
+
+    @property
+    def z_tiles(self):
+        """How the last advance() treated the z bounds in the interior
+        kernel: 0 boundary shell, 1 open bounds in the tiles, 2 cyclic bounds
+        through the periodic images (fields repeating plane 0 at n-1)."""
+        return self._L.mpdx_z_tiles(self._h)
 
     def advector_shape(self, d):
         ext = (ctypes.c_int * 3)()

@@ -199,10 +199,30 @@ struct PerPass {
     return out;
   }
 
-  // > 0: the z faces of the normal-component fill (the zero-divergence or
-  // cyclic faces) are needed only in rows within zband of an x or y edge
-  // (the boundary shell next to z-boundary tiles)
-  int zband = 0;
+  // Rows (x, y) whose z halo of the velocity the interior kernel's
+  // z-boundary tiles compute (open z) or store themselves (cyclic z): the
+  // z-direction fills skip x in zskip_x x y in zskip_y.  zskip: the normal
+  // component's fill; zskip_nrml: the tangential components' too.
+  bool zskip = false, zskip_nrml = false;
+  rng zskip_x{0, -1}, zskip_y{0, -1};
+
+  // the rows of ra x rb (x, y) outside the skipped ones: x bands over all
+  // of rb, then y bands between them
+  std::vector<std::pair<rng, rng>> zrows(rng ra, rng rb) const
+  {
+    std::vector<std::pair<rng, rng>> v;
+    const rng mx = R_(std::max(ra.f, zskip_x.f), std::min(ra.l, zskip_x.l));
+    const rng my = R_(std::max(rb.f, zskip_y.f), std::min(rb.l, zskip_y.l));
+    if (mx.f > mx.l || my.f > my.l) {
+      v.emplace_back(ra, rb);
+      return v;
+    }
+    if (ra.f < mx.f) v.emplace_back(R_(ra.f, mx.f - 1), rb);
+    if (mx.l < ra.l) v.emplace_back(R_(mx.l + 1, ra.l), rb);
+    if (rb.f < my.f) v.emplace_back(mx, R_(rb.f, my.f - 1));
+    if (my.l < rb.l) v.emplace_back(mx, R_(my.l + 1, rb.l));
+    return v;
+  }
 
   // Scalar-style fills along d (mode 0 cyclic / 1 edge copy from the bcond
   // kind, or 2 = zero).  Fills queue up in a batch that runs as one launch;
@@ -210,6 +230,15 @@ struct PerPass {
   FillBatch<R> fb{};
 
   void fill(R *a, int d, int side, rng ra, rng rb, bool nrml)
+  {
+    if (zskip && zskip_nrml && nrml && L.nd == 3 && d == 2) {
+      for (const auto &r : zrows(ra, rb)) fill_rows(a, d, side, r.first, r.second, nrml);
+      return;
+    }
+    fill_rows(a, d, side, ra, rb, nrml);
+  }
+
+  void fill_rows(R *a, int d, int side, rng ra, rng rb, bool nrml)
   {
     const int kind = s->p.bc[d][side];
     int mode = kind == MPDX_BC_CYCLIC ? 0 : (nrml ? 2 : 1);
@@ -313,17 +342,12 @@ struct PerPass {
         bx[side] = make_box(lo, hi, ra.f, ra.l, rb.f, rb.l);
       }
     }
-    if (zband > 0 && nd == 3 && d == 2) {
-      // bands of (x, y) rows: x bands over every y, y bands between them
-      const int Z = zband;
-      const rng bs[4][2] = {{R_(ra.f, ra.f + Z - 1), rb}, {R_(ra.l - Z + 1, ra.l), rb},
-                            {R_(ra.f + Z, ra.l - Z), R_(rb.f, rb.f + Z - 1)},
-                            {R_(ra.f + Z, ra.l - Z), R_(rb.l - Z + 1, rb.l)}};
+    if (zskip && nd == 3 && d == 2) {
       std::vector<std::pair<Box, int>> v;
       for (int side = 0; side < 2; ++side)
-        for (const auto &b : bs)
-          v.emplace_back(make_box(bx[side].lo[0], bx[side].lo[0] + bx[side].ext[0] - 1, b[0].f, b[0].l, b[1].f,
-                                  b[1].l), side);
+        for (const auto &b : zrows(ra, rb))
+          v.emplace_back(make_box(bx[side].lo[0], bx[side].lo[0] + bx[side].ext[0] - 1, b.first.f, b.first.l,
+                                  b.second.f, b.second.l), side);
       if (s->rec) {
         RecGroup grp(s);
         for (const auto &bt : v) record_alng(avd, av1, av2, r, bt.first, nd, bt.second, cyc[bt.second], N, cfirst);
@@ -531,7 +555,7 @@ struct PerPass {
 // shapes, the generator's kernel); whatever changes one of them drops both
 void drop_step_graphs(mpdx_solver *s)
 {
-  for (int q = 0; q < 2; ++q) {
+  for (int q = 0; q < mpdx_solver::kGraphs; ++q) {
     if (s->graph[q]) (void)hipGraphExecDestroy(s->graph[q]);
     s->graph[q] = nullptr;
     if (s->eqplan[q].dev) (void)hipFree(s->eqplan[q].dev);
@@ -822,15 +846,20 @@ struct Fused {
   const Layout &L;
   BcMap m;
   unsigned o;
-  // open bounds on both z sides: with an interior kernel its tiles also
-  // compute the z-boundary cells, and the boundary shell has no z slabs
-  bool zb;
+  // With an interior kernel its tiles also compute the z-boundary cells,
+  // and the boundary shell has no z slabs, for open bounds on both z sides
+  // (zb) and for cyclic ones whose fields repeat plane z = 0 at z = n-1
+  // (seam, decided for each advance call)
+  bool zb, seam;
 
   explicit Fused(mpdx_solver *s_)
       : s(s_), pp(s_), L(s_->L), m(make_bcmap(s_)), o(s_->p.opts),
-        zb(s_->use_zbnd && s_->L.nd == 3 && s_->p.bc[2][0] == MPDX_BC_OPEN && s_->p.bc[2][1] == MPDX_BC_OPEN)
+        zb(s_->use_zbnd && s_->L.nd == 3 && s_->p.bc[2][0] == MPDX_BC_OPEN && s_->p.bc[2][1] == MPDX_BC_OPEN),
+        seam(s_->seam && s_->L.nd == 3 && s_->p.bc[2][0] == MPDX_BC_CYCLIC && s_->p.bc[2][1] == MPDX_BC_CYCLIC)
   {
   }
+
+  bool ztiles() const { return zb || seam; }
 
   R *A(void *p) const { return static_cast<R *>(p); }
 
@@ -1039,7 +1068,7 @@ struct Fused {
       return out;
     }
     int cl[3] = {lo[0], lo[1], lo[2]}, ch[3] = {hi[0], hi[1], hi[2]};
-    for (int d = 0; d < (zb_tiles && zb ? 2 : L.nd); ++d) {
+    for (int d = 0; d < (zb_tiles && ztiles() ? 2 : L.nd); ++d) {
       int a[3] = {cl[0], cl[1], cl[2]}, b[3] = {ch[0], ch[1], ch[2]};
       b[d] = ilo[d] - 1;
       out.push_back(make_box(a[0], b[0], a[1], b[1], a[2], b[2]));
@@ -1094,12 +1123,12 @@ struct Fused {
     }
     a.in_scale = in_scale; a.out_scale = out_scale;
     constexpr int TY = ITile<TYV>::Y, TZ = ITile<TYV>::Z;
-    // zb: the tiles cover every z cell (and store the velocity of every z
-    // face of the domain, face -1/2 included); the last tile is shifted
-    // back to end on the last halo column
-    a.zb = zb ? 1 : 0;
+    // zb / seam: the tiles cover every z cell (and store the velocity of
+    // every z face of the domain, open: face -1/2 included); the last tile
+    // is shifted back to end on the last halo column
+    a.zb = zb ? 1 : (seam ? 2 : 0);
     a.zs_max = std::max(L.n[2] - (TZ - 4), 0);
-    if (zb) {
+    if (ztiles()) {
       a.lo[2] = 0; a.hi[2] = L.n[2] - 1;
       a.vlo[2] = 0; a.vhi[2] = L.n[2] - 1;
     }
@@ -1183,9 +1212,17 @@ struct Fused {
     R **V = pp.corr(it, Vb);
     const bool fct = o & O_FCT;
     // next to z-boundary tiles the shell reads the z halos only near the x
-    // and y edges
-    pp.zband = use_int && zb ? ZBAND : 0;
-    psi_halo(psik, use_int && zb);
+    // and y edges: of psi within ZBAND rows; of the velocity in the rows
+    // the shell computes (open: the tiles compute the z halo of the other
+    // rows; cyclic: the tiles read none, and the tile rows the next pass's
+    // shell reads get theirs from ring_zfill after the tiles are done)
+    const bool zt = use_int && ztiles();
+    pp.zskip = zt;
+    pp.zskip_nrml = zt && seam;
+    const int zs = seam ? 4 : ZBAND;
+    pp.zskip_x = R_(zs, L.n[0] - 1 - zs);
+    pp.zskip_y = R_(zs, L.n[1] - 1 - zs);
+    psi_halo(psik, zt);
     {
       ProfScope ps(s, "fused_antidiff");
       if (use_int)
@@ -1230,6 +1267,26 @@ struct Fused {
     interior(out, psik, psin, U, V, storev, fuse0, in_scale, os);
   }
 
+  // seam mode, after a pass whose tiles stored the velocity: the z halo of
+  // the tile rows next to the boundary shell, which the next pass's shell
+  // reads (the tiles themselves read the z columns' periodic images, and
+  // the shell's own rows got their z halo from the shell's fills)
+  void ring_zfill(int it)
+  {
+    R *Vb[3];
+    R **V = pp.corr(it, Vb);
+    RingFillTask<R> t{};
+    t.v0 = V[0]; t.v1 = V[1]; t.v2 = V[2]; t.L = L;
+    t.x0 = 4; t.x1 = L.n[0] - 5; t.y0 = 4; t.y1 = L.n[1] - 5;
+    const dim3 g((unsigned)((t.threads() + 255) / 256));
+    if (s->rec) {
+      record(s, t, g);
+      return;
+    }
+    ProfScope ps(s, "fused_halo");
+    hipLaunchKernelGGL(k_task<RingFillTask<R>>, g, dim3(256), 0, s->stream, t);
+  }
+
   // one time step of equation e (solver_common.hpp:127-136 with advop inlined)
   void step_eqn(int e)
   {
@@ -1248,7 +1305,7 @@ struct Fused {
     R *Gb[3];
     R **gc = pp.av_of(s->GC, Gb);
     const bool use_int = interior_supported(s->p, L);
-    psi_halo(psin, zb && use_int && passes > 1);
+    psi_halo(psin, ztiles() && use_int && passes > 1);
     if (passes == 1 || !use_int) {
       donor(out_of(0), psin, gc, in_scale, passes == 1 ? out_scale : R(1), 0);
       for (int it = 1; it < passes; ++it)
@@ -1273,6 +1330,7 @@ struct Fused {
         }
         corrective_interior(out_of(it), out_of(it - 1), psin, it, passes, in_scale, os, it == 1);
         if (conc) join(main);
+        if (seam && it != passes - 1) ring_zfill(it);
       }
     }
     s->lev[e] ^= 1;
@@ -1483,6 +1541,60 @@ int rhs_hook(mpdx_solver *s, int hook)
   return 0;
 }
 
+// Seam mode for this advance call: cyclic bounds on both z sides, and every
+// field the interior tiles read repeats plane z = 0 at z = n-1 (psi of every
+// equation) and holds the cyclic images in its z halo (the advector, unless
+// the on-device generator rewrites it each step).  A step keeps this true:
+// cells z = 0 and z = n-1 then see equal neighbourhoods and get equal
+// updates, and the z-independent generator writes equal planes; a host
+// calc_gc callback or forcing could break it, so those keep the slab path.
+template <class R>
+int seam_check(mpdx_solver *s, bool *ok)
+{
+  *ok = false;
+  const Layout &L = s->L;
+  if (!s->use_seam || L.nd != 3 || s->p.bc[2][0] != MPDX_BC_CYCLIC || s->p.bc[2][1] != MPDX_BC_CYCLIC) return 0;
+  if (s->p.n_iters < 2 || !interior_supported(s->p, L) || s->calc_gc || s->rhs_scheme != MPDX_RHS_NONE) return 0;
+  if (!s->seam_flag) {
+    void *p = nullptr;
+    if (int rc = dev_alloc(s, &p, 3 * sizeof(int))) return rc;
+    s->seam_flag = (int *)p;
+  }
+  HIPCHK(hipMemsetAsync(s->seam_flag, 0, 3 * sizeof(int), s->stream));
+  const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
+  // tag: equation e (psi) or 64 + d (advector component d)
+  auto differ = [&](const void *a, int x0, int nx, int y0, int ny, const ZPairs &zp, int tag) {
+    const long t = (long)nx * ny * zp.n;
+    hipLaunchKernelGGL(k_zpairs_differ<R>, dim3((unsigned)((t + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream,
+                       (const R *)a, L, x0, nx, y0, ny, zp, tag, s->seam_flag);
+  };
+  const ZPairs edge{1, {0}, {n2 - 1}};
+  for (int e = 0; e < s->p.n_eqns; ++e) differ(s->psi[e][s->lev[e]], 0, n0, 0, n1, edge, e);
+  if (s->gen_kind == MPDX_GC_NONE) {
+    // rows the tiles read: x -1..n0-1, y -1..n1; tangential components at
+    // cells z (halo cells -m and n-1+m), the normal one at faces z + 1/2
+    // (halo faces -m and n-2+m), m = 1..hw
+    ZPairs tang{1, {0}, {n2 - 1}}, nrml{1, {n2 - 1}, {0}};
+    for (int m = 1; m <= L.hw; ++m) {
+      tang.za[tang.n] = -m; tang.zb[tang.n++] = n2 - 1 - m;
+      tang.za[tang.n] = n2 - 1 + m; tang.zb[tang.n++] = m;
+      nrml.za[nrml.n] = -m; nrml.zb[nrml.n++] = n2 - 1 - m;
+      if (m >= 2) { nrml.za[nrml.n] = n2 - 2 + m; nrml.zb[nrml.n++] = m - 1; }
+    }
+    for (int d = 0; d < 3; ++d) differ(s->GC[d], -1, n0 + 1, -1, n1 + 2, d == 2 ? nrml : tang, 64 + d);
+  }
+  HIPCHK(hipGetLastError());
+  int h[3] = {1, 0, 0};
+  HIPCHK(hipMemcpyAsync(h, s->seam_flag, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  *ok = h[0] == 0;
+  // MPDX_SEAM_DEBUG=1: say which field kept the slab path
+  if (h[0] && std::getenv("MPDX_SEAM_DEBUG"))
+    std::fprintf(stderr, "mpdx seam check: %s %d, column pair %d, row (%d, %d) differ\n",
+                 h[0] / 16 >= 64 ? "advector component" : "equation", h[0] / 16 % 64, h[0] % 16 - 1, h[1], h[2]);
+  return 0;
+}
+
 template <class R, class G>
 int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
 {
@@ -1500,6 +1612,10 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
   const bool fused = s->active_path == MPDX_PATH_FUSED;
   if (!fused)
     if (int rc = ensure_perpass(s)) return rc;
+  bool seam = false;
+  if (fused)
+    if (int rc = seam_check<R>(s, &seam)) return rc;
+  s->seam = seam;
   PerPass<R, G> pp(s);
   // on-device advector generator: the amplitude of every step of this call,
   // evaluated on the host like the oracle's, and a device step index
@@ -1558,7 +1674,7 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
     }
     if (panicked(s)) break;
     if (graphs) {
-      const int key = s->lev[0] & 1;
+      const int key = (s->lev[0] & 1) | (s->seam ? 2 : 0);
       if (!s->graph[key]) {
         int lev_save[MPDX_MAX_EQNS];
         for (int e = 0; e < s->p.n_eqns; ++e) lev_save[e] = s->lev[e];
@@ -1673,6 +1789,16 @@ int ante_dispatch(mpdx_solver *s)
   return ante_loop<float, GuardFast>(s);
 }
 
+int seam_dispatch(mpdx_solver *s)
+{
+  bool ok = false;
+  int rc = 0;
+  if (s->esz == 8) rc = seam_check<double>(s, &ok);
+  else rc = seam_check<float>(s, &ok);
+  s->seam = ok;
+  return rc;
+}
+
 int step_dispatch(mpdx_solver *s)
 {
   if (s->esz == 8) return fused_step<double, double>(s);
@@ -1724,11 +1850,14 @@ namespace {
     }                                                                       \
   } while (0)
 
+// members' time levels and seam modes
 std::vector<int> level_key(const mpdx_ensemble *e)
 {
   std::vector<int> k;
-  for (const mpdx_solver *s : e->m)
+  for (const mpdx_solver *s : e->m) {
     for (int q = 0; q < s->p.n_eqns; ++q) k.push_back(s->lev[q]);
+    k.push_back(s->seam ? 1 : 0);
+  }
   return k;
 }
 
@@ -1767,8 +1896,10 @@ int build_plan(mpdx_ensemble *e, EnsPlan &p)
     fail_at = i;
   }
   size_t q = 0;
-  for (mpdx_solver *s : e->m)
+  for (mpdx_solver *s : e->m) {
     for (int r = 0; r < s->p.n_eqns; ++r) s->lev[r] = saved[q++];
+    ++q;   // (the seam mode)
+  }
   if (rc) return member_error(e, fail_at, e->m[fail_at], rc);
 
   if (int rc2 = pack_plan(recs, p.bp, e->err)) return rc2;
@@ -1787,14 +1918,15 @@ int build_plan(mpdx_ensemble *e, EnsPlan &p)
 }
 
 // first-advance work of members that have not stepped yet (hook_ante_loop),
-// and every member's own pending work finished before the ensemble's stream
-// touches its arrays
+// each member's seam mode for this call, and every member's own pending
+// work finished before the ensemble's stream touches its arrays
 int ensemble_ante(mpdx_ensemble *e)
 {
   for (size_t i = 0; i < e->m.size(); ++i) {
     mpdx_solver *s = e->m[i];
     if (s->timestep == 0)
       if (int rc = ante_dispatch(s)) return member_error(e, i, s, rc);
+    if (int rc = seam_dispatch(s)) return member_error(e, i, s, rc);
     if (hipStreamSynchronize(s->stream) != hipSuccess) {
       e->err = "libmpdata++: ensemble member " + std::to_string(i) + ": HIP error";
       return MPDX_EHIP;
@@ -1987,6 +2119,7 @@ int mpdx_create(const mpdx_params *p, mpdx_solver **out)
   if (off("MPDX_NO_GRAPHS")) s->use_graphs = false;
   if (off("MPDX_NO_CONCURRENCY")) s->use_concurrency = false;
   if (off("MPDX_NO_ZBND")) s->use_zbnd = false;
+  if (off("MPDX_NO_SEAM")) s->use_seam = false;
   if (off("MPDX_BATCH_EQNS")) s->batch_eqns = true;   // (a switch that turns the batched path on)
   {
     int dev = 0, ncu = 0;
@@ -2221,6 +2354,15 @@ int mpdx_watch_panic(mpdx_solver *s, const volatile unsigned char *flag)
 }
 
 int mpdx_active_path(const mpdx_solver *s) { return s ? s->active_path : -1; }
+
+int mpdx_z_tiles(const mpdx_solver *s)
+{
+  if (!s) return -1;
+  if (s->active_path != MPDX_PATH_FUSED || s->p.n_iters < 2 || !interior_supported(s->p, s->L)) return 0;
+  const int *b = s->p.bc[2];
+  if (s->use_zbnd && b[0] == MPDX_BC_OPEN && b[1] == MPDX_BC_OPEN) return 1;
+  return s->seam && b[0] == MPDX_BC_CYCLIC && b[1] == MPDX_BC_CYCLIC ? 2 : 0;
+}
 
 int mpdx_set_calc_gc(mpdx_solver *s, mpdx_calc_gc_fn fn, void *user)
 {

@@ -487,9 +487,12 @@ struct InteriorArgs {
   int nb[3];            // tiles along z, y and x chunks
   int chunk;            // output planes per x chunk
   R in_scale, out_scale;
-  // zb: open bounds on both z sides and the tiles also cover the z-boundary
-  // cells (lo[2] = 0, hi[2] = n-1); the last z tile starts at zs_max so that
-  // it ends on the last halo column
+  // zb != 0: the tiles also cover the z-boundary cells (lo[2] = 0,
+  // hi[2] = n-1); the last z tile starts at zs_max so that it ends on the
+  // last halo column.  zb = 1: open bounds on both z sides.  zb = 2: cyclic
+  // bounds on both z sides with planes z = 0 and z = n-1 of psi and of the
+  // tangential advector components equal (the duplicated edge of a period
+  // of n-1 cells)
   int zb, zs_max;
 };
 
@@ -604,30 +607,40 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
   const int nsteps = x1 - x0 + 5;   // psi^k planes x0-2 .. x1+2
   // loads: y >= lo-2 >= 0; on the far side rows and columns past the tile's
   // last output cell are clamped to the first halo row (cell n), which the
-  // FUSE0 donor needs for psi^1 at cell n-1.  z-boundary tiles (zb) also
-  // hold the halo columns: scalars (psi^k, psi^n) are read at the open
-  // boundary's edge copy (bcond/open_3d.hpp:32-42, clamp to 0..n-1), the
-  // advector from its halo faces in memory (clamped to -hw..n)
+  // FUSE0 donor needs for psi^1 at cell n-1.  z-boundary tiles also hold
+  // the halo columns.  Open (zb = 1): scalars (psi^k, psi^n) are read at the
+  // open boundary's edge copy (bcond/open_3d.hpp:32-42, clamp to 0..n-1),
+  // the advector from its halo faces in memory (clamped to -hw..n).
+  // Cyclic (zb = 2): every field is read at the periodic image of the
+  // column in 0..n-1 (z < 0: z + n-1, z > n-1: z - (n-1)), the value the
+  // cyclic halo fill copies there (bcond/cyclic_3d.hpp:29-47,120-138); column n-1
+  // itself holds the image of column 0.  (Columns past n+1 occur only in a
+  // tile wider than the domain and feed no output: clamped to n-1.)
+  const bool seam = a.zb == 2;
+  auto zwrap = [n2](int c) { return c < 0 ? c + (n2 - 1) : (c > n2 - 1 ? min(c - (n2 - 1), n2 - 1) : c); };
   const int yl = min(y, L.n[1]);
-  const int zl = a.zb ? min(max(z, -L.hw), n2) : min(z, n2);
-  const int zls = a.zb ? min(max(z, 0), n2 - 1) : min(z, n2);
+  const int zl = seam ? zwrap(z) : (a.zb ? min(max(z, -L.hw), n2) : min(z, n2));
+  const int zls = seam ? zwrap(z) : (a.zb ? min(max(z, 0), n2 - 1) : min(z, n2));
+  // column the per-thread byte offsets count from (all offsets >= 0)
+  const int zbase = seam ? -L.hw : zlo;
   const long sx = L.st[0];
-  const long base0 = L.org + (long)(x0 - 2) * sx + (long)ylo * L.st[1] + zlo;
-  const unsigned toff = (unsigned)(((long)(yl - ylo) * L.st[1] + (zl - zlo)) * (long)sizeof(R));
-  const unsigned toff_s = (unsigned)(((long)(yl - ylo) * L.st[1] + (zls - zlo)) * (long)sizeof(R));
+  const long base0 = L.org + (long)(x0 - 2) * sx + (long)ylo * L.st[1] + zbase;
+  const unsigned toff = (unsigned)(((long)(yl - ylo) * L.st[1] + (zl - zbase)) * (long)sizeof(R));
+  const unsigned toff_s = (unsigned)(((long)(yl - ylo) * L.st[1] + (zls - zbase)) * (long)sizeof(R));
   const bool out_ok = ty >= 2 && ty < TY - 2 && tz >= 2 && tz < TZ - 2 && y <= a.hi[1] && z <= a.hi[2];
   const bool v_ok = out_ok && y >= a.vlo[1] && y <= a.vhi[1] && z >= a.vlo[2] && z <= a.vhi[2];
-  // z-boundary tiles: block flag (uniform), halo columns (z < 0 or z > n-1),
-  // the offset from a halo column to the edge column it copies, and the
-  // velocity of face -1/2 (cell -1), which belongs to the domain's velocity
-  const bool blk_zb = a.zb && (zlo < 0 || zlo + TZ - 1 > n2 - 1);
-  const bool zh = a.zb && (z < 0 || z > n2 - 1);
+  // open z-boundary tiles: block flag (uniform), halo columns (z < 0 or
+  // z > n-1), the offset from a halo column to the edge column it copies,
+  // and the velocity of face -1/2 (cell -1), which belongs to the domain's
+  // velocity
+  const bool blk_zb = a.zb == 1 && (zlo < 0 || zlo + TZ - 1 > n2 - 1);
+  const bool zh = a.zb == 1 && (z < 0 || z > n2 - 1);
   const int e_off = z < 0 ? -z : (z > n2 - 1 ? n2 - 1 - z : 0);
   // lane offsets of the zero-divergence inputs of halo cells -1 and n: the
   // edge column (V_x, V_y) and the V_z face (face 1/2 of cell 0, face
   // n-3/2 of cell n-2)
   const int zx1 = z == -1 ? 1 : (z == n2 ? -1 : 0), zx2 = z == -1 ? 1 : (z == n2 ? -2 : 0);
-  const bool vzm_ok = a.zb && z == -1 && ty >= 2 && ty < TY - 2 && y <= a.hi[1] && y >= a.vlo[1] &&
+  const bool vzm_ok = a.zb == 1 && z == -1 && ty >= 2 && ty < TY - 2 && y <= a.hi[1] && y >= a.vlo[1] &&
                       y <= a.vhi[1];
   // steps whose velocity plane (p-1) is stored: planes x0..x1 within vlo..vhi
   const int vq0 = max(3, a.vlo[0] - x0 + 3), vq1 = min(nsteps - 2, a.vhi[0] - x0 + 3);
@@ -689,9 +702,10 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
     else if (tid < 3 * TZ + 2 * TY) { kind = 1; fy = -1; fz = tid - 2 * TZ - 2 * TY; }
     else { kind = 2; fy = tid - 3 * TZ - 2 * TY; fz = -1; }
     const int gy = min(ylo + fy, L.n[1]);
-    const int gz = !a.zb ? min(zlo + fz, n2) : (kind == 0 ? min(max(zlo + fz, 0), n2 - 1)
-                                                          : min(max(zlo + fz, -L.hw), n2));
-    const long off = ((long)(gy - ylo) * L.st[1] + (gz - zlo)) * (long)sizeof(R);
+    const int gz = seam ? zwrap(zlo + fz)
+                        : (!a.zb ? min(zlo + fz, n2) : (kind == 0 ? min(max(zlo + fz, 0), n2 - 1)
+                                                                  : min(max(zlo + fz, -L.hw), n2)));
+    const long off = ((long)(gy - ylo) * L.st[1] + (gz - zbase)) * (long)sizeof(R);
     const R *arr = kind == 0 ? a.psin : (kind == 1 ? a.U1 : a.U2);
     fr_g = at(arr, 0) + off;
     fr_lds = (kind == 0 ? IL_PN : (kind == 1 ? IL_UY : IL_UZ)) * PL + (fy + 1) * PW + fz + 1;

@@ -8,6 +8,8 @@
 // Citations are path:line in the reference tree (libmpdata++/...).
 #pragma once
 
+#include <type_traits>
+
 #include "mpdx_common.hpp"
 
 namespace mpdx {
@@ -362,6 +364,34 @@ __global__ void k_gen_deform(R *g0, R *g1, R *g2, Layout L, Box bx, const double
   g0[p] = R(c - deform_corner(i, j - 1, sx2, sy2, amp));
   g1[p] = R(deform_corner(i - 1, j, sx2, sy2, amp) - c);
   if (L.nd == 3) g2[p] = R(0);
+}
+
+// pairs of z columns compared by k_zpairs_differ
+struct ZPairs {
+  int n;
+  int za[5], zb[5];
+};
+
+// *flag = tag * 16 + q + 1 where a(x, y, za[q]) and a(x, y, zb[q]) differ bit
+// for bit, for a row (x, y) of [x0, x0 + nx) x [y0, y0 + ny) and a pair q
+// (any one of the differing rows); flag[1], flag[2] = its x and y
+template <class R>
+__global__ void k_zpairs_differ(const R *a, Layout L, int x0, int nx, int y0, int ny, ZPairs zp, int tag,
+                                int *flag)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long rows = (long)nx * ny;
+  if (t >= rows * zp.n) return;
+  const int q = (int)(t / rows);
+  const long r = t - q * rows;
+  const int i = x0 + (int)(r / ny), j = y0 + (int)(r % ny);
+  const long p = L.org + (long)i * L.st[0] + (long)j * L.st[1];
+  using B = typename std::conditional<sizeof(R) == 4, unsigned, unsigned long long>::type;
+  if (__builtin_bit_cast(B, a[p + (long)zp.za[q] * L.st[2]]) != __builtin_bit_cast(B, a[p + (long)zp.zb[q] * L.st[2]])) {
+    flag[0] = tag * 16 + q + 1;
+    flag[1] = i;
+    flag[2] = j;
+  }
 }
 
 // explicit forcing (solvers/mpdata_rhs.hpp:52-93): zero the forcing array

@@ -108,11 +108,13 @@ struct mpdx_solver {
   mpdx_fused_state *fz = nullptr;
   mpdx::TaskRecorder *rec = nullptr;   // set: launches are recorded, not made
   int tile_y = 0;            // interior tile height (0: the element type's default)
-  // one captured time step per time-level parity (fused path); with several
-  // equations the step's launches are batched across them (eqplan: the
-  // device tables the captured step reads)
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
-  mpdx::BatchPlan eqplan[2];
+  // one captured time step per time-level parity and seam mode (fused
+  // path, index parity + 2 seam); with several equations the step's
+  // launches are batched across them (eqplan: the device tables the
+  // captured step reads)
+  static constexpr int kGraphs = 4;
+  hipGraphExec_t graph[kGraphs] = {};
+  mpdx::BatchPlan eqplan[kGraphs];
   bool use_graphs = true;
   bool batch_eqns = false;   // MPDX_BATCH_EQNS=1: measured slower for 8 tracers (shell loses its overlap)
   // auxiliary stream (boundary shell next to the interior kernel) and its
@@ -123,6 +125,12 @@ struct mpdx_solver {
   // with open bounds on both z sides the interior tiles also compute the
   // z-boundary cells (no z slabs in the boundary shell)
   bool use_zbnd = true;
+  // with cyclic bounds on both z sides the tiles also compute the
+  // z-boundary cells while the fields repeat plane z = 0 at z = n-1 (seam:
+  // checked on the device at the start of each advance call)
+  bool use_seam = true;
+  bool seam = false;
+  int *seam_flag = nullptr;
   int ix_chunk = 0;         // output planes per x chunk of the interior kernel (0: automatic)
   int n_cu = 256;           // compute units of the device
   // explicit forcing (mpdata_rhs): scheme, user update_rhs, forcing arrays

@@ -149,6 +149,55 @@ template <class R> struct AlngTask : TaskGrid {
   }
 };
 
+// z halo of the velocity under cyclic z bounds in a ring of rows: x = x0
+// and x = x1 with y in [y0, y1], and y = y0 and y = y1 with x in (x0, x1).
+// Tangential components: halo cells -m and n-1+m copy cells n-1-m and m;
+// the normal one: halo faces -m and n-2+m copy faces n-1-m and m-1
+// (bcond/cyclic_3d.hpp:44-58,135-149), m = 1..hw.  One thread per row,
+// component, side and m.
+template <class R>
+__device__ inline void ring_zfill_one(R *v0, R *v1, R *v2, const Layout &L, int x0, int x1, int y0, int y1, int t)
+{
+  const int hw = L.hw, items = 6 * hw;
+  const int nrx = y1 - y0 + 1, nry = x1 - x0 - 1;
+  const int r = t / items, k = t - r * items;
+  if (r >= 2 * (nrx + nry)) return;
+  int x, y;
+  if (r < 2 * nrx) {
+    x = r < nrx ? x0 : x1;
+    y = y0 + r % nrx;
+  } else {
+    const int q = r - 2 * nrx;
+    y = q < nry ? y0 : y1;
+    x = x0 + 1 + q % nry;
+  }
+  const int c = k / (2 * hw), side = (k / hw) % 2, m = 1 + k % hw;
+  R *a = c == 0 ? v0 : (c == 1 ? v1 : v2);
+  const int n2 = L.n[2];
+  const long row = L.org + (long)x * L.st[0] + (long)y * L.st[1];
+  int dst, src;
+  if (side == 0) { dst = -m; src = n2 - 1 - m; }
+  else if (c < 2) { dst = n2 - 1 + m; src = m; }
+  else { dst = n2 - 2 + m; src = m - 1; }
+  a[row + (long)dst * L.st[2]] = a[row + (long)src * L.st[2]];
+}
+
+template <class R> struct RingFillTask : TaskGrid {
+  static constexpr int kThreads = 256;
+  R *v0, *v1, *v2;
+  Layout L;
+  int x0, x1, y0, y1;
+  long threads() const { return 2L * ((y1 - y0 + 1) + (x1 - x0 - 1)) * 6 * L.hw; }
+  __device__ void run(TaskBlk b) const { ring_zfill_one(v0, v1, v2, L, x0, x1, y0, y1, (int)(b.x * 256u + threadIdx.x)); }
+};
+
+// one task launched on its own
+template <class T>
+__global__ void __launch_bounds__(T::kThreads) k_task(const T t)
+{
+  t.run(TaskBlk{blockIdx.x, blockIdx.y, blockIdx.z});
+}
+
 // one launch of the interior corrective pass (k_fz_interior); its block
 // count is a multiple of 8, so a batch of these keeps the XCD tile order
 template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV>

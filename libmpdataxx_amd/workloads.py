@@ -151,14 +151,16 @@ def random_field(shape, seed, lo=0.1, hi=1.0):
 
 
 def streamfunction_advector(grid, seed, max_courant=0.5, n_modes=4, periodic=True,
-                            quantum=2.0 ** -20):
+                            quantum=2.0 ** -20, exact_period=False):
     """Discretely divergence-free random advector from a streamfunction.
 
     The streamfunction lives on cell corners and is quantised to a dyadic
     grid so that every advector value and every divergence sum is exact.
     For periodic domains the corner field has period n-1 so the advector is
-    consistent across the duplicated edge cell.  3D: horizontal (x, y) flow
-    per z-level plus a w built from a second streamfunction in (x, z)."""
+    consistent across the duplicated edge cell (bit for bit with
+    exact_period, which evaluates a corner at its index reduced modulo the
+    period).  3D: horizontal (x, y) flow per z-level plus a w built from a
+    second streamfunction in (x, z)."""
     rng = np.random.default_rng(seed)
     nd = len(grid)
     if isinstance(periodic, bool):
@@ -170,6 +172,9 @@ def streamfunction_advector(grid, seed, max_courant=0.5, n_modes=4, periodic=Tru
         ys = np.arange(-2, ny) + 0.5
         px = (nx - 1) if perx else 2 * nx
         py = (ny - 1) if pery else 2 * ny
+        if exact_period:
+            xs = np.mod(np.arange(-2, nx), px) + 0.5
+            ys = np.mod(np.arange(-2, ny), py) + 0.5
         f = np.zeros((xs.size, ys.size))
         for _ in range(n_modes):
             kx = rng.integers(1, 3)
@@ -333,7 +338,8 @@ def level_streamfunction_advector(grid, seed, max_courant=0.5, n_modes=8, quantu
         f += rng.normal() * np.sin(2 * np.pi * kx * xs[:, None] / (nx - 1) + ph1) \
             * np.sin(2 * np.pi * ky * ys[None, :] / (ny - 1) + ph2)
     phz = rng.uniform(0, 2 * np.pi)
-    lev = 1.0 + 0.5 * np.sin(2 * np.pi * np.arange(nz) / (nz - 1) + phz)   # period nz-1
+    # period nz-1, exactly: level nz-1 is evaluated as level 0
+    lev = 1.0 + 0.5 * np.sin(2 * np.pi * (np.arange(nz) % (nz - 1)) / (nz - 1) + phz)
     gx0 = f[:, 1:] - f[:, :-1]
     gy0 = -(f[1:, :] - f[:-1, :])
     scale = max_courant / (max(np.abs(gx0).max(), np.abs(gy0).max()) * lev.max())

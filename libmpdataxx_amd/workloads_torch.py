@@ -34,7 +34,8 @@ def multi_tracer_fields(shape, dev):
     constant 1."""
     n0, n1, n2 = shape
     out = []
-    ix = [torch.arange(n, dtype=torch.float64, device=dev) for n in shape]
+    # cell n-1 is evaluated as cell 0, so the duplicated edge repeats bit for bit
+    ix = [torch.arange(n, dtype=torch.float64, device=dev) % (n - 1) for n in shape]
     X, Y, Z = ix[0][:, None, None], ix[1][None, :, None], ix[2][None, None, :]
     for t in range(8):
         g = torch.Generator().manual_seed(4000 + t)
