@@ -184,8 +184,11 @@ int mpdx_active_path(const mpdx_solver *h);
  * 0 = the boundary shell computed the cells next to them, 1 = open bounds on
  * both z sides, computed in the interior kernel's tiles, 2 = cyclic bounds on
  * both z sides whose fields repeat plane z = 0 at z = n-1, read by the tiles
- * through the periodic images of the halo columns (decided at the start of
- * each call by comparing those planes on the device); -1 for a null handle */
+ * through the periodic images of the halo columns, 3 = cyclic bounds in all
+ * three dimensions with fields repeating plane 0 at n-1 along each: the tiles
+ * compute every cell and there is no boundary shell (both decided at the
+ * start of each call by comparing those planes on the device); -1 for a
+ * null handle */
 int mpdx_z_tiles(const mpdx_solver *h);
 
 /* Time-varying advector (the reference's calc_gc() hook, solver_common.hpp:

@@ -202,7 +202,8 @@ class Solver:
     def z_tiles(self):
         """How the last advance() treated the z bounds in the interior
         kernel: 0 boundary shell, 1 open bounds in the tiles, 2 cyclic bounds
-        through the periodic images (fields repeating plane 0 at n-1)."""
+        through the periodic images (fields repeating plane 0 at n-1), 3 the
+        same in x and y too (no boundary shell)."""
         return self._L.mpdx_z_tiles(self._h)
 
     def advector_shape(self, d):

@@ -26,10 +26,10 @@
 
 // the FP32 default-guard interior kernels are instantiated in mpdx_interior.hip
 namespace mpdx {
-#define MPDX_EXTERN(R, G, F, I, S, Z, T)                                                       \
-  extern template __global__ void k_fz_interior<R, G, F, I, S, Z, T>(InteriorArgs<R>);         \
-  extern template __global__ void k_task_list<InteriorTask<R, G, F, I, S, Z, T>>(              \
-      const InteriorTask<R, G, F, I, S, Z, T> *, const unsigned *, int);
+#define MPDX_EXTERN(R, G, F, I, S, Z, T, W)                                                    \
+  extern template __global__ void k_fz_interior<R, G, F, I, S, Z, T, W>(InteriorArgs<R>);      \
+  extern template __global__ void k_task_list<InteriorTask<R, G, F, I, S, Z, T, W>>(           \
+      const InteriorTask<R, G, F, I, S, Z, T, W> *, const unsigned *, int);
 MPDX_INTERIOR_F32_FAST(MPDX_EXTERN)
 #undef MPDX_EXTERN
 }  // namespace mpdx
@@ -849,13 +849,17 @@ struct Fused {
   // With an interior kernel its tiles also compute the z-boundary cells,
   // and the boundary shell has no z slabs, for open bounds on both z sides
   // (zb) and for cyclic ones whose fields repeat plane z = 0 at z = n-1
-  // (seam, decided for each advance call)
-  bool zb, seam;
+  // (seam, decided for each advance call); with cyclic bounds everywhere
+  // and fields repeating across every period (full) the tiles compute every
+  // cell and there is no boundary shell at all
+  bool zb, seam, full;
 
   explicit Fused(mpdx_solver *s_)
       : s(s_), pp(s_), L(s_->L), m(make_bcmap(s_)), o(s_->p.opts),
         zb(s_->use_zbnd && s_->L.nd == 3 && s_->p.bc[2][0] == MPDX_BC_OPEN && s_->p.bc[2][1] == MPDX_BC_OPEN),
-        seam(s_->seam && s_->L.nd == 3 && s_->p.bc[2][0] == MPDX_BC_CYCLIC && s_->p.bc[2][1] == MPDX_BC_CYCLIC)
+        seam(s_->seam >= 1 && s_->L.nd == 3 && s_->p.bc[2][0] == MPDX_BC_CYCLIC && s_->p.bc[2][1] == MPDX_BC_CYCLIC),
+        full(seam && s_->seam == 2 && s_->p.bc[0][0] == MPDX_BC_CYCLIC && s_->p.bc[0][1] == MPDX_BC_CYCLIC &&
+             s_->p.bc[1][0] == MPDX_BC_CYCLIC && s_->p.bc[1][1] == MPDX_BC_CYCLIC)
   {
   }
 
@@ -1132,27 +1136,43 @@ struct Fused {
       a.lo[2] = 0; a.hi[2] = L.n[2] - 1;
       a.vlo[2] = 0; a.vhi[2] = L.n[2] - 1;
     }
+    // full: every x and y cell too (the last y tile shifted back like z's)
+    a.ys_max = std::max(L.n[1] - (TY - 4), 0);
+    if (full)
+      for (int d = 0; d < 2; ++d) {
+        a.lo[d] = 0; a.hi[d] = L.n[d] - 1;
+        a.vlo[d] = 0; a.vhi[d] = L.n[d] - 1;
+      }
     a.nb[0] = (a.hi[2] - a.lo[2] + 1 + TZ - 5) / (TZ - 4);
-    a.nb[1] = (L.n[1] - 4 + TY - 5) / (TY - 4);
+    a.nb[1] = (a.hi[1] - a.lo[1] + 1 + TY - 5) / (TY - 4);
     // x chunk: as long as possible (the 4-plane ramp of each chunk is
     // overhead) while leaving at least two tiles per CU
+    const int nxc = a.hi[0] - a.lo[0] + 1;
     int chunk = s->ix_chunk;
     if (chunk <= 0) {
       chunk = 256;
-      while (chunk > 16 && (long)a.nb[0] * a.nb[1] * ((L.n[0] - 4 + chunk - 1) / chunk) < 2L * s->n_cu)
+      while (chunk > 16 && (long)a.nb[0] * a.nb[1] * ((nxc + chunk - 1) / chunk) < 2L * s->n_cu)
         chunk /= 2;
     }
     a.chunk = chunk;
-    a.nb[2] = (L.n[0] - 4 + a.chunk - 1) / a.chunk;
+    a.nb[2] = (nxc + a.chunk - 1) / a.chunk;
     const int ntile = a.nb[0] * a.nb[1] * a.nb[2];
     const dim3 g(8 * ((ntile + 7) / 8));   // tiles are spread over the 8 XCDs in the kernel
+    if (full) launch_interior<FCT, IGA, STOREV, FUSE0, TYV, true>(a, g);
+    else launch_interior<FCT, IGA, STOREV, FUSE0, TYV, false>(a, g);
+  }
+
+  template <bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV, bool WXY>
+  void launch_interior(const InteriorArgs<R> &a, dim3 g)
+  {
     if (s->rec) {
-      InteriorTask<R, G, FCT, IGA, STOREV, FUSE0, TYV> t{};
+      InteriorTask<R, G, FCT, IGA, STOREV, FUSE0, TYV, WXY> t{};
       t.a = a;
       record(s, t, g);
       return;
     }
-    hipLaunchKernelGGL((k_fz_interior<R, G, FCT, IGA, STOREV, FUSE0, TYV>), g, dim3(TZ, TY), 0, s->stream, a);
+    hipLaunchKernelGGL((k_fz_interior<R, G, FCT, IGA, STOREV, FUSE0, TYV, WXY>), g,
+                       dim3(ITile<TYV>::Z, ITile<TYV>::Y), 0, s->stream, a);
   }
 
   // psi: psi^k, or unused with fuse0 (psi^1 computed from psin and U = GC)
@@ -1305,6 +1325,15 @@ struct Fused {
     R *Gb[3];
     R **gc = pp.av_of(s->GC, Gb);
     const bool use_int = interior_supported(s->p, L);
+    if (full && use_int && passes > 1) {
+      // every cell in the interior kernel, reading halos at their periodic
+      // images: no halo fills and no boundary shell
+      for (int it = 1; it < passes; ++it)
+        corrective_interior(out_of(it), out_of(it - 1), psin, it, passes, in_scale,
+                            it == passes - 1 ? out_scale : R(1), it == 1);
+      s->lev[e] ^= 1;
+      return;
+    }
     psi_halo(psin, ztiles() && use_int && passes > 1);
     if (passes == 1 || !use_int) {
       donor(out_of(0), psin, gc, in_scale, passes == 1 ? out_scale : R(1), 0);
@@ -1541,57 +1570,69 @@ int rhs_hook(mpdx_solver *s, int hook)
   return 0;
 }
 
-// Seam mode for this advance call: cyclic bounds on both z sides, and every
-// field the interior tiles read repeats plane z = 0 at z = n-1 (psi of every
-// equation) and holds the cyclic images in its z halo (the advector, unless
-// the on-device generator rewrites it each step).  A step keeps this true:
-// cells z = 0 and z = n-1 then see equal neighbourhoods and get equal
-// updates, and the z-independent generator writes equal planes; a host
-// calc_gc callback or forcing could break it, so those keep the slab path.
+// Seam mode for this advance call (1: z, 2: x, y and z): cyclic bounds on
+// both sides of those dimensions, and every field the interior tiles read
+// repeats plane 0 at plane n-1 along each of them (psi of every equation)
+// and holds the periodic images in its halo (the advector, unless the
+// on-device generator rewrites it each step: its corner tables repeat
+// across the period by construction).  A step keeps this true: cells 0 and
+// n-1 then see equal neighbourhoods and get equal updates.  A host calc_gc
+// callback or forcing could break it, so those keep the slab path.
 template <class R>
-int seam_check(mpdx_solver *s, bool *ok)
+int seam_check(mpdx_solver *s, int *mode)
 {
-  *ok = false;
+  *mode = 0;
   const Layout &L = s->L;
-  if (!s->use_seam || L.nd != 3 || s->p.bc[2][0] != MPDX_BC_CYCLIC || s->p.bc[2][1] != MPDX_BC_CYCLIC) return 0;
+  auto cyclic = [&](int d) { return s->p.bc[d][0] == MPDX_BC_CYCLIC && s->p.bc[d][1] == MPDX_BC_CYCLIC; };
+  if (!s->use_seam || L.nd != 3 || !cyclic(2)) return 0;
   if (s->p.n_iters < 2 || !interior_supported(s->p, L) || s->calc_gc || s->rhs_scheme != MPDX_RHS_NONE) return 0;
   if (!s->seam_flag) {
     void *p = nullptr;
-    if (int rc = dev_alloc(s, &p, 3 * sizeof(int))) return rc;
+    if (int rc = dev_alloc(s, &p, 9 * sizeof(int))) return rc;
     s->seam_flag = (int *)p;
   }
-  HIPCHK(hipMemsetAsync(s->seam_flag, 0, 3 * sizeof(int), s->stream));
-  const int n0 = L.n[0], n1 = L.n[1], n2 = L.n[2];
-  // tag: equation e (psi) or 64 + d (advector component d)
-  auto differ = [&](const void *a, int x0, int nx, int y0, int ny, const ZPairs &zp, int tag) {
-    const long t = (long)nx * ny * zp.n;
-    hipLaunchKernelGGL(k_zpairs_differ<R>, dim3((unsigned)((t + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream,
-                       (const R *)a, L, x0, nx, y0, ny, zp, tag, s->seam_flag);
+  HIPCHK(hipMemsetAsync(s->seam_flag, 0, 9 * sizeof(int), s->stream));
+  const bool xyz = cyclic(0) && cyclic(1);
+  // tag: equation e (psi) or 64 + c (advector component c); flags 3 d .. 3 d + 2
+  auto differ = [&](const void *a, int d, bool halo, const PlanePairs &pp, int tag) {
+    const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+    // rows: the interior (psi) or the rows the tiles read, one halo row on
+    // each side (the advector)
+    const int i0 = halo ? -1 : 0, j0 = halo ? -1 : 0;
+    const int ni = L.n[d1] + (halo ? 2 : 0), nj = L.n[d2] + (halo ? 2 : 0);
+    const long t = (long)ni * nj * pp.n;
+    hipLaunchKernelGGL(k_pairs_differ<R>, dim3((unsigned)((t + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream,
+                       (const R *)a, L, d, i0, ni, j0, nj, pp, tag, s->seam_flag + 3 * d);
   };
-  const ZPairs edge{1, {0}, {n2 - 1}};
-  for (int e = 0; e < s->p.n_eqns; ++e) differ(s->psi[e][s->lev[e]], 0, n0, 0, n1, edge, e);
-  if (s->gen_kind == MPDX_GC_NONE) {
-    // rows the tiles read: x -1..n0-1, y -1..n1; tangential components at
-    // cells z (halo cells -m and n-1+m), the normal one at faces z + 1/2
-    // (halo faces -m and n-2+m), m = 1..hw
-    ZPairs tang{1, {0}, {n2 - 1}}, nrml{1, {n2 - 1}, {0}};
+  for (int d = xyz ? 0 : 2; d < 3; ++d) {
+    const int n = L.n[d];
+    const PlanePairs edge{1, {0}, {n - 1}};
+    for (int e = 0; e < s->p.n_eqns; ++e) differ(s->psi[e][s->lev[e]], d, false, edge, e);
+    if (s->gen_kind != MPDX_GC_NONE) continue;
+    // components tangential to d at cells (halo cells -m and n-1+m), the
+    // normal one at faces + 1/2 (halo faces -m and n-2+m), m = 1..hw
+    PlanePairs tang{1, {0}, {n - 1}}, nrml{1, {n - 1}, {0}};
     for (int m = 1; m <= L.hw; ++m) {
-      tang.za[tang.n] = -m; tang.zb[tang.n++] = n2 - 1 - m;
-      tang.za[tang.n] = n2 - 1 + m; tang.zb[tang.n++] = m;
-      nrml.za[nrml.n] = -m; nrml.zb[nrml.n++] = n2 - 1 - m;
-      if (m >= 2) { nrml.za[nrml.n] = n2 - 2 + m; nrml.zb[nrml.n++] = m - 1; }
+      tang.pa[tang.n] = -m; tang.pb[tang.n++] = n - 1 - m;
+      tang.pa[tang.n] = n - 1 + m; tang.pb[tang.n++] = m;
+      nrml.pa[nrml.n] = -m; nrml.pb[nrml.n++] = n - 1 - m;
+      if (m >= 2) { nrml.pa[nrml.n] = n - 2 + m; nrml.pb[nrml.n++] = m - 1; }
     }
-    for (int d = 0; d < 3; ++d) differ(s->GC[d], -1, n0 + 1, -1, n1 + 2, d == 2 ? nrml : tang, 64 + d);
+    for (int c = 0; c < 3; ++c) differ(s->GC[c], d, true, c == d ? nrml : tang, 64 + c);
   }
   HIPCHK(hipGetLastError());
-  int h[3] = {1, 0, 0};
+  int h[9] = {};
   HIPCHK(hipMemcpyAsync(h, s->seam_flag, sizeof(h), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  *ok = h[0] == 0;
-  // MPDX_SEAM_DEBUG=1: say which field kept the slab path
-  if (h[0] && std::getenv("MPDX_SEAM_DEBUG"))
-    std::fprintf(stderr, "mpdx seam check: %s %d, column pair %d, row (%d, %d) differ\n",
-                 h[0] / 16 >= 64 ? "advector component" : "equation", h[0] / 16 % 64, h[0] % 16 - 1, h[1], h[2]);
+  const bool z_ok = h[6] == 0, xy_ok = xyz && h[0] == 0 && h[3] == 0;
+  *mode = !z_ok ? 0 : (xy_ok ? 2 : 1);
+  // MPDX_SEAM_DEBUG=1: say which field keeps which dimension off the seam path
+  if (std::getenv("MPDX_SEAM_DEBUG"))
+    for (int d = 0; d < 3; ++d)
+      if (h[3 * d])
+        std::fprintf(stderr, "mpdx seam check: dimension %d: %s %d, plane pair %d, row (%d, %d) differ\n", d,
+                     h[3 * d] / 16 >= 64 ? "advector component" : "equation", h[3 * d] / 16 % 64,
+                     h[3 * d] % 16 - 1, h[3 * d + 1], h[3 * d + 2]);
   return 0;
 }
 
@@ -1612,7 +1653,7 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
   const bool fused = s->active_path == MPDX_PATH_FUSED;
   if (!fused)
     if (int rc = ensure_perpass(s)) return rc;
-  bool seam = false;
+  int seam = 0;
   if (fused)
     if (int rc = seam_check<R>(s, &seam)) return rc;
   s->seam = seam;
@@ -1674,7 +1715,7 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
     }
     if (panicked(s)) break;
     if (graphs) {
-      const int key = (s->lev[0] & 1) | (s->seam ? 2 : 0);
+      const int key = (s->lev[0] & 1) | (s->seam << 1);
       if (!s->graph[key]) {
         int lev_save[MPDX_MAX_EQNS];
         for (int e = 0; e < s->p.n_eqns; ++e) lev_save[e] = s->lev[e];
@@ -1791,11 +1832,11 @@ int ante_dispatch(mpdx_solver *s)
 
 int seam_dispatch(mpdx_solver *s)
 {
-  bool ok = false;
+  int mode = 0;
   int rc = 0;
-  if (s->esz == 8) rc = seam_check<double>(s, &ok);
-  else rc = seam_check<float>(s, &ok);
-  s->seam = ok;
+  if (s->esz == 8) rc = seam_check<double>(s, &mode);
+  else rc = seam_check<float>(s, &mode);
+  s->seam = mode;
   return rc;
 }
 
@@ -1856,7 +1897,7 @@ std::vector<int> level_key(const mpdx_ensemble *e)
   std::vector<int> k;
   for (const mpdx_solver *s : e->m) {
     for (int q = 0; q < s->p.n_eqns; ++q) k.push_back(s->lev[q]);
-    k.push_back(s->seam ? 1 : 0);
+    k.push_back(s->seam);
   }
   return k;
 }
@@ -2361,7 +2402,8 @@ int mpdx_z_tiles(const mpdx_solver *s)
   if (s->active_path != MPDX_PATH_FUSED || s->p.n_iters < 2 || !interior_supported(s->p, s->L)) return 0;
   const int *b = s->p.bc[2];
   if (s->use_zbnd && b[0] == MPDX_BC_OPEN && b[1] == MPDX_BC_OPEN) return 1;
-  return s->seam && b[0] == MPDX_BC_CYCLIC && b[1] == MPDX_BC_CYCLIC ? 2 : 0;
+  if (b[0] != MPDX_BC_CYCLIC || b[1] != MPDX_BC_CYCLIC) return 0;
+  return s->seam == 2 ? 3 : (s->seam == 1 ? 2 : 0);
 }
 
 int mpdx_set_calc_gc(mpdx_solver *s, mpdx_calc_gc_fn fn, void *user)
@@ -2485,13 +2527,20 @@ int mpdx_set_advector_generator(mpdx_solver *s, int kind, const double *params, 
     return MPDX_EINVAL;
   }
   // sin^2 tables at corners a + 1/2, a = -2 .. n-1 (host libm, shared with
-  // the oracle), and the amplitude that makes the largest |GC| (at cos = 1)
-  // equal the Courant number
+  // the oracle; a corner is evaluated at its index modulo the period n-1, so
+  // the tables repeat bit for bit across the duplicated edge), and the
+  // amplitude that makes the largest |GC| (at cos = 1) equal the Courant
+  // number
   const int n0 = s->L.n[0], n1 = s->L.n[1];
   const double pi = 3.141592653589793;
   std::vector<double> sx2(n0 + 2), sy2(n1 + 2);
-  for (int a = -2; a < n0; ++a) { const double v = std::sin(pi * (a + 0.5) / (n0 - 1)); sx2[a + 2] = v * v; }
-  for (int b = -2; b < n1; ++b) { const double v = std::sin(pi * (b + 0.5) / (n1 - 1)); sy2[b + 2] = v * v; }
+  auto sin2 = [pi](int a, int n) {
+    const int r = ((a % (n - 1)) + (n - 1)) % (n - 1);
+    const double v = std::sin(pi * (r + 0.5) / (n - 1));
+    return v * v;
+  };
+  for (int a = -2; a < n0; ++a) sx2[a + 2] = sin2(a, n0);
+  for (int b = -2; b < n1; ++b) sy2[b + 2] = sin2(b, n1);
   double mx = 0;
   for (int a = -1; a < n0; ++a)
     for (int b = -1; b < n1; ++b) {

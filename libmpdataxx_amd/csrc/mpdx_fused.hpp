@@ -494,6 +494,12 @@ struct InteriorArgs {
   // tangential advector components equal (the duplicated edge of a period
   // of n-1 cells)
   int zb, zs_max;
+  // with the kernels' WXY (and zb = 2): cyclic bounds in x and y too, with
+  // the duplicated planes equal there as well; the tiles also cover every x
+  // and y cell (lo = 0, hi = n-1; the last y tile starts at ys_max) and read
+  // x and y halo planes at their periodic images, so no boundary shell
+  // remains
+  int ys_max;
 };
 
 // max / min without the std::max argument-order semantics (they differ only
@@ -579,7 +585,7 @@ template <class R> __device__ __forceinline__ void buf_st(char *g, unsigned off,
 // block blk of the launch over a's tiles (a launch of 8 * ceil(tiles / 8)
 // blocks; blk % 8 is the XCD the block runs on when the launch starts on a
 // multiple of 8)
-template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV>
+template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV, bool WXY>
 __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned blk)
 {
   constexpr int TY = ITile<TYV>::Y, TZ = ITile<TYV>::Z, PW = ITile<TYV>::PW, PL = ITile<TYV>::PLANE;
@@ -599,15 +605,17 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
   const int tz = threadIdx.x, ty = threadIdx.y;
   const int tid = ty * TZ + tz;
   const int n2 = L.n[2];
-  const int ylo = a.lo[1] - 2 + by * (TY - 4);
+  constexpr bool wxy = WXY;
+  const int ylo = (wxy ? min(a.lo[1] + by * (TY - 4), a.ys_max) : a.lo[1] + by * (TY - 4)) - 2;
   const int zlo = (a.zb ? min(a.lo[2] + bz * (TZ - 4), a.zs_max) : a.lo[2] + bz * (TZ - 4)) - 2;
   const int y = ylo + ty, z = zlo + tz;
   const int x0 = a.lo[0] + bxc * a.chunk;
   const int x1 = min(x0 + a.chunk - 1, a.hi[0]);
   const int nsteps = x1 - x0 + 5;   // psi^k planes x0-2 .. x1+2
-  // loads: y >= lo-2 >= 0; on the far side rows and columns past the tile's
-  // last output cell are clamped to the first halo row (cell n), which the
-  // FUSE0 donor needs for psi^1 at cell n-1.  z-boundary tiles also hold
+  // loads: y >= lo-2 >= 0 (wxy: rows are read at their periodic images,
+  // like the cyclic z columns below); on the far side rows and columns past
+  // the tile's last output cell are clamped to the first halo row (cell n),
+  // which the FUSE0 donor needs for psi^1 at cell n-1.  z-boundary tiles also hold
   // the halo columns.  Open (zb = 1): scalars (psi^k, psi^n) are read at the
   // open boundary's edge copy (bcond/open_3d.hpp:32-42, clamp to 0..n-1),
   // the advector from its halo faces in memory (clamped to -hw..n).
@@ -618,15 +626,18 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
   // tile wider than the domain and feed no output: clamped to n-1.)
   const bool seam = a.zb == 2;
   auto zwrap = [n2](int c) { return c < 0 ? c + (n2 - 1) : (c > n2 - 1 ? min(c - (n2 - 1), n2 - 1) : c); };
-  const int yl = min(y, L.n[1]);
+  const int n1 = L.n[1];
+  auto ywrap = [n1](int c) { return c < 0 ? c + (n1 - 1) : (c > n1 - 1 ? min(c - (n1 - 1), n1 - 1) : c); };
+  const int yl = wxy ? ywrap(y) : min(y, n1);
   const int zl = seam ? zwrap(z) : (a.zb ? min(max(z, -L.hw), n2) : min(z, n2));
   const int zls = seam ? zwrap(z) : (a.zb ? min(max(z, 0), n2 - 1) : min(z, n2));
-  // column the per-thread byte offsets count from (all offsets >= 0)
+  // row and column the per-thread byte offsets count from (all offsets >= 0)
   const int zbase = seam ? -L.hw : zlo;
+  const int ybase = wxy ? -L.hw : ylo;
   const long sx = L.st[0];
-  const long base0 = L.org + (long)(x0 - 2) * sx + (long)ylo * L.st[1] + zbase;
-  const unsigned toff = (unsigned)(((long)(yl - ylo) * L.st[1] + (zl - zbase)) * (long)sizeof(R));
-  const unsigned toff_s = (unsigned)(((long)(yl - ylo) * L.st[1] + (zls - zbase)) * (long)sizeof(R));
+  const long base0 = L.org + (long)(x0 - 2) * sx + (long)ybase * L.st[1] + zbase;
+  const unsigned toff = (unsigned)(((long)(yl - ybase) * L.st[1] + (zl - zbase)) * (long)sizeof(R));
+  const unsigned toff_s = (unsigned)(((long)(yl - ybase) * L.st[1] + (zls - zbase)) * (long)sizeof(R));
   const bool out_ok = ty >= 2 && ty < TY - 2 && tz >= 2 && tz < TZ - 2 && y <= a.hi[1] && z <= a.hi[2];
   const bool v_ok = out_ok && y >= a.vlo[1] && y <= a.vhi[1] && z >= a.vlo[2] && z <= a.vhi[2];
   // open z-boundary tiles: block flag (uniform), halo columns (z < 0 or
@@ -654,9 +665,21 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
   auto at = [&](const R *arr, int plane) -> char * {
     return reinterpret_cast<char *>(const_cast<R *>(arr + base0)) + plane * sxb;
   };
+  // load streams (wxy): plane x is read at its periodic image in 0..n0-1,
+  // so a stream steps back n0-2 planes where the next plane is x = 0 or
+  // x = n0 (the only crossings of a chunk's planes x0-3 .. x1+2)
+  const int n0 = L.n[0];
+  auto atl = [&](const R *arr, int plane) -> char * {
+    const int X = x0 - 2 + plane;
+    const long c = !wxy ? 0 : (X < 0 ? (long)(n0 - 1) * sxb : (X > n0 - 1 ? -(long)(n0 - 1) * sxb : 0));
+    return at(arr, plane) + c;
+  };
+  auto dstep = [&](int X) -> long {   // stride to plane X from plane X - 1
+    return wxy && (X == 0 || X == n0) ? sxb - (long)(n0 - 1) * sxb : sxb;
+  };
   constexpr int AHEAD = FUSE0 ? 1 : 0;   // plane of the first "next" register set
-  char *g_ps = FUSE0 ? nullptr : at(a.psi, 0), *g_pn = FCT || FUSE0 ? at(a.psin, AHEAD) : nullptr;
-  char *g_ux = at(a.U0, AHEAD), *g_uy = at(a.U1, AHEAD), *g_uz = at(a.U2, AHEAD);
+  char *g_ps = FUSE0 ? nullptr : atl(a.psi, 0), *g_pn = FCT || FUSE0 ? atl(a.psin, AHEAD) : nullptr;
+  char *g_ux = atl(a.U0, AHEAD), *g_uy = atl(a.U1, AHEAD), *g_uz = atl(a.U2, AHEAD);
   char *g_out = at(a.out, -2);
   char *g_v0 = STOREV ? at(a.V0, -1) : nullptr, *g_v1 = STOREV ? at(a.V1, -1) : nullptr,
        *g_v2 = STOREV ? at(a.V2, -1) : nullptr;
@@ -694,20 +717,20 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
   // branch.
   R fr_sc = 1;
   int fr_lds = IL_N * PL + tid;
-  const char *fr_g = at(a.psin, 0) + toff_s;
+  const char *fr_g = atl(a.psin, 0) + toff_s;
   if (FUSE0 && tid < ITile<TYV>::NFRAME) {
     int fy, fz, kind;
     if (tid < 2 * TZ) { kind = 0; fy = tid < TZ ? -1 : TY; fz = tid % TZ; }
     else if (tid < 2 * TZ + 2 * TY) { kind = 0; const int r = tid - 2 * TZ; fy = r % TY; fz = r < TY ? -1 : TZ; }
     else if (tid < 3 * TZ + 2 * TY) { kind = 1; fy = -1; fz = tid - 2 * TZ - 2 * TY; }
     else { kind = 2; fy = tid - 3 * TZ - 2 * TY; fz = -1; }
-    const int gy = min(ylo + fy, L.n[1]);
+    const int gy = wxy ? ywrap(ylo + fy) : min(ylo + fy, n1);
     const int gz = seam ? zwrap(zlo + fz)
                         : (!a.zb ? min(zlo + fz, n2) : (kind == 0 ? min(max(zlo + fz, 0), n2 - 1)
                                                                   : min(max(zlo + fz, -L.hw), n2)));
-    const long off = ((long)(gy - ylo) * L.st[1] + (gz - zbase)) * (long)sizeof(R);
+    const long off = ((long)(gy - ybase) * L.st[1] + (gz - zbase)) * (long)sizeof(R);
     const R *arr = kind == 0 ? a.psin : (kind == 1 ? a.U1 : a.U2);
-    fr_g = at(arr, 0) + off;
+    fr_g = atl(arr, 0) + off;
     fr_lds = (kind == 0 ? IL_PN : (kind == 1 ? IL_UY : IL_UZ)) * PL + (fy + 1) * PW + fz + 1;
     fr_sc = kind == 0 ? a.in_scale : R(1);
   }
@@ -728,14 +751,13 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
     n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
   } else {
     // prologue: psi^1 on plane 0 from psi^n planes -1..1 and GC planes -1, 0
-    const char *gp0 = at(a.psin, 0);
-    pn[5] = LDs(gp0 - sxb) * sc;
-    pn[0] = LDs(gp0) * sc;
-    pn[1] = LDs(gp0 + sxb) * sc;
-    ux[5] = LD(at(a.U0, -1));
-    ux[0] = LD(at(a.U0, 0));
-    uy[0] = LD(at(a.U1, 0));
-    uz[0] = LD(at(a.U2, 0));
+    pn[5] = LDs(atl(a.psin, -1)) * sc;
+    pn[0] = LDs(atl(a.psin, 0)) * sc;
+    pn[1] = LDs(atl(a.psin, 1)) * sc;
+    ux[5] = LD(atl(a.U0, -1));
+    ux[0] = LD(atl(a.U0, 0));
+    uy[0] = LD(atl(a.U1, 0));
+    uz[0] = LD(atl(a.U2, 0));
     S(IL_PN + 0, 0, 0) = pn[0];
     S(IL_UX + 0, 0, 0) = ux[0];
     S(IL_UY + 0, 0, 0) = uy[0];
@@ -743,7 +765,7 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
     lds[fr_lds] = FLD(0) * fr_sc;
     // plane 1 for the first step's publish
     n_pn = LDs(g_pn); n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
-    fr_g += sxb;
+    fr_g += dstep(x0 - 1);
     n_fr = *reinterpret_cast<const R *>(fr_g);
     __syncthreads();
     const R pc = pn[0];
@@ -781,9 +803,10 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
         S(IL_PN + su(0), 0, 0) = pn[s6(0)];
       }
       if (q + 1 < nsteps) {
-        g_ps += sxb; g_ux += sxb; g_uy += sxb; g_uz += sxb;
+        const long ds = dstep(x0 - 1 + q);   // to plane q + 1
+        g_ps += ds; g_ux += ds; g_uy += ds; g_uz += ds;
         n_ps = LDs(g_ps);
-        if (FCT) { g_pn += sxb; n_pn = LDs(g_pn); }
+        if (FCT) { g_pn += ds; n_pn = LDs(g_pn); }
         n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
       }
     } else {
@@ -793,11 +816,12 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
       S(IL_UZ + su(1), 0, 0) = n_uz;
       S(IL_PN + su(1), 0, 0) = pn[s6(1)];
       lds[fr_lds + (fr_lds < IL_N * PL ? su(1) * PL : 0)] = n_fr * fr_sc;
-      if (q + 1 < nsteps) { g_pn += sxb; n_pn = LDs(g_pn); }
+      const long ds = dstep(x0 + q);   // to plane q + 2
+      if (q + 1 < nsteps) { g_pn += ds; n_pn = LDs(g_pn); }
       if (q + 2 < nsteps) {
-        g_ux += sxb; g_uy += sxb; g_uz += sxb;
+        g_ux += ds; g_uy += ds; g_uz += ds;
         n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
-        fr_g += sxb;
+        fr_g += ds;
         n_fr = *reinterpret_cast<const R *>(fr_g);
       }
     }
@@ -960,11 +984,11 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
   }
 }
 
-template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV>
+template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV, bool WXY>
 __global__ void __launch_bounds__(ITile<TYV>::Y * ITile<TYV>::Z) MPDX_INTERIOR_ATTR
 k_fz_interior(InteriorArgs<R> a)
 {
-  fz_interior_block<R, G, FCT, IGA, STOREV, FUSE0, TYV>(a, blockIdx.x);
+  fz_interior_block<R, G, FCT, IGA, STOREV, FUSE0, TYV, WXY>(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- host side

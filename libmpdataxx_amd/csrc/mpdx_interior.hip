@@ -11,10 +11,10 @@
 #include "mpdx_interior_list.hpp"
 
 namespace mpdx {
-#define MPDX_INST(R, G, F, I, S, Z, T)                                                        \
-  template __global__ void k_fz_interior<R, G, F, I, S, Z, T>(InteriorArgs<R>);               \
-  template __global__ void k_task_list<InteriorTask<R, G, F, I, S, Z, T>>(                    \
-      const InteriorTask<R, G, F, I, S, Z, T> *, const unsigned *, int);
+#define MPDX_INST(R, G, F, I, S, Z, T, W)                                                     \
+  template __global__ void k_fz_interior<R, G, F, I, S, Z, T, W>(InteriorArgs<R>);            \
+  template __global__ void k_task_list<InteriorTask<R, G, F, I, S, Z, T, W>>(                 \
+      const InteriorTask<R, G, F, I, S, Z, T, W> *, const unsigned *, int);
 MPDX_INTERIOR_F32_FAST(MPDX_INST)
 #undef MPDX_INST
 }  // namespace mpdx

@@ -1,12 +1,15 @@
 // The FP32 interior kernels with the default (fast FP64-epsilon) guards:
-// k_fz_interior<float, GuardFast, FCT, IGA, STOREV, FUSE0, TY> for every
-// option combination the library launches and both tile heights (32, 16).
-// X(R, G, FCT, IGA, STOREV, FUSE0, TY) is applied to each.
+// k_fz_interior<float, GuardFast, FCT, IGA, STOREV, FUSE0, TY, WXY> for
+// every option combination the library launches, both tile heights (32,
+// 16) and both x / y bound treatments (WXY: wrapped to the periodic images).
+// X(R, G, FCT, IGA, STOREV, FUSE0, TY, WXY) is applied to each.
 #pragma once
 
 #define MPDX_IV_T(X, F, I, S, Z)                                      \
-  X(float, ::mpdx::GuardFast, F, I, S, Z, 32)                         \
-  X(float, ::mpdx::GuardFast, F, I, S, Z, 16)
+  X(float, ::mpdx::GuardFast, F, I, S, Z, 32, false)                  \
+  X(float, ::mpdx::GuardFast, F, I, S, Z, 16, false)                  \
+  X(float, ::mpdx::GuardFast, F, I, S, Z, 32, true)                   \
+  X(float, ::mpdx::GuardFast, F, I, S, Z, 16, true)
 
 #define MPDX_IV_S(X, F, I, Z) MPDX_IV_T(X, F, I, true, Z) MPDX_IV_T(X, F, I, false, Z)
 

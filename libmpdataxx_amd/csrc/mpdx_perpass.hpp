@@ -366,28 +366,30 @@ __global__ void k_gen_deform(R *g0, R *g1, R *g2, Layout L, Box bx, const double
   if (L.nd == 3) g2[p] = R(0);
 }
 
-// pairs of z columns compared by k_zpairs_differ
-struct ZPairs {
+// pairs of planes along one dimension compared by k_pairs_differ
+struct PlanePairs {
   int n;
-  int za[5], zb[5];
+  int pa[5], pb[5];
 };
 
-// *flag = tag * 16 + q + 1 where a(x, y, za[q]) and a(x, y, zb[q]) differ bit
-// for bit, for a row (x, y) of [x0, x0 + nx) x [y0, y0 + ny) and a pair q
-// (any one of the differing rows); flag[1], flag[2] = its x and y
+// flag[0] = tag * 16 + q + 1 where a's planes pa[q] and pb[q] along
+// dimension d differ bit for bit, for a row (i, j) of [i0, i0 + ni) x
+// [j0, j0 + nj) over the other two dimensions in order (any one of the
+// differing rows); flag[1], flag[2] = its i and j
 template <class R>
-__global__ void k_zpairs_differ(const R *a, Layout L, int x0, int nx, int y0, int ny, ZPairs zp, int tag,
-                                int *flag)
+__global__ void k_pairs_differ(const R *a, Layout L, int d, int i0, int ni, int j0, int nj, PlanePairs pp,
+                               int tag, int *flag)
 {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long rows = (long)nx * ny;
-  if (t >= rows * zp.n) return;
+  const long rows = (long)ni * nj;
+  if (t >= rows * pp.n) return;
   const int q = (int)(t / rows);
   const long r = t - q * rows;
-  const int i = x0 + (int)(r / ny), j = y0 + (int)(r % ny);
-  const long p = L.org + (long)i * L.st[0] + (long)j * L.st[1];
+  const int i = i0 + (int)(r / nj), j = j0 + (int)(r % nj);
+  const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+  const long p = L.org + (long)i * L.st[d1] + (long)j * L.st[d2];
   using B = typename std::conditional<sizeof(R) == 4, unsigned, unsigned long long>::type;
-  if (__builtin_bit_cast(B, a[p + (long)zp.za[q] * L.st[2]]) != __builtin_bit_cast(B, a[p + (long)zp.zb[q] * L.st[2]])) {
+  if (__builtin_bit_cast(B, a[p + (long)pp.pa[q] * L.st[d]]) != __builtin_bit_cast(B, a[p + (long)pp.pb[q] * L.st[d]])) {
     flag[0] = tag * 16 + q + 1;
     flag[1] = i;
     flag[2] = j;

@@ -112,7 +112,7 @@ struct mpdx_solver {
   // path, index parity + 2 seam); with several equations the step's
   // launches are batched across them (eqplan: the device tables the
   // captured step reads)
-  static constexpr int kGraphs = 4;
+  static constexpr int kGraphs = 6;
   hipGraphExec_t graph[kGraphs] = {};
   mpdx::BatchPlan eqplan[kGraphs];
   bool use_graphs = true;
@@ -126,10 +126,12 @@ struct mpdx_solver {
   // z-boundary cells (no z slabs in the boundary shell)
   bool use_zbnd = true;
   // with cyclic bounds on both z sides the tiles also compute the
-  // z-boundary cells while the fields repeat plane z = 0 at z = n-1 (seam:
+  // z-boundary cells while the fields repeat plane z = 0 at z = n-1, and
+  // with cyclic bounds everywhere the x- and y-boundary cells as well while
+  // the fields repeat those planes too (seam: 0 no, 1 z, 2 x, y and z;
   // checked on the device at the start of each advance call)
   bool use_seam = true;
-  bool seam = false;
+  int seam = 0;
   int *seam_flag = nullptr;
   int ix_chunk = 0;         // output planes per x chunk of the interior kernel (0: automatic)
   int n_cu = 256;           // compute units of the device

@@ -200,11 +200,11 @@ __global__ void __launch_bounds__(T::kThreads) k_task(const T t)
 
 // one launch of the interior corrective pass (k_fz_interior); its block
 // count is a multiple of 8, so a batch of these keeps the XCD tile order
-template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV>
+template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV, bool WXY>
 struct InteriorTask : TaskGrid {
   static constexpr int kThreads = ITile<TYV>::Y * ITile<TYV>::Z, kBlockY = ITile<TYV>::Y;
   InteriorArgs<R> a;
-  __device__ void run(TaskBlk b) const { fz_interior_block<R, G, FCT, IGA, STOREV, FUSE0, TYV>(a, b.x); }
+  __device__ void run(TaskBlk b) const { fz_interior_block<R, G, FCT, IGA, STOREV, FUSE0, TYV, WXY>(a, b.x); }
 };
 
 }  // namespace mpdx

@@ -608,7 +608,8 @@ static void seterr(char *err, int len, const char *msg)
 /* Time-varying advector of the reversing_deform test family
  * (tests/mp3_paper_2018_JCP/reversing_deform/reversing_deform.hpp:64-105):
  * the discrete curl of the streamfunction sin^2(pi x) sin^2(pi y) cos(pi t/T)
- * sampled at cell corners x = (a + 1/2)/(n0 - 1), y = (b + 1/2)/(n1 - 1),
+ * sampled at cell corners x = (a + 1/2)/(n0 - 1), y = (b + 1/2)/(n1 - 1)
+ * (a and b taken modulo n - 1, the period of sin^2 in them),
  * each corner value rounded to a multiple of 2^-17 (so the discrete
  * divergence is exactly zero and the same values come out of any correctly
  * rounded evaluation), written over i, j = -1 .. n-1 at t + dt/2 and
@@ -647,8 +648,18 @@ int orc_set_generator(orc_t *s, int kind, double T, double max_courant)
   free(s->gen_sy2);
   s->gen_sx2 = (double *)malloc(sizeof(double) * (size_t)(n0 + 2));
   s->gen_sy2 = (double *)malloc(sizeof(double) * (size_t)(n1 + 2));
-  for (int a = -2; a < n0; ++a) { const double v = sin(pi * (a + 0.5) / (n0 - 1)); s->gen_sx2[a + 2] = v * v; }
-  for (int b = -2; b < n1; ++b) { const double v = sin(pi * (b + 0.5) / (n1 - 1)); s->gen_sy2[b + 2] = v * v; }
+  /* each corner at its index modulo the period n-1 (the tables repeat bit
+     for bit across the duplicated edge) */
+  for (int a = -2; a < n0; ++a) {
+    const int r = ((a % (n0 - 1)) + (n0 - 1)) % (n0 - 1);
+    const double v = sin(pi * (r + 0.5) / (n0 - 1));
+    s->gen_sx2[a + 2] = v * v;
+  }
+  for (int b = -2; b < n1; ++b) {
+    const int r = ((b % (n1 - 1)) + (n1 - 1)) % (n1 - 1);
+    const double v = sin(pi * (r + 0.5) / (n1 - 1));
+    s->gen_sy2[b + 2] = v * v;
+  }
   double mx = 0;
   for (int a = -1; a < n0; ++a)
     for (int b = -1; b < n1; ++b) {

@@ -122,7 +122,7 @@ def test_c4_multi_tracer_full_size():
     m0 = [float(f[:-1, :-1, :-1].double().sum()) for f in fields]
     del fields
     s.advance(steps)
-    assert s.z_tiles == 2   # repeating planes: the interior tiles take the cyclic z bounds
+    assert s.z_tiles == 3   # repeating planes everywhere: the interior tiles take every cyclic bound
     out = torch.empty(shape, dtype=torch.float32, device=DEV)
     for e in range(8):
         s.get_advectee_device(out, e)

@@ -1,13 +1,16 @@
-"""Cyclic z bounds in the interior kernel's tiles ("seam" mode).
+"""Cyclic bounds in the interior kernel's tiles ("seam" modes).
 
 With cyclic bounds on both z sides and fields that repeat plane z = 0 at
 z = n-1 (the duplicated edge of a period of n-1 cells), the tiles also compute
 the cells next to the z bounds, reading every halo column at its periodic
 image (the value the cyclic halo fill puts there, bcond/cyclic_3d.hpp:29-47,
-120-138), and the boundary shell has no z slabs.  advance() compares those
-planes on the device at the start of each call and takes this path only when
-they are equal bit for bit (psi of every equation and, without the on-device
-generator, the advector's z halo against its images).
+120-138), and the boundary shell has no z slabs (z_tiles 2).  With cyclic
+bounds everywhere and fields repeating across every period, the tiles do the
+same along x and y and compute every cell: no boundary shell, no halo fills
+(z_tiles 3).  advance() compares those planes on the device at the start of
+each call and takes these paths only when they are equal bit for bit (psi of
+every equation and, without the on-device generator, the advector's halo
+against its images).
 
 In FP64 the results must match the oracle for every option set of the
 interior kernel and z extents where one tile holds both bounds (12, 27, 28) or
@@ -30,7 +33,8 @@ NZ = [12, 27, 28, 29, 45, 61]
 
 
 def consistent_case(grid, seed, xb, yb):
-    """psi and a divergence-free advector repeating plane z = 0 at z = n-1.
+    """psi and a divergence-free advector repeating plane 0 at n-1 along z
+    and along each cyclic one of x and y.
     Cyclic x and y: a 3D streamfunction flow (w != 0).  An open x or y
     bound: horizontal flow (w = 0), since the reference's zero-divergence
     halo faces at an open x or y edge are computed before the cyclic z fill
@@ -40,6 +44,10 @@ def consistent_case(grid, seed, xb, yb):
     bcs = [(xb, xb), (yb, yb), (M.CYCLIC, M.CYCLIC)]
     psi = W.random_field(grid, seed)
     psi[:, :, -1] = psi[:, :, 0]
+    if xb == M.CYCLIC:
+        psi[-1] = psi[0]
+    if yb == M.CYCLIC:
+        psi[:, -1] = psi[:, 0]
     if xb == M.CYCLIC and yb == M.CYCLIC:
         gc = W.streamfunction_advector(grid, seed + 100, periodic=(True, True, True), max_courant=0.45,
                                        exact_period=True)
@@ -48,6 +56,10 @@ def consistent_case(grid, seed, xb, yb):
     for d in range(2):
         assert np.array_equal(gc[d][:, :, 0], gc[d][:, :, -1])
     return bcs, psi, gc
+
+
+def expected_mode(bcs):
+    return 3 if bcs[0][0] == M.CYCLIC and bcs[1][0] == M.CYCLIC else 2
 
 
 def case(nz, seed):
@@ -99,11 +111,13 @@ def test_seam_tiles_match_oracle_fp64(nz, opts, iters, monkeypatch):
     grid, bcs, psi, gc = case(nz, 100)
     print(f"seam vs oracle nz={nz} opts={opts} n_iters={iters} x/y bounds={bcs[0][0]}/{bcs[1][0]}", flush=True)
     got, mode = run(grid, opts, iters, np.float64, bcs, psi, gc, 4, monkeypatch, True)
-    assert mode == 2
+    assert mode == expected_mode(bcs)
     want = oracle_run(grid, opts, iters, bcs, psi, gc, 4)
     assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
     # the step keeps the planes equal
     assert np.array_equal(got[:, :, 0], got[:, :, -1])
+    if mode == 3:
+        assert np.array_equal(got[0], got[-1]) and np.array_equal(got[:, 0], got[:, -1])
 
 
 @pytest.mark.parametrize("nz", NZ)
@@ -114,20 +128,22 @@ def test_seam_tiles_round_like_slab_path(nz, dtype, opts, iters, monkeypatch):
     print(f"seam vs slab nz={nz} dtype={np.dtype(dtype).name} opts={opts} n_iters={iters}", flush=True)
     a, ma = run(grid, opts, iters, dtype, bcs, psi, gc, 4, monkeypatch, True)
     b, mb = run(grid, opts, iters, dtype, bcs, psi, gc, 4, monkeypatch, False)
-    assert (ma, mb) == (2, 0)
+    assert (ma, mb) == (expected_mode(bcs), 0)
     eps = np.finfo(dtype).eps
     assert np.abs(a.astype(np.float64) - b).max() <= 16 * eps * np.abs(b).max()
 
 
+@pytest.mark.parametrize("yb", [M.OPEN, M.CYCLIC])
 @pytest.mark.parametrize("iters", [3, 4])
-def test_seam_tiles_several_tiles_and_chunks_fp64(iters, monkeypatch):
-    """A grid with several tiles along y and z and several x chunks (the
-    ring of tile rows next to the shell spans many tiles)."""
+def test_seam_tiles_several_tiles_and_chunks_fp64(iters, yb, monkeypatch):
+    """A grid with several tiles along y and z and several x chunks (open
+    y: the ring of tile rows next to the shell spans many tiles; cyclic y:
+    the tiles wrap around in every dimension)."""
     grid = (70, 66, 93)
-    bcs, psi, gc = consistent_case(grid, 17, M.CYCLIC, M.OPEN)
-    print(f"seam several tiles grid={grid} n_iters={iters}", flush=True)
+    bcs, psi, gc = consistent_case(grid, 17, M.CYCLIC, yb)
+    print(f"seam several tiles grid={grid} n_iters={iters} y bounds={yb}", flush=True)
     got, mode = run(grid, M.FCT, iters, np.float64, bcs, psi, gc, 3, monkeypatch, True)
-    assert mode == 2
+    assert mode == expected_mode(bcs)
     want = oracle_run(grid, M.FCT, iters, bcs, psi, gc, 3)
     assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
 
@@ -156,6 +172,14 @@ def test_seam_needs_repeating_planes(monkeypatch):
     gc3 = W.streamfunction_advector(grid, 9, periodic=(True, False, True), max_courant=0.45, exact_period=True)
     psi3 = W.random_field(grid, 10)
     psi3[:, :, -1] = psi3[:, :, 0]
+    print("seam fallback: cyclic everywhere, psi repeating along z only", flush=True)
+    grid4 = (24, 20, 27)
+    bcs4, psi4, gc4 = consistent_case(grid4, 11, M.CYCLIC, M.CYCLIC)
+    psi4[-1, 3, 5] += 0.125
+    e4, me = run(grid4, M.FCT, 3, np.float64, bcs4, psi4, gc4, 3, monkeypatch, True)
+    assert me == 2
+    want4 = oracle_run(grid4, M.FCT, 3, bcs4, psi4, gc4, 3)
+    assert np.abs(e4 - want4).max() <= 1e-12 * np.abs(want4).max()
     print("seam fallback: open y with w != 0", flush=True)
     d, md = run(grid, M.FCT, 3, np.float64, bcs, psi3, gc3, 3, monkeypatch, True)
     assert md == 0
@@ -179,7 +203,7 @@ def test_seam_decided_per_call(monkeypatch):
             x.set_advector(gc[d], d)
     s.advance(2)
     o.advance(2)
-    assert s.z_tiles == 2
+    assert s.z_tiles == expected_mode(bcs)
     p = s.advectee()
     p[2, 3, 0] += 0.5
     s.set_advectee(p)
@@ -193,16 +217,22 @@ def test_seam_decided_per_call(monkeypatch):
     assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("planes", ["z", "xyz"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_seam_with_deform_generator(dtype, monkeypatch):
-    """The z-independent on-device generator keeps the advector's planes
-    equal: cyclic in all dimensions (the multi-tracer workload's bounds)."""
+def test_seam_with_deform_generator(dtype, planes, monkeypatch):
+    """The on-device generator (z-independent, its corner tables repeating
+    across the x and y periods) keeps the advector's planes equal: cyclic in
+    all dimensions (the multi-tracer workload's bounds), psi repeating along
+    z only (z_tiles 2) or along every dimension (3)."""
     grid = (40, 34, 14)
     bcs = [(M.CYCLIC, M.CYCLIC)] * 3
     psi = W.random_field(grid, 801)
     psi[:, :, -1] = psi[:, :, 0]
+    if planes == "xyz":
+        psi[-1] = psi[0]
+        psi[:, -1] = psi[:, 0]
     T, c, nt = 16.0, 0.45, 6
-    print(f"seam deform generator dtype={np.dtype(dtype).name}", flush=True)
+    print(f"seam deform generator dtype={np.dtype(dtype).name} planes={planes}", flush=True)
     monkeypatch.delenv("MPDX_NO_SEAM", raising=False)
     o = O.Oracle(grid, opts=O.FCT, n_iters=3, bcs=[(O.CYCLIC, O.CYCLIC)] * 3, dt=1.0, max_abs_div_eps=1e300)
     s = M.Solver(grid, opts=M.FCT, n_iters=3, bcs=bcs, dt=1.0, dtype=dtype, max_abs_div_eps=1e300)
@@ -214,7 +244,7 @@ def test_seam_with_deform_generator(dtype, monkeypatch):
     s.set_advector_generator(M.GC_DEFORM, T, c)
     o.advance(nt)
     s.advance(nt)
-    assert s.z_tiles == 2
+    assert s.z_tiles == (3 if planes == "xyz" else 2)
     got = s.advectee().astype(np.float64)
     s.close()
     want = o.advectee()
@@ -235,7 +265,7 @@ def test_seam_two_equations_hint_scale_fp64(batched, monkeypatch):
         monkeypatch.delenv("MPDX_BATCH_EQNS", raising=False)
     got, mode = run(grid, M.FCT, 4, np.float64, bcs, [psi, psi2], gc, 4, monkeypatch, True, n_eqns=2,
                     hint_scale=[2, -1])
-    assert mode == 2
+    assert mode == 3
     want = oracle_run(grid, M.FCT, 4, bcs, [psi, psi2], gc, 4, n_eqns=2, hint_scale=[2, -1])
     for g, w in zip(got, want):
         assert np.abs(g - w).max() <= 1e-12 * np.abs(w).max()
@@ -263,14 +293,14 @@ def test_seam_ensemble_member_matches_solo(monkeypatch):
     for g, b, p, u in ((grid, bcs, psi, gc), (grid2, bcs2, psi2, gc2)):
         s = make(g, b, p, u)
         s.advance(5)
-        assert s.z_tiles == 2
+        assert s.z_tiles == expected_mode(b)
         solo.append(s.advectee())
         s.close()
     members = [make(grid, bcs, psi, gc), make(grid2, bcs2, psi2, gc2)]
     ens = M.Ensemble(members)
     ens.advance(5)
-    for s, want in zip(members, solo):
-        assert s.z_tiles == 2
+    for s, b, want in zip(members, (bcs, bcs2), solo):
+        assert s.z_tiles == expected_mode(b)
         assert np.array_equal(s.advectee(), want)
     ens.close()
     for s in members:
