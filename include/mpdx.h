@@ -187,8 +187,9 @@ int mpdx_active_path(const mpdx_solver *h);
  * through the periodic images of the halo columns, 3 = cyclic bounds in all
  * three dimensions with fields repeating plane 0 at n-1 along each: the tiles
  * compute every cell and there is no boundary shell (both decided at the
- * start of each call by comparing those planes on the device); -1 for a
- * null handle */
+ * start of each call by comparing those planes on the device); 2D: 0, or 3
+ * when the 2D pass kernel's tiles cover every cell of such an all-cyclic
+ * domain; -1 for a null handle */
 int mpdx_z_tiles(const mpdx_solver *h);
 
 /* Time-varying advector (the reference's calc_gc() hook, solver_common.hpp:

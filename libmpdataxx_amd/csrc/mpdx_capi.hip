@@ -858,8 +858,9 @@ struct Fused {
       : s(s_), pp(s_), L(s_->L), m(make_bcmap(s_)), o(s_->p.opts),
         zb(s_->use_zbnd && s_->L.nd == 3 && s_->p.bc[2][0] == MPDX_BC_OPEN && s_->p.bc[2][1] == MPDX_BC_OPEN),
         seam(s_->seam >= 1 && s_->L.nd == 3 && s_->p.bc[2][0] == MPDX_BC_CYCLIC && s_->p.bc[2][1] == MPDX_BC_CYCLIC),
-        full(seam && s_->seam == 2 && s_->p.bc[0][0] == MPDX_BC_CYCLIC && s_->p.bc[0][1] == MPDX_BC_CYCLIC &&
-             s_->p.bc[1][0] == MPDX_BC_CYCLIC && s_->p.bc[1][1] == MPDX_BC_CYCLIC)
+        full((seam || s_->L.nd == 2) && s_->seam == 2 && s_->p.bc[0][0] == MPDX_BC_CYCLIC &&
+             s_->p.bc[0][1] == MPDX_BC_CYCLIC && s_->p.bc[1][0] == MPDX_BC_CYCLIC &&
+             s_->p.bc[1][1] == MPDX_BC_CYCLIC)
   {
   }
 
@@ -1175,9 +1176,50 @@ struct Fused {
                        dim3(ITile<TYV>::Z, ITile<TYV>::Y), 0, s->stream, a);
   }
 
+  // the 2D pass kernel over the interior cells 2..n-3 (velocity stored for
+  // 4..n-5, like the 3D kernel), or with full over every cell
+  void interior2d(R *out, R *psi, R *psin, R **U, R **V, bool storev, bool fuse0, R in_scale, R out_scale)
+  {
+    Pass2dArgs<R> a{};
+    a.out = out; a.psi = psi; a.psin = psin;
+    a.U0 = U[0]; a.U1 = U[1]; a.V0 = V[0]; a.V1 = V[1];
+    a.L = L;
+    for (int d = 0; d < 2; ++d) {
+      a.lo[d] = full ? 0 : 2; a.hi[d] = full ? L.n[d] - 1 : L.n[d] - 3;
+      a.vlo[d] = full ? 0 : 4; a.vhi[d] = full ? L.n[d] - 1 : L.n[d] - 5;
+    }
+    a.nb[0] = (a.hi[1] - a.lo[1] + kP2T) / kP2T;
+    a.nb[1] = (a.hi[0] - a.lo[0] + kP2T) / kP2T;
+    a.in_scale = in_scale; a.out_scale = out_scale;
+    a.fuse0 = fuse0; a.storev = storev; a.wrap = full;
+    const dim3 g((unsigned)(a.nb[0] * a.nb[1]));
+    const bool fct = o & O_FCT, iga = o & O_IGA;
+    if (fct && iga) launch_pass2d<true, true>(a, g);
+    else if (fct) launch_pass2d<true, false>(a, g);
+    else if (iga) launch_pass2d<false, true>(a, g);
+    else launch_pass2d<false, false>(a, g);
+  }
+
+  template <bool FCT, bool IGA>
+  void launch_pass2d(const Pass2dArgs<R> &a, dim3 g)
+  {
+    if (s->rec) {
+      Pass2dTask<R, G, FCT, IGA> t{};
+      t.a = a;
+      record(s, t, g);
+      return;
+    }
+    hipLaunchKernelGGL((k_fz_pass2d<R, G, FCT, IGA>), g, dim3(kP2Threads), 0, s->stream, a);
+  }
+
   // psi: psi^k, or unused with fuse0 (psi^1 computed from psin and U = GC)
   void interior(R *out, R *psi, R *psin, R **U, R **V, bool storev, bool fuse0, R in_scale, R out_scale)
   {
+    if (L.nd == 2) {
+      ProfScope ps(s, fuse0 ? "fused_first_2d" : "fused_interior_2d");
+      interior2d(out, psi, psin, U, V, storev, fuse0, in_scale, out_scale);
+      return;
+    }
     ProfScope ps(s, fuse0 ? ((o & O_FCT) ? (storev ? "fused_first_fct_v" : "fused_first_fct")
                                           : (storev ? "fused_first_v" : "fused_first"))
                           : ((o & O_FCT) ? (storev ? "fused_interior_fct_v" : "fused_interior_fct")
@@ -1324,7 +1366,7 @@ struct Fused {
     auto out_of = [&](int it) { return ((passes - 1 - it) % 2 == 0) ? fin : tmp; };
     R *Gb[3];
     R **gc = pp.av_of(s->GC, Gb);
-    const bool use_int = interior_supported(s->p, L);
+    const bool use_int = interior_supported(s->p, L) || (s->use_pass2d && pass2d_supported(s->p, L));
     if (full && use_int && passes > 1) {
       // every cell in the interior kernel, reading halos at their periodic
       // images: no halo fills and no boundary shell
@@ -1579,32 +1621,50 @@ int rhs_hook(mpdx_solver *s, int hook)
 // n-1 then see equal neighbourhoods and get equal updates.  A host calc_gc
 // callback or forcing could break it, so those keep the slab path.
 template <class R>
+int seam_check_device(mpdx_solver *s, int *mode);
+
+template <class R>
 int seam_check(mpdx_solver *s, int *mode)
+{
+  if (s->seam_known && !s->seam_exposed) {
+    *mode = s->seam;
+    return 0;
+  }
+  if (int rc = seam_check_device<R>(s, mode)) return rc;
+  s->seam_known = true;
+  return 0;
+}
+
+template <class R>
+int seam_check_device(mpdx_solver *s, int *mode)
 {
   *mode = 0;
   const Layout &L = s->L;
   auto cyclic = [&](int d) { return s->p.bc[d][0] == MPDX_BC_CYCLIC && s->p.bc[d][1] == MPDX_BC_CYCLIC; };
-  if (!s->use_seam || L.nd != 3 || !cyclic(2)) return 0;
-  if (s->p.n_iters < 2 || !interior_supported(s->p, L) || s->calc_gc || s->rhs_scheme != MPDX_RHS_NONE) return 0;
+  const int nd = L.nd;
+  const bool xyz = cyclic(0) && cyclic(1);
+  // 3D: z cyclic for mode 1, x and y too for mode 2; 2D: x and y for mode 2
+  if (!s->use_seam || (nd == 3 ? !cyclic(2) : !xyz)) return 0;
+  const bool kernel = nd == 3 ? interior_supported(s->p, L) : s->use_pass2d && pass2d_supported(s->p, L);
+  if (s->p.n_iters < 2 || !kernel || s->calc_gc || s->rhs_scheme != MPDX_RHS_NONE) return 0;
   if (!s->seam_flag) {
     void *p = nullptr;
     if (int rc = dev_alloc(s, &p, 9 * sizeof(int))) return rc;
     s->seam_flag = (int *)p;
   }
   HIPCHK(hipMemsetAsync(s->seam_flag, 0, 9 * sizeof(int), s->stream));
-  const bool xyz = cyclic(0) && cyclic(1);
   // tag: equation e (psi) or 64 + c (advector component c); flags 3 d .. 3 d + 2
   auto differ = [&](const void *a, int d, bool halo, const PlanePairs &pp, int tag) {
     const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
     // rows: the interior (psi) or the rows the tiles read, one halo row on
-    // each side (the advector)
-    const int i0 = halo ? -1 : 0, j0 = halo ? -1 : 0;
-    const int ni = L.n[d1] + (halo ? 2 : 0), nj = L.n[d2] + (halo ? 2 : 0);
+    // each side (the advector); 2D rows run along the other dimension only
+    const int i0 = halo ? -1 : 0, j0 = halo && nd == 3 ? -1 : 0;
+    const int ni = L.n[d1] + (halo ? 2 : 0), nj = nd == 3 ? L.n[d2] + (halo ? 2 : 0) : 1;
     const long t = (long)ni * nj * pp.n;
     hipLaunchKernelGGL(k_pairs_differ<R>, dim3((unsigned)((t + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream,
                        (const R *)a, L, d, i0, ni, j0, nj, pp, tag, s->seam_flag + 3 * d);
   };
-  for (int d = xyz ? 0 : 2; d < 3; ++d) {
+  for (int d = xyz ? 0 : 2; d < nd; ++d) {
     const int n = L.n[d];
     const PlanePairs edge{1, {0}, {n - 1}};
     for (int e = 0; e < s->p.n_eqns; ++e) differ(s->psi[e][s->lev[e]], d, false, edge, e);
@@ -1618,14 +1678,14 @@ int seam_check(mpdx_solver *s, int *mode)
       nrml.pa[nrml.n] = -m; nrml.pb[nrml.n++] = n - 1 - m;
       if (m >= 2) { nrml.pa[nrml.n] = n - 2 + m; nrml.pb[nrml.n++] = m - 1; }
     }
-    for (int c = 0; c < 3; ++c) differ(s->GC[c], d, true, c == d ? nrml : tang, 64 + c);
+    for (int c = 0; c < nd; ++c) differ(s->GC[c], d, true, c == d ? nrml : tang, 64 + c);
   }
   HIPCHK(hipGetLastError());
   int h[9] = {};
   HIPCHK(hipMemcpyAsync(h, s->seam_flag, sizeof(h), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  const bool z_ok = h[6] == 0, xy_ok = xyz && h[0] == 0 && h[3] == 0;
-  *mode = !z_ok ? 0 : (xy_ok ? 2 : 1);
+  const bool z_ok = nd == 2 || h[6] == 0, xy_ok = xyz && h[0] == 0 && h[3] == 0;
+  *mode = !z_ok ? 0 : (xy_ok ? 2 : (nd == 3 ? 1 : 0));
   // MPDX_SEAM_DEBUG=1: say which field keeps which dimension off the seam path
   if (std::getenv("MPDX_SEAM_DEBUG"))
     for (int d = 0; d < 3; ++d)
@@ -1714,15 +1774,22 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
       if (ev) HIPCHK(hipEventSynchronize(ev));
     }
     if (panicked(s)) break;
+    int adv = 1;   // steps this iteration advances
     if (graphs) {
-      const int key = (s->lev[0] & 1) | (s->seam << 1);
+      const bool batched = s->batch_eqns && s->p.n_eqns > 1;
+      if (s->graph_steps == 0) {
+        long cells = (long)s->p.n_eqns;
+        for (int d = 0; d < s->L.nd; ++d) cells *= s->L.n[d];
+        s->graph_steps = cells <= (1L << 21) ? 8 : 1;
+      }
+      const int gs = !var_dt && !batched && nt - s->timestep >= s->graph_steps ? s->graph_steps : 1;
+      const int key = (gs > 1 ? 6 : 0) + ((s->lev[0] & 1) | (s->seam << 1));
       if (!s->graph[key]) {
         int lev_save[MPDX_MAX_EQNS];
         for (int e = 0; e < s->p.n_eqns; ++e) lev_save[e] = s->lev[e];
         // several equations: the launches of each equation's step are
         // recorded and merged into batched launches (one per kernel and
         // round, covering every equation), uploaded before the capture
-        const bool batched = s->batch_eqns && s->p.n_eqns > 1;
         if (batched) {
           // equations 1.. get velocity scratch of their own
           for (int e = 1; e < s->p.n_eqns; ++e)
@@ -1748,12 +1815,15 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
         }
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-        int rc = calc_gc_step<R, G>(s);
-        if (!rc) {
-          if (batched) launch_plan(s->eqplan[key], s->stream);
-          else rc = fused_step<R, G>(s);
+        int rc = 0;
+        for (int q = 0; q < gs && !rc; ++q) {
+          rc = calc_gc_step<R, G>(s);
+          if (!rc) {
+            if (batched) launch_plan(s->eqplan[key], s->stream);
+            else rc = fused_step<R, G>(s);
+          }
+          if (gen) hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, s->stream, s->gen_step, -1);
         }
-        if (gen) hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, s->stream, s->gen_step, -1);
         hipError_t ec = hipStreamEndCapture(s->stream, &g);
         for (int e = 0; e < s->p.n_eqns; ++e) s->lev[e] = lev_save[e];
         if (rc) return rc;
@@ -1763,7 +1833,8 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
         HIPCHK(ei);
       }
       HIPCHK(hipGraphLaunch(s->graph[key], s->stream));
-      for (int e = 0; e < s->p.n_eqns; ++e) s->lev[e] ^= 1;
+      for (int e = 0; e < s->p.n_eqns; ++e) s->lev[e] ^= gs & 1;
+      adv = gs;
     } else {
       bool var_gc = false;
       if (int rc = calc_gc_step<R, G>(s, &var_gc)) return rc;
@@ -1805,7 +1876,7 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
     HIPCHK(hipGetLastError());
     if (hipEvent_t ev = s->inflight[queued % mpdx_solver::kInflight]) HIPCHK(hipEventRecord(ev, s->stream));
     ++queued;
-    s->timestep++;
+    s->timestep += adv;
     s->time = var_dt ? s->time + s->p.dt : s->timestep * s->p.dt;
     // mpdata_rhs::hook_post_step (solvers/mpdata_rhs.hpp:162-183)
     if (rs == MPDX_RHS_EULER_A) {
@@ -2161,12 +2232,17 @@ int mpdx_create(const mpdx_params *p, mpdx_solver **out)
   if (off("MPDX_NO_CONCURRENCY")) s->use_concurrency = false;
   if (off("MPDX_NO_ZBND")) s->use_zbnd = false;
   if (off("MPDX_NO_SEAM")) s->use_seam = false;
+  if (off("MPDX_NO_PASS2D")) s->use_pass2d = false;
   if (off("MPDX_BATCH_EQNS")) s->batch_eqns = true;   // (a switch that turns the batched path on)
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
       s->n_cu = ncu;
+  }
+  if (const char *ev = std::getenv("MPDX_GRAPH_STEPS")) {
+    const int g = std::atoi(ev);
+    if (g == 1 || (g >= 2 && g <= 64 && g % 2 == 0)) s->graph_steps = g;
   }
   if (const char *ev = std::getenv("MPDX_IX_CHUNK")) {
     const int c = std::atoi(ev);
@@ -2268,6 +2344,7 @@ int mpdx_set_advectee(mpdx_solver *s, int e, const void *buf, const int64_t stri
   MPDX_NEED(buf, "buffer");
   MPDX_NEED(strides, "strides");
   if (e < 0 || e >= s->p.n_eqns) { s->err = "libmpdata++: bad equation index"; return MPDX_EINVAL; }
+  s->seam_known = false;
   int lo[3] = {0, 0, 0}, hi[3] = {s->L.n[0] - 1, s->L.n[1] - 1, s->L.nd == 3 ? s->L.n[2] - 1 : 0};
   return copy_view(s, s->psi[e][s->lev[e]], lo, hi, const_cast<void *>(buf), strides, on_device, true);
 }
@@ -2288,6 +2365,7 @@ int mpdx_set_advector(mpdx_solver *s, int d, const void *buf, const int64_t stri
   MPDX_NEED(buf, "buffer");
   MPDX_NEED(strides, "strides");
   if (d < 0 || d >= s->L.nd) { s->err = "libmpdata++: bad advector component"; return MPDX_EINVAL; }
+  s->seam_known = false;
   int lo[3], hi[3];
   adv_bounds(s, d, lo, hi);
   return copy_view(s, s->GC[d], lo, hi, const_cast<void *>(buf), strides, on_device, true);
@@ -2399,6 +2477,8 @@ int mpdx_active_path(const mpdx_solver *s) { return s ? s->active_path : -1; }
 int mpdx_z_tiles(const mpdx_solver *s)
 {
   if (!s) return -1;
+  if (s->active_path == MPDX_PATH_FUSED && s->p.n_iters >= 2 && s->use_pass2d && pass2d_supported(s->p, s->L))
+    return s->seam == 2 ? 3 : 0;
   if (s->active_path != MPDX_PATH_FUSED || s->p.n_iters < 2 || !interior_supported(s->p, s->L)) return 0;
   const int *b = s->p.bc[2];
   if (s->use_zbnd && b[0] == MPDX_BC_OPEN && b[1] == MPDX_BC_OPEN) return 1;
@@ -2409,6 +2489,7 @@ int mpdx_z_tiles(const mpdx_solver *s)
 int mpdx_set_calc_gc(mpdx_solver *s, mpdx_calc_gc_fn fn, void *user)
 {
   MPDX_NEED(s, "solver handle");
+  s->seam_known = false;
   s->calc_gc = fn;
   s->calc_gc_user = user;
   return MPDX_OK;
@@ -2421,6 +2502,7 @@ int mpdx_set_rhs_scheme(mpdx_solver *s, int scheme, mpdx_update_rhs_fn fn, void 
     s->err = "libmpdata++: unsupported rhs scheme";
     return MPDX_EUNSUPPORTED;
   }
+  s->seam_known = false;
   if (scheme != MPDX_RHS_NONE)
     for (int e = 0; e < s->p.n_eqns; ++e)
       if (int rc = alloc_missing(s, &s->rhs[e])) return rc;
@@ -2494,6 +2576,7 @@ int mpdx_advectee_device(mpdx_solver *s, int e, void **ptr, int64_t strides[3])
   MPDX_NEED(ptr, "output pointer");
   MPDX_NEED(strides, "strides");
   if (e < 0 || e >= s->p.n_eqns) { s->err = "libmpdata++: bad equation index"; return MPDX_EINVAL; }
+  s->seam_exposed = true;
   return interior_device(s, s->psi[e][s->lev[e]], ptr, strides);
 }
 
@@ -2503,6 +2586,7 @@ int mpdx_advector_device(mpdx_solver *s, int d, void **ptr, int64_t strides[3])
   MPDX_NEED(ptr, "output pointer");
   MPDX_NEED(strides, "strides");
   if (d < 0 || d >= s->L.nd) { s->err = "libmpdata++: bad advector component"; return MPDX_EINVAL; }
+  s->seam_exposed = true;
   int lo[3], hi[3];
   adv_bounds(s, d, lo, hi);
   const long off = s->L.org + lo[0] * s->L.st[0] + lo[1] * s->L.st[1] + (s->L.nd == 3 ? lo[2] * s->L.st[2] : 0);
@@ -2517,6 +2601,7 @@ int mpdx_set_advector_generator(mpdx_solver *s, int kind, const double *params, 
   // a captured step holds the generator's kernel (or its absence)
   HIPCHK(hipStreamSynchronize(s->stream));
   drop_step_graphs(s);
+  s->seam_known = false;
   if (kind == MPDX_GC_NONE) { s->gen_kind = kind; return MPDX_OK; }
   if (s->p.var_dt) {
     s->err = "libmpdata++: the built-in advector generator does not follow a variable dt";

@@ -109,13 +109,17 @@ struct mpdx_solver {
   mpdx::TaskRecorder *rec = nullptr;   // set: launches are recorded, not made
   int tile_y = 0;            // interior tile height (0: the element type's default)
   // one captured time step per time-level parity and seam mode (fused
-  // path, index parity + 2 seam); with several equations the step's
-  // launches are batched across them (eqplan: the device tables the
-  // captured step reads)
-  static constexpr int kGraphs = 6;
+  // path, index parity + 2 seam; + 6: graph_steps steps in one graph); with
+  // several equations the step's launches are batched across them (eqplan:
+  // the device tables the captured step reads)
+  static constexpr int kGraphs = 12;
   hipGraphExec_t graph[kGraphs] = {};
   mpdx::BatchPlan eqplan[kGraphs];
   bool use_graphs = true;
+  // steps per captured graph: several for small steps, whose launch
+  // overhead would dominate (0: chosen at the first capture; an even number,
+  // so the time levels return to the same parity)
+  int graph_steps = 0;
   bool batch_eqns = false;   // MPDX_BATCH_EQNS=1: measured slower for 8 tracers (shell loses its overlap)
   // auxiliary stream (boundary shell next to the interior kernel) and its
   // fork / join events
@@ -132,6 +136,12 @@ struct mpdx_solver {
   // checked on the device at the start of each advance call)
   bool use_seam = true;
   int seam = 0;
+  // the seam mode stays valid while only the library's own steps change the
+  // fields (they keep the planes equal); setting a field or the advector's
+  // source drops it, and once a raw device pointer to a field has been
+  // handed out every advance call checks again
+  bool seam_known = false, seam_exposed = false;
+  bool use_pass2d = true;   // 2D: the pass kernel (MPDX_NO_PASS2D=1: the shell kernels over every cell)
   int *seam_flag = nullptr;
   int ix_chunk = 0;         // output planes per x chunk of the interior kernel (0: automatic)
   int n_cu = 256;           // compute units of the device

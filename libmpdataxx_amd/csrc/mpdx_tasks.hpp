@@ -11,6 +11,7 @@
 #pragma once
 
 #include "mpdx_fused.hpp"
+#include "mpdx_pass2d.hpp"
 
 namespace mpdx {
 
@@ -197,6 +198,14 @@ __global__ void __launch_bounds__(T::kThreads) k_task(const T t)
 {
   t.run(TaskBlk{blockIdx.x, blockIdx.y, blockIdx.z});
 }
+
+// one launch of the 2D corrective pass (k_fz_pass2d)
+template <class R, class G, bool FCT, bool IGA>
+struct Pass2dTask : TaskGrid {
+  static constexpr int kThreads = kP2Threads;
+  Pass2dArgs<R> a;
+  __device__ void run(TaskBlk b) const { fz_pass2d_block<R, G, FCT, IGA>(a, b.x); }
+};
 
 // one launch of the interior corrective pass (k_fz_interior); its block
 // count is a multiple of 8, so a batch of these keeps the XCD tile order

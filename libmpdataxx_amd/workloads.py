@@ -390,7 +390,8 @@ def ensemble_domains(n2=32, n3=32, seed=5000, max_courant=0.5):
             psi[tuple(dst)] = psi[tuple(src)]
         adv_seed = int(field_seed % (1 << 63))
         if nd == 2:
-            gc = streamfunction_advector(grid, adv_seed, max_courant=max_courant, n_modes=8, periodic=True)
+            gc = streamfunction_advector(grid, adv_seed, max_courant=max_courant, n_modes=8, periodic=True,
+                                         exact_period=True)
         else:
             gc = level_streamfunction_advector(grid, adv_seed, max_courant=max_courant)
         out.append({"grid": grid, "n_iters": n_iters, "fct": fct, "psi": psi, "gc": gc})

@@ -36,6 +36,7 @@ def test_c1_gaussian_2d_cyclic_matches_oracle():
             x.set_advector(gc[d], d)
     s.advance(200)
     o.advance(200)
+    assert s.z_tiles == 3   # all-cyclic and repeating: the 2D pass kernel alone
     got, want = s.advectee(), o.advectee()
     assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
     # cyclic with period n-1: the duplicated edge stays a copy, the mass of
