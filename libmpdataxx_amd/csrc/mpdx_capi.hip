@@ -1107,7 +1107,8 @@ struct Fused {
       if (tile_y() == 16) return interior_ty<FCT, IGA, STOREV, FUSE0, 16>(out, psi, psin, U, V, in_scale, out_scale);
       return interior_ty<FCT, IGA, STOREV, FUSE0, 32>(out, psi, psin, U, V, in_scale, out_scale);
     } else {
-      return interior_ty<FCT, IGA, STOREV, FUSE0, itile_y_default<R>()>(out, psi, psin, U, V, in_scale, out_scale);
+      return interior_ty<FCT, IGA, STOREV, FUSE0, itile_y_default<R, FUSE0, STOREV>()>(out, psi, psin, U, V, in_scale,
+                                                                                      out_scale);
     }
   }
 
@@ -2861,7 +2862,11 @@ int mpdx_ensemble_tile_y(const mpdx_ensemble *e, int i, int *rows)
   MPDX_NEED(rows, "output pointer");
   if (i < 0 || i >= (int)e->m.size()) return MPDX_EINVAL;
   const mpdx_solver *s = e->m[i];
-  *rows = s->tile_y ? s->tile_y : (s->esz == 8 ? 16 : 32);
+  // FP64: 24 rows; a single corrective pass (n_iters = 2) 16
+  *rows = s->tile_y ? s->tile_y
+                    : (s->esz == 8 ? (s->p.n_iters == 2 ? itile_y_default<double, true, false>()
+                                                        : itile_y_default<double>())
+                                   : itile_y_default<float>());
   return MPDX_OK;
 }
 

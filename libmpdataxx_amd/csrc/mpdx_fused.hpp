@@ -452,27 +452,33 @@ k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R 
 // own LDS address plus an immediate offset; frame cells that nobody loads
 // hold zeros and reach only frame results, which are discarded.
 // Tile of TYV (y) x 32 (z) threads.  FP32 tiles are 32 high (one
-// 1024-thread block per CU, 106-116 KB of LDS) or 16 high (512 threads, two
-// blocks per CU); FP64 tiles are 16 high (the 32-high rings would need 212 KB
-// of LDS).
+// 1024-thread block per CU, 72-99 KB of LDS) or 16 high (512 threads, two
+// blocks per CU); FP64 tiles are 24 high (768 threads, 12 waves: one block
+// per CU, 110-151 KB of LDS; 32-high rings would need up to 198 KB, more
+// than a CU's 160 KB) or 16 high (below).
 template <int TYV> struct ITile {
   static constexpr int Y = TYV, Z = 32;
   static constexpr int PW = Z + 2, PLANE = (Y + 2) * PW;
   static constexpr int NFRAME = 3 * Z + 3 * Y;   // FUSE0 frame loads per plane
 };
-template <class R> constexpr int itile_y_default() { return sizeof(R) == 8 ? 16 : 32; }
+// the default tile height of a kernel variant: FP32 32 rows; FP64 24, but
+// 16 for the first pass that is also the last (FUSE0 without STOREV, n_iters
+// = 2), which at 24 rows needs more than the 168 VGPRs three waves per SIMD
+// allow and would spill
+template <class R, bool FUSE0 = false, bool STOREV = true> constexpr int itile_y_default()
+{
+  return sizeof(R) == 8 ? (FUSE0 && !STOREV ? 16 : 24) : 32;
+}
 
-// LDS planes of the interior kernel
-enum {
-  IL_PSI = 0,     // psi^k ring, 3 planes
-  IL_UX = 3,      // U_x ring, 3
-  IL_UY = 6,      // U_y ring, 2 (3 with FUSE0)
-  IL_UZ = 9,      // U_z ring, 2 (3 with FUSE0)
-  IL_PN = 12,     // psi^n ring, 2 (3 with FUSE0)
-  IL_FY = 15,     // limited y flux ring, 2
-  IL_FZ = 17,     // limited z flux ring, 2
-  IL_VY = 19, IL_VZ = 20, IL_BU = 21, IL_BD = 22,
-  IL_N = 23
+// LDS planes of the interior kernel: the psi^k and U_x rings (3 planes),
+// the U_y, U_z and psi^n rings (2 planes, 3 with FUSE0), the limited y and z
+// fluxes (FCT: one plane each, read by the update of the plane before they
+// are overwritten; without FCT a 2-plane ring each) and, with FCT, V_y,
+// V_z, beta-up and beta-down (one plane each)
+template <bool FCT, bool FUSE0> struct ILds {
+  static constexpr int RU = FUSE0 ? 3 : 2, RF = FCT ? 1 : 2;
+  static constexpr int PSI = 0, UX = 3, UY = 6, UZ = UY + RU, PN = UZ + RU, FY = PN + RU, FZ = FY + RF,
+                       VY = FZ + RF, VZ = VY + 1, BU = VY + 2, BD = VY + 3, N = VY + (FCT ? 4 : 0);
 };
 
 template <class R>
@@ -589,6 +595,10 @@ template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV
 __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned blk)
 {
   constexpr int TY = ITile<TYV>::Y, TZ = ITile<TYV>::Z, PW = ITile<TYV>::PW, PL = ITile<TYV>::PLANE;
+  using IL = ILds<FCT, FUSE0>;
+  constexpr int IL_PSI = IL::PSI, IL_UX = IL::UX, IL_UY = IL::UY, IL_UZ = IL::UZ, IL_PN = IL::PN,
+                IL_FY = IL::FY, IL_FZ = IL::FZ, IL_VY = IL::VY, IL_VZ = IL::VZ, IL_BU = IL::BU, IL_BD = IL::BD,
+                IL_N = IL::N;
   __shared__ R lds[IL_N * PL + (FUSE0 ? TY * TZ : 0)];
 
   // XCD-aware tile order: workgroup w runs on XCD w % 8, and XCD x takes the
@@ -865,8 +875,10 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
           vz = antidiff_core<R, G, 3, IGA, false, false, decltype(P), true>(uz[s6(-1)], s1, s2, P);
         }
         if (zh) vy = R(0);
-        S(IL_VY, 0, 0) = vy;
-        S(IL_VZ, 0, 0) = vz;
+        if (FCT) {
+          S(IL_VY, 0, 0) = vy;
+          S(IL_VZ, 0, 0) = vz;
+        }
       }
       if (STOREV && q >= vq0 && q <= vq1) {
         if (v_ok) {
@@ -917,23 +929,38 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
       S(IL_BU, 0, 0) = bu;
       S(IL_BD, 0, 0) = bd;
     }
+    // FCT: the x flux of the face owned by p-2 (between planes p-2 and p-1,
+    // limited with beta of both planes, in registers) and psi^{k+1} on
+    // plane p-2, whose y and z fluxes the previous step published; stage 4
+    // overwrites them only after the barrier
+    R fx = 0;
+    if (FCT && q >= 3) {
+      const R mx = imono(vx_[s2(-1)], bu_[s2(-1)], bd_[s2(-1)], bu, bd);
+      fx = IGA ? mx : iflux_v(ps[s6(-2)], ps[s6(-1)], mx, vx_[s2(-1)]);
+      if (q >= 4 && out_ok) {
+        const R r = ps[s6(-2)] + ((-fx + fx_[s2(-1)]) + (-S(IL_FY, 0, 0) + S(IL_FY, -1, 0))
+                                  + (-S(IL_FZ, 0, 0) + S(IL_FZ, 0, -1))) / 1;
+        ST(g_out, r * a.out_scale);
+      }
+    }
     if (FCT) __syncthreads();
 
-    // ---- stage 4: limited fluxes; psi^{k+1} on plane p-2
+    // ---- stage 4: limited y and z fluxes of plane p-1; without FCT also
+    // the x flux of the face owned by p-2 and psi^{k+1} on plane p-2
     if (q >= 3) {
-      // x face owned by p-2 (between planes p-2 and p-1)
-      const R mx = FCT ? imono(vx_[s2(-1)], bu_[s2(-1)], bd_[s2(-1)], bu, bd) : vx_[s2(-1)];
-      const R fx = IGA ? mx : iflux_v(ps[s6(-2)], ps[s6(-1)], mx, vx_[s2(-1)]);
-      // y and z faces of plane p-1 (flux ring slot s2(-1)); plane p-2 is s2(-2)
       if (!FCT) {
+        fx = IGA ? vx_[s2(-1)] : iflux_v(ps[s6(-2)], ps[s6(-1)], vx_[s2(-1)], vx_[s2(-1)]);
         yp = S(IL_PSI + s3(-1), 1, 0);
         zp = S(IL_PSI + s3(-1), 0, 1);
       }
+      // flux planes: FCT one each; without FCT a ring, plane p-1 in slot
+      // s2(-1) and plane p-2 in s2(-2)
+      const int wf = FCT ? 0 : s2(-1);
       const R my = FCT ? imono(vy, bu, bd, S(IL_BU, 1, 0), S(IL_BD, 1, 0)) : vy;
       const R mz = FCT ? imono(vz, bu, bd, S(IL_BU, 0, 1), S(IL_BD, 0, 1)) : vz;
-      S(IL_FY + s2(-1), 0, 0) = IGA ? my : iflux_v(ps[s6(-1)], yp, my, vy);
-      S(IL_FZ + s2(-1), 0, 0) = IGA ? mz : iflux_v(ps[s6(-1)], zp, mz, vz);
-      if (q >= 4 && out_ok) {
+      S(IL_FY + wf, 0, 0) = IGA ? my : iflux_v(ps[s6(-1)], yp, my, vy);
+      S(IL_FZ + wf, 0, 0) = IGA ? mz : iflux_v(ps[s6(-1)], zp, mz, vz);
+      if (!FCT && q >= 4 && out_ok) {
         const R r = ps[s6(-2)] + ((-fx + fx_[s2(-1)]) + (-S(IL_FY + s2(-2), 0, 0) + S(IL_FY + s2(-2), -1, 0))
                                   + (-S(IL_FZ + s2(-2), 0, 0) + S(IL_FZ + s2(-2), 0, -1))) / 1;
         ST(g_out, r * a.out_scale);
