@@ -116,19 +116,19 @@ template <bool NPA, class R> __device__ inline R negpart(R x)
 // (v_rcp_f32, <= 1 ulp) times the numerator.
 struct GuardFast {};
 
-// FP64 quotient of the guarded ratios: the hardware reciprocal (v_rcp_f64)
-// refined by two Newton-Raphson steps r <- r + r (1 - d r), the quotient
-// q = n r, and one correction q + r (n - d q) whose residual the fused
-// multiply-add forms exactly.  Within 1 ulp of the IEEE quotient (and equal
-// to it in nearly every case) for DBL_MIN <= |d| and a finite quotient, in 8
-// VALU instructions where the IEEE sequence (div_scale, div_fmas,
-// div_fixup) takes 11 and hazard waits.  A denominator that cancels to 0 or
-// to a subnormal (a guarded sum within 2 DBL_MIN of zero) gives NaN where
-// IEEE division gives an infinity.
+// FP64 quotient of the guarded ratios: the hardware reciprocal (v_rcp_f64,
+// relative error below 2^-24 on gfx950: 2^-24.4 at most over 4M random
+// operands) refined by one Newton-Raphson step r <- r + r (1 - d r) to about
+// 2^-48, the quotient q = n r, and one correction q + r (n - d q) whose
+// residual the fused multiply-add forms exactly.  Within 1 ulp of the IEEE
+// quotient (equal to it for each of those 4M operands) for DBL_MIN <= |d|
+// and a finite quotient, in 6 VALU instructions where the IEEE sequence
+// (div_scale, div_fmas, div_fixup) takes 11 and hazard waits.  A
+// denominator that cancels to 0 or to a subnormal (a guarded sum within
+// 2 DBL_MIN of zero) gives NaN where IEEE division gives an infinity.
 __device__ __forceinline__ double ddiv(double n, double d)
 {
   double r = __builtin_amdgcn_rcp(d);
-  r = fma(r, fma(-d, r, 1.0), r);
   r = fma(r, fma(-d, r, 1.0), r);
   const double q = n * r;
   return fma(fma(-d, q, n), r, q);
