@@ -933,6 +933,7 @@ struct Fused {
   void donor_boxes(R *out, R *in, R **C, R in_scale, R out_scale, int flux_is_C, const std::vector<Box> &bxs)
   {
     R *c2 = L.nd == 3 ? C[2] : C[0];
+    const int khn = (o & O_KHN) ? 1 : 0;
     RecGroup grp(s);
     for (const Box &bx : bxs) {
       if (bx.total <= 0) continue;
@@ -940,7 +941,7 @@ struct Fused {
       if (s->rec) {
         auto fill = [&](auto t) {
           t.out = out; t.in = in; t.C0 = C[0]; t.C1 = C[1]; t.C2 = c2; t.L = L;
-          t.in_scale = in_scale; t.out_scale = out_scale; t.flux_is_C = flux_is_C; t.bx = bx;
+          t.in_scale = in_scale; t.out_scale = out_scale; t.flux_is_C = flux_is_C; t.khn = khn; t.bx = bx;
           record(s, t, g);
         };
         if (o & O_NPA) fill(DonorTask<R, true>{});
@@ -949,10 +950,12 @@ struct Fused {
       }
       if (o & O_NPA)
         hipLaunchKernelGGL((k_fz_donor<R, true>), g, dim3(256), 0, s->stream, out, (const R *)in,
-                           (const R *)C[0], (const R *)C[1], (const R *)c2, L, in_scale, out_scale, flux_is_C, bx);
+                           (const R *)C[0], (const R *)C[1], (const R *)c2, L, in_scale, out_scale, flux_is_C, khn,
+                           bx);
       else
         hipLaunchKernelGGL((k_fz_donor<R, false>), g, dim3(256), 0, s->stream, out, (const R *)in,
-                           (const R *)C[0], (const R *)C[1], (const R *)c2, L, in_scale, out_scale, flux_is_C, bx);
+                           (const R *)C[0], (const R *)C[1], (const R *)c2, L, in_scale, out_scale, flux_is_C, khn,
+                           bx);
     }
   }
 
@@ -1005,18 +1008,19 @@ struct Fused {
   template <int ND, int SHAPE, bool NPA, bool ABS, bool IGA>
   void fct_s(R *out, R *psi, R *psin, R **V, R in_scale, R out_scale, const Box &bx)
   {
+    const int khn = (o & O_KHN) ? 1 : 0;
     constexpr int TX = FzBrick<ND, SHAPE>::TX, TY = FzBrick<ND, SHAPE>::TY, TZ = FzBrick<ND, SHAPE>::TZ;
     dim3 g(ND == 3 ? (bx.ext[2] + TZ - 1) / TZ : 1, (bx.ext[1] + TY - 1) / TY, (bx.ext[0] + TX - 1) / TX);
     if (s->rec) {
       FctTask<R, G, ND, SHAPE, NPA, ABS, IGA> t{};
       t.out = out; t.psi = psi; t.psin = psin; t.V0 = V[0]; t.V1 = V[1]; t.V2 = V[ND == 3 ? 2 : 0];
-      t.L = L; t.in_scale = in_scale; t.out_scale = out_scale; t.bx = bx;
+      t.L = L; t.in_scale = in_scale; t.out_scale = out_scale; t.khn = khn; t.bx = bx;
       record(s, t, g);
       return;
     }
     hipLaunchKernelGGL((k_fz_fct<R, G, ND, SHAPE, NPA, ABS, IGA>), g, dim3(TX * TY * TZ), 0, s->stream, out,
                        (const R *)psi, (const R *)psin, (const R *)V[0], (const R *)V[1],
-                       (const R *)V[ND == 3 ? 2 : 0], L, in_scale, out_scale, bx);
+                       (const R *)V[ND == 3 ? 2 : 0], L, in_scale, out_scale, khn, bx);
   }
 
   // brick oriented along the box's thinnest dimension

@@ -88,10 +88,35 @@ __device__ __forceinline__ R rel(const R *A, long p, const Layout &L, int dx, in
 
 // ---------------------------------------------------------------- donor-cell pass
 
+// psi + the flux divergence (donorcell_sum, donorcell_formulae.hpp:105-165):
+// f[0..5] are the fluxes through the +x, -x, +y, -y, +z, -z faces.  Without
+// khn the reference's parenthesisation; with khn (opts::khn) the
+// compensated sum of psi and the signed fluxes in the reference's order,
+// each step kahan_add (formulae/kahan_sum.hpp:13-27); the switch is uniform
+// across a launch
+template <class R, int ND>
+__device__ __forceinline__ R donor_sum(R pc, const R *f, int khn)
+{
+  if (!khn) {
+    if (ND == 3) return pc + ((-f[0] + f[1]) + (-f[2] + f[3]) + (-f[4] + f[5])) / 1;
+    return pc + ((-f[0] + f[1]) + (-f[2] + f[3])) / 1;
+  }
+  R sum = 0, c = 0;
+  auto add = [&](R in) {
+    const R y = in - c;
+    const R t = sum + y;
+    c = (t - sum) - y;
+    sum = t;
+  };
+  add(pc);
+  for (int q = 0; q < 2 * ND; ++q) add(q % 2 == 0 ? -f[q] / 1 : f[q] / 1);
+  return sum;
+}
+
 template <class R, bool NPA>
 __device__ __forceinline__ void fz_donor_cell(R *out, const R *in, const R *C0, const R *C1, const R *C2,
-                                              const Layout &L, R in_scale, R out_scale, int flux_is_C, int i,
-                                              int j, int k)
+                                              const Layout &L, R in_scale, R out_scale, int flux_is_C, int khn,
+                                              int i, int j, int k)
 {
   const long p = cell_index(L, i, j, k);
   const R pc = in[p] * in_scale;
@@ -113,21 +138,19 @@ __device__ __forceinline__ void fz_donor_cell(R *out, const R *in, const R *C0, 
       f[5] = donor_flux<NPA>(zm, pc, rel(C2, p, L, 0, 0, -1));
     }
   }
-  R r;
-  if (L.nd == 3) r = pc + ((-f[0] + f[1]) + (-f[2] + f[3]) + (-f[4] + f[5])) / 1;
-  else r = pc + ((-f[0] + f[1]) + (-f[2] + f[3])) / 1;
+  const R r = L.nd == 3 ? donor_sum<R, 3>(pc, f, khn) : donor_sum<R, 2>(pc, f, khn);
   out[p] = r * out_scale;
 }
 
 template <class R, bool NPA>
 __global__ void __launch_bounds__(256)
 k_fz_donor(R *out, const R *in, const R *C0, const R *C1, const R *C2, Layout L,
-           R in_scale, R out_scale, int flux_is_C, Box bx)
+           R in_scale, R out_scale, int flux_is_C, int khn, Box bx)
 {
   // cells of the box bx (last coordinate fastest)
   int i, j, k;
   if (!box_idx(bx, i, j, k)) return;
-  fz_donor_cell<R, NPA>(out, in, C0, C1, C2, L, in_scale, out_scale, flux_is_C, i, j, k);
+  fz_donor_cell<R, NPA>(out, in, C0, C1, C2, L, in_scale, out_scale, flux_is_C, khn, i, j, k);
 }
 
 // ---------------------------------------------------------------- antidiffusive velocity
@@ -352,7 +375,7 @@ __device__ inline void beta_at(R &bup, R &bdn, const R *psi, const R *psin, cons
 // brick (gz, gy, gx) of the box: gz along z (3D), gy along y, gx along x
 template <class R, class G, int ND, int SHAPE, bool NPA, bool ABS, bool IGA>
 __device__ __forceinline__ void fz_fct_block(R *out, const R *psi, const R *psin, const R *V0, const R *V1,
-                                             const R *V2, const Layout &L, R in_scale, R out_scale,
+                                             const R *V2, const Layout &L, R in_scale, R out_scale, int khn,
                                              const Box &bx, unsigned gz, unsigned gy, unsigned gx)
 {
   constexpr int TX = FzBrick<ND, SHAPE>::TX, TY = FzBrick<ND, SHAPE>::TY, TZ = FzBrick<ND, SHAPE>::TZ;
@@ -408,9 +431,9 @@ __device__ __forceinline__ void fz_fct_block(R *out, const R *psi, const R *psin
     const R mm = mono(rel(V2, p, L, 0, 0, -1), B(sbu, 0, 0, -1), B(sbd, 0, 0, -1), bu0, bd0, zm, pc);
     f[4] = IGA ? mp : donor_flux<NPA>(pc, zp, mp);
     f[5] = IGA ? mm : donor_flux<NPA>(zm, pc, mm);
-    r = pc + ((-f[0] + f[1]) + (-f[2] + f[3]) + (-f[4] + f[5])) / 1;
+    r = donor_sum<R, 3>(pc, f, khn);
   } else {
-    r = pc + ((-f[0] + f[1]) + (-f[2] + f[3])) / 1;
+    r = donor_sum<R, 2>(pc, f, khn);
   }
   out[p] = r * out_scale;
 }
@@ -418,9 +441,9 @@ __device__ __forceinline__ void fz_fct_block(R *out, const R *psi, const R *psin
 template <class R, class G, int ND, int SHAPE, bool NPA, bool ABS, bool IGA>
 __global__ void __launch_bounds__((fz_brick_threads<ND, SHAPE>()))
 k_fz_fct(R *out, const R *psi, const R *psin, const R *V0, const R *V1, const R *V2, Layout L,
-         R in_scale, R out_scale, Box bx)
+         R in_scale, R out_scale, int khn, Box bx)
 {
-  fz_fct_block<R, G, ND, SHAPE, NPA, ABS, IGA>(out, psi, psin, V0, V1, V2, L, in_scale, out_scale, bx,
+  fz_fct_block<R, G, ND, SHAPE, NPA, ABS, IGA>(out, psi, psin, V0, V1, V2, L, in_scale, out_scale, khn, bx,
                                                blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
@@ -1020,10 +1043,11 @@ k_fz_interior(InteriorArgs<R> a)
 
 // ---------------------------------------------------------------- host side
 
-inline bool fused_supported(const mpdx_params &p, const Layout &)
+inline bool fused_supported(const mpdx_params &, const Layout &)
 {
-  // khn (element-wise Kahan donor sums) stays on the per-pass path
-  return !(p.opts & O_KHN);
+  // every option: the interior and 2D pass kernels take the common ones,
+  // the boundary-shell kernels (over the whole grid) the rest, khn included
+  return true;
 }
 
 // the interior kernel covers 3D, standard or iga antidiff without abs/npa/pfc

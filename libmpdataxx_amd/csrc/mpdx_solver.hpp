@@ -120,7 +120,7 @@ struct mpdx_solver {
   // overhead would dominate (0: chosen at the first capture; an even number,
   // so the time levels return to the same parity)
   int graph_steps = 0;
-  bool batch_eqns = false;   // MPDX_BATCH_EQNS=1: measured slower for 8 tracers (shell loses its overlap)
+  bool batch_eqns = false;   // MPDX_BATCH_EQNS=1: 2 % faster with no boundary shell, slower beside one; velocity scratch per equation
   // auxiliary stream (boundary shell next to the interior kernel) and its
   // fork / join events
   hipStream_t aux = nullptr;

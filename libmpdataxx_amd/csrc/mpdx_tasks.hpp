@@ -66,13 +66,13 @@ template <class R, bool NPA> struct DonorTask : TaskGrid {
   const R *in, *C0, *C1, *C2;
   Layout L;
   R in_scale, out_scale;
-  int flux_is_C;
+  int flux_is_C, khn;
   Box bx;
   __device__ void run(TaskBlk b) const
   {
     int i, j, k;
     if (box_idx_at(bx, b.x, b.y, i, j, k))
-      fz_donor_cell<R, NPA>(out, in, C0, C1, C2, L, in_scale, out_scale, flux_is_C, i, j, k);
+      fz_donor_cell<R, NPA>(out, in, C0, C1, C2, L, in_scale, out_scale, flux_is_C, khn, i, j, k);
   }
 };
 
@@ -97,10 +97,12 @@ template <class R, class G, int ND, int SHAPE, bool NPA, bool ABS, bool IGA> str
   const R *psi, *psin, *V0, *V1, *V2;
   Layout L;
   R in_scale, out_scale;
+  int khn;
   Box bx;
   __device__ void run(TaskBlk b) const
   {
-    fz_fct_block<R, G, ND, SHAPE, NPA, ABS, IGA>(out, psi, psin, V0, V1, V2, L, in_scale, out_scale, bx, b.x, b.y, b.z);
+    fz_fct_block<R, G, ND, SHAPE, NPA, ABS, IGA>(out, psi, psin, V0, V1, V2, L, in_scale, out_scale, khn, bx, b.x,
+                                                 b.y, b.z);
   }
 };
 

@@ -40,15 +40,37 @@ def _case(nd, bcs, name, seed):
 @pytest.mark.parametrize("nd", [2, 3])
 @pytest.mark.parametrize("path", [1, 2])
 def test_fp64_matches_oracle(path, nd, bi, name):
-    """path 1 = per-pass kernels, path 2 = fused kernels (khn is per-pass only)."""
-    if path == 2 and name == "khn_fct":
-        pytest.skip("khn runs on the per-pass path")
+    """path 1 = per-pass kernels, path 2 = fused kernels (abs, npa, pfc and khn
+    on the boundary-shell kernels over the whole grid)."""
     bcs = (BCS2 if nd == 2 else BCS3)[bi]
     grid, opts, iters, psi, gc = _case(nd, bcs, name, 10 * bi + nd)
     want = run_oracle(grid, bcs, opts, iters, psi, gc, 4)[0]
     got = run_gpu(grid, bcs, opts, iters, psi, gc, 4, path=path)[0]
     assert not np.isnan(got).any()
     assert rel_err(got, want) <= 1e-12
+
+
+@pytest.mark.parametrize("bi", [0, 1])
+@pytest.mark.parametrize("nd", [2, 3])
+@pytest.mark.parametrize("path", [1, 2])
+def test_khn_donor_bit_exact(path, nd, bi):
+    """khn with the donor-cell pass alone (n_iters = 1): no division occurs
+    and no product can be contracted differently (one of pospart / negpart of
+    each face velocity is zero), so the GPU's compensated sums equal the
+    oracle's (pinned against math.fsum) bit for bit.  Control: the same run
+    without khn differs from them (psi spans eight decades, so the pairwise
+    sum rounds differently in some cells)."""
+    bcs = (BCS2 if nd == 2 else BCS3)[bi]
+    grid = (40, 36) if nd == 2 else (20, 18, 22)
+    rng = np.random.default_rng(17 + 2 * nd + bi)
+    psi = 10.0 ** rng.uniform(-4.0, 4.0, grid)
+    _, gc = setup_case(grid, bcs, 3 + bi)
+    print(f"khn donor nd={nd} bcs={bcs} path={path}", flush=True)
+    want = run_oracle(grid, bcs, O.KHN, 1, psi, gc, 3)[0]
+    got = run_gpu(grid, bcs, O.KHN, 1, psi, gc, 3, path=path)[0]
+    assert np.array_equal(got, want)
+    plain = run_gpu(grid, bcs, 0, 1, psi, gc, 3, path=path)[0]
+    assert not np.array_equal(plain, want)
 
 
 @pytest.mark.parametrize("name", ["basic", "fct", "fct3", "iga_fct"])
