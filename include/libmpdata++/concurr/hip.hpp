@@ -221,9 +221,13 @@ namespace libmpdataxx
         }
         else if constexpr (with_output)
         {
+          // hook_ante_loop (output_common.hpp:47-62: const.h5 and a record)
+          // runs only when the solver is at timestep 0 (solver_common.hpp:277-285);
           // records between steps (hook_post_step, output_common.hpp:145-156):
           // one library call per stretch of steps up to the next record
-          out_.start(h_, nt);
+          int64_t ts_start;
+          mpdx_timestep(h_, &ts_start);
+          if (ts_start == 0) out_.start(h_, nt);
           int64_t left = int64_t(nt);
           while (left > 0 && !panic_)
           {

@@ -12,8 +12,10 @@
  * describes the output; concurr::hip records through the C ABI of
  * include/mpdx_h5.h (link libmpdx_h5 and HDF5).
  *
- * Records are taken when the reference takes them: at the start of every
- * advance() (hook_ante_loop, output_common.hpp:47-62) and after each step
+ * Records are taken when the reference takes them: at the start of the first
+ * advance(), from timestep 0 (hook_ante_loop, output_common.hpp:47-62, which
+ * solver_common.hpp:277-285 runs only then; it also writes const.h5 with that
+ * call's nt), and after each step
  * whose number, less some t in [0, outwindow), is a multiple of outfreq and
  * at least outstart (hook_post_step, output_common.hpp:145-156).  Output with
  * var_dt (interpolated records, output_common.hpp:72-140) is not implemented.

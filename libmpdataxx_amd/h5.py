@@ -3,8 +3,10 @@ include/mpdx_h5.h; libmpdx_h5.so is built where HDF5 is installed).
 
 ``Hdf5Xdmf`` plays the reference's ``output::hdf5_xdmf<solver>`` decorator
 (output/hdf5_xdmf.hpp, output/hdf5.hpp, output/detail/output_common.hpp):
-its ``advance(nt)`` writes const.h5 and a record at the start, then records
-after every step that output_common's hook_post_step would record
+its first ``advance(nt)`` (from timestep 0) writes const.h5 and a record at
+the start, as hook_ante_loop does (output_common.hpp:47-62, run only from
+timestep 0 by solver_common.hpp:277-285); every advance then records after
+each step that output_common's hook_post_step would record
 (output_common.hpp:145-156).
 """
 from __future__ import annotations
@@ -159,12 +161,22 @@ class Hdf5Xdmf:
         if self.setup.xdmf:
             self.records.append(ts)
 
+    def _records_after(self, ts):
+        """Records hook_post_step makes after step ts (one per matching t)."""
+        return sum(1 for t in range(self.outwindow) if (ts - t) % self.outfreq == 0 and ts - t >= self.outstart)
+
     def advance(self, nt):
-        start(self.setup, self.outdir, nt)
-        self.record_all()
-        for _ in range(int(nt)):
-            self.solver.advance(1)
+        if self.solver.timestep == 0:
+            start(self.setup, self.outdir, nt)
+            self.record_all()
+        left = int(nt)
+        while left > 0:
+            # the steps up to the next recording one go to the library in one call
             ts = self.solver.timestep
-            for t in range(self.outwindow):
-                if (ts - t) % self.outfreq == 0 and ts - t >= self.outstart:
-                    self.record_all()
+            k = 1
+            while k < left and self._records_after(ts + k) == 0:
+                k += 1
+            self.solver.advance(k)
+            left -= k
+            for _ in range(self._records_after(self.solver.timestep)):
+                self.record_all()

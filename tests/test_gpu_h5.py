@@ -95,3 +95,30 @@ def test_python_decorator_records_scheduled_states(tmp_path, dtype):
         np.testing.assert_array_equal(H.read(path, "psi", grid), want[ts][0].astype(np.float32))
         np.testing.assert_array_equal(H.read(path, "chi", grid), want[ts][1].astype(np.float32))
     s.close()
+
+
+def test_python_decorator_two_advance_calls(tmp_path):
+    """advance(4) then advance(3) records what advance(7) records: the start
+    record and const.h5 come only from the call at timestep 0 (hook_ante_loop,
+    which solver_common.hpp:277-285 runs only then), so const.h5's time axis
+    is the first call's (the records of an advance over 4 steps)."""
+    from libmpdataxx_amd import workloads as W
+
+    grid = (24, 20, 22)
+    psi = W.random_field(grid, 21)
+    gc = W.streamfunction_advector(grid, 22, periodic=False, max_courant=0.4)
+    s = M.Solver(grid, opts=M.FCT, n_iters=2, dtype=np.float64, dt=0.5)
+    s.set_advectee(psi)
+    for d in range(3):
+        s.set_advector(gc[d], d)
+    out = H.Hdf5Xdmf(s, tmp_path, outfreq=3, outwindow=2, dijk=(0.5, 0.5, 0.5))
+    out.advance(4)
+    mid = s.advectee()
+    out.advance(3)
+    assert out.records == [0, 1, 3, 4, 6, 7]
+    assert s.timestep == 7
+    t = H.read(str(tmp_path / "const.h5"), "T", (4,))
+    np.testing.assert_array_equal(t, np.array([0, 1, 3, 4], dtype=np.float32) * np.float32(0.5))
+    np.testing.assert_array_equal(H.read(str(tmp_path / "timestep0000000004.h5"), "psi", grid),
+                                  mid.astype(np.float32))
+    s.close()
