@@ -1152,13 +1152,33 @@ struct Fused {
     a.nb[0] = (a.hi[2] - a.lo[2] + 1 + TZ - 5) / (TZ - 4);
     a.nb[1] = (a.hi[1] - a.lo[1] + 1 + TY - 5) / (TY - 4);
     // x chunk: as long as possible (the 4-plane ramp of each chunk is
-    // overhead) while leaving at least two tiles per CU
+    // overhead; long blocks also hold CUs the concurrent boundary shell
+    // waits for) while leaving at least two tiles per CU.  Every block
+    // marches one chunk, one block per CU at a time, so the launch takes
+    // ceil(blocks / CUs) rounds; when that leaves the last of two or more
+    // rounds less than a third full, a nearby chunk count with the smallest
+    // rounds x (planes + 4-step ramp + ~8 steps of block start-up) is taken
+    // instead (256^3 FP64: 4 chunks of 63 planes took 3 rounds, the last
+    // holding 8 of 520 blocks; 5 of 51 take 3 full ones, 9 % faster)
     const int nxc = a.hi[0] - a.lo[0] + 1;
     int chunk = s->ix_chunk;
     if (chunk <= 0) {
+      const long tiles = (long)a.nb[0] * a.nb[1];
       chunk = 256;
-      while (chunk > 16 && (long)a.nb[0] * a.nb[1] * ((nxc + chunk - 1) / chunk) < 2L * s->n_cu)
-        chunk /= 2;
+      while (chunk > 16 && tiles * ((nxc + chunk - 1) / chunk) < 2L * s->n_cu) chunk /= 2;
+      const int c0 = (nxc + chunk - 1) / chunk;
+      const long blocks = tiles * c0, rounds = (blocks + s->n_cu - 1) / s->n_cu;
+      if (rounds >= 2 && 3 * (blocks - (rounds - 1) * s->n_cu) < s->n_cu) {
+        long best = -1;
+        for (int c = std::max(1, c0 - 2); c <= std::min(nxc, c0 + 3); ++c) {
+          const int len = (nxc + c - 1) / c;
+          const long cost = (tiles * c + s->n_cu - 1) / s->n_cu * (len + 12);
+          if (best < 0 || cost < best) {
+            best = cost;
+            chunk = len;
+          }
+        }
+      }
     }
     a.chunk = chunk;
     a.nb[2] = (nxc + a.chunk - 1) / a.chunk;
