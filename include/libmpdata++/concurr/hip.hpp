@@ -217,7 +217,12 @@ namespace libmpdataxx
         upload();
         if constexpr (ct_params_t::var_dt)
         {
+          // at least one step, and out_intrp_ord - 1 more once time has
+          // reached the end (additional_steps, solver_common.hpp:293-294,352)
           if (!panic_) check(mpdx_advance_time(h_, nt));
+          static_assert(ct_params_t::out_intrp_ord == 1 || ct_params_t::out_intrp_ord == 2,
+                        "out_intrp_ord is 1 or 2 (output_common.hpp:78-79)");
+          if (ct_params_t::out_intrp_ord == 2 && !panic_) check(mpdx_advance(h_, 1));
         }
         else if constexpr (with_output)
         {

@@ -136,17 +136,23 @@ int mpdx_get_advectee(mpdx_solver *h, int e, void *buf, const int64_t strides[3]
 int mpdx_set_advector(mpdx_solver *h, int d, const void *buf, const int64_t strides[3], int on_device);
 int mpdx_get_advector(mpdx_solver *h, int d, void *buf, const int64_t strides[3], int on_device);
 
-/* advance by nt time steps (synchronous: returns when the work is done) */
+/* advance by nt time steps (synchronous: returns when the work is done).
+ * A var_dt solver takes nt steps of its current dt (with the same
+ * Courant-number adjustment as mpdx_advance_time). */
 int mpdx_advance(mpdx_solver *h, int64_t nt);
 
 /* var_dt: advance by a duration (advance(real_t) of a var_dt solver,
- * solver_common.hpp:270-274,312-336).  On the first call dt and the
+ * solver_common.hpp:270-274,292-355).  On the first call dt and the
  * advector are rescaled so that the Courant number (solver_3d.hpp:390-398)
  * equals max_courant (hook_ante_loop, solver_common.hpp:199-210); with a
  * calc_gc hook dt is re-adjusted every step until the Courant number is at
  * most max_courant.  Steps are taken while time < time at the call +
- * duration.  mpdx_advance is refused for var_dt solvers and this call for
- * the others.  mpdx_dt returns the current dt. */
+ * duration, and at least one: the reference's loop runs out_intrp_ord
+ * (default 1) further steps once time has reached the end, so
+ * advance(0) takes one step and a step that ends at or past the end is
+ * the last.  The C++ wrapper takes the second extra step of
+ * out_intrp_ord == 2 with mpdx_advance(h, 1).  Refused (EINVAL) for
+ * constant-dt solvers.  mpdx_dt returns the current dt. */
 int mpdx_advance_time(mpdx_solver *h, double duration);
 int mpdx_dt(const mpdx_solver *h, double *dt);
 

@@ -290,7 +290,12 @@ class Solver:
         return t.value
 
     def advance_time(self, duration):
-        """var_dt solvers: advance by a duration with the adaptive step."""
+        """var_dt solvers: advance by a duration with the adaptive step.
+
+        Steps while time < time at the call + duration, and at least one
+        (advance(real_t), solver_common.hpp:292-355, with out_intrp_ord 1):
+        advance_time(0) takes one step.  advance(n) takes n steps of the
+        current dt."""
         self._check(self._L.mpdx_advance_time(self._h, float(duration)))
 
     @property

@@ -32,7 +32,7 @@ SOURCES = {
 }
 
 
-CPP_TESTS = ["revolving_sphere_3d", "rotating_cone_2d", "sphere_c3_bench"]
+CPP_TESTS = ["revolving_sphere_3d", "rotating_cone_2d", "sphere_c3_bench", "var_dt_steps"]
 # HDF5 (output only): the image's HDF5 C library; the writer is host code
 HDF5_ROOT = os.environ.get("MPDX_HDF5_ROOT", "/opt/conda")
 H5LIB = os.path.join(HERE, "libmpdx_h5.so")

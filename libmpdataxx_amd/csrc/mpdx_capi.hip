@@ -1721,8 +1721,12 @@ int seam_check_device(mpdx_solver *s, int *mode)
   return 0;
 }
 
+// nt steps, or with by_time the reference's advance(real_t) of a var_dt
+// solver (solver_common.hpp:292-355): steps while time < t_end or
+// additional_steps > 0, which starts at out_intrp_ord (1, the default) and
+// drops on each step ending at or past t_end, so at least one step is taken
 template <class R, class G>
-int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
+int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0, bool by_time = false)
 {
   const bool var_dt = s->p.var_dt;
   nt += s->timestep;
@@ -1793,7 +1797,9 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0)
         (void)hipGetLastError();
       }
   int64_t queued = 0;
-  while (var_dt ? s->time < t_end : s->timestep < nt) {
+  bool stepped = false;
+  while (by_time ? (s->time < t_end || !stepped) : s->timestep < nt) {
+    stepped = true;
     if (queued >= mpdx_solver::kInflight) {
       hipEvent_t ev = s->inflight[queued % mpdx_solver::kInflight];
       if (ev) HIPCHK(hipEventSynchronize(ev));
@@ -2411,7 +2417,6 @@ int mpdx_advance(mpdx_solver *s, int64_t nt)
 {
   MPDX_NEED(s, "solver handle");
   if (nt < 0) { s->err = "libmpdata++: negative number of steps"; return MPDX_EINVAL; }
-  if (s->p.var_dt) { s->err = "libmpdata++: a var_dt solver advances by a duration (mpdx_advance_time)"; return MPDX_EINVAL; }
   if (s->esz == 8) return advance_impl<double, double>(s, nt);
   if (s->p.guard == MPDX_GUARD_NATIVE) return advance_impl<float, float>(s, nt);
   if (s->p.guard == MPDX_GUARD_F64_EXACT) return advance_impl<float, double>(s, nt);
@@ -2424,10 +2429,10 @@ int mpdx_advance_time(mpdx_solver *s, double duration)
   if (!s->p.var_dt) { s->err = "libmpdata++: advancing by a duration needs var_dt"; return MPDX_EINVAL; }
   if (!(duration >= 0)) { s->err = "libmpdata++: negative duration"; return MPDX_EINVAL; }
   const double t_end = s->time + duration;
-  if (s->esz == 8) return advance_impl<double, double>(s, 0, t_end);
-  if (s->p.guard == MPDX_GUARD_NATIVE) return advance_impl<float, float>(s, 0, t_end);
-  if (s->p.guard == MPDX_GUARD_F64_EXACT) return advance_impl<float, double>(s, 0, t_end);
-  return advance_impl<float, GuardFast>(s, 0, t_end);
+  if (s->esz == 8) return advance_impl<double, double>(s, 0, t_end, true);
+  if (s->p.guard == MPDX_GUARD_NATIVE) return advance_impl<float, float>(s, 0, t_end, true);
+  if (s->p.guard == MPDX_GUARD_F64_EXACT) return advance_impl<float, double>(s, 0, t_end, true);
+  return advance_impl<float, GuardFast>(s, 0, t_end, true);
 }
 
 int mpdx_dt(const mpdx_solver *s, double *dt)

@@ -752,35 +752,38 @@ static int ante_loop(orc_t *s, char *err, int errlen)
   return 0;
 }
 
-static int solve(orc_t *s, long nt, double t_end, char *err, int errlen);
+static int solve(orc_t *s, long nt, double t_end, int by_time, char *err, int errlen);
 
-/* solve(): solver_common.hpp:270-358 */
+/* solve(): solver_common.hpp:270-358; nt steps (with var_dt too, each with
+   the current dt) */
 int orc_advance(orc_t *s, long nt, char *err, int errlen)
 {
-  if (s->p.var_dt) {
-    seterr(err, errlen, "libmpdata++: with var_dt advance by a duration");
-    return 1;
-  }
-  return solve(s, s->timestep + nt, 0, err, errlen);
+  return solve(s, s->timestep + nt, 0, 0, err, errlen);
 }
 
+/* advance(real_t) of a var_dt solver: solver_common.hpp:292-355 steps while
+   time < t_end or additional_steps > 0, which starts at out_intrp_ord (1 by
+   default) and drops on each step that ends at or past t_end, so at least one
+   step is taken */
 int orc_advance_time(orc_t *s, double duration, char *err, int errlen)
 {
   if (!s->p.var_dt) {
     seterr(err, errlen, "libmpdata++: advance by a duration needs var_dt");
     return 1;
   }
-  return solve(s, 0, s->time + duration, err, errlen);
+  return solve(s, 0, s->time + duration, 1, err, errlen);
 }
 
 double orc_dt(const orc_t *s) { return s->p.dt; }
 
-static int solve(orc_t *s, long nt, double t_end, char *err, int errlen)
+static int solve(orc_t *s, long nt, double t_end, int by_time, char *err, int errlen)
 {
   const int var_dt = s->p.var_dt;
+  int stepped = 0;
   if (s->timestep == 0)
     if (ante_loop(s, err, errlen)) return 1;
-  while (var_dt ? s->time < t_end : s->timestep < nt) {
+  while (by_time ? (s->time < t_end || !stepped) : s->timestep < nt) {
+    stepped = 1;
     if (s->panic) break;
     /* calc_gc() at the top of each step (solver_common.hpp:306-308) */
     if (s->gen_kind == 1) calc_gc_deform(s);

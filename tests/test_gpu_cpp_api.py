@@ -43,6 +43,42 @@ def test_rotating_cone_cpp(tmp_path, golden_dir):
         assert got == want, v
 
 
+@pytest.mark.parametrize("ord_", [1, 2])
+def test_var_dt_cpp_out_intrp_ord(tmp_path, ord_):
+    """A var_dt solver of the C++ API (tests/cpp/var_dt_steps.cpp) advanced by
+    0, 1.3 and 0.25 with out_intrp_ord 1 and 2: each call steps while time <
+    end or additional_steps > 0 (solver_common.hpp:292-355), so it takes at
+    least one step and, with out_intrp_ord 2, one more once time has reached
+    the end.  The oracle is driven the same way (advance by a duration, then
+    one step for ord 2) from the initial state the program wrote; times agree
+    to 1e-15 and the FP64 fields to 1e-12 relative."""
+    import oracle as O
+    _run("var_dt_steps", tmp_path)
+    grid = (40, 30)
+    o = O.Oracle(grid, opts=O.FCT, n_iters=2, bcs=[(O.OPEN, O.OPEN)] * 2, dt=0.3,
+                 var_dt=True, max_courant=0.4)
+    raw = np.fromfile(tmp_path / f"var_dt_ord{ord_}_in.f64", dtype=np.float64)
+    sizes = [40 * 30] + [int(np.prod(o.advector_shape(d))) for d in range(2)]
+    assert raw.size == sum(sizes)
+    parts = np.split(raw, np.cumsum(sizes)[:-1])
+    o.set_advectee(parts[0].reshape(grid))
+    for d in range(2):
+        o.set_advector(parts[1 + d].reshape(o.advector_shape(d)), d)
+    times = []
+    for dur in (0.0, 1.3, 0.25):
+        t0 = o.time
+        o.advance_time(dur)
+        if ord_ == 2:
+            o.advance(1)
+        assert o.time >= t0 + dur
+        times.append(o.time)
+    got_t = [float(x) for x in open(tmp_path / f"var_dt_ord{ord_}.txt").read().split()]
+    assert np.allclose(got_t, times, rtol=1e-15, atol=0), (got_t, times)
+    got = np.fromfile(tmp_path / f"var_dt_ord{ord_}.f64", dtype=np.float64).reshape(grid)
+    want = o.advectee()
+    assert np.sqrt(np.sum((got - want) ** 2) / np.sum(want ** 2)) <= 1e-12
+
+
 def test_headline_through_cpp_api_at_device_speed():
     """The 512^3 FP32 MPDATA-FCT workload written as a libmpdata++ program
     with concurr::hip (tests/cpp/sphere_c3_bench.cpp): advance(K) is one

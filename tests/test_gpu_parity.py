@@ -274,7 +274,8 @@ def test_calc_gc_callback_matches_oracle(dtype):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_var_dt_matches_oracle(dtype, path):
     """var_dt: the initial rescaling of dt and the advector to max_courant,
-    then steps until the requested duration, against the oracle."""
+    then steps until the requested duration, a duration of 0 and a step
+    count, against the oracle."""
     import libmpdataxx_amd as M
     grid, bcs = (30, 26, 20), [(1, 1), (0, 0), (1, 1)]
     psi, gc = setup_case(grid, bcs, 55)
@@ -291,8 +292,15 @@ def test_var_dt_matches_oracle(dtype, path):
     assert s.timestep == o.timestep
     assert abs(s.dt - o.dt) <= 1e-15 * o.dt * (1 if dtype == np.float64 else 1e8)
     assert rel_err(s.advectee().astype(np.float64), o.advectee()) <= (1e-12 if dtype == np.float64 else 1e-5)
-    with pytest.raises(Exception):
-        s.advance(1)
+    # a duration of 0 takes one step (solver_common.hpp:293-294,352); a step
+    # count takes that many steps of the current dt
+    ts = o.timestep
+    for x in (o, s):
+        x.advance_time(0.0)
+        x.advance(2)
+    assert s.timestep == o.timestep == ts + 3
+    assert abs(s.time - o.time) <= 1e-12 * o.time
+    assert rel_err(s.advectee().astype(np.float64), o.advectee()) <= (1e-12 if dtype == np.float64 else 1e-5)
     s.close()
 
 

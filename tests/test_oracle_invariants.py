@@ -292,8 +292,17 @@ def test_var_dt_matches_prescaled_fixed_dt():
     # the rescaled advector's Courant number is max_courant (interior
     # cells whose faces are all inside the views)
     assert abs(_courant(v, grid) - 0.45) < 1e-12
+    # advance(0) takes one step (additional_steps = out_intrp_ord = 1,
+    # solver_common.hpp:293-294,352), advance by a step count takes that
+    # many: the run stays equal to the fixed-dt run
+    v.advance_time(0.0)
+    assert v.timestep == steps + 1
+    v.advance(2)
+    assert v.timestep == steps + 3 and v.time == pytest.approx((steps + 3) * v.dt, rel=1e-14)
+    f.advance(3)
+    assert np.array_equal(v.advectee(), f.advectee())
     with pytest.raises(RuntimeError):
-        v.advance(1)
+        f.advance_time(1.0)
 
 
 def _deform_closed_form(grid, T, courant, t_half, half=0.5, swap=False):
