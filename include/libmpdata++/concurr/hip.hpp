@@ -12,7 +12,8 @@
  * advectee_global() gives a read-only view that triggers no upload.
  * advance(nt) is one library call for all nt steps (the library polls the
  * panic flag between steps); with an output decorator it is one call per
- * stretch of steps between records.
+ * stretch of steps between records (with var_dt one call, the library calling
+ * the decorator back for each record).
  * All work goes through the C ABI of include/mpdx.h; a solver decorated with
  * output::hdf5 / output::hdf5_xdmf records through include/mpdx_h5.h.
  */
@@ -188,6 +189,11 @@ namespace libmpdataxx
         mp.var_dt = ct_params_t::var_dt ? 1 : 0;
         mp.max_courant = p.max_courant;
         if (mpdx_create(&mp, &h_) != MPDX_OK) throw std::runtime_error(mpdx_last_error(nullptr));
+        if constexpr (with_output)
+        {
+          try { out_.attach(h_); }
+          catch (...) { mpdx_destroy(h_); throw; }
+        }
         std::size_t n = 1;
         for (int d = 0; d < n_dims; ++d) n *= grid_[d];
         psi_host_.assign(n_eqns, std::vector<real_t>(n, real_t(0)));
@@ -218,11 +224,16 @@ namespace libmpdataxx
         if constexpr (ct_params_t::var_dt)
         {
           // at least one step, and out_intrp_ord - 1 more once time has
-          // reached the end (additional_steps, solver_common.hpp:293-294,352)
-          if (!panic_) check(mpdx_advance_time(h_, nt));
+          // reached the end (additional_steps, solver_common.hpp:293-294,352);
+          // an output decorator records from the library's callbacks
           static_assert(ct_params_t::out_intrp_ord == 1 || ct_params_t::out_intrp_ord == 2,
                         "out_intrp_ord is 1 or 2 (output_common.hpp:78-79)");
-          if (ct_params_t::out_intrp_ord == 2 && !panic_) check(mpdx_advance(h_, 1));
+          for (int call = 0; call < ct_params_t::out_intrp_ord && !panic_; ++call)
+          {
+            const int rc = call == 0 ? mpdx_advance_time(h_, nt) : mpdx_advance(h_, 1);
+            if constexpr (with_output) out_.rethrow();
+            check(rc);
+          }
         }
         else if constexpr (with_output)
         {

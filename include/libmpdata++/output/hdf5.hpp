@@ -17,8 +17,11 @@
  * solver_common.hpp:277-285 runs only then; it also writes const.h5 with that
  * call's nt), and after each step
  * whose number, less some t in [0, outwindow), is a multiple of outfreq and
- * at least outstart (hook_post_step, output_common.hpp:145-156).  Output with
- * var_dt (interpolated records, output_common.hpp:72-140) is not implemented.
+ * at least outstart (hook_post_step, output_common.hpp:145-156).  With var_dt
+ * the library keeps the reference's interpolation schedule
+ * (output_common.hpp:72-140, mpdx_set_var_dt_output) and calls the recorder
+ * back: at the start (after dt was rescaled, so const.h5 carries that dt) and
+ * for each record at k * outfreq, time-interpolated to out_intrp_ord 1 or 2.
  */
 #pragma once
 
@@ -46,6 +49,7 @@ namespace libmpdataxx
         using real_t = typename solver_t::real_t;
         using advance_arg_t = typename solver_t::advance_arg_t;
         enum { n_dims = solver_t::n_dims, n_eqns = solver_t::n_eqns };
+        static constexpr bool var_dt = solver_t::ct_params_t_::var_dt;
 
         mpdx_h5_setup s_{};
         std::map<int, info_t> outvars_;
@@ -54,6 +58,7 @@ namespace libmpdataxx
         int outwindow_ = 1;
         std::vector<int64_t> records_;   // timesteps already in temp.xmf
         std::vector<std::vector<real_t>> buf_;
+        std::string cb_err_;             // exception text from a library callback
 
         static void check(int rc)
         {
@@ -67,13 +72,18 @@ namespace libmpdataxx
         template <class rt_params_t>
         void init(rt_params_t &p)
         {
-          static_assert(!solver_t::ct_params_t_::var_dt, "output with var_dt is not implemented by the HIP backend");
           outfreq_ = p.outfreq;
           outstart_ = p.outstart;
           outwindow_ = p.outwindow;
           outvars_ = p.outvars;
           outdir_ = p.outdir;
-          if (int(outfreq_) <= 0 || int(outstart_) % int(outfreq_) != 0)
+          if constexpr (var_dt)
+          {
+            // output_common.hpp:193-194
+            if (!(outfreq_ > 0)) throw std::runtime_error("libmpdata++: outfreq must be positive");
+            if (outstart_ != 0) throw std::runtime_error("libmpdata++: outstart must be 0 with var_dt");
+          }
+          else if (int(outfreq_) <= 0 || int(outstart_) % int(outfreq_) != 0)
             throw std::runtime_error("libmpdata++: outstart must be a multiple of outfreq");
           if (outvars_.empty() && n_eqns == 1) outvars_ = {{0, {"", ""}}};
           for (const auto &v : outvars_)
@@ -89,7 +99,7 @@ namespace libmpdataxx
           s_.dt = p.dt;
           s_.n_iters = p.n_iters;
           s_.opts = (uint32_t)solver_t::ct_params_t_::opts;
-          s_.var_dt = 0;
+          s_.var_dt = var_dt ? 1 : 0;
           s_.max_courant = p.max_courant;
           s_.outfreq = double(outfreq_);
           s_.outstart = double(outstart_);
@@ -98,6 +108,28 @@ namespace libmpdataxx
           std::size_t n = 1;
           for (int d = 0; d < n_dims; ++d) n *= std::size_t(p.grid_size[d]);
           buf_.assign(outvars_.size(), std::vector<real_t>(n));
+        }
+
+        // var_dt: the library runs the interpolation schedule and calls back
+        void attach(mpdx_solver *h)
+        {
+          if constexpr (var_dt)
+          {
+            std::vector<int> vars;
+            for (const auto &v : outvars_) vars.push_back(v.first);
+            if (mpdx_set_var_dt_output(h, double(outfreq_), solver_t::ct_params_t_::out_intrp_ord, int(vars.size()),
+                                       vars.data(), &recorder::on_output, this) != MPDX_OK)
+              throw std::runtime_error(mpdx_last_error(h));
+          }
+        }
+
+        // an exception a callback turned into an error code
+        void rethrow()
+        {
+          if (cb_err_.empty()) return;
+          const std::string m = cb_err_;
+          cb_err_.clear();
+          throw std::runtime_error(m);
         }
 
         // hdf5.hpp:66-235 start() followed by record_all()
@@ -130,12 +162,39 @@ namespace libmpdataxx
           return horizon;
         }
 
-        void record(mpdx_solver *h)
+        // with var_dt, the start (hook_ante_loop, output_common.hpp:47-62:
+        // const.h5 with the rescaled dt, then a record at the current time)
+        // and each interpolated record (output_common.hpp:104-156)
+        static int on_output(mpdx_solver *h, int kind, double t, void *user)
+        {
+          auto *r = static_cast<recorder *>(user);
+          try
+          {
+            double rt = t;
+            if (kind == MPDX_OUT_START)
+            {
+              mpdx_dt(h, &r->s_.dt);
+              r->check(mpdx_h5_start(&r->s_, r->outdir_.c_str(), double(real_t(t))));
+              mpdx_time(h, &rt);
+            }
+            r->record(h, true, double(real_t(rt)));
+            return 0;
+          }
+          catch (const std::exception &e)
+          {
+            r->cb_err_ = e.what();
+            return 1;
+          }
+        }
+
+        // interp: the library's interpolation buffers at time t (var_dt),
+        // else the current state at timestep * dt
+        void record(mpdx_solver *h, bool interp = false, double t = 0)
         {
           int64_t ts;
           mpdx_timestep(h, &ts);
           // time = timestep * dt in the solver's real_t (solver_common.hpp:348)
-          const real_t record_time = real_t(ts) * real_t(s_.dt);
+          const double record_time = interp ? t : double(real_t(ts) * real_t(s_.dt));
           int64_t st[3] = {0, 0, 0};
           int64_t s = 1;
           for (int d = n_dims - 1; d >= 0; --d) { st[d] = s; s *= s_.grid[d]; }
@@ -144,13 +203,14 @@ namespace libmpdataxx
           std::size_t q = 0;
           for (const auto &v : outvars_)
           {
-            if (mpdx_get_advectee(h, v.first, buf_[q].data(), st, 0) != MPDX_OK)
+            if ((interp ? mpdx_get_output(h, v.first, buf_[q].data(), st, 0)
+                        : mpdx_get_advectee(h, v.first, buf_[q].data(), st, 0)) != MPDX_OK)
               throw std::runtime_error(mpdx_last_error(h));
             names.push_back(v.second.name.c_str());
             fields.push_back(buf_[q].data());
             ++q;
           }
-          check(mpdx_h5_record(&s_, outdir_.c_str(), ts, double(record_time), int(names.size()), names.data(),
+          check(mpdx_h5_record(&s_, outdir_.c_str(), ts, record_time, int(names.size()), names.data(),
                                fields.data(), int(records_.size()), records_.data()));
           if (xdmf) records_.push_back(ts);
         }

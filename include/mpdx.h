@@ -259,6 +259,30 @@ typedef int (*mpdx_rhs_hook_fn)(mpdx_solver *h, int hook, void *user);
 int mpdx_set_rhs_hooks(mpdx_solver *h, mpdx_rhs_hook_fn hook, void *user);
 int mpdx_update_rhs(mpdx_solver *h, double dt, int at);
 int mpdx_apply_rhs(mpdx_solver *h, double dt_arg);
+/* Output with var_dt (output/detail/output_common.hpp:32-37,47-62,72-140):
+ * records at the times k * outfreq, interpolated in time from the states
+ * around them (intrp_ord 1: linear from the states before and after the
+ * step that reaches k * outfreq; 2: quadratic, with the state after one
+ * more step).  The library keeps the interpolation buffers on the device
+ * for the equations listed in vars and calls fn(h, kind, t, user):
+ *   MPDX_OUT_START:  hook_ante_loop of the first advance from timestep 0,
+ *                    after dt was rescaled; t = the end time of that call
+ *                    (time + duration; time + nt * dt for mpdx_advance);
+ *                    the buffers hold the current state;
+ *   MPDX_OUT_RECORD: after the step that completes a record (hook_post_step),
+ *                    with the solver's timestep and time already advanced;
+ *                    t = the record's time k * outfreq.
+ * In the callback mpdx_get_output copies the buffer of equation e (interior,
+ * like mpdx_get_advectee).  A nonzero return from fn stops the advance with
+ * MPDX_EINVAL.  Only for var_dt solvers; fn = NULL removes the schedule.
+ * Times are compared and coefficients formed in double. */
+#define MPDX_OUT_START 0
+#define MPDX_OUT_RECORD 1
+typedef int (*mpdx_output_fn)(mpdx_solver *h, int kind, double t, void *user);
+int mpdx_set_var_dt_output(mpdx_solver *h, double outfreq, int intrp_ord, int n_vars, const int *vars,
+                           mpdx_output_fn fn, void *user);
+int mpdx_get_output(mpdx_solver *h, int e, void *buf, const int64_t strides[3], int on_device);
+
 /* copy the forcing of equation e (interior, like mpdx_set_advectee) */
 int mpdx_set_rhs(mpdx_solver *h, int e, const void *buf, const int64_t strides[3], int on_device);
 /* device address of element (0, 0, 0) of the forcing / the current state of

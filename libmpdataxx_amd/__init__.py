@@ -35,7 +35,7 @@ EXPORTED = [
     "mpdx_panic_ptr", "mpdx_watch_panic", "mpdx_active_path", "mpdx_z_tiles", "mpdx_profile", "mpdx_profile_read",
     "mpdx_set_calc_gc", "mpdx_advector_device", "mpdx_set_advector_generator",
     "mpdx_set_rhs_scheme", "mpdx_set_rhs", "mpdx_rhs_device", "mpdx_advectee_device",
-    "mpdx_set_rhs_hooks", "mpdx_update_rhs", "mpdx_apply_rhs",
+    "mpdx_set_rhs_hooks", "mpdx_update_rhs", "mpdx_apply_rhs", "mpdx_set_var_dt_output", "mpdx_get_output",
     "mpdx_advance_time", "mpdx_dt", "mpdx_set_x_chunk", "mpdx_set_tile_y",
     "mpdx_ensemble_create", "mpdx_ensemble_destroy", "mpdx_ensemble_advance", "mpdx_ensemble_tune",
     "mpdx_ensemble_x_chunk", "mpdx_ensemble_tile_y", "mpdx_ensemble_stats", "mpdx_ensemble_last_error",
@@ -46,6 +46,8 @@ RHS_NONE, RHS_EULER_A, RHS_EULER_B, RHS_TRAPEZ, RHS_MIXED = 0, 1, 2, 3, 4
 HOOK_ANTE_LOOP, HOOK_ANTE_STEP, HOOK_POST_STEP = 0, 1, 2
 UPDATE_RHS_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_void_p)
 RHS_HOOK_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p)
+OUT_START, OUT_RECORD = 0, 1
+OUTPUT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p)
 
 
 class Params(ctypes.Structure):
@@ -88,7 +90,7 @@ def lib():
         L.mpdx_set_stream.argtypes = [P, P]
         L.mpdx_advector_extents.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         for name in ("mpdx_set_advectee", "mpdx_get_advectee", "mpdx_set_advector", "mpdx_get_advector",
-                     "mpdx_set_rhs"):
+                     "mpdx_set_rhs", "mpdx_get_output"):
             getattr(L, name).argtypes = [P, ctypes.c_int, P, i64p, ctypes.c_int]
         L.mpdx_advance.argtypes = [P, ctypes.c_int64]
         L.mpdx_advance_time.argtypes = [P, ctypes.c_double]
@@ -108,6 +110,8 @@ def lib():
         L.mpdx_advectee_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), i64p]
         L.mpdx_set_rhs_scheme.argtypes = [P, ctypes.c_int, UPDATE_RHS_FN, P]
         L.mpdx_set_rhs_hooks.argtypes = [P, RHS_HOOK_FN, P]
+        L.mpdx_set_var_dt_output.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_int), OUTPUT_FN, P]
         L.mpdx_update_rhs.argtypes = [P, ctypes.c_double, ctypes.c_int]
         L.mpdx_apply_rhs.argtypes = [P, ctypes.c_double]
         L.mpdx_set_advector_generator.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
@@ -266,7 +270,14 @@ class Solver:
         self._check(self._L.mpdx_set_stream(self._h, stream_ptr))
 
     def advance(self, nt):
-        self._check(self._L.mpdx_advance(self._h, int(nt)))
+        self._step_check(self._L.mpdx_advance(self._h, int(nt)))
+
+    def _step_check(self, rc):
+        # an exception raised by an output callback during the call
+        exc, self._out_exc = getattr(self, "_out_exc", None), None
+        if rc and exc is not None:
+            raise exc
+        self._check(rc)
 
     def reduce(self, op, e=0):
         out = ctypes.c_double()
@@ -296,7 +307,37 @@ class Solver:
         (advance(real_t), solver_common.hpp:292-355, with out_intrp_ord 1):
         advance_time(0) takes one step.  advance(n) takes n steps of the
         current dt."""
-        self._check(self._L.mpdx_advance_time(self._h, float(duration)))
+        self._step_check(self._L.mpdx_advance_time(self._h, float(duration)))
+
+    def set_var_dt_output(self, fn, outfreq, intrp_ord=1, outvars=(0,)):
+        """var_dt output (output_common.hpp:47-62,72-140): fn(kind, t) is
+        called with OUT_START (first advance from timestep 0, t = the call's
+        end time) and OUT_RECORD (a record at time t = k * outfreq,
+        interpolated to order intrp_ord); inside it output(e) is the field to
+        write.  fn = None removes the schedule."""
+        if fn is None:
+            self._out_cb = None
+            self._check(self._L.mpdx_set_var_dt_output(self._h, 1.0, 1, 0, None, OUTPUT_FN(), None))
+            return
+
+        def tramp(h, kind, t, user):
+            try:
+                fn(kind, t)
+                return 0
+            except Exception as exc:   # noqa: BLE001 - re-raised by the advance call
+                self._out_exc = exc
+                return -1
+        vs = (ctypes.c_int * len(outvars))(*[int(e) for e in outvars])
+        cb = OUTPUT_FN(tramp)
+        self._check(self._L.mpdx_set_var_dt_output(self._h, float(outfreq), int(intrp_ord), len(outvars), vs, cb,
+                                                   None))
+        self._out_cb = cb
+
+    def output(self, e=0):
+        """The interpolation buffer of output equation e (interior)."""
+        out = np.empty(self.grid, dtype=self.dtype)
+        self._check(self._L.mpdx_get_output(self._h, e, out.ctypes.data, _strides(out), 0))
+        return out
 
     @property
     def dt(self):

@@ -36,7 +36,7 @@ CPP_TESTS = ["revolving_sphere_3d", "rotating_cone_2d", "sphere_c3_bench", "var_
 # HDF5 (output only): the image's HDF5 C library; the writer is host code
 HDF5_ROOT = os.environ.get("MPDX_HDF5_ROOT", "/opt/conda")
 H5LIB = os.path.join(HERE, "libmpdx_h5.so")
-CPP_TESTS_H5 = ["revolving_sphere_3d_h5"]
+CPP_TESTS_H5 = ["revolving_sphere_3d_h5", "var_dt_h5"]
 
 
 def have_hdf5() -> bool:

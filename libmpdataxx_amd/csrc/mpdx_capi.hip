@@ -1721,6 +1721,88 @@ int seam_check_device(mpdx_solver *s, int *mode)
   return 0;
 }
 
+// ---------------------------------------------------------------- var_dt output
+
+int out_call(mpdx_solver *s, int kind, double t)
+{
+  if (s->out_fn(s, kind, t, s->out_user) != 0) {
+    if (s->err.empty()) s->err = "libmpdata++: output callback failed";
+    return MPDX_EINVAL;
+  }
+  return 0;
+}
+
+// the current state of the output equations into interpolation slot k
+template <class R>
+int out_save(mpdx_solver *s, int k)
+{
+  for (int v = 0; v < s->out_n; ++v) {
+    const int e = s->out_var[v];
+    HIPCHK(hipMemcpyAsync(s->out_buf[k][v], s->psi[e][s->lev[e]], sizeof(R) * (size_t)s->L.size,
+                          hipMemcpyDeviceToDevice, s->stream));
+  }
+  return 0;
+}
+
+// hook_ante_loop (output_common.hpp:47-62): the buffers take the state and
+// the caller writes const.h5 and the first record
+template <class R>
+int out_start(mpdx_solver *s, double t_end)
+{
+  if (int rc = out_save<R>(s, 0)) return rc;
+  s->out_cnt = 0;
+  s->out_rec = s->time;
+  return out_call(s, MPDX_OUT_START, t_end);
+}
+
+// hook_ante_step (output_common.hpp:72-101), once the step's dt is set: a
+// step that reaches the next multiple of outfreq starts a record (its time
+// k * outfreq, out_ord states to collect), and while one is open the state
+// before the step is saved
+template <class R>
+int out_ante(mpdx_solver *s)
+{
+  const double next_time = s->time + s->p.dt;
+  const long next_idx = (long)std::floor(next_time / s->out_freq);
+  const long curr_idx = (long)std::floor(s->time / s->out_freq);
+  s->out_cnt = s->out_cnt > 0 ? s->out_cnt - 1 : 0;
+  if (next_idx > curr_idx || s->out_cnt > 0) {
+    if (s->out_cnt == 0) {
+      s->out_cnt = s->out_ord;
+      s->out_rec = (double)next_idx * s->out_freq;
+    }
+    const int k = s->out_ord - s->out_cnt;
+    s->out_t[k] = s->time;
+    if (int rc = out_save<R>(s, k)) return rc;
+  }
+  return 0;
+}
+
+// hook_post_step (output_common.hpp:104-156): the step that completes a
+// record interpolates it into slot 0 and hands it to the caller
+template <class R>
+int out_post(mpdx_solver *s)
+{
+  if (s->out_cnt != 1) return 0;
+  const double t = s->out_rec, t0 = s->out_t[0];
+  for (int v = 0; v < s->out_n; ++v) {
+    const int e = s->out_var[v];
+    R *y0 = (R *)s->out_buf[0][v];
+    const R *psi = (const R *)s->psi[e][s->lev[e]];
+    if (s->out_ord == 1) {
+      const double t1 = s->time;
+      launch(s->stream, s->L.size, k_out_lin<R>, y0, psi, s->L.size, R((t1 - t) / (t1 - t0)),
+             R((t - t0) / (t1 - t0)));
+    } else {
+      const double t1 = s->out_t[1], t2 = s->time;
+      launch(s->stream, s->L.size, k_out_quad<R>, y0, (const R *)s->out_buf[1][v], psi, s->L.size,
+             R(t1 - t0), R(t2 - t0), R(t2 - t1), R(t - t0), R(t - t1));
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return out_call(s, MPDX_OUT_RECORD, t);
+}
+
 // nt steps, or with by_time the reference's advance(real_t) of a var_dt
 // solver (solver_common.hpp:292-355): steps while time < t_end or
 // additional_steps > 0, which starts at out_intrp_ord (1, the default) and
@@ -1737,6 +1819,8 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0, bool by_time = fa
       if (int rc = update_rhs<R>(s, s->p.dt / 2, 0)) return rc;
     if (s->rhs_scheme == MPDX_RHS_MIXED)
       if (int rc = rhs_hook(s, MPDX_HOOK_ANTE_LOOP)) return rc;
+    if (s->out_fn)
+      if (int rc = out_start<R>(s, by_time ? t_end : s->time + (double)(nt - s->timestep) * s->p.dt)) return rc;
   }
   const int rs = s->rhs_scheme;
   const bool fused = s->active_path == MPDX_PATH_FUSED;
@@ -1863,6 +1947,8 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0, bool by_time = fa
         (void)hipGraphDestroy(g);
         HIPCHK(ei);
       }
+      if (s->out_fn)
+        if (int rc = out_ante<R>(s)) return rc;
       HIPCHK(hipGraphLaunch(s->graph[key], s->stream));
       for (int e = 0; e < s->p.n_eqns; ++e) s->lev[e] ^= gs & 1;
       adv = gs;
@@ -1897,6 +1983,9 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0, bool by_time = fa
       } else if (rs == MPDX_RHS_MIXED) {
         if (int rc = rhs_hook(s, MPDX_HOOK_ANTE_STEP)) return rc;
       }
+      // output_common::hook_ante_step runs after its parents' (output<rhs<...>>)
+      if (s->out_fn)
+        if (int rc = out_ante<R>(s)) return rc;
       if (fused) {
         if (int rc = fused_step<R, G>(s)) return rc;
       } else {
@@ -1918,6 +2007,8 @@ int advance_impl(mpdx_solver *s, int64_t nt, double t_end = 0, bool by_time = fa
     } else if (rs == MPDX_RHS_MIXED) {
       if (int rc = rhs_hook(s, MPDX_HOOK_POST_STEP)) return rc;
     }
+    if (s->out_fn)
+      if (int rc = out_post<R>(s)) return rc;
   }
   HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
@@ -2548,6 +2639,53 @@ int mpdx_set_rhs_hooks(mpdx_solver *s, mpdx_rhs_hook_fn hook, void *user)
   s->rhs_hook = hook;
   s->rhs_hook_user = user;
   return MPDX_OK;
+}
+
+int mpdx_set_var_dt_output(mpdx_solver *s, double outfreq, int ord, int n_vars, const int *vars,
+                           mpdx_output_fn fn, void *user)
+{
+  MPDX_NEED(s, "solver handle");
+  if (!fn) {
+    s->out_fn = nullptr;
+    return MPDX_OK;
+  }
+  MPDX_NEED(vars, "variable list");
+  if (!s->p.var_dt) { s->err = "libmpdata++: interpolated output needs var_dt"; return MPDX_EINVAL; }
+  if (!(outfreq > 0) || !std::isfinite(outfreq)) { s->err = "libmpdata++: outfreq must be positive"; return MPDX_EINVAL; }
+  if (ord != 1 && ord != 2) {
+    s->err = "libmpdata++: only 1st and 2nd order temporal interpolation for output implemented";
+    return MPDX_EINVAL;
+  }
+  if (n_vars < 1 || n_vars > s->p.n_eqns) { s->err = "libmpdata++: bad number of output variables"; return MPDX_EINVAL; }
+  for (int v = 0; v < n_vars; ++v) {
+    if (vars[v] < 0 || vars[v] >= s->p.n_eqns) { s->err = "libmpdata++: bad equation index"; return MPDX_EINVAL; }
+    for (int u = 0; u < v; ++u)
+      if (vars[u] == vars[v]) { s->err = "libmpdata++: output variable listed twice"; return MPDX_EINVAL; }
+  }
+  for (int k = 0; k < ord; ++k)
+    for (int v = 0; v < n_vars; ++v)
+      if (!s->out_buf[k][v])
+        if (int rc = dev_alloc(s, &s->out_buf[k][v], (size_t)s->esz * (size_t)s->L.size)) return rc;
+  for (int v = 0; v < n_vars; ++v) s->out_var[v] = vars[v];
+  s->out_n = n_vars;
+  s->out_ord = ord;
+  s->out_freq = outfreq;
+  s->out_cnt = 0;
+  s->out_fn = fn;
+  s->out_user = user;
+  return MPDX_OK;
+}
+
+int mpdx_get_output(mpdx_solver *s, int e, void *buf, const int64_t strides[3], int on_device)
+{
+  MPDX_NEED(s, "solver handle");
+  MPDX_NEED(buf, "buffer");
+  MPDX_NEED(strides, "strides");
+  int v = 0;
+  while (v < s->out_n && s->out_var[v] != e) ++v;
+  if (!s->out_fn || v == s->out_n) { s->err = "libmpdata++: equation is not an output variable"; return MPDX_EINVAL; }
+  int lo[3] = {0, 0, 0}, hi[3] = {s->L.n[0] - 1, s->L.n[1] - 1, s->L.nd == 3 ? s->L.n[2] - 1 : 0};
+  return copy_view(s, s->out_buf[0][v], lo, hi, buf, strides, on_device, false);
 }
 
 int mpdx_update_rhs(mpdx_solver *s, double dt, int at)

@@ -427,6 +427,33 @@ __global__ void k_fill_value(R *a, long n, R v)
   if (t < n) a[t] = v;
 }
 
+// var_dt output: the record interpolated in time (output_common.hpp:
+// 112-137), element by element over whole arrays, each product and sum
+// rounded on its own like the reference's array statements.  Linear:
+// y = y * c0 + c1 * psi with c0 = (t1 - t) / (t1 - t0), c1 = (t - t0) /
+// (t1 - t0).  Quadratic through (t0, y0), (t1, y1), (t2, y2) with the time
+// differences d10 = t1 - t0, d20 = t2 - t0, d21 = t2 - t1, e0 = t - t0,
+// e1 = t - t1 formed in double on the host.
+template <class R>
+__global__ void k_out_lin(R *y, const R *psi, long n, R c0, R c1)
+{
+#pragma clang fp contract(off)
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const R a = y[q] * c0;
+  y[q] = a + c1 * psi[q];
+}
+
+template <class R>
+__global__ void k_out_quad(R *y0, const R *y1, const R *y2, long n, R d10, R d20, R d21, R e0, R e1)
+{
+#pragma clang fp contract(off)
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const R a = y0[q], b = y1[q], c = y2[q];
+  y0[q] = a + (b - a) / d10 * e0 + ((c - b) / (d21 * d20) - (b - a) / (d10 * d20)) * e0 * e1;
+}
+
 // ---------------------------------------------------------------- copies and reductions
 
 template <class R>

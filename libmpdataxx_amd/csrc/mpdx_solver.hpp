@@ -154,6 +154,18 @@ struct mpdx_solver {
   void *rhs[MPDX_MAX_EQNS] = {};
   mpdx_calc_gc_fn calc_gc = nullptr;
   void *calc_gc_user = nullptr;
+  // var_dt output (output_common.hpp:32-37,72-140): out_buf[k][v] holds the
+  // state saved before step k of a record's interpolation (k = 0 afterwards
+  // the interpolated field) for equation out_var[v]; out_cnt is the
+  // reference's do_record_cnt, out_t its intrp_times, out_rec record_time
+  mpdx_output_fn out_fn = nullptr;
+  void *out_user = nullptr;
+  double out_freq = 1;
+  int out_ord = 1, out_n = 0;
+  int out_var[MPDX_MAX_EQNS] = {};
+  void *out_buf[2][MPDX_MAX_EQNS] = {};
+  int out_cnt = 0;
+  double out_t[2] = {}, out_rec = 0;
   int gen_kind = MPDX_GC_NONE;
   double gen_T = 1, gen_scale = 0;
   double *gen_sx2 = nullptr, *gen_sy2 = nullptr;   // device sin^2 tables from index -2

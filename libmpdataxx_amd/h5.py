@@ -7,7 +7,10 @@ its first ``advance(nt)`` (from timestep 0) writes const.h5 and a record at
 the start, as hook_ante_loop does (output_common.hpp:47-62, run only from
 timestep 0 by solver_common.hpp:277-285); every advance then records after
 each step that output_common's hook_post_step would record
-(output_common.hpp:145-156).
+(output_common.hpp:145-156).  With a var_dt solver the library runs the
+interpolation schedule of output_common.hpp:72-140 and calls the decorator
+back for the start and for each record at k * outfreq (intrp_ord plays
+ct_params_t::out_intrp_ord).
 """
 from __future__ import annotations
 
@@ -131,24 +134,50 @@ class Hdf5Xdmf:
     constructor does."""
 
     def __init__(self, solver, outdir, outfreq=1, outstart=0, outwindow=1, outvars=None, dijk=(0.0, 0.0, 0.0),
-                 xdmf=True):
+                 xdmf=True, intrp_ord=1):
         p = solver.params
-        if p.var_dt:
-            raise RuntimeError("output with var_dt is not implemented by the HIP backend")
-        if outfreq <= 0 or int(outstart) % int(outfreq) != 0:
+        self.var_dt = bool(p.var_dt)
+        if self.var_dt:
+            # output_common.hpp:78-80,193-194
+            if not outfreq > 0:
+                raise ValueError("outfreq must be positive")
+            if outstart != 0:
+                raise ValueError("outstart must be 0 with var_dt")
+            if intrp_ord not in (1, 2):
+                raise ValueError("only 1st and 2nd order temporal interpolation for output implemented")
+        elif outfreq <= 0 or int(outstart) % int(outfreq) != 0:
             raise ValueError("outstart must be a multiple of outfreq")
         if xdmf and solver.n_dims < 2:
             raise ValueError("only 2D and 3D output supported")
         self.solver = solver
         self.outdir = os.fspath(outdir)
         self.outvars = dict(outvars) if outvars else {0: "psi"}
-        self.outfreq, self.outstart, self.outwindow = int(outfreq), int(outstart), int(outwindow)
+        self.outstart, self.outwindow = int(outstart), int(outwindow)
+        self.outfreq = float(outfreq) if self.var_dt else int(outfreq)
+        self.intrp_ord = int(intrp_ord)
         dt = p.dt if p.dt != 0 else 1.0
         dijk = [v if v != 0 else 1.0 for v in list(dijk) + [0.0] * (3 - len(dijk))]
-        self.setup = make_setup(solver.grid, p.real_bytes, dijk, dt, p.n_iters, p.opts, False, p.max_courant,
+        self.setup = make_setup(solver.grid, p.real_bytes, dijk, dt, p.n_iters, p.opts, self.var_dt, p.max_courant,
                                 outfreq, outstart, outwindow, xdmf)
         self._real = np.float64 if p.real_bytes == 8 else np.float32
         self.records = []
+        if self.var_dt:
+            solver.set_var_dt_output(self._on_output, self.outfreq, self.intrp_ord,
+                                     outvars=sorted(self.outvars))
+
+    def _on_output(self, kind, t):
+        # the start (hook_ante_loop: const.h5 with the rescaled dt, a record
+        # at the current time) and each interpolated record
+        from . import OUT_START
+        if kind == OUT_START:
+            self.setup.dt = self.solver.dt
+            start(self.setup, self.outdir, float(self._real(t)))
+            t = self.solver.time
+        ts = self.solver.timestep
+        fields = {name: self.solver.output(e) for e, name in sorted(self.outvars.items())}
+        record(self.setup, self.outdir, ts, float(self._real(t)), fields, self.records if self.setup.xdmf else ())
+        if self.setup.xdmf:
+            self.records.append(ts)
 
     def _time(self, ts):
         # timestep * dt in the solver's real_t (solver_common.hpp:348)
@@ -166,6 +195,12 @@ class Hdf5Xdmf:
         return sum(1 for t in range(self.outwindow) if (ts - t) % self.outfreq == 0 and ts - t >= self.outstart)
 
     def advance(self, nt):
+        if self.var_dt:
+            # a duration; out_intrp_ord - 1 further steps (solver_common.hpp:293-294,352)
+            self.solver.advance_time(nt)
+            if self.intrp_ord == 2:
+                self.solver.advance(1)
+            return
         if self.solver.timestep == 0:
             start(self.setup, self.outdir, nt)
             self.record_all()

@@ -24,6 +24,7 @@ FCT, ABS, TOT, PFC, NPA, IGA, NUG, DFL, KHN = (1 << b for b in range(9))
 CYCLIC, OPEN = 0, 1
 RHS_NONE, RHS_EULER_A, RHS_EULER_B, RHS_TRAPEZ = 0, 1, 2, 3
 RHS_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_void_p)
+OUT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p)
 
 
 class _Params(ctypes.Structure):
@@ -109,6 +110,9 @@ def _lib(fast: bool = False):
         lib.orc_advance_time.argtypes = [P, ctypes.c_double, ctypes.c_char_p, ctypes.c_int]
         lib.orc_dt.argtypes = [P]
         lib.orc_dt.restype = ctypes.c_double
+        lib.orc_set_var_dt_output.argtypes = [P, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int), OUT_FN, ctypes.c_void_p]
+        lib.orc_get_output.argtypes = [P, ctypes.c_int, dp]
         _libs[name] = lib
     return _libs[name]
 
@@ -214,8 +218,31 @@ class Oracle:
         if self._lib.orc_set_rhs(self._h, e, _dp(a)):
             raise ValueError("rhs not enabled or bad equation index")
 
+    def set_var_dt_output(self, fn, outfreq, intrp_ord=1, outvars=(0,)):
+        """var_dt output (output_common.hpp:47-62,72-140): fn(kind, t) with
+        kind 0 at the start (t = end time of the first advance) and 1 for a
+        record at time t; inside it output(e) is the field to write."""
+        def tramp(h, kind, t, user):
+            try:
+                fn(kind, t)
+                return 0
+            except Exception:          # noqa: BLE001 - reported as an error code
+                return -1
+        self._out_cb = OUT_FN(tramp)
+        vs = (ctypes.c_int * len(outvars))(*[int(e) for e in outvars])
+        if self._lib.orc_set_var_dt_output(self._h, float(outfreq), int(intrp_ord), len(outvars), vs, self._out_cb,
+                                           None):
+            raise ValueError("bad var_dt output schedule")
+
+    def output(self, e=0):
+        out = np.empty(tuple(self.grid), dtype=np.float64)
+        if self._lib.orc_get_output(self._h, e, _dp(out)):
+            raise ValueError("not an output variable")
+        return out
+
     def advance_time(self, duration):
-        """var_dt: advance by a duration with the adaptive time step."""
+        """var_dt: advance by a duration with the adaptive time step (at
+        least one step)."""
         err = ctypes.create_string_buffer(256)
         if self._lib.orc_advance_time(self._h, float(duration), err, 256):
             raise RuntimeError(err.value.decode())

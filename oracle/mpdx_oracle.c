@@ -14,9 +14,10 @@
  * Pinning: the reference's own output files pin donor cell, the standard and
  * iga antidiffusive velocities, FCT, open halos and the forcing schemes;
  * exact identities and textbook reductions pin cyclic halos, n_iters >= 3,
- * abs, npa, pfc, khn, hint_scale, upwind_filter_freq and var_dt.  Parity
- * unpinned: the exact field of the deformational advector generator (only
- * its invariants are pinned).
+ * abs, npa, pfc, khn, hint_scale, upwind_filter_freq, var_dt and its
+ * interpolated output (exact on fields linear / quadratic in time); an
+ * independent evaluation of its closed form pins the deformational advector
+ * generator.
  *
  * The heavy sweeps carry OpenMP pragmas so the same code doubles as the
  * multi-core CPU baseline; each element is computed independently, so the
@@ -60,6 +61,15 @@ struct orc {
   int gen_kind;               /* 0 none, 1 reversing deformational flow */
   double gen_T, gen_scale;
   double *gen_sx2, *gen_sy2;  /* sin^2 at corners a + 1/2 from a = -2 */
+  /* var_dt output: intrp_vars (slot k of output variable v), do_record_cnt,
+     intrp_times, record_time (output_common.hpp:32-37) */
+  orc_out_fn out_fn;
+  void *out_user;
+  double out_freq;
+  int out_ord, out_n, out_var[ORC_MAX_EQNS];
+  double *out_buf[2][ORC_MAX_EQNS];
+  int out_cnt;
+  double out_t[2], out_rec;
 };
 
 static inline rng R_(int f, int l) { rng r = {f, l}; return r; }
@@ -776,12 +786,119 @@ int orc_advance_time(orc_t *s, double duration, char *err, int errlen)
 
 double orc_dt(const orc_t *s) { return s->p.dt; }
 
+/* ---- var_dt output: output_common.hpp:47-62,72-140 */
+
+static int out_call(orc_t *s, int kind, double t, char *err, int errlen)
+{
+  if (s->out_fn(s, kind, t, s->out_user)) {
+    seterr(err, errlen, "libmpdata++: output callback failed");
+    return 1;
+  }
+  return 0;
+}
+
+/* intrp_vars[v + k n_eqns](ijk) = advectee(v)(ijk) */
+static void out_save(orc_t *s, int k)
+{
+  for (int v = 0; v < s->out_n; ++v) {
+    const int e = s->out_var[v];
+    memcpy(s->out_buf[k][v], s->psi[e][s->lev[e]], sizeof(double) * (size_t)s->size);
+  }
+}
+
+/* hook_ante_step, output_common.hpp:72-101 */
+static void out_ante(orc_t *s)
+{
+  const double next_time = s->time + s->p.dt;
+  const long next_idx = (long)floor(next_time / s->out_freq);
+  const long curr_idx = (long)floor(s->time / s->out_freq);
+  s->out_cnt = s->out_cnt > 0 ? s->out_cnt - 1 : 0;
+  if (next_idx > curr_idx || s->out_cnt > 0) {
+    if (s->out_cnt == 0) {
+      s->out_cnt = s->out_ord;
+      s->out_rec = next_idx * s->out_freq;
+    }
+    const int inc_ix = s->out_ord - s->out_cnt;
+    s->out_t[inc_ix] = s->time;
+    out_save(s, inc_ix);
+  }
+}
+
+/* hook_post_step, output_common.hpp:104-156 (interpolation over the
+   interior, then record_all) */
+static int out_post(orc_t *s, char *err, int errlen)
+{
+  if (s->out_cnt != 1) return 0;
+  const int n0 = s->n[0], n1 = s->n[1], n2 = s->nd == 3 ? s->n[2] : 1;
+  const double t = s->out_rec;
+  for (int v = 0; v < s->out_n; ++v) {
+    const int e = s->out_var[v];
+    double *y = s->out_buf[0][v];
+    const double *psi = s->psi[e][s->lev[e]];
+    for (int i = 0; i < n0; ++i)
+      for (int j = 0; j < n1; ++j)
+        for (int k = 0; k < n2; ++k) {
+          const long q = ix(s, i, j, k);
+          if (s->out_ord == 1) {
+            const double t0 = s->out_t[0], t1 = s->time;
+            y[q] *= (t1 - t) / (t1 - t0);
+            y[q] += (t - t0) / (t1 - t0) * psi[q];
+          } else {
+            const double t0 = s->out_t[0], t1 = s->out_t[1], t2 = s->time;
+            const double y0 = y[q], y1 = s->out_buf[1][v][q], y2 = psi[q];
+            y[q] = y0 + (y1 - y0) / (t1 - t0) * (t - t0) +
+                   ((y2 - y1) / ((t2 - t1) * (t2 - t0)) - (y1 - y0) / ((t1 - t0) * (t2 - t0))) * (t - t0) * (t - t1);
+          }
+        }
+  }
+  return out_call(s, 1, t, err, errlen);
+}
+
+int orc_set_var_dt_output(orc_t *s, double outfreq, int intrp_ord, int n_vars, const int *vars, orc_out_fn fn,
+                          void *user)
+{
+  if (!fn) { s->out_fn = NULL; return 0; }
+  if (!s->p.var_dt || !(outfreq > 0) || (intrp_ord != 1 && intrp_ord != 2) || n_vars < 1 ||
+      n_vars > s->p.n_eqns)
+    return 1;
+  for (int v = 0; v < n_vars; ++v)
+    if (vars[v] < 0 || vars[v] >= s->p.n_eqns) return 1;
+  for (int k = 0; k < intrp_ord; ++k)
+    for (int v = 0; v < n_vars; ++v)
+      if (!s->out_buf[k][v]) s->out_buf[k][v] = alloc_nan(s->size);
+  for (int v = 0; v < n_vars; ++v) s->out_var[v] = vars[v];
+  s->out_n = n_vars;
+  s->out_ord = intrp_ord;
+  s->out_freq = outfreq;
+  s->out_cnt = 0;
+  s->out_fn = fn;
+  s->out_user = user;
+  return 0;
+}
+
+static int copy_interior(const orc_t *s, double *arr, double *dense, int to_arr);
+
+int orc_get_output(const orc_t *s, int e, double *dense)
+{
+  for (int v = 0; v < s->out_n; ++v)
+    if (s->out_fn && s->out_var[v] == e) return copy_interior(s, s->out_buf[0][v], dense, 0);
+  return 1;
+}
+
 static int solve(orc_t *s, long nt, double t_end, int by_time, char *err, int errlen)
 {
   const int var_dt = s->p.var_dt;
   int stepped = 0;
-  if (s->timestep == 0)
+  if (s->timestep == 0) {
     if (ante_loop(s, err, errlen)) return 1;
+    /* output_common::hook_ante_loop: intrp_vars = advectee, start + record */
+    if (s->out_fn) {
+      out_save(s, 0);
+      s->out_cnt = 0;
+      s->out_rec = s->time;
+      if (out_call(s, 0, by_time ? t_end : s->time + (nt - s->timestep) * s->p.dt, err, errlen)) return 1;
+    }
+  }
   while (by_time ? (s->time < t_end || !stepped) : s->timestep < nt) {
     stepped = 1;
     if (s->panic) break;
@@ -794,6 +911,8 @@ static int solve(orc_t *s, long nt, double t_end, int by_time, char *err, int er
     } else if (s->rhs_scheme == 3) {
       apply_rhs(s, s->p.dt / 2);
     }
+    /* output_common::hook_ante_step after its parents' */
+    if (s->out_fn) out_ante(s);
     for (int e = 0; e < s->p.n_eqns; ++e) {
       scale(s, e, s->p.hint_scale[e]);
       xchng_sclr(s, s->psi[e][s->lev[e]], s->hw);
@@ -810,6 +929,7 @@ static int solve(orc_t *s, long nt, double t_end, int by_time, char *err, int er
       if (update_rhs(s, s->p.dt / 2, 1)) { seterr(err, errlen, "libmpdata++: update_rhs failed"); return 1; }
       apply_rhs(s, s->p.dt / 2);
     }
+    if (s->out_fn && out_post(s, err, errlen)) return 1;
   }
   return 0;
 }
@@ -877,6 +997,8 @@ void orc_destroy(orc_t *s)
   free(s->psi_min); free(s->psi_max); free(s->beta_up); free(s->beta_dn);
   free(s->gen_sx2); free(s->gen_sy2);
   for (int e = 0; e < s->p.n_eqns; ++e) free(s->rhs[e]);
+  for (int k = 0; k < 2; ++k)
+    for (int v = 0; v < ORC_MAX_EQNS; ++v) free(s->out_buf[k][v]);
   free(s);
 }
 

@@ -57,8 +57,9 @@ int orc_get_advector(const orc_t *s, int d, double *dense);
 int orc_advance(orc_t *s, long nt, char *err, int errlen);
 
 /* var_dt: advance by a duration (the reference's advance(real_t) with
- * ct_params_t::var_dt, solver_common.hpp:270-274,312-336); dt is rescaled so
- * that the Courant number is max_courant */
+ * ct_params_t::var_dt, solver_common.hpp:270-274,292-355, out_intrp_ord 1:
+ * at least one step); dt is rescaled so that the Courant number is
+ * max_courant */
 int orc_advance_time(orc_t *s, double duration, char *err, int errlen);
 double orc_dt(const orc_t *s);
 
@@ -74,6 +75,17 @@ int orc_set_generator(orc_t *s, int kind, double T, double max_courant);
 typedef int (*orc_rhs_fn)(orc_t *s, double dt, int at, void *user);
 int orc_set_rhs_scheme(orc_t *s, int scheme, orc_rhs_fn fn, void *user);
 int orc_set_rhs(orc_t *s, int e, const double *dense);
+
+/* Output with var_dt (output/detail/output_common.hpp:32-37,47-62,72-140):
+ * records at the times k * outfreq interpolated in time (intrp_ord 1 or 2)
+ * for the equations in vars.  fn(s, kind, t, user) is called with kind 0 at
+ * hook_ante_loop of the first advance from timestep 0 (t = that call's end
+ * time; the buffers hold the state) and kind 1 after the step completing a
+ * record at time t; orc_get_output reads the buffer of equation e. */
+typedef int (*orc_out_fn)(orc_t *s, int kind, double t, void *user);
+int orc_set_var_dt_output(orc_t *s, double outfreq, int intrp_ord, int n_vars, const int *vars, orc_out_fn fn,
+                          void *user);
+int orc_get_output(const orc_t *s, int e, double *dense);
 
 long orc_timestep(const orc_t *s);
 double orc_time(const orc_t *s);

@@ -84,6 +84,8 @@ def test_null_handles_are_errors():
         "mpdx_ensemble_advance": (None, 1),
         "mpdx_ensemble_tune": (None, 1, None),
         "mpdx_ensemble_x_chunk": (None, 0, ctypes.byref(ctypes.c_int())),
+        "mpdx_set_var_dt_output": (None, 1.0, 1, 1, None, M.OUTPUT_FN(), None),
+        "mpdx_get_output": (None, 0, buf, st, 0),
     }
     for name, args in calls.items():
         fn = getattr(L, name)
@@ -121,6 +123,8 @@ def test_null_buffers_are_errors():
         "mpdx_timestep": [(h, None)],
         "mpdx_reduce": [(h, 0, 0, None)],
         "mpdx_advector_extents": [(h, 0, None)],
+        "mpdx_get_output": [(h, 0, None, st, 0), (h, 0, buf, None, 0)],
+        "mpdx_set_var_dt_output": [(h, 1.0, 1, 1, None, M.OUTPUT_FN(lambda *a: 0), None)],
     }
     for name, variants in calls.items():
         fn = getattr(L, name)

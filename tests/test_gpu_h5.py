@@ -122,3 +122,94 @@ def test_python_decorator_two_advance_calls(tmp_path):
     np.testing.assert_array_equal(H.read(str(tmp_path / "timestep0000000004.h5"), "psi", grid),
                                   mid.astype(np.float32))
     s.close()
+
+
+def _oracle_var_dt_records(init, grid, ord_, durations, outfreq, **kw):
+    """The oracle's var_dt output (start and records: kind, time, timestep,
+    interpolated field) for the given initial advectee and advector."""
+    import oracle as O
+    o = O.Oracle(grid, **kw)
+    o.set_advectee(init[0])
+    for d in range(len(grid)):
+        o.set_advector(init[1 + d], d)
+    recs = []
+    o.set_var_dt_output(lambda kind, t: recs.append((kind, t, o.timestep, o.output(0))), outfreq, ord_)
+    for dur in durations:
+        o.advance_time(dur)
+        if ord_ == 2:
+            o.advance(1)
+    return recs, o
+
+
+def _check_var_dt_files(outdir, recs, grid, name="psi"):
+    import re
+    assert [r[0] for r in recs] == [0] + [1] * (len(recs) - 1) and len(recs) >= 4
+    temp = open(os.path.join(outdir, "temp.xmf")).read()
+    listed = re.findall(r"timestep(\d{10})\.xmf", temp)
+    assert [int(x) for x in listed] == [r[2] for r in recs]
+    for kind, t, ts, want in recs:
+        base = os.path.join(outdir, f"timestep{ts:010d}")
+        xmf = open(base + ".xmf").read()
+        # the start record is at the initial time (its callback's t is the
+        # first call's end time)
+        rt = t if kind == 1 else 0.0
+        assert float(re.search(r'<Time Value="([^"]+)"', xmf).group(1)) == pytest.approx(rt, abs=1e-6)
+        got = H.read(base + ".h5", name, grid).astype(np.float64)
+        assert np.all(np.abs(got - want) <= np.abs(want) * 2.0 ** -22 + 1e-12), (ts, t)
+
+
+def test_var_dt_cpp_output(tmp_path):
+    """output::hdf5_xdmf over a var_dt solver through the C++ API
+    (tests/cpp/var_dt_h5.cpp, out_intrp_ord 1 and 2): the start record and the
+    records at k * outfreq, time-interpolated (output_common.hpp:47-62,
+    72-140), hold the oracle's interpolated fields (float32 storage, 2^-22),
+    are named by the timestep they were written after and carry their record
+    time in the XDMF markup; const.h5's time axis covers the first call
+    (var_dt: multiples of outfreq up to its end time, hdf5.hpp:199-210)."""
+    import oracle as O
+    exe = os.path.join(ROOT, "tests", "cpp", "bin", "var_dt_h5")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} not built (python -m libmpdataxx_amd.build)")
+    print("running var_dt_h5", flush=True)
+    subprocess.run([exe, str(tmp_path)], check=True, timeout=300)
+    grid = (40, 30)
+    probe = O.Oracle(grid, opts=O.FCT, n_iters=2, bcs=[(O.OPEN, O.OPEN)] * 2, dt=0.3)
+    shapes = [grid] + [probe.advector_shape(d) for d in range(2)]
+    for ord_ in (1, 2):
+        out = tmp_path / f"ord{ord_}"
+        raw = np.fromfile(out / "in.f64", dtype=np.float64)
+        parts = np.split(raw, np.cumsum([int(np.prod(s)) for s in shapes])[:-1])
+        assert sum(p.size for p in parts) == raw.size
+        init = [p.reshape(s) for p, s in zip(parts, shapes)]
+        recs, o = _oracle_var_dt_records(init, grid, ord_, (1.3, 0.9), 0.5, opts=O.FCT, n_iters=2,
+                                         bcs=[(O.OPEN, O.OPEN)] * 2, dt=0.3, var_dt=True, max_courant=0.4)
+        _check_var_dt_files(str(out), recs, grid)
+        np.testing.assert_array_equal(H.read(str(out / "const.h5"), "T", (3,)),
+                                      np.array([0.0, 0.5, 1.0], dtype=np.float32))
+        dump = subprocess.run([H5DUMP, "-A", "const.h5"], cwd=out, capture_output=True, check=True).stdout.decode()
+        assert "max_courant" in dump and "var_dt" in dump
+
+
+def test_python_decorator_var_dt(tmp_path):
+    """The Python decorator over a var_dt solver (3D, two equations, FP64,
+    out_intrp_ord 2): files hold the oracle's interpolated records."""
+    import oracle as O
+    from libmpdataxx_amd import workloads as W
+
+    grid = (24, 20, 22)
+    psi = W.random_field(grid, 31)
+    gc = W.streamfunction_advector(grid, 32, periodic=False, max_courant=0.4)
+    kw = dict(opts=M.FCT, n_iters=3, dt=0.5, var_dt=True, max_courant=0.45)
+    s = M.Solver(grid, dtype=np.float64, **kw)
+    s.set_advectee(psi)
+    for d in range(3):
+        s.set_advector(gc[d], d)
+    out = H.Hdf5Xdmf(s, tmp_path, outfreq=0.75, intrp_ord=2, dijk=(0.5, 0.5, 0.5))
+    out.advance(2.0)
+    out.advance(1.1)
+    recs, o = _oracle_var_dt_records([psi] + list(gc), grid, 2, (2.0, 1.1), 0.75, bcs=[(O.OPEN, O.OPEN)] * 3,
+                                     **kw)
+    assert s.timestep == o.timestep
+    assert out.records == [r[2] for r in recs]
+    _check_var_dt_files(str(tmp_path), recs, grid)
+    s.close()

@@ -304,6 +304,49 @@ def test_var_dt_matches_oracle(dtype, path):
     s.close()
 
 
+@pytest.mark.parametrize("ord_", [1, 2])
+@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_var_dt_output_matches_oracle(dtype, path, ord_):
+    """var_dt output (output_common.hpp:47-62,72-140): the start call and
+    every record (kind, time, timestep, interpolated field of both
+    equations) of the library equal the oracle's; FP64 <= 1e-12, FP32 <= 1e-5
+    relative.  Two advance calls (durations 1.3 and 0.9, plus the further
+    step of out_intrp_ord 2), records every 0.5."""
+    import libmpdataxx_amd as M
+    grid, bcs = (30, 26, 20), [(1, 1), (0, 0), (1, 1)]
+    psi, gc = setup_case(grid, bcs, 56)
+    psi1 = W_random(grid, 57)
+    print(f"var_dt output dtype={np.dtype(dtype).name} path={path} ord={ord_}", flush=True)
+    kw = dict(opts=O.FCT, n_iters=3, n_eqns=2, bcs=bcs, dt=0.2, max_abs_div_eps=1e300, var_dt=True,
+              max_courant=0.42)
+    o = O.Oracle(grid, **kw)
+    s = M.Solver(grid, dtype=dtype, path=path, **kw)
+    recs = {id(o): [], id(s): []}
+    for x in (o, s):
+        x.set_advectee(psi, 0)
+        x.set_advectee(psi1, 1)
+        for d in range(3):
+            x.set_advector(gc[d], d)
+
+        def out(kind, t, x=x):
+            recs[id(x)].append((kind, t, x.timestep, [x.output(e).astype(np.float64) for e in (1, 0)]))
+        x.set_var_dt_output(out, 0.5, ord_, outvars=(1, 0))
+    for dur in (1.3, 0.9):
+        for x in (o, s):
+            x.advance_time(dur)
+            if ord_ == 2:
+                x.advance(1)
+    ro, rs = recs[id(o)], recs[id(s)]
+    assert [r[0] for r in ro] == [0] + [1] * (len(ro) - 1) and len(ro) >= 4
+    assert [r[:3] for r in rs] == [r[:3] for r in ro]
+    tol = 1e-12 if dtype == np.float64 else 1e-5
+    for a, b in zip(rs, ro):
+        for fa, fb in zip(a[3], b[3]):
+            assert rel_err(fa, fb) <= tol, (a[:3], rel_err(fa, fb))
+    s.close()
+
+
 def test_var_dt_with_calc_gc_rescales_every_step():
     """var_dt with a time-varying advector: the callback writes velocity x
     dt; the library shrinks dt until the Courant number is max_courant
