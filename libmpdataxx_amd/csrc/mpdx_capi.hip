@@ -1526,7 +1526,8 @@ void free_batch_plan(BatchPlan &p)
 // those groups as one batch; a member moves to its next group once its
 // current one is done.  Each member's groups therefore run in order, and a
 // batch holds tasks of different members (or independent tasks of one).
-void schedule(const std::vector<TaskRecorder> &recs, std::vector<std::vector<const RecOp *>> &out)
+void schedule(const std::vector<TaskRecorder> &recs, std::vector<std::vector<const RecOp *>> &out,
+              std::vector<std::vector<int>> &who)
 {
   const size_t n = recs.size();
   std::vector<size_t> g(n, 0);
@@ -1563,6 +1564,7 @@ void schedule(const std::vector<TaskRecorder> &recs, std::vector<std::vector<con
     for (const auto &c : count)
       if (c.second > most) { best = c.first; most = c.second; }
     std::vector<const RecOp *> batch;
+    std::vector<int> seqs;
     for (size_t i = 0; i < n; ++i) {
       if (g[i] >= recs[i].groups.size()) continue;
       const auto &grp = recs[i].groups[g[i]];
@@ -1570,10 +1572,12 @@ void schedule(const std::vector<TaskRecorder> &recs, std::vector<std::vector<con
         if (!taken[i][q] && grp[q].fn == best) {
           batch.push_back(&grp[q]);
           taken[i][q] = 1;
+          if (seqs.empty() || seqs.back() != (int)i) seqs.push_back((int)i);
         }
       settle(i);
     }
     out.push_back(std::move(batch));
+    who.push_back(std::move(seqs));
   }
 }
 
@@ -1582,7 +1586,9 @@ void schedule(const std::vector<TaskRecorder> &recs, std::vector<std::vector<con
 int pack_plan(const std::vector<TaskRecorder> &recs, BatchPlan &p, std::string &err)
 {
   std::vector<std::vector<const RecOp *>> batches;
-  schedule(recs, batches);
+  std::vector<std::vector<int>> who;
+  schedule(recs, batches, who);
+  std::vector<int> last_of(recs.size(), -1);   // last batch holding tasks of each sequence
   // per batch its tasks, then its prefix table, each 256-byte aligned
   std::vector<unsigned char> blob;
   auto align = [&]() { blob.resize((blob.size() + 255) / 256 * 256); };
@@ -1602,6 +1608,12 @@ int pack_plan(const std::vector<TaskRecorder> &recs, BatchPlan &p, std::string &
       first += op->blocks;
     }
     eb.blocks = first;
+    const int j = (int)p.batches.size();
+    for (int i : who[j]) {
+      if (last_of[i] >= 0 && std::find(eb.deps.begin(), eb.deps.end(), last_of[i]) == eb.deps.end())
+        eb.deps.push_back(last_of[i]);
+      last_of[i] = j;
+    }
     p.batches.push_back(eb);
     p.tasks += b.size();
   }
@@ -1620,6 +1632,54 @@ void launch_plan(const BatchPlan &p, hipStream_t st)
   char *dev = static_cast<char *>(p.dev);
   for (const EnsBatch &b : p.batches)
     b.fn(st, dev + b.toff, reinterpret_cast<const unsigned *>(dev + b.foff), b.n, b.blocks);
+}
+
+// The same batches spread over st and the streams aux[0..naux-1], for
+// capture into one graph: a batch follows only the batches it depends on
+// (its sequences' earlier batches), so batches of unrelated sequences (2D
+// and 3D members, members with other options) run side by side and fill
+// each other's last block rounds.  A batch goes to the stream whose last
+// batch it depends on, else to the stream idle longest; dependencies on
+// other streams become event waits.  ev holds batches.size() + 1 events
+// (one per batch and the fork).  Every auxiliary stream that gets a batch
+// forks from st and joins it again at the end.
+hipError_t launch_plan_streams(const BatchPlan &p, hipStream_t st, const hipStream_t *aux, int naux,
+                               const hipEvent_t *ev)
+{
+  const int nb = (int)p.batches.size(), ns = naux + 1;
+  auto S = [&](int k) { return k == 0 ? st : aux[k - 1]; };
+  std::vector<int> where(nb, -1), last(ns, -1), used(ns, 0);
+  for (int j = 0; j < nb; ++j) {
+    int k = -1;
+    for (int d : p.batches[j].deps)
+      if (last[where[d]] == d) { k = where[d]; break; }
+    if (k < 0) {
+      k = 0;
+      for (int q = 1; q < ns; ++q)
+        if (last[q] < last[k]) k = q;
+    }
+    where[j] = k;
+    last[k] = j;
+    used[k] = 1;
+  }
+  char *dev = static_cast<char *>(p.dev);
+  hipError_t e = hipEventRecord(ev[nb], st);
+  for (int k = 1; k < ns && e == hipSuccess; ++k)
+    if (used[k]) e = hipStreamWaitEvent(S(k), ev[nb], 0);
+  std::fill(last.begin(), last.end(), -1);
+  for (int j = 0; j < nb && e == hipSuccess; ++j) {
+    const EnsBatch &b = p.batches[j];
+    const int k = where[j];
+    for (int d : b.deps)
+      if (where[d] != k && e == hipSuccess) e = hipStreamWaitEvent(S(k), ev[d], 0);
+    if (e != hipSuccess) break;
+    b.fn(S(k), dev + b.toff, reinterpret_cast<const unsigned *>(dev + b.foff), b.n, b.blocks);
+    e = hipEventRecord(ev[j], S(k));
+    last[k] = j;
+  }
+  for (int k = 1; k < ns && e == hipSuccess; ++k)
+    if (last[k] >= 0) e = hipStreamWaitEvent(st, ev[last[k]], 0);
+  return e;
 }
 
 // the user's hooks of the mixed scheme (mpdata_rhs.hpp:112-124): they
@@ -2058,6 +2118,7 @@ struct EnsPlan {
   std::vector<int> key;              // members' time levels the plan was recorded at
   BatchPlan bp;
   hipGraphExec_t graph = nullptr;
+  std::vector<hipEvent_t> ev;        // the capture's fork and per-batch events
 };
 
 }  // namespace
@@ -2065,6 +2126,12 @@ struct EnsPlan {
 struct mpdx_ensemble {
   std::vector<mpdx_solver *> m;
   hipStream_t stream = nullptr;
+  // streams that batches of unrelated members are spread over while a
+  // step is captured (MPDX_ENS_STREAMS: streams in all, 1 = one linear
+  // sequence; default 4)
+  static constexpr int kAux = 7;
+  hipStream_t aux[kAux] = {};
+  int n_aux = 0;
   EnsPlan plan[2];
   int next_slot = 0;
   static constexpr int kInflight = 6;
@@ -2099,6 +2166,8 @@ void free_plan(EnsPlan &p)
 {
   if (p.graph) (void)hipGraphExecDestroy(p.graph);
   free_batch_plan(p.bp);
+  for (hipEvent_t ev : p.ev)
+    if (ev) (void)hipEventDestroy(ev);
   p = EnsPlan();
 }
 
@@ -2137,11 +2206,23 @@ int build_plan(mpdx_ensemble *e, EnsPlan &p)
   if (rc) return member_error(e, fail_at, e->m[fail_at], rc);
 
   if (int rc2 = pack_plan(recs, p.bp, e->err)) return rc2;
+  if (e->n_aux > 0) {
+    p.ev.assign(p.bp.batches.size() + 1, nullptr);
+    for (hipEvent_t &ev : p.ev) EHIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
   hipGraph_t graph = nullptr;
   EHIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-  launch_plan(p.bp, e->stream);
-  const hipError_t el = hipGetLastError();
+  hipError_t el = hipSuccess;
+  if (e->n_aux > 0)
+    el = launch_plan_streams(p.bp, e->stream, e->aux, e->n_aux, p.ev.data());
+  else
+    launch_plan(p.bp, e->stream);
+  if (el == hipSuccess) el = hipGetLastError();
   const hipError_t ec = hipStreamEndCapture(e->stream, &graph);
+  if (el != hipSuccess && graph) {
+    (void)hipGraphDestroy(graph);
+    graph = nullptr;
+  }
   EHIPCHK(el);
   EHIPCHK(ec);
   const hipError_t ei = hipGraphInstantiate(&p.graph, graph, nullptr, nullptr, 0);
@@ -2903,6 +2984,17 @@ int mpdx_ensemble_create(mpdx_solver *const *members, int n, mpdx_ensemble **out
     g_create_error = "libmpdata++: cannot create the ensemble's stream";
     return MPDX_EHIP;
   }
+  int ns = 4;
+  if (const char *v = std::getenv("MPDX_ENS_STREAMS"))
+    if (atoi(v) > 0) ns = std::min(atoi(v), mpdx_ensemble::kAux + 1);
+  for (int k = 0; k + 1 < ns; ++k) {
+    if (hipStreamCreateWithFlags(&e->aux[k], hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
+      e->aux[k] = nullptr;
+      break;   // fewer streams: the batches still run, with less overlap
+    }
+    e->n_aux = k + 1;
+  }
   *out = e;
   return MPDX_OK;
 }
@@ -2914,6 +3006,8 @@ int mpdx_ensemble_destroy(mpdx_ensemble *e)
   drop_plans(e);
   for (hipEvent_t ev : e->inflight)
     if (ev) (void)hipEventDestroy(ev);
+  for (hipStream_t a : e->aux)
+    if (a) (void)hipStreamDestroy(a);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return MPDX_OK;

@@ -46,12 +46,15 @@ struct TaskRecorder {
 };
 
 // a batched launch of k_task_list: its kernel, the byte offsets of its
-// tasks and of their block prefix table in BatchPlan::dev, tasks, blocks
+// tasks and of their block prefix table in BatchPlan::dev, tasks, blocks,
+// and the earlier batches it must follow (for each sequence it holds tasks
+// of, the last earlier batch holding tasks of that sequence)
 struct EnsBatch {
   TaskLaunchFn fn;
   size_t toff, foff;
   int n;
   unsigned blocks;
+  std::vector<int> deps;
 };
 
 // recorded launches of several independent sequences merged into batches

@@ -4,9 +4,10 @@ n_iters 2-4, FCT on/off) stepped by one graph of batched launches per step
 (each launch runs one kernel's tasks of many members).
 
 The ensemble must give exactly what each member gives when advanced on its
-own (same device code, any x-chunk length and tile height: results do not
-depend on them), match the oracle domain by domain in FP64, and conserve
-mass and positivity."""
+own (same device code, any x-chunk length and tile height and any number of
+streams the captured step is spread over: results do not depend on them),
+match the oracle domain by domain in FP64, and conserve mass and
+positivity."""
 import numpy as np
 import pytest
 
@@ -58,10 +59,14 @@ def test_x_chunk_does_not_change_results(dtype, bc, opts, iters):
         np.testing.assert_array_equal(o, outs[0])
 
 
+@pytest.mark.parametrize("streams", ["1", "4", "8"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_ensemble_matches_members_alone(domains, dtype):
+def test_ensemble_matches_members_alone(domains, dtype, streams, monkeypatch):
+    """Equal to the members advanced alone, with the step captured as one
+    linear sequence of batched launches or spread over 4 or 8 streams."""
     pick = [0, 3, 7, 31, 32, 33, 40, 63]
-    print(f"ensemble of {len(pick)} dtype={np.dtype(dtype).name}", flush=True)
+    print(f"ensemble of {len(pick)} dtype={np.dtype(dtype).name} streams={streams}", flush=True)
+    monkeypatch.setenv("MPDX_ENS_STREAMS", streams)
     ens_m = [make(domains[i], dtype) for i in pick]
     alone = [make(domains[i], dtype) for i in pick]
     e = M.Ensemble(ens_m)
