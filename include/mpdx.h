@@ -305,11 +305,14 @@ int mpdx_profile_read(mpdx_solver *h, int idx, char *name, int name_len,
  * do not depend on it. */
 int mpdx_set_x_chunk(mpdx_solver *h, int planes);
 
-/* Interior tile height of the 3D interior kernel: 32 (one 1024-thread block
- * per CU) or 16 rows (two 512-thread blocks per CU) for FP32 solvers with the
- * default guards; 0 = the default (32 in FP32; FP64 and the other guard
- * modes always use their own default).  A tuning knob with no reference
- * counterpart; results do not depend on it. */
+/* Interior tile height of the 3D interior kernel for FP32 solvers with the
+ * default guards: 32 (one 1024-thread block per CU) or 16 rows (two
+ * 512-thread blocks per CU) for every corrective pass; 0 = the default: 32
+ * rows for the first corrective pass, 24 (two 768-thread blocks per CU) for
+ * the later ones (32 for all of them when the solver is stepped as an
+ * ensemble member).  FP64 and the other guard modes always use their own
+ * default.  A tuning knob with no reference counterpart; results do not
+ * depend on it. */
 int mpdx_set_tile_y(mpdx_solver *h, int rows);
 
 /* Ensembles: independent solvers (domains of any size, dimensionality,
@@ -319,7 +322,8 @@ int mpdx_set_tile_y(mpdx_solver *h, int rows);
  * (same kernels, same arguments), launches of the same kernel are merged
  * across members into single batched launches (a device table of per-member
  * tasks), and the batched launches are captured on the ensemble's own stream
- * as one linear hipGraph that is replayed once per step.  Members stay owned
+ * as one linear hipGraph that is replayed once per step (MPDX_ENS_STREAMS=n
+ * spreads them over n streams by member dependencies instead).  Members stay owned
  * by the caller, must outlive the ensemble, live on one device, and keep
  * working on their own between ensemble calls (their pending work is
  * finished before an ensemble step starts).  They must use the fused path

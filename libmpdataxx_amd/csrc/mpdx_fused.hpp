@@ -606,8 +606,17 @@ template <class R> __device__ __forceinline__ void buf_st(char *g, unsigned off,
                                           buf_rsrc(g), (int)off, 0, 0);
 }
 
-#ifdef MPDX_INTERIOR_WPE
-#define MPDX_INTERIOR_ATTR __attribute__((amdgpu_waves_per_eu(MPDX_INTERIOR_WPE, MPDX_INTERIOR_WPE)))
+// Waves per SIMD an interior kernel is compiled for: FP32 tiles 24 rows
+// high (768 threads) run two blocks per CU, 6 waves per SIMD, which caps
+// them at 80 VGPRs; 1 leaves the others to their launch bounds.  Applied in
+// the translation unit that instantiates the FP32 default-guard kernels
+// (MPDX_INTERIOR_TU); the others carry no such attribute.
+template <class R, int TYV> constexpr int interior_waves_per_eu()
+{
+  return sizeof(R) == 4 && TYV == 24 ? 6 : 1;
+}
+#ifdef MPDX_INTERIOR_TU
+#define MPDX_INTERIOR_ATTR __attribute__((amdgpu_waves_per_eu(interior_waves_per_eu<R, TYV>())))
 #else
 #define MPDX_INTERIOR_ATTR
 #endif

@@ -3,10 +3,13 @@
 // with the max-ILP instruction scheduler: their occupancy is fixed by LDS
 // (one 1024-thread block per CU, or two 512-thread ones), so registers are
 // better spent on instruction-level parallelism than on the occupancy the
-// default scheduler aims for.  Both the single launches (k_fz_interior) and
+// default scheduler aims for; the 24-row tiles of the later corrective
+// passes are compiled for two 768-thread blocks per CU (6 waves per SIMD,
+// interior_waves_per_eu).  Both the single launches (k_fz_interior) and
 // the batched ones of ensembles (k_task_list over InteriorTask) are here.
 // The FP64 and other-guard variants keep the default scheduler (with
 // max-ILP they need up to 250 VGPRs or spill).
+#define MPDX_INTERIOR_TU 1
 #include "mpdx_tasks.hpp"
 #include "mpdx_interior_list.hpp"
 

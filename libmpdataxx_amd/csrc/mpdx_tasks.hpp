@@ -19,9 +19,23 @@ struct TaskBlk {
   unsigned x, y, z;
 };
 
+// waves per SIMD a batched launch is compiled for: the interior tasks'
+// own (interior_waves_per_eu), 1 (the launch bounds decide) for the others
+template <class T> struct TaskWaves {
+  static constexpr int v = 1;
+};
+
+// the interior tasks' batched launches carry their kernel's occupancy
+// attribute in the translation unit that instantiates them
+#ifdef MPDX_INTERIOR_TU
+#define MPDX_TASK_ATTR __attribute__((amdgpu_waves_per_eu(TaskWaves<T>::v)))
+#else
+#define MPDX_TASK_ATTR
+#endif
+
 // blocks of a task's own grid; first = prefix sums of blocks, n tasks
 template <class T>
-__global__ void __launch_bounds__(T::kThreads) k_task_list(const T *tasks, const unsigned *first, int n)
+__global__ void __launch_bounds__(T::kThreads) MPDX_TASK_ATTR k_task_list(const T *tasks, const unsigned *first, int n)
 {
   const unsigned b = blockIdx.x;
   int lo = 0, hi = n - 1;
@@ -216,6 +230,10 @@ struct InteriorTask : TaskGrid {
   static constexpr int kThreads = ITile<TYV>::Y * ITile<TYV>::Z, kBlockY = ITile<TYV>::Y;
   InteriorArgs<R> a;
   __device__ void run(TaskBlk b) const { fz_interior_block<R, G, FCT, IGA, STOREV, FUSE0, TYV, WXY>(a, b.x); }
+};
+template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV, bool WXY>
+struct TaskWaves<InteriorTask<R, G, FCT, IGA, STOREV, FUSE0, TYV, WXY>> {
+  static constexpr int v = interior_waves_per_eu<R, TYV>();
 };
 
 }  // namespace mpdx
