@@ -1317,7 +1317,21 @@ struct Fused {
     // shell reads get theirs from ring_zfill after the tiles are done)
     const bool zt = use_int && ztiles();
     pp.zskip = zt;
-    pp.zskip_nrml = zt && seam;
+    // open z: the tangential components' z halo is zero (the normal fill)
+    // and nothing but that fill writes it outside the edge bands (the tiles
+    // store z = 0..n-1, the shell only its bands), so once an array has been
+    // filled in full the later fills refill only the bands; a recorded step
+    // (batched launches) always fills in full
+    auto zeroed = [&](const R *a) {
+      return std::find(s->zhalo_zeroed.begin(), s->zhalo_zeroed.end(), (const void *)a) != s->zhalo_zeroed.end();
+    };
+    const bool open_zt = zt && !seam && !s->rec;
+    pp.zskip_nrml = zt && (seam || (open_zt && zeroed(V[0]) && zeroed(V[1])));
+    auto mark_zeroed = [&]() {
+      if (open_zt)
+        for (int d = 0; d < 2; ++d)
+          if (!zeroed(V[d])) s->zhalo_zeroed.push_back(V[d]);
+    };
     const int zs = seam ? 4 : ZBAND;
     pp.zskip_x = R_(zs, L.n[0] - 1 - zs);
     pp.zskip_y = R_(zs, L.n[1] - 1 - zs);
@@ -1335,6 +1349,7 @@ struct Fused {
         pp.xchng_vctr_alng(V, 1);
         pp.xchng_vctr_nrml(V, 0);
       }
+      mark_zeroed();
       ProfScope ps(s, "fused_fct");
       if (use_int) {
         RecGroup grp(s);
@@ -1345,6 +1360,7 @@ struct Fused {
       if (it != passes - 1) {
         ProfScope ps(s, "fused_halo");
         pp.xchng_vctr_nrml(V, 0);
+        mark_zeroed();
       }
       if (use_int) {
         ProfScope ps(s, "fused_donor_shell");

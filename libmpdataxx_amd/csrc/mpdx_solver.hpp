@@ -147,6 +147,10 @@ struct mpdx_solver {
   bool use_pass2d = true;   // 2D: the pass kernel (MPDX_NO_PASS2D=1: the shell kernels over every cell)
   int *seam_flag = nullptr;
   int ix_chunk = 0;         // output planes per x chunk of the interior kernel (0: automatic)
+  // velocity arrays whose tangential z halo (open z bounds) has been
+  // zero-filled in full at least once (later fills only refill the bands
+  // next to the x and y edges)
+  std::vector<const void *> zhalo_zeroed;
   int n_cu = 256;           // compute units of the device
   // explicit forcing (mpdata_rhs): scheme, user update_rhs, forcing arrays
   int rhs_scheme = MPDX_RHS_NONE;
