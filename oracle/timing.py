@@ -6,7 +6,7 @@ solid-body-rotation workload and prints one JSON object.  Only ``bench.py``'s
 ``cpu_baseline`` leg runs it, as a child process, once per thread count (the
 OpenMP team size is fixed when a process first enters a parallel region).
 
-Timing follows the survey's plan for the reference's CPU path: one warm-up
+Timing is that of a libmpdata++ user timing concurr::openmp: one warm-up
 step (the first advance runs hook_ante_loop: halos, divergence check), then
 ``std::chrono``-style wall clock around ``advance(k)``; the median of
 ``runs`` repetitions, with k chosen so that one repetition takes about

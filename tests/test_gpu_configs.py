@@ -90,7 +90,7 @@ def _sphere(n, dtype, steps):
 def test_c2_c3_rotation_sphere_full_size(n):
     """FCT creates no new extrema (the field starts in {0, 1}); nothing
     reaches the open boundaries, so mass is conserved; FP32 stays within the
-    north-star bounds of the FP64 run (L2 <= 1e-5 relative, mass <= 1e-6)."""
+    accuracy bounds against the FP64 run (L2 <= 1e-5 relative, mass <= 1e-6)."""
     steps = 100
     print(f"C2/C3 {n}^3 FCT n_iters=3 {steps} steps FP64 and FP32", flush=True)
     s64, a64, m64_0 = _sphere(n, torch.float64, steps)
