@@ -315,6 +315,18 @@ int mpdx_set_x_chunk(mpdx_solver *h, int planes);
  * depend on it. */
 int mpdx_set_tile_y(mpdx_solver *h, int rows);
 
+/* Autotuner for one solver (no reference counterpart): tries the interior
+ * tile heights (FP32 with the default guards: 0, 32 and 16 rows; otherwise
+ * the default) with the x-chunk lengths 0, 16, 32, 64, 128, 256 (those up to
+ * the x extent rounded up to a power of two), timing steps_per_trial steps
+ * of each after two untimed ones (which capture the step graphs), and keeps
+ * the fastest pair.  The trial steps are real steps of the solver (use them
+ * as warm-up); *steps_taken returns their number, *tile_y and *x_chunk the
+ * pair kept (each output may be NULL).  A solver without the 3D interior
+ * kernel (2D, the per-pass path, options the kernel does not cover) takes
+ * no steps.  Results do not depend on the pair. */
+int mpdx_tune(mpdx_solver *h, int steps_per_trial, int64_t *steps_taken, int *tile_y, int *x_chunk);
+
 /* Ensembles: independent solvers (domains of any size, dimensionality,
  * options and precision) advanced together, as the reference advances each
  * solver object with its own advance(nt) (concurr_common.hpp:380-385).  One

@@ -36,7 +36,7 @@ EXPORTED = [
     "mpdx_set_calc_gc", "mpdx_advector_device", "mpdx_set_advector_generator",
     "mpdx_set_rhs_scheme", "mpdx_set_rhs", "mpdx_rhs_device", "mpdx_advectee_device",
     "mpdx_set_rhs_hooks", "mpdx_update_rhs", "mpdx_apply_rhs", "mpdx_set_var_dt_output", "mpdx_get_output",
-    "mpdx_advance_time", "mpdx_dt", "mpdx_set_x_chunk", "mpdx_set_tile_y",
+    "mpdx_advance_time", "mpdx_dt", "mpdx_set_x_chunk", "mpdx_set_tile_y", "mpdx_tune",
     "mpdx_ensemble_create", "mpdx_ensemble_destroy", "mpdx_ensemble_advance", "mpdx_ensemble_tune",
     "mpdx_ensemble_x_chunk", "mpdx_ensemble_tile_y", "mpdx_ensemble_stats", "mpdx_ensemble_last_error",
 ]
@@ -117,6 +117,7 @@ def lib():
         L.mpdx_set_advector_generator.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.mpdx_set_x_chunk.argtypes = [P, ctypes.c_int]
         L.mpdx_set_tile_y.argtypes = [P, ctypes.c_int]
+        L.mpdx_tune.argtypes = [P, ctypes.c_int, i64p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.mpdx_ensemble_tile_y.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.mpdx_ensemble_stats.argtypes = [P, ctypes.POINTER(ctypes.c_int), i64p]
         L.mpdx_ensemble_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(P)]
@@ -437,6 +438,15 @@ class Solver:
     def set_tile_y(self, rows):
         """Interior tile height (0 = default, 16 or 32; FP32 default guards)."""
         self._check(self._L.mpdx_set_tile_y(self._h, int(rows)))
+
+    def tune(self, steps_per_trial=3):
+        """Autotune the interior tile height and x-chunk length for this
+        grid (mpdx_tune); the trials are real steps.  Returns (steps taken,
+        tile height kept, x chunk kept)."""
+        n, ty, xc = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.mpdx_tune(self._h, int(steps_per_trial), ctypes.byref(n), ctypes.byref(ty),
+                                      ctypes.byref(xc)))
+        return n.value, ty.value, xc.value
 
 
 class Ensemble:

@@ -2981,6 +2981,70 @@ int mpdx_set_tile_y(mpdx_solver *s, int rows)
   return MPDX_OK;
 }
 
+int mpdx_tune(mpdx_solver *s, int steps_per_trial, int64_t *steps_taken, int *tile_y, int *x_chunk)
+{
+  MPDX_NEED(s, "solver handle");
+  if (steps_taken) *steps_taken = 0;
+  if (steps_per_trial < 1) { s->err = "libmpdata++: steps_per_trial must be >= 1"; return MPDX_EINVAL; }
+  auto report = [&]() {
+    if (tile_y) *tile_y = s->tile_y;
+    if (x_chunk) *x_chunk = s->ix_chunk;
+  };
+  // only the 3D interior kernel has these knobs
+  if (s->active_path != MPDX_PATH_FUSED || s->L.nd != 3 || !interior_supported(s->p, s->L)) {
+    report();
+    return MPDX_OK;
+  }
+  int span = 16;
+  while (span < s->L.n[0] - 4) span *= 2;
+  const bool tiles = s->esz == 4 && s->p.guard == MPDX_GUARD_F64;
+  std::vector<std::pair<int, int>> trials;   // (tile height, x chunk)
+  for (int ty : tiles ? std::vector<int>{0, 32, 16} : std::vector<int>{0})
+    for (int c : {0, 16, 32, 64, 128, 256})
+      if (c <= span) trials.emplace_back(ty, c);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  HIPCHK(hipEventCreate(&t0));
+  if (hipEventCreate(&t1) != hipSuccess) {
+    (void)hipEventDestroy(t0);
+    s->err = "libmpdata++: cannot create timing events";
+    return MPDX_EHIP;
+  }
+  const std::pair<int, int> start(s->tile_y, s->ix_chunk);
+  std::pair<int, int> best = start;
+  float best_ms = -1;
+  int64_t taken = 0;
+  int rc = 0;
+  for (const auto &tr : trials) {
+    if ((rc = mpdx_set_tile_y(s, tr.first)) || (rc = mpdx_set_x_chunk(s, tr.second))) break;
+    if ((rc = mpdx_advance(s, 2))) break;   // captures the step graphs
+    taken += 2;
+    float ms = 0;
+    if (hipEventRecord(t0, s->stream) != hipSuccess) rc = MPDX_EHIP;
+    if (!rc) rc = mpdx_advance(s, steps_per_trial);
+    if (!rc && (hipEventRecord(t1, s->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess
+                || hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
+      rc = MPDX_EHIP;
+    if (rc) break;
+    taken += steps_per_trial;
+    if (best_ms < 0 || ms < best_ms) {
+      best_ms = ms;
+      best = tr;
+    }
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (steps_taken) *steps_taken = taken;
+  if (rc == MPDX_EHIP && s->err.empty()) s->err = "libmpdata++: HIP error while tuning";
+  // keep the fastest pair (the starting one if no trial finished)
+  if (best_ms < 0) best = start;
+  const std::string err = s->err;
+  int rc2 = mpdx_set_tile_y(s, best.first);
+  if (!rc2) rc2 = mpdx_set_x_chunk(s, best.second);
+  if (rc) s->err = err;
+  report();
+  return rc ? rc : rc2;
+}
+
 int mpdx_ensemble_create(mpdx_solver *const *members, int n, mpdx_ensemble **out)
 {
   MPDX_NEED(out, "output pointer");

@@ -72,6 +72,8 @@ def test_null_handles_are_errors():
         "mpdx_time": (None, ctypes.byref(d)),
         "mpdx_timestep": (None, ctypes.byref(i64)),
         "mpdx_set_x_chunk": (None, 0),
+        "mpdx_set_tile_y": (None, 0),
+        "mpdx_tune": (None, 1, None, None, None),
         "mpdx_profile": (None, 1),
         "mpdx_profile_read": (None, 0, None, 0, None, None),
         "mpdx_set_calc_gc": (None, M.CALC_GC_FN(), None),
