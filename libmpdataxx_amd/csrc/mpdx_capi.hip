@@ -32,6 +32,11 @@ namespace mpdx {
       const InteriorTask<R, G, F, I, S, Z, T, W> *, const unsigned *, int);
 MPDX_INTERIOR_F32_FAST(MPDX_EXTERN)
 #undef MPDX_EXTERN
+#define MPDX_EXTERN_MIX(R, G, T, W)                                                            \
+  extern template __global__ void k_task_list<InteriorMixTask<R, G, T, W>>(                     \
+      const InteriorMixTask<R, G, T, W> *, const unsigned *, int);
+MPDX_INTERIOR_MIX_F32_FAST(MPDX_EXTERN_MIX)
+#undef MPDX_EXTERN_MIX
 }  // namespace mpdx
 
 using namespace mpdx;
@@ -1204,6 +1209,18 @@ struct Fused {
   void launch_interior(const InteriorArgs<R> &a, dim3 g)
   {
     if (s->rec) {
+      // FP32 default guards: the mixed-variant task, which merges with
+      // every other member's interior pass into one batched launch
+      if constexpr (std::is_same<R, float>::value && std::is_same<G, GuardFast>::value && (TYV == 32 || TYV == 16)
+                    && interior_mix_variant(FCT, IGA, STOREV, FUSE0)) {
+        if (s->mix_tasks) {
+          InteriorMixTask<R, G, TYV, WXY> t{};
+          t.a = a;
+          t.variant = (int)FCT | (int)IGA << 1 | (int)STOREV << 2 | (int)FUSE0 << 3;
+          record(s, t, g);
+          return;
+        }
+      }
       InteriorTask<R, G, FCT, IGA, STOREV, FUSE0, TYV, WXY> t{};
       t.a = a;
       record(s, t, g);
@@ -2465,6 +2482,7 @@ int mpdx_create(const mpdx_params *p, mpdx_solver **out)
   if (off("MPDX_NO_SEAM")) s->use_seam = false;
   if (off("MPDX_NO_PASS2D")) s->use_pass2d = false;
   if (off("MPDX_BATCH_EQNS")) s->batch_eqns = true;   // (a switch that turns the batched path on)
+  if (off("MPDX_NO_TASK_MIX")) s->mix_tasks = false;
   {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&

@@ -504,6 +504,13 @@ template <bool FCT, bool FUSE0> struct ILds {
                        VY = FZ + RF, VZ = VY + 1, BU = VY + 2, BD = VY + 3, N = VY + (FCT ? 4 : 0);
 };
 
+// LDS elements of an interior block: the planes, and with FUSE0 one more
+// tile of psi^1 columns
+template <bool FCT, bool FUSE0, int TYV> constexpr int interior_lds_elems()
+{
+  return ILds<FCT, FUSE0>::N * ITile<TYV>::PLANE + (FUSE0 ? ITile<TYV>::Y * ITile<TYV>::Z : 0);
+}
+
 template <class R>
 struct InteriorArgs {
   R *out;
@@ -623,7 +630,14 @@ template <class R, int TYV> constexpr int interior_waves_per_eu()
 // block blk of the launch over a's tiles (a launch of 8 * ceil(tiles / 8)
 // blocks; blk % 8 is the XCD the block runs on when the launch starts on a
 // multiple of 8)
-template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV, bool WXY>
+// the LDS array of an interior block, one per element type and size: a
+// kernel whose blocks run one of several variants (InteriorMixTask) gives
+// them all the largest size and so holds one array
+template <class R, int N> __shared__ R interior_lds[N];
+
+// LDS_N: the size of the block's LDS array (0: interior_lds_elems of this
+// variant)
+template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV, bool WXY, int LDS_N = 0>
 __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned blk)
 {
   constexpr int TY = ITile<TYV>::Y, TZ = ITile<TYV>::Z, PW = ITile<TYV>::PW, PL = ITile<TYV>::PLANE;
@@ -631,7 +645,8 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
   constexpr int IL_PSI = IL::PSI, IL_UX = IL::UX, IL_UY = IL::UY, IL_UZ = IL::UZ, IL_PN = IL::PN,
                 IL_FY = IL::FY, IL_FZ = IL::FZ, IL_VY = IL::VY, IL_VZ = IL::VZ, IL_BU = IL::BU, IL_BD = IL::BD,
                 IL_N = IL::N;
-  __shared__ R lds[IL_N * PL + (FUSE0 ? TY * TZ : 0)];
+  static_assert(LDS_N == 0 || LDS_N >= interior_lds_elems<FCT, FUSE0, TYV>(), "LDS array too small");
+  constexpr int LN = LDS_N ? LDS_N : interior_lds_elems<FCT, FUSE0, TYV>();
 
   // XCD-aware tile order: workgroup w runs on XCD w % 8, and XCD x takes the
   // x-th contiguous run of tiles (z fastest), so neighbouring tiles that
@@ -746,11 +761,11 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
     int c;
     if (r < 2 * PW) c = (r < PW ? 0 : (TY + 1) * PW) + r % PW;
     else c = (1 + (r - 2 * PW) / 2) * PW + ((r - 2 * PW) % 2 ? PW - 1 : 0);
-    lds[pl * PL + c] = R(0);
+    interior_lds<R, LN>[pl * PL + c] = R(0);
   }
   __syncthreads();
   const int ti = (ty + 1) * PW + tz + 1;
-  auto S = [&](int plane, int dy, int dz) -> R & { return lds[plane * PL + ti + dy * PW + dz]; };
+  auto S = [&](int plane, int dy, int dz) -> R & { return interior_lds<R, LN>[plane * PL + ti + dy * PW + dz]; };
 
   // FUSE0 frame job of this thread: one psi^n, GC_y or GC_z value outside
   // the tile per plane (rows y = -1 and Y, columns z = -1 and Z of psi^n;
@@ -804,7 +819,7 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
     S(IL_UX + 0, 0, 0) = ux[0];
     S(IL_UY + 0, 0, 0) = uy[0];
     S(IL_UZ + 0, 0, 0) = uz[0];
-    lds[fr_lds] = FLD(0) * fr_sc;
+    interior_lds<R, LN>[fr_lds] = FLD(0) * fr_sc;
     // plane 1 for the first step's publish
     n_pn = LDs(g_pn); n_ux = LD(g_ux); n_uy = LD(g_uy); n_uz = LD(g_uz);
     fr_g += dstep(x0 - 1);
@@ -857,7 +872,7 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
       S(IL_UY + su(1), 0, 0) = n_uy;
       S(IL_UZ + su(1), 0, 0) = n_uz;
       S(IL_PN + su(1), 0, 0) = pn[s6(1)];
-      lds[fr_lds + (fr_lds < IL_N * PL ? su(1) * PL : 0)] = n_fr * fr_sc;
+      interior_lds<R, LN>[fr_lds + (fr_lds < IL_N * PL ? su(1) * PL : 0)] = n_fr * fr_sc;
       const long ds = dstep(x0 + q);   // to plane q + 2
       if (q + 1 < nsteps) { g_pn += ds; n_pn = LDs(g_pn); }
       if (q + 2 < nsteps) {
@@ -948,7 +963,7 @@ __device__ __forceinline__ void fz_interior_block(InteriorArgs<R> a, unsigned bl
         // tile row lies in one wave), V_y from the published plane.
         const int ln = (int)__lane_id();
         const R ex = __shfl(vx, ln + zx1), exp_ = __shfl(vx_[s2(-1)], ln + zx1), ez = __shfl(vz, ln + zx2);
-        const R eyp = lds[IL_VY * PL + ti + zx1], eym = lds[IL_VY * PL + ti - PW + zx1];
+        const R eyp = interior_lds<R, LN>[IL_VY * PL + ti + zx1], eym = interior_lds<R, LN>[IL_VY * PL + ti - PW + zx1];
         if (z == -1) vzm = ez - (exp_ - ex) - (eym - eyp);
         if (z == n2) vzp = ez + (exp_ - ex) + (eym - eyp);
       }

@@ -6,7 +6,8 @@
 // default scheduler aims for; the 24-row tiles of the later corrective
 // passes are compiled for two 768-thread blocks per CU (6 waves per SIMD,
 // interior_waves_per_eu).  Both the single launches (k_fz_interior) and
-// the batched ones of ensembles (k_task_list over InteriorTask) are here.
+// the batched ones of ensembles (k_task_list over InteriorTask and
+// InteriorMixTask) are here.
 // The FP64 and other-guard variants keep the default scheduler (with
 // max-ILP they need up to 250 VGPRs or spill).
 #define MPDX_INTERIOR_TU 1
@@ -20,4 +21,9 @@ namespace mpdx {
       const InteriorTask<R, G, F, I, S, Z, T, W> *, const unsigned *, int);
 MPDX_INTERIOR_F32_FAST(MPDX_INST)
 #undef MPDX_INST
+#define MPDX_INST_MIX(R, G, T, W)                                                             \
+  template __global__ void k_task_list<InteriorMixTask<R, G, T, W>>(const InteriorMixTask<R, G, T, W> *, \
+                                                                    const unsigned *, int);
+MPDX_INTERIOR_MIX_F32_FAST(MPDX_INST_MIX)
+#undef MPDX_INST_MIX
 }  // namespace mpdx

@@ -30,3 +30,12 @@
   MPDX_IV_Z(X, true, true)                                            \
   MPDX_IV_Z(X, false, false)                                          \
   MPDX_IV_Z(X, false, true)
+
+// the mixed-variant interior tasks of ensembles (InteriorMixTask): FP32
+// default guards, both tile heights of batched launches, both x / y bound
+// treatments.  M(R, G, TY, WXY) is applied to each.
+#define MPDX_INTERIOR_MIX_F32_FAST(M)                                 \
+  M(float, ::mpdx::GuardFast, 32, false)                              \
+  M(float, ::mpdx::GuardFast, 32, true)                               \
+  M(float, ::mpdx::GuardFast, 16, false)                              \
+  M(float, ::mpdx::GuardFast, 16, true)

@@ -236,4 +236,41 @@ struct TaskWaves<InteriorTask<R, G, FCT, IGA, STOREV, FUSE0, TYV, WXY>> {
   static constexpr int v = interior_waves_per_eu<R, TYV>();
 };
 
+// An interior pass of any option variant (variant = FCT | IGA << 1 |
+// STOREV << 2 | FUSE0 << 3) with one element type, guard, tile height and
+// x / y treatment, so that one batched launch holds the interior passes of
+// every member whatever options and pass each is at.  The variants share
+// one LDS array of the largest size.  iga takes only its single corrective
+// pass (FUSE0 without STOREV; iga allows n_iters <= 2), so ten variants.
+constexpr bool interior_mix_variant(bool fct, bool iga, bool storev, bool fuse0)
+{
+  (void)fct;
+  return !iga || (fuse0 && !storev);
+}
+template <class R, class G, int TYV, bool WXY> struct InteriorMixTask : TaskGrid {
+  static constexpr int kThreads = ITile<TYV>::Y * ITile<TYV>::Z, kBlockY = ITile<TYV>::Y;
+  static constexpr int cmax(int x, int y) { return x > y ? x : y; }
+  static constexpr int kLds = cmax(cmax(interior_lds_elems<true, true, TYV>(), interior_lds_elems<true, false, TYV>()),
+                                   cmax(interior_lds_elems<false, true, TYV>(), interior_lds_elems<false, false, TYV>()));
+  InteriorArgs<R> a;
+  int variant;
+  __device__ __forceinline__ void run(TaskBlk b) const
+  {
+    switch (variant) {
+#define MPDX_MIX(F, I, S, Z)                                                                     \
+  case (F) | (I) << 1 | (S) << 2 | (Z) << 3:                                                     \
+    fz_interior_block<R, G, F, I, S, Z, TYV, WXY, kLds>(a, b.x);                                 \
+    break;
+      MPDX_MIX(1, 0, 1, 1) MPDX_MIX(1, 0, 1, 0) MPDX_MIX(1, 0, 0, 1) MPDX_MIX(1, 0, 0, 0)
+      MPDX_MIX(0, 0, 1, 1) MPDX_MIX(0, 0, 1, 0) MPDX_MIX(0, 0, 0, 1) MPDX_MIX(0, 0, 0, 0)
+      MPDX_MIX(1, 1, 0, 1) MPDX_MIX(0, 1, 0, 1)
+#undef MPDX_MIX
+      default: break;
+    }
+  }
+};
+template <class R, class G, int TYV, bool WXY> struct TaskWaves<InteriorMixTask<R, G, TYV, WXY>> {
+  static constexpr int v = interior_waves_per_eu<R, TYV>();
+};
+
 }  // namespace mpdx
