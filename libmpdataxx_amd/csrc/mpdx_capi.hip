@@ -1258,6 +1258,13 @@ struct Fused {
   void launch_pass2d(const Pass2dArgs<R> &a, dim3 g)
   {
     if (s->rec) {
+      if (s->mix_tasks) {   // merges with every other 2D member's pass
+        Pass2dMixTask<R, G> t{};
+        t.a = a;
+        t.variant = (int)FCT | (int)IGA << 1;
+        record(s, t, g);
+        return;
+      }
       Pass2dTask<R, G, FCT, IGA> t{};
       t.a = a;
       record(s, t, g);

@@ -41,11 +41,23 @@ struct Pass2dArgs {
   int fuse0, storev, wrap;
 };
 
+// the LDS planes of a 2D pass block (K: psi^n, psi^k, the two advector
+// components, the two velocity components, beta up and down), one set per
+// element type, so that a kernel whose blocks run one of several variants
+// (Pass2dMixTask) holds one set
+template <class R, int K> __shared__ R pass2d_lds[kP2W * kP2W];
+
 template <class R, class G, bool FCT, bool IGA>
 __device__ __forceinline__ void fz_pass2d_block(const Pass2dArgs<R> &a, unsigned blk)
 {
-  __shared__ R s_pn[kP2W * kP2W], s_pk[kP2W * kP2W], s_ux[kP2W * kP2W], s_uy[kP2W * kP2W];
-  __shared__ R s_vx[kP2W * kP2W], s_vy[kP2W * kP2W], s_bu[kP2W * kP2W], s_bd[kP2W * kP2W];
+#define s_pn (pass2d_lds<R, 0>)
+#define s_pk (pass2d_lds<R, 1>)
+#define s_ux (pass2d_lds<R, 2>)
+#define s_uy (pass2d_lds<R, 3>)
+#define s_vx (pass2d_lds<R, 4>)
+#define s_vy (pass2d_lds<R, 5>)
+#define s_bu (pass2d_lds<R, 6>)
+#define s_bd (pass2d_lds<R, 7>)
   const Layout &L = a.L;
   const int n0 = L.n[0], n1 = L.n[1];
   const int by = (int)(blk % (unsigned)a.nb[0]), bx = (int)(blk / (unsigned)a.nb[0]);
@@ -180,6 +192,14 @@ __device__ __forceinline__ void fz_pass2d_block(const Pass2dArgs<R> &a, unsigned
   }
   const R r = pc + ((-f0 + f1) + (-f2 + f3)) / 1;
   a.out[L.org + (long)gx * L.st[0] + (long)gy * L.st[1]] = r * a.out_scale;
+#undef s_pn
+#undef s_pk
+#undef s_ux
+#undef s_uy
+#undef s_vx
+#undef s_vy
+#undef s_bu
+#undef s_bd
 }
 
 template <class R, class G, bool FCT, bool IGA>

@@ -111,8 +111,9 @@ struct mpdx_solver {
   mpdx_fused_state *fz = nullptr;
   mpdx::TaskRecorder *rec = nullptr;   // set: launches are recorded, not made
   int tile_y = 0;            // interior tile height (0: the element type's default)
-  // recorded FP32 interior passes as mixed-variant tasks (one batched launch
-  // for every member's pass; MPDX_NO_TASK_MIX=1: one launch per variant)
+  // recorded FP32 interior passes and 2D passes as mixed-variant tasks (one
+  // batched launch for every member's pass; MPDX_NO_TASK_MIX=1: one launch
+  // per variant)
   bool mix_tasks = true;
   // one captured time step per time-level parity and seam mode (fused
   // path, index parity + 2 seam; + 6: graph_steps steps in one graph); with

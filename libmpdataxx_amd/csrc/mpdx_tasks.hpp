@@ -223,6 +223,25 @@ struct Pass2dTask : TaskGrid {
   __device__ void run(TaskBlk b) const { fz_pass2d_block<R, G, FCT, IGA>(a, b.x); }
 };
 
+// a 2D pass of any option variant (variant = FCT | IGA << 1): one batched
+// launch holds the pass of every 2D member whatever its options (the
+// variants share one set of LDS planes)
+template <class R, class G> struct Pass2dMixTask : TaskGrid {
+  static constexpr int kThreads = kP2Threads;
+  Pass2dArgs<R> a;
+  int variant;
+  __device__ __forceinline__ void run(TaskBlk b) const
+  {
+    switch (variant) {
+      case 0: fz_pass2d_block<R, G, false, false>(a, b.x); break;
+      case 1: fz_pass2d_block<R, G, true, false>(a, b.x); break;
+      case 2: fz_pass2d_block<R, G, false, true>(a, b.x); break;
+      case 3: fz_pass2d_block<R, G, true, true>(a, b.x); break;
+      default: break;
+    }
+  }
+};
+
 // one launch of the interior corrective pass (k_fz_interior); its block
 // count is a multiple of 8, so a batch of these keeps the XCD tile order
 template <class R, class G, bool FCT, bool IGA, bool STOREV, bool FUSE0, int TYV, bool WXY>
