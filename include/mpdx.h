@@ -351,8 +351,11 @@ int mpdx_ensemble_stats(const mpdx_ensemble *e, int *launches, int64_t *tasks);
  * extent rounded up to a power of two) tries the interior tile heights
  * (FP32 default-guard classes: 32 and 16 rows) with the x-chunk lengths 0,
  * 16, 32, 64, 128, 256 (those that fit), timing steps_per_trial ensemble
- * steps of each after two untimed ones, and keeps the fastest pair.  The
- * trial steps are real steps of every member (use them as warm-up);
+ * steps of each after two untimed ones, and keeps the fastest pair; if the
+ * classes chose different heights, it then times one height (the default,
+ * then 16 rows) for all of them against that choice and keeps the fastest
+ * (one height lets every member's interior pass share a batched launch).
+ * The trial steps are real steps of every member (use them as warm-up);
  * *steps_taken returns their number. */
 int mpdx_ensemble_tune(mpdx_ensemble *e, int steps_per_trial, int64_t *steps_taken);
 /* x-chunk length and interior tile height chosen for member i */

@@ -3216,6 +3216,49 @@ int mpdx_ensemble_tune(mpdx_ensemble *e, int steps_per_trial, int64_t *steps_tak
     }
     if (rc) break;
   }
+  // classes that chose different tile heights run their interior passes as
+  // different batched launches (a mixed-variant task holds one tile
+  // height): time the chosen set against one height for every class that
+  // can take it (keeping the chunks), and keep the fastest
+  auto trial = [&](float *ms) {
+    int r = 0;
+    if (hipStreamSynchronize(e->stream) != hipSuccess) r = MPDX_EHIP;
+    drop_plans(e);
+    if (!r) r = ensemble_run(e, 2);
+    if (!r && hipEventRecord(t0, e->stream) != hipSuccess) r = MPDX_EHIP;
+    if (!r) r = ensemble_run(e, steps_per_trial);
+    if (!r && hipEventRecord(t1, e->stream) != hipSuccess) r = MPDX_EHIP;
+    if (!r && (hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(ms, t0, t1) != hipSuccess)) r = MPDX_EHIP;
+    if (!r) taken += 2 + steps_per_trial;
+    return r;
+  };
+  std::vector<mpdx_solver *> tunable;
+  for (auto &kv : cls)
+    for (mpdx_solver *s : kv.second)
+      if (s->esz == 4 && s->p.guard == MPDX_GUARD_F64) tunable.push_back(s);
+  bool mixed = false;
+  for (mpdx_solver *s : tunable) mixed = mixed || s->tile_y != tunable.front()->tile_y;
+  if (!rc && mixed) {
+    std::vector<int> chosen;
+    for (mpdx_solver *s : tunable) chosen.push_back(s->tile_y);
+    float best_ms = 0;
+    int best_ty = -1;   // -1: the per-class choice
+    rc = trial(&best_ms);
+    for (int ty : {0, 16}) {
+      if (rc) break;
+      for (mpdx_solver *s : tunable) s->tile_y = ty;
+      float ms = 0;
+      rc = trial(&ms);
+      if (!rc && ms < best_ms) {
+        best_ms = ms;
+        best_ty = ty;
+      }
+    }
+    for (size_t i = 0; i < tunable.size(); ++i) {
+      tunable[i]->tile_y = best_ty < 0 ? chosen[i] : best_ty;
+      drop_step_graphs(tunable[i]);
+    }
+  }
   (void)hipStreamSynchronize(e->stream);
   drop_plans(e);
   (void)hipEventDestroy(t0);
