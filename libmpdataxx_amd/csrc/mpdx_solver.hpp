@@ -127,7 +127,13 @@ struct mpdx_solver {
   // overhead would dominate (0: chosen at the first capture; an even number,
   // so the time levels return to the same parity)
   int graph_steps = 0;
-  bool batch_eqns = false;   // MPDX_BATCH_EQNS=1: 2 % faster with no boundary shell, slower beside one; velocity scratch per equation
+  bool batch_eqns = false;   // MPDX_BATCH_EQNS=1: batched equations always (slower beside a boundary shell); velocity scratch per equation
+  // batched equations by default where they pay: all-cyclic seam mode (no
+  // boundary shell) with the extra velocity scratch in an eighth of the
+  // device's memory (MPDX_NO_AUTO_BATCH=1: off)
+  bool auto_batch_eqns = true;
+  int auto_batch_fits = -1;  // the scratch fits the bound (-1: not tested yet)
+  bool rec_tile16 = false;   // recording a batched-equation step: 16-row tiles when tile_y is 0
   // auxiliary stream (boundary shell next to the interior kernel) and its
   // fork / join events
   hipStream_t aux = nullptr;
